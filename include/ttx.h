@@ -1,0 +1,140 @@
+/*
+ * ttx -- MI355X-native TT-SVD / TR-SVD engine, C ABI.
+ *
+ * Every entry point takes plain pointers and sizes.  Tensors are FP32, column-major: element
+ * (i1, ..., id) of an N1 x ... x Nd tensor lives at i1 + N1 (i2 + N2 (i3 + ...)).  Device pointers
+ * are HIP device memory of the current device; all work is queued on the handle's stream and the
+ * call returns after the stream has been synchronised, unless stated otherwise.
+ *
+ * Core layouts (the reference's):
+ *   TT core k (k = 0..d-1) is the (r_k N_{k+1}) x r_{k+1} column-major matrix of the 3-way array
+ *   (r_k, N_{k+1}, r_{k+1}) (ttdecomposition/high_order_GPUs/ttdec6_singleGPU.cu:194-263, contraction
+ *   order :141-172), with r_0 = r_d = 1.
+ *   TR core 0 is N1 x (r_0 r_1) with column a + r_0 b (trdecomposition/high_order_GPUs/
+ *   trdec6_singleGPU.cu:865,873); cores 1..d-2 as TT; core d-1 is r_{d-1} x (N_d r_0).
+ */
+#ifndef TTX_H
+#define TTX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TTX_MAX_ORDER 16
+#define TTX_MAX_ELL 64 /* sketch width and rank ceiling of one step */
+
+typedef enum {
+  TTX_OK = 0,
+  TTX_ERR_BAD_ARG = 1,
+  TTX_ERR_OOM = 2,
+  TTX_ERR_NO_CONVERGE = 3,
+  TTX_RESIDUAL_FALLBACK_USED = 4, /* success; the adaptive range finder had to add passes */
+  TTX_ERR_HIP = 5,
+  TTX_ERR_UNSUPPORTED = 6
+} ttx_status;
+
+typedef enum { TTX_PREC_FP32 = 0, TTX_PREC_BF16 = 1, TTX_PREC_FP16 = 2 } ttx_precision;
+typedef enum { TTX_RANK_FIXED = 0, TTX_RANK_DELTA = 1, TTX_RANK_LEARNED = 2 } ttx_rank_mode;
+typedef enum { TTX_METHOD_AUTO = 0, TTX_METHOD_EXACT = 1, TTX_METHOD_RSVD = 2 } ttx_method;
+typedef enum { TTX_PATH_EXACT_M = 0, TTX_PATH_EXACT_N = 1, TTX_PATH_RSVD = 2 } ttx_path;
+
+typedef struct ttx_handle_s* ttx_handle;
+
+typedef struct {
+  int32_t order;                    /* d, 2..TTX_MAX_ORDER */
+  int64_t dims[TTX_MAX_ORDER];      /* N_1..N_d */
+  int32_t ranks[TTX_MAX_ORDER + 1]; /* r_0..r_d: fixed ranks, or ceilings in DELTA / LEARNED mode.
+                                       TT: r_0 = r_d = 1.  TR: r_0 = r_d = ring rank. */
+  int32_t oversampling;             /* s; sketch width ell = r + s */
+  int32_t power_iters;              /* q */
+  uint64_t omega_seed;              /* Philox key of the Gaussian sketch */
+  int32_t precision;                /* ttx_precision */
+  int32_t method;                   /* ttx_method: AUTO takes the exact Gram path when ell >= min(m, n) */
+  int32_t rank_mode;                /* ttx_rank_mode */
+  double eps;                       /* relative accuracy of DELTA / LEARNED ranks: delta = eps / sqrt(d-1) */
+  int32_t flags;                    /* TTX_FLAG_* */
+} ttx_params;
+
+#define TTX_FLAG_FAST_RANGE 1 /* BF16 mode: single bf16 MFMA per product in the range finder (no hi/lo split) */
+#define TTX_FLAG_TIMING 2     /* record per-step times into the report */
+
+typedef struct {
+  int32_t ranks[TTX_MAX_ORDER + 1];   /* ranks used */
+  int32_t ell[TTX_MAX_ORDER];         /* sketch width per step (0 on exact paths) */
+  int32_t path[TTX_MAX_ORDER];        /* ttx_path per step */
+  int32_t power_iters[TTX_MAX_ORDER]; /* power iterations run per step */
+  int32_t zero_pivots[TTX_MAX_ORDER]; /* CholeskyQR pivots that were not positive */
+  int32_t jacobi_sweeps[TTX_MAX_ORDER];
+  double norm2_in[TTX_MAX_ORDER];     /* ||A_k||_F^2 of the unfolding entering step k (when known, else -1) */
+  double step_ms[TTX_MAX_ORDER];      /* with TTX_FLAG_TIMING */
+  double total_ms;                    /* with TTX_FLAG_TIMING: whole call on the device */
+  int64_t bytes_alg;                  /* algorithmic HBM bytes of the passes that ran */
+} ttx_report;
+
+/* handle ---------------------------------------------------------------------------------------- */
+ttx_status ttx_create(ttx_handle* h, void* hip_stream);
+ttx_status ttx_destroy(ttx_handle h);
+ttx_status ttx_set_stream(ttx_handle h, void* hip_stream);
+const char* ttx_status_string(int32_t status);
+const char* ttx_last_error(ttx_handle h);
+int32_t ttx_version(void);
+
+/* default parameters: ranks 1, s = 10, q = 2, seed 1234, BF16, AUTO, FIXED */
+void ttx_params_init(ttx_params* p, int32_t order, const int64_t* dims, const int32_t* ranks);
+/* number of floats of core k with the ranks in p (TT layout, or TR layout when tr != 0) */
+int64_t ttx_core_size(const ttx_params* p, int32_t tr, int32_t k);
+
+/* decompositions -------------------------------------------------------------------------------- */
+/* Replaces ttdec_half_lu_device_6 (ttdecomposition/high_order_GPUs/ttdec6_singleGPU.cu:799-958)
+ * generalised to order d.  d_A: device input (read only).  d_cores[k]: caller-allocated device
+ * buffers of ttx_core_size(p, 0, k) floats (with the ceiling ranks).  h_sigma (optional, host):
+ * (d-1) x TTX_MAX_ELL singular values per step, descending, zero-filled. */
+ttx_status ttx_tt_svd(ttx_handle h, const float* d_A, const ttx_params* p, float* const* d_cores, float* h_sigma,
+                      ttx_report* rep);
+/* Replaces ttdec_half_lu_device_6 of trdecomposition/high_order_GPUs/trdec6_singleGPU.cu:827-1004:
+ * first step of rank r_0 r_1, the 231 relayout, then the TT sweep; TR core layouts above. */
+ttx_status ttx_tr_svd(ttx_handle h, const float* d_A, const ttx_params* p, float* const* d_cores, float* h_sigma,
+                      ttx_report* rep);
+/* Replaces basicRandSVD_LU (ttdecomposition/third_order_GPUs/opt/ttdec_gpu_float_rsvd_lu_half.cu:557-775):
+ * ell = k + s; d_U m x ell, d_S ell, d_SVT ell x n (all device, column-major), A ~= U diag(S)... with
+ * SVT = diag(S) V^T. */
+ttx_status ttx_rsvd(ttx_handle h, const float* d_A, int64_t m, int64_t n, int32_t k, int32_t s, int32_t q,
+                    uint64_t seed, int32_t precision, float* d_U, float* d_S, float* d_SVT);
+/* One sweep step on an m x n unfolding: core (m x r) and remainder (r x n); r chosen as in p. */
+ttx_status ttx_step(ttx_handle h, const float* d_A, int64_t m, int64_t n, int32_t r, int32_t s, int32_t q,
+                    uint64_t seed, uint32_t stream_id, int32_t precision, int32_t method, float* d_core,
+                    float* d_next, float* h_sigma, int32_t* path_out);
+
+/* relayout and utilities ------------------------------------------------------------------------- */
+/* tensorToMode231 (trdecomposition/high_order_GPUs/trdec6_singleGPU.cu:49-58): (r0, r1, n) -> (r1, n, r0) */
+ttx_status ttx_permute_231(ttx_handle h, const float* d_in, float* d_out, int64_t r0, int64_t r1, int64_t n);
+/* element e = scale * normal4(seed, stream, e / 4, 0)[e % 4] (Philox4x32-10 + Box-Muller) */
+ttx_status ttx_fill_normal(ttx_handle h, float* d_out, int64_t count, uint64_t seed, uint32_t stream, float scale);
+/* the sketch matrix Omega, n x ell row-major (entry (j, c) at j * ell + c) */
+ttx_status ttx_omega(ttx_handle h, float* d_out, int64_t n, int32_t ell, uint64_t seed, uint32_t stream);
+/* synthetic tensors: cores N(0, std^2) from streams 0x100 + k (TT) / 0x200 + k (TR), contracted,
+ * then A += noise_rel * rms(A) * N(0,1) from stream 0xFFFF.  d_cores (optional) receives the cores. */
+ttx_status ttx_tt_generate(ttx_handle h, int32_t order, const int64_t* dims, const int32_t* ranks, float std_,
+                           float noise_rel, uint64_t seed, float* d_A, float* const* d_cores);
+ttx_status ttx_tr_generate(ttx_handle h, int32_t order, const int64_t* dims, const int32_t* ranks, float std_,
+                           float noise_rel, uint64_t seed, float* d_A, float* const* d_cores);
+/* ||A - reconstruction(cores)||_F / ||A||_F in FP64 without writing the reconstruction (calMSE,
+ * ttdecomposition/high_order_GPUs/ttdec6_singleGPU.cu:134-182, non-destructive) */
+ttx_status ttx_tt_rel_error(ttx_handle h, int32_t order, const int64_t* dims, const int32_t* ranks,
+                            const float* const* d_cores, const float* d_A, double* err);
+ttx_status ttx_tr_rel_error(ttx_handle h, int32_t order, const int64_t* dims, const int32_t* ranks,
+                            const float* const* d_cores, const float* d_A, double* err);
+/* contract cores into a full tensor (d_out: prod N floats) */
+ttx_status ttx_tt_full(ttx_handle h, int32_t order, const int64_t* dims, const int32_t* ranks,
+                       const float* const* d_cores, float* d_out);
+ttx_status ttx_tr_full(ttx_handle h, int32_t order, const int64_t* dims, const int32_t* ranks,
+                       const float* const* d_cores, float* d_out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TTX_H */

@@ -1,0 +1,340 @@
+"""FP64 CPU restatement of the reference TT-SVD / TR-SVD hot path.
+
+TEST INFRASTRUCTURE ONLY.  This is the checker the HIP engine is compared against; only
+``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s CPU-baseline leg may import it, and the
+engine never calls it.
+
+Storage is column-major (Fortran order) throughout, as in the reference: a tensor element
+(i1..id) lives at i1 + N1*(i2 + N2*(i3 + ...)).  A TT core G_k is the (r_{k-1}*N_k) x r_k matrix of
+the 3-way array (r_{k-1}, N_k, r_k) (ttdec6_singleGPU.cu:194-263, contraction order :141-172).
+
+Functions and the reference code each one restates:
+
+* ``gram_eig_step``      -- longMatrixSVD_Eign_once_device, ttdec6_singleGPU.cu:510-536
+                            (Gram A A^T :388-401, syevd :334-385, top-r column reversal :274-290,
+                            projection G^T A :424-488)
+* ``rsvd_step``          -- basicRandSVD_LU, third_order_GPUs/opt/ttdec_gpu_float_rsvd_lu_half.cu:557-775
+* ``cholqr2``            -- replaces basicQR (geqrf + orgqr, rsvd_lu_half.cu:85-159); the Q factor
+                            with positive-diagonal R is unique, so both name the same matrix
+* ``tt_svd``             -- ttdec_half_lu_device_6, ttdec6_singleGPU.cu:799-958 (order d)
+* ``tr_svd``             -- ttdec_half_lu_device_6 (TR), trdec6_singleGPU.cu:827-1004
+* ``permute_231``        -- tensorToMode231, trdec6_singleGPU.cu:49-58
+* ``permute_312``        -- tensorToMode3_host, trdec6_singleGPU.cu:166-173
+* ``tt_full``/``tr_full``-- the contraction chains of calMSE / generalTTTensor / generalTRTensor,
+                            ttdec6_singleGPU.cu:134-271, trdec6_singleGPU.cu:175-329
+* ``rel_error``          -- calMSE + norm2HH, ttdec6_singleGPU.cu:122-182 (non-destructive here)
+* ``rsvd``               -- the basicRandSVD_LU interface: all ell columns of U and SVT
+* ``delta_rank``         -- the commented truncation rule,
+                            trdecomposition/third_order_GPU/unopt_GPU/trdec_gpu_float_unopt2_rank.cu:92,222-243
+"""
+import numpy as np
+
+from . import philox
+
+EXACT = "exact"
+RSVD = "rsvd"
+
+
+# ----------------------------------------------------------------------------------------------
+# small dense pieces
+# ----------------------------------------------------------------------------------------------
+
+def gauge(G, *rows_of):
+    """Fix the sign freedom of singular vectors: the largest-|.| entry of every column of G is
+    made positive, and the matching rows of each array in ``rows_of`` flip with it."""
+    G = np.array(G, dtype=np.float64, copy=True)
+    idx = np.argmax(np.abs(G), axis=0)
+    sgn = np.sign(G[idx, np.arange(G.shape[1])])
+    sgn[sgn == 0] = 1.0
+    G *= sgn[None, :]
+    out = [G]
+    for R in rows_of:
+        R = np.array(R, dtype=np.float64, copy=True)
+        R[: len(sgn)] *= sgn[:, None]
+        out.append(R)
+    return out if rows_of else G
+
+
+def cholqr2(Y):
+    """Q with orthonormal columns and upper-triangular R (positive diagonal) such that Y = Q R.
+
+    CholeskyQR2 (Fukaya, Nakatsukasa, Yanagisawa, Yamamoto 2014) in FP64; a column whose Gram
+    pivot is not positive is handled by Householder QR instead (same Q up to the sign convention,
+    which is then fixed to a positive diagonal)."""
+    Y = np.asarray(Y, dtype=np.float64)
+    try:
+        R1 = np.linalg.cholesky(Y.T @ Y).T
+        Q1 = np.linalg.solve(R1.T, Y.T).T
+        R2 = np.linalg.cholesky(Q1.T @ Q1).T
+        Q = np.linalg.solve(R2.T, Q1.T).T
+        R = R2 @ R1
+    except np.linalg.LinAlgError:
+        Q, R = np.linalg.qr(Y)
+        s = np.sign(np.diag(R))
+        s[s == 0] = 1.0
+        Q = Q * s[None, :]
+        R = R * s[:, None]
+    return Q, R
+
+
+def qr_pos(Y):
+    """Householder QR with the sign fixed so that diag(R) > 0 (the unique thin QR)."""
+    Q, R = np.linalg.qr(np.asarray(Y, dtype=np.float64))
+    s = np.sign(np.diag(R))
+    s[s == 0] = 1.0
+    return Q * s[None, :], R * s[:, None]
+
+
+def eig_desc(W):
+    """Symmetric eigen-decomposition with eigenvalues in descending order.
+
+    syevd returns ascending eigenpairs (ttdec6_singleGPU.cu:334-385); the reference then reverses
+    the column order (matrixInvertColumn, :274-290).  This returns the reversed form directly."""
+    lam, V = np.linalg.eigh(np.asarray(W, dtype=np.float64))
+    return lam[::-1].copy(), V[:, ::-1].copy()
+
+
+# ----------------------------------------------------------------------------------------------
+# one decomposition step
+# ----------------------------------------------------------------------------------------------
+
+def gram_eig_step(A, r):
+    """Exact step: G = top-r eigenvectors of A A^T (descending), A_next = G^T A.
+
+    ttdec6_singleGPU.cu:510-536.  Returns (G m x r, sigma (all min(m,n)) descending, A_next r x n).
+    When r exceeds the number of rows the extra columns of G are zero (TR first step with
+    r0*r1 > N1)."""
+    A = np.asarray(A, dtype=np.float64)
+    m = A.shape[0]
+    lam, V = eig_desc(A @ A.T)
+    lam = np.maximum(lam, 0.0)
+    G = np.zeros((m, r))
+    kk = min(r, m)
+    G[:, :kk] = V[:, :kk]
+    G = gauge(G)
+    A_next = G.T @ A
+    return G, np.sqrt(lam), A_next
+
+
+def rsvd_step(A, k, s, q, Omega):
+    """Randomized step (EDrSVD), basicRandSVD_LU, rsvd_lu_half.cu:557-775.
+
+    Y = A Omega, Q = qr(Y); q times: Z = A^T Q, Qz = qr(Z), Y = A Qz, Q = qr(Y) (:601-683);
+    B = Q^T A (:690-709); W = B B^T, eig, reverse (:717-728); SVT = U_B^T B (:734-739);
+    U = Q U_B (:745-763).  Returns (U m x ell, sigma ell, SVT ell x n)."""
+    A = np.asarray(A, dtype=np.float64)
+    Om = np.asarray(Omega, dtype=np.float64)
+    Q, _ = cholqr2(A @ Om)
+    for _ in range(q):
+        Qz, _ = cholqr2(A.T @ Q)
+        Q, _ = cholqr2(A @ Qz)
+    B = Q.T @ A
+    lam, Ub = eig_desc(B @ B.T)
+    lam = np.maximum(lam, 0.0)
+    U = Q @ Ub
+    U, Ub_t = gauge(U, Ub.T)
+    SVT = Ub_t @ B
+    return U, np.sqrt(lam), SVT
+
+
+def step(A, r, s, q, seed, stream, method=RSVD):
+    """One sweep step with the engine's path rule: the exact Gram step when ell = r + s reaches
+    min(m, n) or when ``method`` is EXACT; the randomized step otherwise."""
+    m, n = A.shape
+    ell = r + s
+    if method == EXACT or ell >= min(m, n):
+        G, sig, A_next = gram_eig_step(A, r)
+        return G, sig, A_next, EXACT
+    Om = philox.omega(seed, stream, n, ell)
+    U, sig, SVT = rsvd_step(A, r, s, q, Om)
+    return U[:, :r], sig, SVT[:r], RSVD
+
+
+def rsvd(A, k, s, q, seed, stream=0):
+    """basicRandSVD_LU's outputs (rsvd_lu_half.cu:557-775): U (m x ell), sigma (ell), SVT (ell x n),
+    ell = k + s, with the sketch drawn from the project's Philox stream."""
+    A = np.asarray(A, dtype=np.float64)
+    Om = philox.omega(seed, stream, A.shape[1], k + s)
+    return rsvd_step(A, k, s, q, Om)
+
+
+# ----------------------------------------------------------------------------------------------
+# sweeps
+# ----------------------------------------------------------------------------------------------
+
+def delta_rank(lam, total2, delta2, rmax):
+    """Rank of the delta rule on one step (Oseledets 2011, TT-SVD; the reference's commented rule
+    trdecomposition/third_order_GPU/unopt_GPU/trdec_gpu_float_unopt2_rank.cu:92,222-243 with
+    ep = eps / sqrt(d-1)): the smallest r with tail(r) <= delta2, where
+    tail(r) = max(total2 - sum_{i<=r} lam_i, sum_{r<i} lam_i), lam = sigma^2 descending and total2 =
+    ||A_k||_F^2; capped at rmax and at least 1."""
+    lam = np.maximum(np.asarray(lam, dtype=np.float64), 0.0)
+    k = len(lam)
+    r = k
+    cum = np.cumsum(lam)
+    for i in range(k):
+        tail = max(total2 - cum[i], float(lam[i + 1:].sum()))
+        if tail <= delta2:
+            r = i + 1
+            break
+    return max(1, min(r, rmax))
+
+
+def tt_svd(A, dims, ranks, s=10, q=2, seed=1234, method=RSVD, rank_mode="fixed", eps=1e-4):
+    """Order-d TT-SVD sweep (ttdec6_singleGPU.cu:799-958 generalised to order d).
+
+    ranks has d+1 entries with ranks[0] = ranks[d] = 1 (fixed ranks, or ceilings for
+    rank_mode="delta").  Returns (cores, sigmas, paths, used_ranks) where core k is the
+    (r_{k-1} N_k) x r_k matrix (the last is r_{d-1} x N_d) in FP64."""
+    d = len(dims)
+    A = np.asarray(A, dtype=np.float64)
+    cur = A.reshape(dims[0], -1, order="F")
+    norm2 = float(np.sum(A * A))
+    delta2 = eps * eps / (d - 1) * norm2
+    cores, sigmas, paths = [], [], []
+    used = [1]
+    total2 = norm2
+    for k in range(d - 1):
+        m = used[k] * dims[k]
+        cur = cur.reshape(m, -1, order="F")
+        r = ranks[k + 1]
+        if rank_mode == "delta":
+            G, sig, nxt, path = step(cur, min(r, min(cur.shape)), s, q, seed, k, method)
+            rr = delta_rank(sig ** 2, total2, delta2, G.shape[1])
+            G, nxt = G[:, :rr], nxt[:rr]
+        else:
+            G, sig, nxt, path = step(cur, r, s, q, seed, k, method)
+            rr = r
+        total2 = float(np.sum(sig[:rr] ** 2))
+        cores.append(G)
+        sigmas.append(sig)
+        paths.append(path)
+        used.append(rr)
+        cur = nxt
+    used.append(1)
+    cores.append(cur.reshape(used[d - 1], dims[d - 1], order="F"))
+    return cores, sigmas, paths, used
+
+
+def permute_231(T, r0, r1, n):
+    """tensorToMode231 (trdec6_singleGPU.cu:49-58): T1 (r0, r1, n) -> T2 (r1, n, r0), column-major;
+    T2[b + r1*j + r1*n*a] = T1[a + r0*b + r0*r1*j]."""
+    T = np.asarray(T).reshape(-1, order="F")
+    return np.ascontiguousarray(
+        T.reshape((r0, r1, n), order="F").transpose(1, 2, 0).reshape(-1, order="F"))
+
+
+def permute_312(T, r1, n, r0):
+    """tensorToMode3_host (trdec6_singleGPU.cu:166-173): (r1, n, r0) -> (r0, r1, n)."""
+    T = np.asarray(T).reshape(-1, order="F")
+    return np.ascontiguousarray(
+        T.reshape((r1, n, r0), order="F").transpose(2, 0, 1).reshape(-1, order="F"))
+
+
+def tr_svd(A, dims, ranks, s=10, q=2, seed=1234, method=RSVD):
+    """Order-d TR-SVD sweep, ttdec_half_lu_device_6 (TR), trdec6_singleGPU.cu:827-1004.
+
+    ranks has d+1 entries; ranks[0] = ranks[d] = r0 is the ring rank and the first step keeps
+    r0*r1 columns (:865).  G1 is returned as N1 x (r0*r1) with column a + r0*b (not permuted), the
+    first remainder is permuted (r0, r1, N') -> (r1, N', r0) (:873), and the last core is
+    r_{d-1} x (N_d * r0)."""
+    d = len(dims)
+    r0, r1 = ranks[0], ranks[1]
+    A1 = np.asarray(A, dtype=np.float64).reshape(dims[0], -1, order="F")
+    nrest = A1.shape[1]
+    G1, sig1, A2t, path = step(A1, r0 * r1, s, q, seed, 0, method)
+    cur = permute_231(A2t.reshape(-1, order="F"), r0, r1, nrest)
+    cores, sigmas, paths = [G1], [sig1], [path]
+    ncols = nrest * r0
+    for k in range(1, d - 1):
+        m = ranks[k] * dims[k]
+        ncols //= dims[k]
+        cur = np.asarray(cur).reshape(m, ncols, order="F")
+        G, sig, cur, path = step(cur, ranks[k + 1], s, q, seed, k, method)
+        cores.append(G)
+        sigmas.append(sig)
+        paths.append(path)
+    cores.append(np.asarray(cur).reshape(ranks[d - 1], dims[d - 1] * r0, order="F"))
+    return cores, sigmas, paths, list(ranks)
+
+
+# ----------------------------------------------------------------------------------------------
+# reconstruction and error
+# ----------------------------------------------------------------------------------------------
+
+def tt_full(cores, dims, ranks):
+    """Contract TT cores right to left (generalTTTensor / calMSE order, ttdec6_singleGPU.cu:141-172)."""
+    d = len(dims)
+    R = np.asarray(cores[-1], dtype=np.float64).reshape(ranks[d - 1], -1, order="F")
+    for k in range(d - 2, -1, -1):
+        Gk = np.asarray(cores[k], dtype=np.float64).reshape(ranks[k] * dims[k], ranks[k + 1], order="F")
+        R = (Gk @ R).reshape(ranks[k], -1, order="F")
+    return R.reshape(-1, order="F")
+
+
+def tr_full(cores, dims, ranks):
+    """TR reconstruction as calMSE (TR), trdec6_singleGPU.cu:175-228: chain G2..Gd into
+    (r1, N2..Nd, r0), permute to (r0, r1, N'), multiply by G1 (N1 x r0 r1)."""
+    d = len(dims)
+    r0, r1 = ranks[0], ranks[1]
+    R = np.asarray(cores[-1], dtype=np.float64).reshape(ranks[d - 1], -1, order="F")
+    for k in range(d - 2, 0, -1):
+        Gk = np.asarray(cores[k], dtype=np.float64).reshape(ranks[k] * dims[k], ranks[k + 1], order="F")
+        R = (Gk @ R).reshape(ranks[k], -1, order="F")
+    nrest = int(np.prod(dims[1:]))
+    mid = permute_312(R.reshape(-1, order="F"), r1, nrest, r0).reshape(r0 * r1, nrest, order="F")
+    G1 = np.asarray(cores[0], dtype=np.float64).reshape(dims[0], r0 * r1, order="F")
+    return (G1 @ mid).reshape(-1, order="F")
+
+
+def rel_error(A, Ahat):
+    """||A - Ahat||_F / ||A||_F in FP64 (calMSE, ttdec6_singleGPU.cu:134-182, without overwriting A)."""
+    A = np.asarray(A, dtype=np.float64).reshape(-1)
+    Ahat = np.asarray(Ahat, dtype=np.float64).reshape(-1)
+    return float(np.linalg.norm(A - Ahat) / np.linalg.norm(A))
+
+
+# ----------------------------------------------------------------------------------------------
+# synthetic inputs (the BASELINE distributions, drawn from Philox)
+# ----------------------------------------------------------------------------------------------
+
+DATA_SEED = 1134      # mirrors srand(1134), ttdec6_singleGPU.cu:986
+OMEGA_SEED = 1234     # mirrors the cuRAND seed, rsvd_lu_half.cu:586
+NOISE_STREAM = 0xFFFF
+
+
+def tt_core_shapes(dims, ranks):
+    return [(ranks[k] * dims[k], ranks[k + 1]) for k in range(len(dims))]
+
+
+def make_tt_cores(dims, ranks, std=1.0, seed=DATA_SEED):
+    """Core k drawn iid N(0, std^2) from Philox stream 0x100 + k (float32)."""
+    cores = []
+    for k, (a, b) in enumerate(tt_core_shapes(dims, ranks)):
+        z = philox.normal_flat(seed, 0x100 + k, a * b) * np.float32(std)
+        cores.append(z.astype(np.float32).reshape(a, b, order="F"))
+    return cores
+
+
+def make_tr_cores(dims, ranks, std=1.0, seed=DATA_SEED):
+    """Ring cores in the reference's storage: G1 is N1 x (r0 r1), G2..G_{d-1} are
+    (r_{k-1} N_k) x r_k and G_d is r_{d-1} x (N_d r0); iid N(0, std^2), Philox stream 0x200 + k."""
+    d = len(dims)
+    shapes = [(dims[0], ranks[0] * ranks[1])]
+    shapes += [(ranks[k] * dims[k], ranks[k + 1]) for k in range(1, d - 1)]
+    shapes += [(ranks[d - 1], dims[d - 1] * ranks[0])]
+    cores = []
+    for k, (a, b) in enumerate(shapes):
+        z = philox.normal_flat(seed, 0x200 + k, a * b) * np.float32(std)
+        cores.append(z.astype(np.float32).reshape(a, b, order="F"))
+    return cores
+
+
+def add_noise(A, rel, seed=DATA_SEED):
+    """A + rel * rms(A) * N(0,1) (float32), noise from Philox stream NOISE_STREAM."""
+    A = np.asarray(A, dtype=np.float32).reshape(-1)
+    if rel == 0:
+        return A.copy()
+    rms = np.float32(np.sqrt(np.mean(A.astype(np.float64) ** 2)))
+    z = philox.normal_flat(seed, NOISE_STREAM, A.size)
+    return (A + np.float32(rel) * rms * z).astype(np.float32)

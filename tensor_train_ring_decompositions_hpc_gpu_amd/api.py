@@ -1,0 +1,216 @@
+"""Python mirror of the reference's TT-SVD / TR-SVD entry points over the ttx C ABI.
+
+PyTorch is used only for device memory and the stream: every tensor argument is a contiguous
+float32 CUDA tensor holding column-major data (element (i1..id) at i1 + N1 (i2 + N2 (...)));
+results come back as flat tensors together with their column-major shapes.  All arithmetic runs in
+``libttx.so``; a missing library raises instead of falling back.
+
+Reference entry points mirrored here:
+* ``tt_svd``  -- ttdec_half_lu_device_6, ttdecomposition/high_order_GPUs/ttdec6_singleGPU.cu:799-958
+* ``tr_svd``  -- ttdec_half_lu_device_6 (TR), trdecomposition/high_order_GPUs/trdec6_singleGPU.cu:827-1004
+* ``rsvd``    -- basicRandSVD_LU, ttdecomposition/third_order_GPUs/opt/ttdec_gpu_float_rsvd_lu_half.cu:557-775
+* ``permute_231`` -- tensorToMode231, trdecomposition/high_order_GPUs/trdec6_singleGPU.cu:49-58
+* ``tt_rel_error`` / ``tr_rel_error`` -- calMSE (non-destructive), ttdec6_singleGPU.cu:134-182,
+  trdec6_singleGPU.cu:175-228
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib as L
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def _check(lib, h, status):
+    if status not in (L.OK, L.RESIDUAL_FALLBACK_USED):
+        msg = lib.ttx_last_error(h)
+        raise L.TtxError(status, msg.decode() if msg else "")
+    return status
+
+
+def _dev_ptr(t):
+    torch = _torch()
+    if not isinstance(t, torch.Tensor) or not t.is_cuda:
+        raise TypeError("ttx: expected a CUDA tensor")
+    if t.dtype != torch.float32 or not t.is_contiguous():
+        raise TypeError("ttx: expected a contiguous float32 tensor")
+    return t.data_ptr()
+
+
+class Engine:
+    """One ttx handle bound to the current torch stream of the current device."""
+
+    def __init__(self, stream=None):
+        torch = _torch()
+        self.lib = L.load()
+        if stream is None:
+            stream = torch.cuda.current_stream()
+        self.stream = stream
+        h = ctypes.c_void_p()
+        st = self.lib.ttx_create(ctypes.byref(h), ctypes.c_void_p(stream.cuda_stream))
+        if st != L.OK:
+            raise L.TtxError(st, "ttx_create failed")
+        self.h = h
+
+    def close(self):
+        if self.h:
+            self.lib.ttx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ------------------------------------------------------------------------------------------
+    def params(self, dims, ranks, s=10, q=2, seed=1234, precision="bf16", method="auto", rank_mode="fixed",
+               eps=1e-4, flags=0):
+        p = L.Params()
+        d = len(dims)
+        self.lib.ttx_params_init(ctypes.byref(p), d, L.i64_array(dims), L.i32_array(ranks))
+        p.oversampling = int(s)
+        p.power_iters = int(q)
+        p.omega_seed = int(seed)
+        p.precision = L.PRECISIONS[precision] if isinstance(precision, str) else int(precision)
+        p.method = L.METHODS[method] if isinstance(method, str) else int(method)
+        p.rank_mode = L.RANK_MODES[rank_mode] if isinstance(rank_mode, str) else int(rank_mode)
+        p.eps = float(eps)
+        p.flags = int(flags)
+        return p
+
+    def _sweep(self, fn, A, dims, ranks, tr, **kw):
+        torch = _torch()
+        p = self.params(dims, ranks, **kw)
+        d = len(dims)
+        cores = [torch.empty(int(self.lib.ttx_core_size(ctypes.byref(p), int(tr), k)), device=A.device,
+                             dtype=torch.float32) for k in range(d)]
+        sigma = np.zeros((d - 1) * L.MAX_ELL, dtype=np.float32)
+        rep = L.Report()
+        st = fn(self.h, ctypes.c_void_p(_dev_ptr(A)), ctypes.byref(p), L.ptr_array([c.data_ptr() for c in cores]),
+                sigma.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), ctypes.byref(rep))
+        _check(self.lib, self.h, st)
+        used = [int(rep.ranks[k]) for k in range(d + 1)]
+        out = []
+        for k in range(d):
+            if tr and k == 0:
+                shape = (dims[0], used[0] * used[1])
+            elif tr and k == d - 1:
+                shape = (used[d - 1], dims[d - 1] * used[0])
+            else:
+                shape = (used[k] * dims[k], used[k + 1])
+            out.append((cores[k][: shape[0] * shape[1]], shape))
+        info = report_dict(rep, d)
+        return out, sigma.reshape(d - 1, L.MAX_ELL), info
+
+    def tt_svd(self, A, dims, ranks, **kw):
+        """TT-SVD of the column-major tensor A; returns ([(core, (rows, cols))], sigma, report)."""
+        return self._sweep(self.lib.ttx_tt_svd, A, dims, ranks, False, **kw)
+
+    def tr_svd(self, A, dims, ranks, **kw):
+        """TR-SVD (first step rank ranks[0] * ranks[1]); returns ([(core, shape)], sigma, report)."""
+        return self._sweep(self.lib.ttx_tr_svd, A, dims, ranks, True, **kw)
+
+    def rsvd(self, A, m, n, k, s=10, q=2, seed=1234, precision="bf16"):
+        torch = _torch()
+        ell = k + s
+        U = torch.empty(m * ell, device=A.device, dtype=torch.float32)
+        S = torch.empty(ell, device=A.device, dtype=torch.float32)
+        SVT = torch.empty(ell * n, device=A.device, dtype=torch.float32)
+        prec = L.PRECISIONS[precision] if isinstance(precision, str) else int(precision)
+        st = self.lib.ttx_rsvd(self.h, ctypes.c_void_p(_dev_ptr(A)), m, n, k, s, q, seed, prec,
+                               ctypes.c_void_p(U.data_ptr()), ctypes.c_void_p(S.data_ptr()),
+                               ctypes.c_void_p(SVT.data_ptr()))
+        _check(self.lib, self.h, st)
+        return U, S, SVT
+
+    def step(self, A, m, n, r, s=10, q=2, seed=1234, stream_id=0, precision="bf16", method="auto"):
+        torch = _torch()
+        core = torch.empty(m * r, device=A.device, dtype=torch.float32)
+        nxt = torch.empty(r * n, device=A.device, dtype=torch.float32)
+        sigma = np.zeros(L.MAX_ELL, dtype=np.float32)
+        path = ctypes.c_int32(0)
+        prec = L.PRECISIONS[precision] if isinstance(precision, str) else int(precision)
+        meth = L.METHODS[method] if isinstance(method, str) else int(method)
+        st = self.lib.ttx_step(self.h, ctypes.c_void_p(_dev_ptr(A)), m, n, r, s, q, seed, stream_id, prec, meth,
+                               ctypes.c_void_p(core.data_ptr()), ctypes.c_void_p(nxt.data_ptr()),
+                               sigma.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), ctypes.byref(path))
+        _check(self.lib, self.h, st)
+        return core, nxt, sigma, int(path.value)
+
+    def permute_231(self, x, r0, r1, n):
+        torch = _torch()
+        out = torch.empty(r0 * r1 * n, device=x.device, dtype=torch.float32)
+        st = self.lib.ttx_permute_231(self.h, ctypes.c_void_p(_dev_ptr(x)), ctypes.c_void_p(out.data_ptr()),
+                                      r0, r1, n)
+        _check(self.lib, self.h, st)
+        return out
+
+    def fill_normal(self, count, seed, stream, scale=1.0, device="cuda"):
+        torch = _torch()
+        out = torch.empty(count, device=device, dtype=torch.float32)
+        _check(self.lib, self.h, self.lib.ttx_fill_normal(self.h, ctypes.c_void_p(out.data_ptr()), count, seed,
+                                                          stream, scale))
+        return out
+
+    def omega(self, n, ell, seed, stream, device="cuda"):
+        torch = _torch()
+        out = torch.empty(n * ell, device=device, dtype=torch.float32)
+        _check(self.lib, self.h, self.lib.ttx_omega(self.h, ctypes.c_void_p(out.data_ptr()), n, ell, seed, stream))
+        return out
+
+    def generate(self, dims, ranks, std=1.0, noise_rel=0.0, seed=1134, tr=False, return_cores=False,
+                 device="cuda"):
+        torch = _torch()
+        d = len(dims)
+        total = int(np.prod(dims))
+        A = torch.empty(total, device=device, dtype=torch.float32)
+        cores = None
+        cptr = None
+        if return_cores:
+            p = self.params(dims, ranks)
+            cores = [torch.empty(int(self.lib.ttx_core_size(ctypes.byref(p), int(tr), k)), device=device,
+                                 dtype=torch.float32) for k in range(d)]
+            cptr = L.ptr_array([c.data_ptr() for c in cores])
+        fn = self.lib.ttx_tr_generate if tr else self.lib.ttx_tt_generate
+        st = fn(self.h, d, L.i64_array(dims), L.i32_array(ranks), std, noise_rel, seed,
+                ctypes.c_void_p(A.data_ptr()), cptr)
+        _check(self.lib, self.h, st)
+        return (A, cores) if return_cores else A
+
+    def rel_error(self, cores, dims, ranks, A, tr=False):
+        err = ctypes.c_double(0.0)
+        fn = self.lib.ttx_tr_rel_error if tr else self.lib.ttx_tt_rel_error
+        ptrs = L.ptr_array([(c[0] if isinstance(c, tuple) else c).data_ptr() for c in cores])
+        st = fn(self.h, len(dims), L.i64_array(dims), L.i32_array(ranks), ptrs, ctypes.c_void_p(_dev_ptr(A)),
+                ctypes.byref(err))
+        _check(self.lib, self.h, st)
+        return float(err.value)
+
+    def full(self, cores, dims, ranks, tr=False):
+        torch = _torch()
+        out = torch.empty(int(np.prod(dims)), device="cuda", dtype=torch.float32)
+        fn = self.lib.ttx_tr_full if tr else self.lib.ttx_tt_full
+        ptrs = L.ptr_array([(c[0] if isinstance(c, tuple) else c).data_ptr() for c in cores])
+        st = fn(self.h, len(dims), L.i64_array(dims), L.i32_array(ranks), ptrs, ctypes.c_void_p(out.data_ptr()))
+        _check(self.lib, self.h, st)
+        return out
+
+
+def report_dict(rep, d):
+    return {
+        "ranks": [int(rep.ranks[k]) for k in range(d + 1)],
+        "ell": [int(rep.ell[k]) for k in range(d - 1)],
+        "path": [int(rep.path[k]) for k in range(d - 1)],
+        "power_iters": [int(rep.power_iters[k]) for k in range(d - 1)],
+        "zero_pivots": [int(rep.zero_pivots[k]) for k in range(d - 1)],
+        "jacobi_sweeps": [int(rep.jacobi_sweeps[k]) for k in range(d - 1)],
+        "norm2_in": [float(rep.norm2_in[k]) for k in range(d - 1)],
+        "step_ms": [float(rep.step_ms[k]) for k in range(d - 1)],
+        "total_ms": float(rep.total_ms),
+        "bytes_alg": int(rep.bytes_alg),
+    }

@@ -1,0 +1,833 @@
+// Host side of the ttx engine: workspace management, one decomposition step, the TT and TR sweeps,
+// the randomized SVD entry point, synthetic inputs and reconstruction errors.  Everything is queued
+// on the handle's stream; the only host synchronisations are the per-step rank decisions of the
+// DELTA rank mode and the end of each public call.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/ttx.h"
+#include "ttx_device.h"
+#include "ttx_kernels.h"
+
+namespace ttx {
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+};
+
+struct Handle {
+  hipStream_t stream = nullptr;
+  std::string err;
+  DevBuf part, Y, Qc, Qn, Z, W1, W2, G, Rinv, lam, V, Ur, gpart, status, next[2], sig, scratch[4];
+  int* h_status = nullptr;       // pinned
+  float* h_sigma = nullptr;      // pinned, TTX_MAX_ELL
+  double* h_lam = nullptr;       // pinned
+  int dev = 0;
+  std::vector<float*> last_cores;  // device buffers kept for the reference-name shims
+};
+
+static ttx_status fail(Handle* h, ttx_status s, const std::string& msg) {
+  if (h) h->err = msg;
+  return s;
+}
+
+#define TTX_HIP(h, call)                                                                      \
+  do {                                                                                        \
+    hipError_t e_ = (call);                                                                   \
+    if (e_ != hipSuccess) return fail((h), TTX_ERR_HIP, std::string(#call ": ") + hipGetErrorString(e_)); \
+  } while (0)
+
+static ttx_status ensure(Handle* h, DevBuf& b, size_t bytes) {
+  if (b.bytes >= bytes && b.p) return TTX_OK;
+  if (b.p) {
+    (void)hipStreamSynchronize(h->stream);
+    (void)hipFree(b.p);
+    b.p = nullptr;
+    b.bytes = 0;
+  }
+  size_t want = bytes < 256 ? 256 : bytes;
+  if (hipMalloc(&b.p, want) != hipSuccess) {
+    (void)hipGetLastError();
+    return fail(h, TTX_ERR_OOM, "hipMalloc of " + std::to_string(want) + " bytes failed");
+  }
+  b.bytes = want;
+  return TTX_OK;
+}
+
+template <typename T>
+static T* P(DevBuf& b) { return reinterpret_cast<T*>(b.p); }
+
+static ttx_status check_launch(Handle* h, const char* where) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(h, TTX_ERR_HIP, std::string(where) + ": " + hipGetErrorString(e));
+  return TTX_OK;
+}
+
+#define TTX_TRY(x)                 \
+  do {                             \
+    ttx_status s_ = (x);           \
+    if (s_ != TTX_OK) return s_;   \
+  } while (0)
+
+static int kp_for(int k) { return k <= 32 ? 32 : 64; }
+
+// ------------------------------------------------------------------------------------------------
+// arithmetic choices
+// ------------------------------------------------------------------------------------------------
+struct Arith3 {
+  int range;    // sketch and power passes
+  int accurate; // remainder / exact Gram / exact projection
+};
+static Arith3 arith_for(int precision, int flags) {
+  Arith3 a;
+  if (precision == TTX_PREC_FP32) { a.range = AR_FP32; a.accurate = AR_FP32; }
+  else if (precision == TTX_PREC_FP16) { a.range = AR_FP16; a.accurate = AR_BF16X3; }
+  else { a.range = (flags & TTX_FLAG_FAST_RANGE) ? AR_BF16 : AR_BF16X3; a.accurate = AR_BF16X3; }
+  return a;
+}
+
+// ------------------------------------------------------------------------------------------------
+// orthonormal basis of the columns of Y (m x k, row-major stride kpy): shifted CholeskyQR then two
+// CholeskyQR rounds, all in FP64; output Qc column-major (kp x m), columns k..kp-1 zero
+// ------------------------------------------------------------------------------------------------
+static ttx_status orth(Handle* h, const float* Y, int64_t kpy, int m, int k, int kp, float* Qc) {
+  hipStream_t st = h->stream;
+  const int64_t gg = gram_tall_grid(m);
+  TTX_TRY(ensure(h, h->gpart, (size_t)gg * k * k * sizeof(double)));
+  TTX_TRY(ensure(h, h->G, (size_t)k * k * sizeof(double)));
+  TTX_TRY(ensure(h, h->Rinv, (size_t)k * k * sizeof(double)));
+  TTX_TRY(ensure(h, h->W1, (size_t)m * k * sizeof(double)));
+  TTX_TRY(ensure(h, h->W2, (size_t)m * k * sizeof(double)));
+  double* G = P<double>(h->G);
+  double* Ri = P<double>(h->Rinv);
+  double* W1 = P<double>(h->W1);
+  double* W2 = P<double>(h->W2);
+  int* stat = P<int>(h->status);
+  launch_gram_tall_f32(Y, kpy, m, k, P<double>(h->gpart), gg, st);
+  launch_reduce_f64(P<double>(h->gpart), gg, (int64_t)k * k, G, st);
+  launch_chol_inv(G, k, m, 1, Ri, stat, st);
+  launch_apply_rinv_f32(Y, kpy, m, k, Ri, W1, st);
+  launch_gram_tall_f64(W1, k, m, k, P<double>(h->gpart), gg, st);
+  launch_reduce_f64(P<double>(h->gpart), gg, (int64_t)k * k, G, st);
+  launch_chol_inv(G, k, m, 0, Ri, stat, st);
+  launch_apply_rinv_f64(W1, m, k, Ri, W2, st);
+  launch_gram_tall_f64(W2, k, m, k, P<double>(h->gpart), gg, st);
+  launch_reduce_f64(P<double>(h->gpart), gg, (int64_t)k * k, G, st);
+  launch_chol_inv(G, k, m, 0, Ri, stat, st);
+  launch_apply_rinv_out(W2, m, k, kp, Ri, Qc, m, st);
+  return check_launch(h, "orth");
+}
+
+// ------------------------------------------------------------------------------------------------
+// one step
+// ------------------------------------------------------------------------------------------------
+struct StepArgs {
+  const float* A = nullptr;
+  int64_t m = 0, n = 0;
+  int r = 1;            // rank (fixed) or ceiling
+  int s = 10, q = 2;
+  uint64_t seed = 1234;
+  uint32_t stream_id = 0;
+  int precision = TTX_PREC_BF16;
+  int method = TTX_METHOD_AUTO;
+  int flags = 0;
+  int rank_mode = TTX_RANK_FIXED;
+  double delta2 = 0.0;  // absolute squared truncation threshold for DELTA
+  double norm2 = -1.0;  // ||A||_F^2 when known
+  float* core = nullptr;   // m x r_out
+  float* next = nullptr;   // r_out x n
+};
+struct StepOut {
+  int r = 0, ell = 0, path = 0, power_iters = 0, zero_pivots = 0, sweeps = 0;
+  double kept2 = 0.0;      // sum of kept sigma^2 = ||A_next||_F^2
+  double total2 = -1.0;    // ||A||_F^2 when the step knows it
+  int64_t bytes = 0;
+  float sigma[TTX_MAX_ELL];
+};
+
+static int choose_rank(const StepArgs& a, int k, const double* lam, double total2, double* kept2) {
+  int r = a.r;
+  if (a.rank_mode == TTX_RANK_DELTA) {
+    // smallest r with tail(r) <= delta^2, tail(r) = total - sum_{i<=r} lam_i (Oseledets 2011,
+    // TT-SVD; the commented rule of trdec_gpu_float_unopt2_rank.cu:222-243)
+    double cum = 0.0;
+    r = k;
+    for (int i = 0; i < k; ++i) {
+      cum += std::max(lam[i], 0.0);
+      double tail = total2 - cum;
+      double cap_tail = 0.0;
+      for (int j = i + 1; j < k; ++j) cap_tail += std::max(lam[j], 0.0);
+      tail = std::max(tail, cap_tail);
+      if (tail <= a.delta2) { r = i + 1; break; }
+    }
+    r = std::min(r, a.r);
+    r = std::max(r, 1);
+  }
+  double s = 0.0;
+  for (int i = 0; i < std::min(r, k); ++i) s += std::max(lam[i], 0.0);
+  *kept2 = s;
+  return r;
+}
+
+// tail of a randomized / exact-n step: Z = A^T Q (accurate), eig of Z^T Z, core and remainder
+static ttx_status step_tail(Handle* h, const StepArgs& a, const Arith3& ar, int k, int kp, StepOut& o) {
+  hipStream_t st = h->stream;
+  const int m = (int)a.m;
+  const int64_t n = a.n;
+  TTX_TRY(ensure(h, h->Z, (size_t)n * kp * sizeof(float)));
+  float* Z = P<float>(h->Z);
+  launch_proj(ar.accurate, kp, a.A, a.m, m, n, P<float>(h->Qc), m, 0, Z, kp, 0, st);
+  o.bytes += (int64_t)(a.m * n * 4);
+  const int64_t gg = gram_tall_grid(n);
+  TTX_TRY(ensure(h, h->gpart, (size_t)gg * k * k * sizeof(double)));
+  launch_gram_tall_f32(Z, kp, n, k, P<double>(h->gpart), gg, st);
+  launch_reduce_f64(P<double>(h->gpart), gg, (int64_t)k * k, P<double>(h->G), st);
+  launch_syevj(P<double>(h->G), k, P<double>(h->lam), P<double>(h->V), P<int>(h->status), nullptr, st);
+  TTX_TRY(check_launch(h, "step tail"));
+  // rank decision on the host when needed
+  int r = a.r;
+  if (a.rank_mode != TTX_RANK_FIXED) {
+    TTX_HIP(h, hipMemcpyAsync(h->h_lam, h->lam.p, k * sizeof(double), hipMemcpyDeviceToHost, st));
+    TTX_HIP(h, hipStreamSynchronize(st));
+    double kept;
+    r = choose_rank(a, k, h->h_lam, a.norm2, &kept);
+  }
+  if (r > k) return fail(h, TTX_ERR_BAD_ARG, "rank exceeds the sketch width");
+  TTX_TRY(ensure(h, h->Ur, (size_t)kp * r * sizeof(float)));
+  TTX_TRY(ensure(h, h->sig, TTX_MAX_ELL * sizeof(float)));
+  // G = Q U_B[:, :r] with the sign gauge; sigma = sqrt(eig(B B^T)) (basicRandSVD_LU :717-763)
+  launch_assemble(P<float>(h->Qc), m, m, k, P<double>(h->lam), P<double>(h->V), r, a.core, P<float>(h->Ur), nullptr,
+                  0, P<float>(h->sig), st);
+  // remainder S V^T = U_r^T B = U_r^T Z^T (:734-739)
+  launch_next_from_z(Z, kp, n, k, P<float>(h->Ur), r, a.next, st);
+  o.bytes += (int64_t)(n * kp * 4 + (int64_t)r * n * 4);
+  o.r = r;
+  return check_launch(h, "step tail assemble");
+}
+
+static ttx_status run_step(Handle* h, const StepArgs& a, StepOut& o) {
+  hipStream_t st = h->stream;
+  const Arith3 ar = arith_for(a.precision, a.flags);
+  const int m = (int)a.m;
+  const int64_t n = a.n;
+  if (a.m <= 0 || a.n <= 0 || a.r <= 0) return fail(h, TTX_ERR_BAD_ARG, "step: empty unfolding or rank");
+  if (a.m > (int64_t)1 << 30) return fail(h, TTX_ERR_UNSUPPORTED, "step: more than 2^30 rows");
+  int ell = a.r + a.s;
+  const int64_t mn = std::min<int64_t>(a.m, a.n);
+  const bool exact = (a.method == TTX_METHOD_EXACT) || (a.method == TTX_METHOD_AUTO && ell >= mn);
+  memset(o.sigma, 0, sizeof(o.sigma));
+  o.bytes = 0;
+  TTX_TRY(ensure(h, h->status, 4 * sizeof(int)));
+  TTX_HIP(h, hipMemsetAsync(h->status.p, 0, 4 * sizeof(int), st));
+  TTX_TRY(ensure(h, h->lam, TTX_MAX_ELL * sizeof(double)));
+  TTX_TRY(ensure(h, h->V, TTX_MAX_ELL * TTX_MAX_ELL * sizeof(double)));
+  TTX_TRY(ensure(h, h->G, TTX_MAX_ELL * TTX_MAX_ELL * sizeof(double)));
+  int k = 0;
+  if (exact && a.m <= TTX_MAX_ELL) {
+    // exact Gram path on the short side (ttdec6_singleGPU.cu:510-536): C = A A^T, eig, top r, G^T A
+    o.path = TTX_PATH_EXACT_M;
+    o.ell = 0;
+    k = m;
+    const int64_t grid = gram_small_grid(n);
+    TTX_TRY(ensure(h, h->gpart, (size_t)grid * 4 * m * m * sizeof(double)));
+    launch_gram_small(ar.accurate, a.A, a.m, m, n, P<double>(h->gpart), grid, nullptr, st);
+    launch_reduce_f64(P<double>(h->gpart), grid * 4, (int64_t)m * m, P<double>(h->G), st);
+    launch_syevj(P<double>(h->G), m, P<double>(h->lam), P<double>(h->V), P<int>(h->status), nullptr, st);
+    TTX_TRY(check_launch(h, "exact gram"));
+    o.bytes += a.m * n * 4;
+    TTX_HIP(h, hipMemcpyAsync(h->h_lam, h->lam.p, m * sizeof(double), hipMemcpyDeviceToHost, st));
+    TTX_HIP(h, hipStreamSynchronize(st));
+    double total = 0.0;
+    for (int i = 0; i < m; ++i) total += std::max(h->h_lam[i], 0.0);
+    o.total2 = total;
+    StepArgs aa = a;
+    if (aa.norm2 < 0) aa.norm2 = total;
+    double kept = 0.0;
+    const int r = choose_rank(aa, m, h->h_lam, total, &kept);
+    if (r > TTX_MAX_ELL) return fail(h, TTX_ERR_UNSUPPORTED, "rank above TTX_MAX_ELL");
+    const int kpn = kp_for(r);
+    TTX_TRY(ensure(h, h->Qn, (size_t)kpn * m * sizeof(float)));
+    TTX_TRY(ensure(h, h->sig, TTX_MAX_ELL * sizeof(float)));
+    launch_assemble(nullptr, 0, m, m, P<double>(h->lam), P<double>(h->V), r, a.core, nullptr, P<float>(h->Qn), kpn,
+                    P<float>(h->sig), st);
+    launch_proj(ar.accurate, kpn, a.A, a.m, m, n, P<float>(h->Qn), m, 1, a.next, 0, r, st);
+    o.bytes += a.m * n * 4 + (int64_t)r * n * 4;
+    o.r = r;
+    o.kept2 = kept;
+    TTX_TRY(check_launch(h, "exact projection"));
+  } else if (exact && a.n <= TTX_MAX_ELL) {
+    // short-and-wide on the other side: Q = orth(A), then the common tail
+    o.path = TTX_PATH_EXACT_N;
+    o.ell = 0;
+    k = (int)n;
+    const int kp = kp_for(k);
+    TTX_TRY(ensure(h, h->Y, (size_t)m * kp * sizeof(float)));
+    TTX_TRY(ensure(h, h->Qc, (size_t)kp * m * sizeof(float)));
+    launch_copy_cols(a.A, a.m, m, k, P<float>(h->Y), kp, st);
+    TTX_TRY(orth(h, P<float>(h->Y), kp, m, k, kp, P<float>(h->Qc)));
+    TTX_TRY(step_tail(h, a, ar, k, kp, o));
+  } else {
+    if (exact) return fail(h, TTX_ERR_UNSUPPORTED, "exact path needs min(m, n) <= TTX_MAX_ELL");
+    if (ell > TTX_MAX_ELL) ell = TTX_MAX_ELL;
+    if (ell > mn) ell = (int)mn;
+    o.path = TTX_PATH_RSVD;
+    o.ell = ell;
+    k = ell;
+    const int kp = kp_for(k);
+    const SketchPlan sp = plan_sketch(m, n, kp, 0);
+    TTX_TRY(ensure(h, h->part, (size_t)sp.slabs * m * kp * sizeof(float)));
+    TTX_TRY(ensure(h, h->Y, (size_t)m * kp * sizeof(float)));
+    TTX_TRY(ensure(h, h->Qc, (size_t)kp * m * sizeof(float)));
+    TTX_TRY(ensure(h, h->Z, (size_t)n * kp * sizeof(float)));
+    float* Y = P<float>(h->Y);
+    // Y = A Omega (basicRandSVD_LU :585-620; Omega from the project's Philox stream)
+    launch_sketch(ar.range, sp, a.A, a.m, m, n, nullptr, 0, a.seed, a.stream_id, ell, P<float>(h->part), st);
+    launch_reduce_f32(P<float>(h->part), sp.slabs, (int64_t)m * kp, Y, st);
+    TTX_TRY(check_launch(h, "sketch"));
+    o.bytes += a.m * n * 4;
+    TTX_TRY(orth(h, Y, kp, m, k, kp, P<float>(h->Qc)));
+    // power iterations (:628-683): Z = A^T Q, Y = A Z, Q = orth(Y)
+    for (int it = 0; it < a.q; ++it) {
+      launch_proj(ar.range, kp, a.A, a.m, m, n, P<float>(h->Qc), m, 0, P<float>(h->Z), kp, 0, st);
+      launch_sketch(ar.range, sp, a.A, a.m, m, n, P<float>(h->Z), kp, 0, 0, ell, P<float>(h->part), st);
+      launch_reduce_f32(P<float>(h->part), sp.slabs, (int64_t)m * kp, Y, st);
+      TTX_TRY(check_launch(h, "power iteration"));
+      o.bytes += 2 * a.m * n * 4;
+      TTX_TRY(orth(h, Y, kp, m, k, kp, P<float>(h->Qc)));
+    }
+    o.power_iters = a.q;
+    TTX_TRY(step_tail(h, a, ar, k, kp, o));
+  }
+  // sigma and status back to the host
+  TTX_HIP(h, hipMemcpyAsync(h->h_sigma, h->sig.p, k * sizeof(float), hipMemcpyDeviceToHost, st));
+  TTX_HIP(h, hipMemcpyAsync(h->h_status, h->status.p, 4 * sizeof(int), hipMemcpyDeviceToHost, st));
+  TTX_HIP(h, hipStreamSynchronize(st));
+  for (int i = 0; i < std::min(k, TTX_MAX_ELL); ++i) o.sigma[i] = h->h_sigma[i];
+  o.zero_pivots = h->h_status[0];
+  o.sweeps = h->h_status[1];
+  if (o.path != TTX_PATH_EXACT_M) {
+    double kept = 0.0;
+    for (int i = 0; i < o.r; ++i) kept += (double)o.sigma[i] * o.sigma[i];
+    o.kept2 = kept;
+  }
+  return TTX_OK;
+}
+
+}  // namespace ttx
+
+// ------------------------------------------------------------------------------------------------
+// sweeps
+// ------------------------------------------------------------------------------------------------
+namespace ttx {
+
+static ttx_status validate(Handle* h, const ttx_params* p, bool tr) {
+  if (!p) return fail(h, TTX_ERR_BAD_ARG, "null params");
+  if (p->order < 2 || p->order > TTX_MAX_ORDER) return fail(h, TTX_ERR_BAD_ARG, "order out of range");
+  for (int k = 0; k < p->order; ++k)
+    if (p->dims[k] <= 0) return fail(h, TTX_ERR_BAD_ARG, "non-positive mode size");
+  for (int k = 0; k <= p->order; ++k)
+    if (p->ranks[k] <= 0) return fail(h, TTX_ERR_BAD_ARG, "non-positive rank");
+  if (!tr && (p->ranks[0] != 1 || p->ranks[p->order] != 1)) return fail(h, TTX_ERR_BAD_ARG, "TT ranks need r_0 = r_d = 1");
+  if (tr && p->ranks[0] != p->ranks[p->order]) return fail(h, TTX_ERR_BAD_ARG, "TR ranks need r_0 = r_d");
+  if (p->oversampling < 0 || p->power_iters < 0) return fail(h, TTX_ERR_BAD_ARG, "negative oversampling / power iterations");
+  if (p->precision < 0 || p->precision > 2) return fail(h, TTX_ERR_BAD_ARG, "unknown precision");
+  if (p->rank_mode < 0 || p->rank_mode > 2) return fail(h, TTX_ERR_BAD_ARG, "unknown rank mode");
+  if (p->method < 0 || p->method > 2) return fail(h, TTX_ERR_BAD_ARG, "unknown method");
+  return TTX_OK;
+}
+
+static ttx_status input_norm2(Handle* h, const float* A, int64_t count, double* out) {
+  const int64_t g = sumsq_grid(count);
+  TTX_TRY(ensure(h, h->scratch[3], (size_t)(g + 1) * sizeof(double)));
+  double* part = P<double>(h->scratch[3]);
+  launch_sumsq(A, count, part + g, part, g, h->stream);
+  TTX_TRY(check_launch(h, "sumsq"));
+  TTX_HIP(h, hipMemcpyAsync(h->h_lam, part + g, sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  TTX_HIP(h, hipStreamSynchronize(h->stream));
+  *out = h->h_lam[0];
+  return TTX_OK;
+}
+
+static ttx_status sweep(Handle* h, const float* d_A, const ttx_params* p, float* const* d_cores, float* h_sigma,
+                        ttx_report* rep, bool tr) {
+  TTX_TRY(validate(h, p, tr));
+  if (!d_A || !d_cores) return fail(h, TTX_ERR_BAD_ARG, "null tensor or core pointers");
+  if (tr && p->rank_mode != TTX_RANK_FIXED) return fail(h, TTX_ERR_UNSUPPORTED, "TR-SVD supports fixed ranks only");
+  const int d = p->order;
+  hipStream_t st = h->stream;
+  ttx_report R;
+  memset(&R, 0, sizeof(R));
+  for (int k = 0; k < TTX_MAX_ORDER; ++k) R.norm2_in[k] = -1.0;
+  if (h_sigma) memset(h_sigma, 0, sizeof(float) * (d - 1) * TTX_MAX_ELL);
+  int64_t total = 1;
+  for (int k = 0; k < d; ++k) total *= p->dims[k];
+  hipEvent_t ev[TTX_MAX_ORDER + 1];
+  const bool timing = (p->flags & TTX_FLAG_TIMING) != 0;
+  if (timing)
+    for (int k = 0; k < d; ++k) TTX_HIP(h, hipEventCreate(&ev[k]));
+  double norm2 = -1.0;
+  if (p->rank_mode != TTX_RANK_FIXED) TTX_TRY(input_norm2(h, d_A, total, &norm2));
+  const double delta2 = (p->rank_mode != TTX_RANK_FIXED) ? p->eps * p->eps / (double)(d - 1) * norm2 : 0.0;
+
+  const float* cur = d_A;
+  int cur_buf = -1;   // which of next[0..1] holds cur (-1: the input)
+  int r_prev = 1;
+  int64_t n = total;
+  R.ranks[0] = p->ranks[0];
+  R.ranks[d] = p->ranks[d];
+  if (timing) TTX_HIP(h, hipEventRecord(ev[0], st));
+  for (int k = 0; k < d - 1; ++k) {
+    StepArgs a;
+    const int64_t m = (int64_t)r_prev * p->dims[k];
+    n = n / p->dims[k];
+    a.A = cur;
+    a.m = m;
+    a.n = n;
+    a.r = (tr && k == 0) ? p->ranks[0] * p->ranks[1] : p->ranks[k + 1];
+    a.s = p->oversampling;
+    a.q = p->power_iters;
+    a.seed = p->omega_seed;
+    a.stream_id = (uint32_t)k;
+    a.precision = p->precision;
+    a.method = p->method;
+    a.flags = p->flags;
+    a.rank_mode = p->rank_mode;
+    a.delta2 = delta2;
+    a.norm2 = norm2;
+    a.core = d_cores[k];
+    const bool last = (k == d - 2);
+    float* next = nullptr;
+    int wbuf = -1;
+    if (last && !tr) {
+      next = d_cores[d - 1];
+    } else if (last && tr) {
+      next = d_cores[d - 1];
+    } else {
+      wbuf = (cur_buf == 0) ? 1 : 0;
+      TTX_TRY(ensure(h, h->next[wbuf], (size_t)a.r * n * sizeof(float)));
+      next = P<float>(h->next[wbuf]);
+    }
+    a.next = next;
+    StepOut o;
+    TTX_TRY(run_step(h, a, o));
+    R.norm2_in[k] = (o.total2 >= 0) ? o.total2 : norm2;
+    R.ell[k] = o.ell;
+    R.path[k] = o.path;
+    R.power_iters[k] = o.power_iters;
+    R.zero_pivots[k] = o.zero_pivots;
+    R.jacobi_sweeps[k] = o.sweeps;
+    R.bytes_alg += o.bytes;
+    if (h_sigma) memcpy(h_sigma + (size_t)k * TTX_MAX_ELL, o.sigma, sizeof(float) * TTX_MAX_ELL);
+    norm2 = o.kept2;
+    if (tr && k == 0) {
+      // (r0, r1, N') -> (r1, N', r0) (trdec6_singleGPU.cu:865-877); the ring rank joins the columns
+      const int64_t r0 = p->ranks[0], r1 = p->ranks[1];
+      if (o.r != r0 * r1) return fail(h, TTX_ERR_BAD_ARG, "TR first step rank mismatch");
+      const int pbuf = (wbuf == 0) ? 1 : 0;
+      TTX_TRY(ensure(h, h->next[pbuf], (size_t)o.r * n * sizeof(float)));
+      launch_permute_231(next, P<float>(h->next[pbuf]), r0, r1, n, st);
+      TTX_TRY(check_launch(h, "permute_231"));
+      R.bytes_alg += 2 * o.r * n * 4;
+      cur = P<float>(h->next[pbuf]);
+      cur_buf = pbuf;
+      r_prev = (int)r1;
+      n = n * r0;
+      R.ranks[1] = (int)r1;
+    } else {
+      cur = next;
+      cur_buf = wbuf;
+      r_prev = o.r;
+      R.ranks[k + 1] = o.r;
+    }
+    if (timing) TTX_HIP(h, hipEventRecord(ev[k + 1], st));
+  }
+  TTX_HIP(h, hipStreamSynchronize(st));
+  if (timing) {
+    float ms = 0.0f;
+    for (int k = 0; k < d - 1; ++k) {
+      TTX_HIP(h, hipEventElapsedTime(&ms, ev[k], ev[k + 1]));
+      R.step_ms[k] = ms;
+    }
+    TTX_HIP(h, hipEventElapsedTime(&ms, ev[0], ev[d - 1]));
+    R.total_ms = ms;
+    for (int k = 0; k < d; ++k) (void)hipEventDestroy(ev[k]);
+  }
+  if (rep) *rep = R;
+  return TTX_OK;
+}
+
+// ------------------------------------------------------------------------------------------------
+// contractions: L (left modes) x R (right modes)
+// ------------------------------------------------------------------------------------------------
+struct Halves {
+  const float* L = nullptr;   // M x K
+  const float* R = nullptr;   // K x N
+  int64_t M = 0, N = 0, K = 0;
+};
+
+static ttx_status contract_halves(Handle* h, int d, const int64_t* dims, const int32_t* ranks,
+                                  const float* const* cores, bool tr, Halves& out) {
+  hipStream_t st = h->stream;
+  const int hsplit = d / 2;   // left modes 0..hsplit-1, right modes hsplit..d-1
+  const int64_t r0 = tr ? ranks[0] : 1;
+  // left chain
+  int64_t P_ = dims[0] * r0;   // rows of the running left factor (ring index after mode 0)
+  const float* Lc = cores[0];  // (N0 r0) x r1
+  int li = 0;
+  for (int k = 1; k < hsplit; ++k) {
+    const int64_t cols = dims[k] * ranks[k + 1];
+    TTX_TRY(ensure(h, h->scratch[li], (size_t)P_ * cols * sizeof(float)));
+    float* Ln = P<float>(h->scratch[li]);
+    launch_gemm(P_, cols, ranks[k], 1.0f, Lc, P_, cores[k], ranks[k], 0.0f, Ln, P_, st);
+    Lc = Ln;
+    P_ *= dims[k];
+    li ^= 1;
+  }
+  // right chain
+  int64_t Q_ = dims[d - 1] * r0;
+  const float* Rc = cores[d - 1];   // r_{d-1} x (N_{d-1} r0)
+  int ri = 2;
+  for (int k = d - 2; k >= hsplit; --k) {
+    const int64_t rows = (int64_t)ranks[k] * dims[k];
+    TTX_TRY(ensure(h, h->scratch[ri], (size_t)rows * Q_ * sizeof(float)));
+    float* Rn = P<float>(h->scratch[ri]);
+    launch_gemm(rows, Q_, ranks[k + 1], 1.0f, cores[k], rows, Rc, ranks[k + 1], 0.0f, Rn, rows, st);
+    Rc = Rn;
+    Q_ *= dims[k];
+    ri = (ri == 2) ? 3 : 2;
+  }
+  TTX_TRY(check_launch(h, "contract chains"));
+  const int64_t rh = ranks[hsplit];
+  if (!tr) {
+    out.L = Lc; out.R = Rc; out.M = P_; out.N = Q_; out.K = rh;
+    return TTX_OK;
+  }
+  // TR: move the ring index next to the bond index on both sides
+  const int64_t N0 = dims[0];
+  const int64_t Pp = P_ / (N0 * r0);
+  const int64_t Qp = Q_ / r0;
+  // left: (N0, r0, Pp, rh) -> (N0, Pp, r0, rh); right: (rh, Qp, r0) -> (r0, rh, Qp)
+  DevBuf& lb = h->scratch[li];
+  TTX_TRY(ensure(h, lb, (size_t)P_ * rh * sizeof(float)));
+  {
+    const int64_t dl[4] = {N0, r0, Pp, rh};
+    const int pl[4] = {0, 2, 1, 3};
+    launch_permute(Lc, P<float>(lb), 4, dl, pl, st);
+  }
+  DevBuf& rb = h->scratch[ri];
+  TTX_TRY(ensure(h, rb, (size_t)rh * Q_ * sizeof(float)));
+  {
+    const int64_t dr[3] = {rh, Qp, r0};
+    const int pr[3] = {2, 0, 1};
+    launch_permute(Rc, P<float>(rb), 3, dr, pr, st);
+  }
+  TTX_TRY(check_launch(h, "TR permutes"));
+  out.L = P<float>(lb); out.R = P<float>(rb); out.M = N0 * Pp; out.N = Qp; out.K = r0 * rh;
+  return TTX_OK;
+}
+
+static ttx_status rel_error(Handle* h, int d, const int64_t* dims, const int32_t* ranks, const float* const* cores,
+                            const float* A, bool tr, double* err) {
+  Halves hv;
+  TTX_TRY(contract_halves(h, d, dims, ranks, cores, tr, hv));
+  const int64_t nb = gemm_resid_blocks(hv.M, hv.N);
+  TTX_TRY(ensure(h, h->gpart, (size_t)nb * 2 * sizeof(double)));
+  launch_gemm_resid(hv.M, hv.N, hv.K, hv.L, hv.M, hv.R, hv.K, A, hv.M, P<double>(h->gpart), h->stream);
+  TTX_TRY(check_launch(h, "residual"));
+  std::vector<double> buf((size_t)nb * 2);
+  TTX_HIP(h, hipMemcpyAsync(buf.data(), h->gpart.p, nb * 2 * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  TTX_HIP(h, hipStreamSynchronize(h->stream));
+  double r2 = 0.0, x2 = 0.0;
+  for (int64_t i = 0; i < nb; ++i) { r2 += buf[2 * i]; x2 += buf[2 * i + 1]; }
+  *err = (x2 > 0) ? std::sqrt(r2 / x2) : std::sqrt(r2);
+  return TTX_OK;
+}
+
+static ttx_status full(Handle* h, int d, const int64_t* dims, const int32_t* ranks, const float* const* cores,
+                       float* out, bool tr) {
+  Halves hv;
+  TTX_TRY(contract_halves(h, d, dims, ranks, cores, tr, hv));
+  launch_gemm(hv.M, hv.N, hv.K, 1.0f, hv.L, hv.M, hv.R, hv.K, 0.0f, out, hv.M, h->stream);
+  TTX_TRY(check_launch(h, "full"));
+  TTX_HIP(h, hipStreamSynchronize(h->stream));
+  return TTX_OK;
+}
+
+static int64_t core_elems(int d, const int64_t* dims, const int32_t* ranks, bool tr, int k) {
+  if (!tr) return (int64_t)ranks[k] * dims[k] * ranks[k + 1];
+  if (k == 0) return dims[0] * (int64_t)ranks[0] * ranks[1];
+  if (k == d - 1) return (int64_t)ranks[d - 1] * dims[d - 1] * ranks[0];
+  return (int64_t)ranks[k] * dims[k] * ranks[k + 1];
+}
+
+static ttx_status generate(Handle* h, int d, const int64_t* dims, const int32_t* ranks, float std_, float noise_rel,
+                           uint64_t seed, float* A, float* const* d_cores, bool tr) {
+  if (d < 2 || d > TTX_MAX_ORDER || !A) return fail(h, TTX_ERR_BAD_ARG, "generate: bad order or null output");
+  for (int k = 0; k < d; ++k)
+    if (dims[k] <= 0) return fail(h, TTX_ERR_BAD_ARG, "generate: bad mode size");
+  for (int k = 0; k <= d; ++k)
+    if (ranks[k] <= 0) return fail(h, TTX_ERR_BAD_ARG, "generate: bad rank");
+  // cores in one scratch block unless the caller wants them
+  std::vector<int64_t> sz(d), off(d + 1, 0);
+  for (int k = 0; k < d; ++k) {
+    sz[k] = core_elems(d, dims, ranks, tr, k);
+    off[k + 1] = off[k] + ((sz[k] + 63) / 64) * 64;
+  }
+  DevBuf cb;
+  std::vector<float*> cp(d);
+  if (!d_cores) {
+    if (hipMalloc(&cb.p, off[d] * sizeof(float)) != hipSuccess) return fail(h, TTX_ERR_OOM, "generate: cores");
+    for (int k = 0; k < d; ++k) cp[k] = P<float>(cb) + off[k];
+  } else {
+    for (int k = 0; k < d; ++k) cp[k] = d_cores[k];
+  }
+  for (int k = 0; k < d; ++k)
+    launch_fill_normal(cp[k], sz[k], seed, (uint32_t)((tr ? 0x200 : 0x100) + k), std_, h->stream);
+  ttx_status s = full(h, d, dims, ranks, cp.data(), A, tr);
+  if (s == TTX_OK && noise_rel != 0.0f) {
+    int64_t total = 1;
+    for (int k = 0; k < d; ++k) total *= dims[k];
+    double n2 = 0.0;
+    s = input_norm2(h, A, total, &n2);
+    if (s == TTX_OK) {
+      const float rms = (float)std::sqrt(n2 / (double)total);
+      const float coef = noise_rel * rms;
+      launch_add_noise(A, total, seed, 0xFFFFu, coef, h->stream);
+      s = check_launch(h, "noise");
+      if (s == TTX_OK) {
+        hipError_t e = hipStreamSynchronize(h->stream);
+        if (e != hipSuccess) s = fail(h, TTX_ERR_HIP, hipGetErrorString(e));
+      }
+    }
+  }
+  if (cb.p) {
+    (void)hipStreamSynchronize(h->stream);
+    (void)hipFree(cb.p);
+  }
+  return s;
+}
+
+}  // namespace ttx
+
+// ------------------------------------------------------------------------------------------------
+// C ABI
+// ------------------------------------------------------------------------------------------------
+using ttx::Handle;
+
+static Handle* H(ttx_handle h) { return reinterpret_cast<Handle*>(h); }
+
+extern "C" {
+
+int32_t ttx_version(void) { return 100; }
+
+const char* ttx_status_string(int32_t s) {
+  switch (s) {
+    case TTX_OK: return "ok";
+    case TTX_ERR_BAD_ARG: return "bad argument";
+    case TTX_ERR_OOM: return "out of device memory";
+    case TTX_ERR_NO_CONVERGE: return "no convergence";
+    case TTX_RESIDUAL_FALLBACK_USED: return "ok (adaptive fallback used)";
+    case TTX_ERR_HIP: return "HIP runtime error";
+    case TTX_ERR_UNSUPPORTED: return "unsupported";
+    default: return "unknown status";
+  }
+}
+
+ttx_status ttx_create(ttx_handle* out, void* stream) {
+  if (!out) return TTX_ERR_BAD_ARG;
+  Handle* h = new Handle();
+  h->stream = reinterpret_cast<hipStream_t>(stream);
+  if (hipGetDevice(&h->dev) != hipSuccess ||
+      hipHostMalloc((void**)&h->h_status, 16 * sizeof(int), 0) != hipSuccess ||
+      hipHostMalloc((void**)&h->h_sigma, TTX_MAX_ELL * sizeof(float), 0) != hipSuccess ||
+      hipHostMalloc((void**)&h->h_lam, TTX_MAX_ELL * sizeof(double), 0) != hipSuccess) {
+    (void)hipGetLastError();
+    delete h;
+    *out = nullptr;
+    return TTX_ERR_HIP;
+  }
+  (void)ttx::num_cus();
+  *out = reinterpret_cast<ttx_handle>(h);
+  return TTX_OK;
+}
+
+ttx_status ttx_destroy(ttx_handle hh) {
+  Handle* h = H(hh);
+  if (!h) return TTX_OK;
+  (void)hipStreamSynchronize(h->stream);
+  ttx::DevBuf* bufs[] = {&h->part, &h->Y, &h->Qc, &h->Qn, &h->Z, &h->W1, &h->W2, &h->G, &h->Rinv, &h->lam,
+                         &h->V, &h->Ur, &h->gpart, &h->status, &h->next[0], &h->next[1], &h->sig,
+                         &h->scratch[0], &h->scratch[1], &h->scratch[2], &h->scratch[3]};
+  for (auto* b : bufs)
+    if (b->p) (void)hipFree(b->p);
+  for (float* c : h->last_cores) (void)hipFree(c);
+  if (h->h_status) (void)hipHostFree(h->h_status);
+  if (h->h_sigma) (void)hipHostFree(h->h_sigma);
+  if (h->h_lam) (void)hipHostFree(h->h_lam);
+  delete h;
+  return TTX_OK;
+}
+
+ttx_status ttx_set_stream(ttx_handle h, void* stream) {
+  if (!h) return TTX_ERR_BAD_ARG;
+  H(h)->stream = reinterpret_cast<hipStream_t>(stream);
+  return TTX_OK;
+}
+
+const char* ttx_last_error(ttx_handle h) { return h ? H(h)->err.c_str() : "null handle"; }
+
+void ttx_params_init(ttx_params* p, int32_t order, const int64_t* dims, const int32_t* ranks) {
+  if (!p) return;
+  memset(p, 0, sizeof(*p));
+  p->order = order;
+  for (int k = 0; k < order && k < TTX_MAX_ORDER; ++k) p->dims[k] = dims ? dims[k] : 1;
+  for (int k = 0; k <= order && k <= TTX_MAX_ORDER; ++k) p->ranks[k] = ranks ? ranks[k] : 1;
+  p->oversampling = 10;
+  p->power_iters = 2;
+  p->omega_seed = 1234;
+  p->precision = TTX_PREC_BF16;
+  p->method = TTX_METHOD_AUTO;
+  p->rank_mode = TTX_RANK_FIXED;
+  p->eps = 1e-4;
+  p->flags = 0;
+}
+
+int64_t ttx_core_size(const ttx_params* p, int32_t tr, int32_t k) {
+  if (!p || k < 0 || k >= p->order) return 0;
+  return ttx::core_elems(p->order, p->dims, p->ranks, tr != 0, k);
+}
+
+ttx_status ttx_tt_svd(ttx_handle h, const float* d_A, const ttx_params* p, float* const* d_cores, float* h_sigma,
+                      ttx_report* rep) {
+  if (!h) return TTX_ERR_BAD_ARG;
+  return ttx::sweep(H(h), d_A, p, d_cores, h_sigma, rep, false);
+}
+
+ttx_status ttx_tr_svd(ttx_handle h, const float* d_A, const ttx_params* p, float* const* d_cores, float* h_sigma,
+                      ttx_report* rep) {
+  if (!h) return TTX_ERR_BAD_ARG;
+  return ttx::sweep(H(h), d_A, p, d_cores, h_sigma, rep, true);
+}
+
+ttx_status ttx_step(ttx_handle hh, const float* d_A, int64_t m, int64_t n, int32_t r, int32_t s, int32_t q,
+                    uint64_t seed, uint32_t stream_id, int32_t precision, int32_t method, float* d_core, float* d_next,
+                    float* h_sigma, int32_t* path_out) {
+  Handle* h = H(hh);
+  if (!h || !d_A || !d_core || !d_next || m <= 0 || n <= 0 || r <= 0 || s < 0 || q < 0) {
+    if (h) h->err = "ttx_step: bad argument";
+    return TTX_ERR_BAD_ARG;
+  }
+  ttx::StepArgs a;
+  a.A = d_A; a.m = m; a.n = n; a.r = r; a.s = s; a.q = q; a.seed = seed; a.stream_id = stream_id;
+  a.precision = precision; a.method = method; a.core = d_core; a.next = d_next;
+  ttx::StepOut o;
+  ttx_status st = ttx::run_step(h, a, o);
+  if (st != TTX_OK) return st;
+  if (h_sigma) memcpy(h_sigma, o.sigma, sizeof(float) * TTX_MAX_ELL);
+  if (path_out) *path_out = o.path;
+  return TTX_OK;
+}
+
+ttx_status ttx_rsvd(ttx_handle hh, const float* d_A, int64_t m, int64_t n, int32_t k, int32_t s, int32_t q,
+                    uint64_t seed, int32_t precision, float* d_U, float* d_S, float* d_SVT) {
+  Handle* h = H(hh);
+  if (!h || !d_A || !d_U || !d_SVT || m <= 0 || n <= 0 || k <= 0 || s < 0 || q < 0) {
+    if (h) h->err = "ttx_rsvd: bad argument";
+    return TTX_ERR_BAD_ARG;
+  }
+  const int ell = k + s;
+  if (ell > TTX_MAX_ELL || ell > std::min(m, n)) {
+    h->err = "ttx_rsvd: k + s must not exceed min(m, n) or TTX_MAX_ELL";
+    return TTX_ERR_BAD_ARG;
+  }
+  ttx::StepArgs a;
+  a.A = d_A; a.m = m; a.n = n; a.r = ell; a.s = 0; a.q = q; a.seed = seed; a.stream_id = 0;
+  a.precision = precision; a.method = TTX_METHOD_RSVD; a.core = d_U; a.next = d_SVT;
+  ttx::StepOut o;
+  ttx_status st = ttx::run_step(h, a, o);
+  if (st != TTX_OK) return st;
+  if (d_S) {
+    hipError_t e = hipMemcpyAsync(d_S, o.sigma, ell * sizeof(float), hipMemcpyHostToDevice, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    if (e != hipSuccess) { h->err = hipGetErrorString(e); return TTX_ERR_HIP; }
+  }
+  return TTX_OK;
+}
+
+ttx_status ttx_permute_231(ttx_handle hh, const float* d_in, float* d_out, int64_t r0, int64_t r1, int64_t n) {
+  Handle* h = H(hh);
+  if (!h || !d_in || !d_out || r0 <= 0 || r1 <= 0 || n <= 0) return TTX_ERR_BAD_ARG;
+  ttx::launch_permute_231(d_in, d_out, r0, r1, n, h->stream);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+  if (e != hipSuccess) { h->err = hipGetErrorString(e); return TTX_ERR_HIP; }
+  return TTX_OK;
+}
+
+ttx_status ttx_fill_normal(ttx_handle hh, float* d_out, int64_t count, uint64_t seed, uint32_t stream, float scale) {
+  Handle* h = H(hh);
+  if (!h || !d_out || count < 0) return TTX_ERR_BAD_ARG;
+  if (count == 0) return TTX_OK;
+  ttx::launch_fill_normal(d_out, count, seed, stream, scale, h->stream);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+  if (e != hipSuccess) { h->err = hipGetErrorString(e); return TTX_ERR_HIP; }
+  return TTX_OK;
+}
+
+ttx_status ttx_omega(ttx_handle hh, float* d_out, int64_t n, int32_t ell, uint64_t seed, uint32_t stream) {
+  Handle* h = H(hh);
+  if (!h || !d_out || n <= 0 || ell <= 0) return TTX_ERR_BAD_ARG;
+  ttx::launch_omega(d_out, n, ell, seed, stream, h->stream);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+  if (e != hipSuccess) { h->err = hipGetErrorString(e); return TTX_ERR_HIP; }
+  return TTX_OK;
+}
+
+ttx_status ttx_tt_generate(ttx_handle h, int32_t order, const int64_t* dims, const int32_t* ranks, float std_,
+                           float noise_rel, uint64_t seed, float* d_A, float* const* d_cores) {
+  if (!h || !dims || !ranks) return TTX_ERR_BAD_ARG;
+  if (ranks[0] != 1 || ranks[order] != 1) { H(h)->err = "TT ranks need r_0 = r_d = 1"; return TTX_ERR_BAD_ARG; }
+  return ttx::generate(H(h), order, dims, ranks, std_, noise_rel, seed, d_A, d_cores, false);
+}
+
+ttx_status ttx_tr_generate(ttx_handle h, int32_t order, const int64_t* dims, const int32_t* ranks, float std_,
+                           float noise_rel, uint64_t seed, float* d_A, float* const* d_cores) {
+  if (!h || !dims || !ranks) return TTX_ERR_BAD_ARG;
+  if (ranks[0] != ranks[order]) { H(h)->err = "TR ranks need r_0 = r_d"; return TTX_ERR_BAD_ARG; }
+  return ttx::generate(H(h), order, dims, ranks, std_, noise_rel, seed, d_A, d_cores, true);
+}
+
+ttx_status ttx_tt_rel_error(ttx_handle h, int32_t order, const int64_t* dims, const int32_t* ranks,
+                            const float* const* d_cores, const float* d_A, double* err) {
+  if (!h || !dims || !ranks || !d_cores || !d_A || !err || order < 2 || order > TTX_MAX_ORDER) return TTX_ERR_BAD_ARG;
+  return ttx::rel_error(H(h), order, dims, ranks, d_cores, d_A, false, err);
+}
+
+ttx_status ttx_tr_rel_error(ttx_handle h, int32_t order, const int64_t* dims, const int32_t* ranks,
+                            const float* const* d_cores, const float* d_A, double* err) {
+  if (!h || !dims || !ranks || !d_cores || !d_A || !err || order < 2 || order > TTX_MAX_ORDER) return TTX_ERR_BAD_ARG;
+  return ttx::rel_error(H(h), order, dims, ranks, d_cores, d_A, true, err);
+}
+
+ttx_status ttx_tt_full(ttx_handle h, int32_t order, const int64_t* dims, const int32_t* ranks,
+                       const float* const* d_cores, float* d_out) {
+  if (!h || !dims || !ranks || !d_cores || !d_out || order < 2 || order > TTX_MAX_ORDER) return TTX_ERR_BAD_ARG;
+  return ttx::full(H(h), order, dims, ranks, d_cores, d_out, false);
+}
+
+ttx_status ttx_tr_full(ttx_handle h, int32_t order, const int64_t* dims, const int32_t* ranks,
+                       const float* const* d_cores, float* d_out) {
+  if (!h || !dims || !ranks || !d_cores || !d_out || order < 2 || order > TTX_MAX_ORDER) return TTX_ERR_BAD_ARG;
+  return ttx::full(H(h), order, dims, ranks, d_cores, d_out, true);
+}
+
+}  // extern "C"

@@ -1,0 +1,80 @@
+// Host-side launchers for every ttx kernel.  All launches are asynchronous on the given stream.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace ttx {
+
+int num_cus();
+
+// ---- streaming passes (ttx_stream.hip) ---------------------------------------------------------
+struct SketchPlan {
+  int RB = 1, CG = 1, ysplit = 1, grid_x = 1, kp = 32, lds_bytes = 0;
+  int64_t chunks_per_wg = 1, slabs = 1;
+};
+SketchPlan plan_sketch(int m, int64_t n, int kp, int wgs_hint);
+// Y partials: part[slab][m][kp].  X == nullptr selects the in-kernel Philox sketch (seed, stream, ell).
+void launch_sketch(int ar, const SketchPlan& p, const float* A, int64_t lda, int m, int64_t n, const float* X,
+                   int64_t ldx, uint64_t seed, uint32_t stream, int ell, float* part, hipStream_t st);
+// D = Q^T A; mode 0 writes Z (n x ldo row-major, kp columns), mode 1 writes r_out x n column-major
+void launch_proj(int ar, int kp, const float* A, int64_t lda, int m, int64_t n, const float* Qc, int64_t ldq,
+                 int mode, float* out, int64_t ldo, int r_out, hipStream_t st);
+int64_t gram_small_grid(int64_t n);
+// partial A A^T: part[4 * grid][m][m] (FP64); absmax (optional) receives max |A| as float bits
+void launch_gram_small(int ar, const float* A, int64_t lda, int m, int64_t n, double* part, int64_t grid,
+                       float* absmax, hipStream_t st);
+void launch_reduce_f32(const float* part, int64_t S, int64_t count, float* out, hipStream_t st);
+void launch_reduce_f64(const double* part, int64_t S, int64_t count, double* out, hipStream_t st);
+
+// ---- FP64 small dense kernels (ttx_dense.hip) --------------------------------------------------
+// Gram of a tall matrix X (rows x k, row stride ldx): part[grid][k][k] FP64.
+int64_t gram_tall_grid(int64_t rows);
+void launch_gram_tall_f32(const float* X, int64_t ldx, int64_t rows, int k, double* part, int64_t grid,
+                          hipStream_t st);
+void launch_gram_tall_f64(const double* X, int64_t ldx, int64_t rows, int k, double* part, int64_t grid,
+                          hipStream_t st);
+// Cholesky of G (+ shift) and the explicit inverse of the upper factor: G + s I = R^T R, Rinv = R^-1.
+// shift_mode 1 adds s = 11 (m k + k (k+1)) u ||G||_2-bound; pivots that are not positive zero
+// their row and column of Rinv.  status[0] counts the zeroed pivots.
+void launch_chol_inv(const double* G, int k, int64_t m, int shift_mode, double* Rinv, int* status, hipStream_t st);
+// W_out = W_in Rinv (rows x k); W_in float (row stride ldi) or double (row stride k); output double row-major
+// (stride k) or float column-major (Qc[c * ldq + i], columns k..kp-1 zeroed)
+void launch_apply_rinv_f32(const float* Win, int64_t ldi, int64_t rows, int k, const double* Rinv, double* Wout,
+                           hipStream_t st);
+void launch_apply_rinv_f64(const double* Win, int64_t rows, int k, const double* Rinv, double* Wout, hipStream_t st);
+void launch_apply_rinv_out(const double* Win, int64_t rows, int k, int kp, const double* Rinv, float* Qc,
+                           int64_t ldq, hipStream_t st);
+// symmetric eigen-decomposition (cyclic Jacobi, FP64): lam descending, V column-major k x k
+void launch_syevj(const double* G, int k, double* lam, double* V, int* status, double* work, hipStream_t st);
+// core assembly after the eigen-decomposition of the projected Gram:
+//   Gc = Q V[:, :r] (m x r column-major, FP32) with the sign of each column fixed so that its largest
+//   |entry| is positive (first index on ties); Ur = V[:, :r] diag(sign) (kp x r, FP32, row-major);
+//   Qn = Gc padded with zero columns to kpn (column-major kpn x m) for a following projection;
+//   sigma = sqrt(max(lam, 0)) for the first k values.  Qc == nullptr means Q = I (exact path).
+void launch_assemble(const float* Qc, int64_t ldq, int m, int k, const double* lam, const double* V, int r,
+                     float* Gc, float* Ur, float* Qn, int kpn, float* sigma, hipStream_t st);
+// A_next[c + r j] = sum_a Ur[a][c] Z[j][a]  (c < r, j < n); Z row-major with stride ldz
+void launch_next_from_z(const float* Z, int64_t ldz, int64_t n, int k, const float* Ur, int r, float* out,
+                        hipStream_t st);
+// Y (m x kp row-major) <- A (m x n column-major, n <= kp), zero columns n..kp-1
+void launch_copy_cols(const float* A, int64_t lda, int m, int n, float* Y, int kp, hipStream_t st);
+
+// ---- generators / reconstruction (ttx_rng.hip, ttx_gemm.hip) -----------------------------------
+void launch_fill_normal(float* out, int64_t count, uint64_t seed, uint32_t stream, float scale, hipStream_t st);
+void launch_add_noise(float* A, int64_t count, uint64_t seed, uint32_t stream, float coef, hipStream_t st);
+void launch_sumsq(const float* A, int64_t count, double* out2 /* [1] */, double* part, int64_t grid, hipStream_t st);
+int64_t sumsq_grid(int64_t count);
+void launch_omega(float* out, int64_t n, int ell, uint64_t seed, uint32_t stream, hipStream_t st);
+// C = alpha A B + beta C (column-major, FP32); resid != nullptr: no C write, accumulate
+// sum (X - A B)^2 and sum X^2 per block into resid[2 * block] (X with leading dimension ldc)
+void launch_gemm(int64_t M, int64_t N, int64_t K, float alpha, const float* A, int64_t lda, const float* B,
+                 int64_t ldb, float beta, float* C, int64_t ldc, hipStream_t st);
+int64_t gemm_resid_blocks(int64_t M, int64_t N);
+void launch_gemm_resid(int64_t M, int64_t N, int64_t K, const float* A, int64_t lda, const float* B, int64_t ldb,
+                       const float* X, int64_t ldx, double* resid, hipStream_t st);
+// generic permutation of a column-major tensor with up to 6 modes: out[perm] = in
+void launch_permute(const float* in, float* out, int nd, const int64_t* dims, const int* perm, hipStream_t st);
+// (r0, r1, n) -> (r1, n, r0), column-major (trdec6_singleGPU.cu:49-58)
+void launch_permute_231(const float* in, float* out, int64_t r0, int64_t r1, int64_t n, hipStream_t st);
+
+}  // namespace ttx

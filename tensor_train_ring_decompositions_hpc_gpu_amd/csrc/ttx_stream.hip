@@ -1,0 +1,668 @@
+// Streaming kernels: the passes that read a whole unfolding A (m x n, column-major, leading
+// dimension lda) from HBM once.
+//
+//   k_sketch     Y = A X          X = Omega (Philox, generated in-kernel) or X = Z (n x kp, row-major)
+//                                 -> per-slab partial sums, reduced by k_reduce_f32
+//   k_proj       D = Q^T A        Q column-major (kp x m); D written either as Z = D^T (n x kp,
+//                                 row-major: the power pass) or as the next unfolding r x n
+//                                 (column-major: the remainder / projection)
+//   k_gram_small C = A A^T        m <= 64, exact path (ttdec6_singleGPU.cu:388-401)
+//
+// Rows of A are spread over lanes so that every global load is a 16-byte vector along a column
+// (rows 4r..4r+3 of one column per lane); the MFMA row index r of tile s then maps to physical
+// row 4r+s.  The 32x32 MFMA output layout is documented in ttx_device.h.
+#include "ttx_device.h"
+#include "ttx_kernels.h"
+
+namespace ttx {
+
+// ------------------------------------------------------------------------------------------------
+// sketch: Y = A X
+// ------------------------------------------------------------------------------------------------
+// WG = 8 waves.  RB = row blocks of 128 rows per WG (1, 2, 4 or 8), CG = 8 / RB column groups.
+// Wave w: rb = w % RB, cg = w / RB; it accumulates rows [y*128*RB + 128 rb, +128) over columns
+// jc + 16 cg + [0, 16) of every chunk (chunk width KC = 16 CG).
+// Partial output: part[slab][row][kp], slab = blockIdx.x * CG + cg.
+
+template <int LT>
+struct SketchAcc {
+  f32x16 acc[4][LT];
+};
+
+template <bool VEC>
+__device__ __forceinline__ void sketch_load(const float* __restrict__ A, int64_t lda, int m, int64_t n,
+                                            int row, int64_t col0, float (&a)[8][4]) {
+#pragma unroll
+  for (int t = 0; t < 8; ++t) {
+    const int64_t j = col0 + t;
+    if (VEC) {
+      // full rows: row + 3 < m guaranteed by the caller
+      if (j < n) {
+        const f32x4 v = *reinterpret_cast<const f32x4*>(A + row + j * lda);
+        a[t][0] = v[0]; a[t][1] = v[1]; a[t][2] = v[2]; a[t][3] = v[3];
+      } else {
+        a[t][0] = a[t][1] = a[t][2] = a[t][3] = 0.0f;
+      }
+    } else {
+#pragma unroll
+      for (int s = 0; s < 4; ++s) a[t][s] = (j < n && row + s < m) ? A[row + s + j * lda] : 0.0f;
+    }
+  }
+}
+
+template <int LT, int AR, bool OMEGA>
+__device__ __forceinline__ void sketch_compute(const float (&a)[8][4], const float* __restrict__ xs,
+                                               int pitch, int kofs, int lane, SketchAcc<LT>& S) {
+  const int c = lane & 31;
+  const int b = lane >> 5;
+  // B operands: X[k = kofs + 8b + t][col = 32 lt + c]
+  float xv[LT][8];
+#pragma unroll
+  for (int lt = 0; lt < LT; ++lt) {
+    const float* p = xs + (32 * lt + c) * pitch + kofs + 8 * b;
+    const f32x4 u0 = *reinterpret_cast<const f32x4*>(p);
+    const f32x4 u1 = *reinterpret_cast<const f32x4*>(p + 4);
+    xv[lt][0] = u0[0]; xv[lt][1] = u0[1]; xv[lt][2] = u0[2]; xv[lt][3] = u0[3];
+    xv[lt][4] = u1[0]; xv[lt][5] = u1[1]; xv[lt][6] = u1[2]; xv[lt][7] = u1[3];
+  }
+  if constexpr (AR == AR_FP32) {
+#pragma unroll
+    for (int t = 0; t < 8; ++t)
+#pragma unroll
+      for (int s = 0; s < 4; ++s)
+#pragma unroll
+        for (int lt = 0; lt < LT; ++lt) S.acc[s][lt] = mfma_f32(a[t][s], xv[lt][t], S.acc[s][lt]);
+  } else if constexpr (AR == AR_FP16) {
+    f16x8 bh[LT];
+#pragma unroll
+    for (int lt = 0; lt < LT; ++lt) round8h(xv[lt], bh[lt]);
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      float v[8];
+#pragma unroll
+      for (int t = 0; t < 8; ++t) v[t] = a[t][s];
+      f16x8 ah;
+      round8h(v, ah);
+#pragma unroll
+      for (int lt = 0; lt < LT; ++lt) S.acc[s][lt] = mfma_f16(ah, bh[lt], S.acc[s][lt]);
+    }
+  } else {
+    bf16x8 bh[LT], bl[LT];
+#pragma unroll
+    for (int lt = 0; lt < LT; ++lt) {
+      if (AR == AR_BF16X3 && !OMEGA) split8(xv[lt], bh[lt], bl[lt]);
+      else round8(xv[lt], bh[lt]);   // Omega entries are exact in bf16
+    }
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      float v[8];
+#pragma unroll
+      for (int t = 0; t < 8; ++t) v[t] = a[t][s];
+      bf16x8 ah, al;
+      if (AR == AR_BF16X3) split8(v, ah, al);
+      else round8(v, ah);
+#pragma unroll
+      for (int lt = 0; lt < LT; ++lt) {
+        S.acc[s][lt] = mfma_bf16(ah, bh[lt], S.acc[s][lt]);
+        if (AR == AR_BF16X3) {
+          S.acc[s][lt] = mfma_bf16(al, bh[lt], S.acc[s][lt]);
+          if (!OMEGA) S.acc[s][lt] = mfma_bf16(ah, bl[lt], S.acc[s][lt]);
+        }
+      }
+    }
+  }
+}
+
+// Fill the LDS chunk xs[c][k] (c < kp = 32 LT, k < KC) for columns jc .. jc+KC-1 of X.
+template <int LT, bool OMEGA>
+__device__ __forceinline__ void sketch_fill(float* __restrict__ xs, int pitch, int KC, int64_t jc, int64_t n,
+                                            const float* __restrict__ X, int64_t ldx,
+                                            uint64_t seed, uint32_t stream, int ell) {
+  constexpr int KP = 32 * LT;
+  constexpr int NG = KP / 4;
+  const int tid = threadIdx.x;
+  const int total = KC * NG;
+  for (int idx = tid; idx < total; idx += blockDim.x) {
+    const int jj = idx % KC;
+    const int g = idx / KC;
+    const int64_t j = jc + jj;
+    float z[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+    if (j < n) {
+      if (OMEGA) {
+        if (4 * g < ell) {
+          normal4(seed, stream, (uint64_t)j, (uint32_t)g, z);
+#pragma unroll
+          for (int e = 0; e < 4; ++e) z[e] = (4 * g + e < ell) ? omega_round(z[e]) : 0.0f;
+        }
+      } else {
+        const f32x4 v = *reinterpret_cast<const f32x4*>(X + j * ldx + 4 * g);
+        z[0] = v[0]; z[1] = v[1]; z[2] = v[2]; z[3] = v[3];
+      }
+    }
+#pragma unroll
+    for (int e = 0; e < 4; ++e) xs[(4 * g + e) * pitch + jj] = z[e];
+  }
+}
+
+// LT = 1 runs 8 waves per WG (2 per SIMD); LT = 2 runs 4 waves per WG so that one wave per SIMD
+// can hold its 128 accumulators in AGPRs.
+template <int LT>
+constexpr int sketch_waves() { return LT == 1 ? 8 : 4; }
+
+template <int LT, int AR, bool OMEGA>
+__global__ __launch_bounds__(64 * sketch_waves<LT>()) void k_sketch(const float* __restrict__ A, int64_t lda, int m, int64_t n,
+                                                const float* __restrict__ X, int64_t ldx, uint64_t seed,
+                                                uint32_t stream, int ell, int RB, int CG, int64_t chunks_per_wg,
+                                                float* __restrict__ part) {
+  constexpr int KP = 32 * LT;
+  const int KC = 16 * CG;
+  const int pitch = KC + 4;
+  extern __shared__ __align__(16) float smem[];
+  float* xs0 = smem;
+  float* xs1 = smem + KP * pitch;
+
+  const int lane = threadIdx.x & 63;
+  const int w = threadIdx.x >> 6;
+  const int rb = w % RB;
+  const int cg = w / RB;
+  const int r = lane & 31;
+  const int b = lane >> 5;
+  const int row = blockIdx.y * 128 * RB + 128 * rb + 4 * r;
+  const bool vec = ((lda & 3) == 0) && (row + 3 < m);
+
+  const int64_t nchunks = (n + KC - 1) / KC;
+  const int64_t c0 = (int64_t)blockIdx.x * chunks_per_wg;
+  const int64_t c1 = (c0 + chunks_per_wg < nchunks) ? c0 + chunks_per_wg : nchunks;
+
+  SketchAcc<LT> S;
+#pragma unroll
+  for (int s = 0; s < 4; ++s)
+#pragma unroll
+    for (int lt = 0; lt < LT; ++lt) S.acc[s][lt] = zero16();
+
+  float a0[8][4], a1[8][4];
+  const int kofs = 16 * cg;
+  if (c0 < c1) {
+    sketch_fill<LT, OMEGA>(xs0, pitch, KC, c0 * KC, n, X, ldx, seed, stream, ell);
+    const int64_t col = c0 * KC + kofs + 8 * b;
+    if (vec) sketch_load<true>(A, lda, m, n, row, col, a0);
+    else sketch_load<false>(A, lda, m, n, row, col, a0);
+  }
+  __syncthreads();
+  for (int64_t c = c0; c < c1; c += 2) {
+    // even step: compute chunk c from a0 / xs0, prefetch chunk c+1 into a1 / xs1
+    if (c + 1 < c1) {
+      sketch_fill<LT, OMEGA>(xs1, pitch, KC, (c + 1) * KC, n, X, ldx, seed, stream, ell);
+      const int64_t col = (c + 1) * KC + kofs + 8 * b;
+      if (vec) sketch_load<true>(A, lda, m, n, row, col, a1);
+      else sketch_load<false>(A, lda, m, n, row, col, a1);
+    }
+    sketch_compute<LT, AR, OMEGA>(a0, xs0, pitch, kofs, lane, S);
+    __syncthreads();
+    if (c + 1 >= c1) break;
+    // odd step: compute chunk c+1 from a1 / xs1, prefetch chunk c+2 into a0 / xs0
+    if (c + 2 < c1) {
+      sketch_fill<LT, OMEGA>(xs0, pitch, KC, (c + 2) * KC, n, X, ldx, seed, stream, ell);
+      const int64_t col = (c + 2) * KC + kofs + 8 * b;
+      if (vec) sketch_load<true>(A, lda, m, n, row, col, a0);
+      else sketch_load<false>(A, lda, m, n, row, col, a0);
+    }
+    sketch_compute<LT, AR, OMEGA>(a1, xs1, pitch, kofs, lane, S);
+    __syncthreads();
+  }
+
+  // partial output (row-major, kp columns)
+  const int64_t slab = (int64_t)blockIdx.x * CG + cg;
+  float* out = part + slab * (int64_t)m * KP;
+#pragma unroll
+  for (int s = 0; s < 4; ++s)
+#pragma unroll
+    for (int lt = 0; lt < LT; ++lt)
+#pragma unroll
+      for (int v = 0; v < 16; ++v) {
+        const int prow = blockIdx.y * 128 * RB + 128 * rb + 4 * mfma32_row(v, lane) + s;
+        if (prow < m) out[(int64_t)prow * KP + 32 * lt + r] = S.acc[s][lt][v];
+      }
+}
+
+// ------------------------------------------------------------------------------------------------
+// projection: D = Q^T A (kp x n), Q column-major kp x m (Qc[c*ldq + i])
+// ------------------------------------------------------------------------------------------------
+// WG = 8 waves; wave w owns columns jb + 32 w + [0, 32) of the current column block (256 columns).
+// K loop in chunks of 64 rows; the Q chunk (64 rows x kp) is staged in LDS and shared by the waves.
+// Lane (jl = lane%32, b = lane/32) loads rows 64 kc + 32 b + [0, 32) of column jb + 32 w + jl
+// (one 128-byte line); K-step t uses rows 32 b + 8 t + [0, 8) of the chunk for both operands.
+template <int LT, int AR>
+__global__ __launch_bounds__(512) void k_proj(const float* __restrict__ A, int64_t lda, int m, int64_t n,
+                                              const float* __restrict__ Qc, int64_t ldq,
+                                              int out_mode, float* __restrict__ out, int64_t ldo, int r_out,
+                                              int64_t blocks_total) {
+  constexpr int KP = 32 * LT;
+  constexpr int QP = 64 + 4;   // LDS pitch of a Q column chunk
+  __shared__ float qs[2][KP * QP];
+  const int lane = threadIdx.x & 63;
+  const int w = threadIdx.x >> 6;
+  const int jl = lane & 31;
+  const int b = lane >> 5;
+  const int nk = (m + 63) / 64;
+  const bool vec = ((lda & 3) == 0) && ((m & 3) == 0);
+
+  for (int64_t blk = blockIdx.x; blk < blocks_total; blk += gridDim.x) {
+    const int64_t j = blk * 256 + 32 * w + jl;
+    f32x16 acc[LT];
+#pragma unroll
+    for (int lt = 0; lt < LT; ++lt) acc[lt] = zero16();
+
+    auto fill = [&](float* q, int kc) {
+      // 64 rows x KP columns; thread handles float4 pieces along rows
+      for (int idx = threadIdx.x; idx < KP * 16; idx += 512) {
+        const int c = idx / 16;
+        const int i4 = (idx % 16) * 4;
+        const int i = kc * 64 + i4;
+        f32x4 v;
+        if (((ldq & 3) == 0) && i + 3 < m) {
+          v = *reinterpret_cast<const f32x4*>(Qc + (int64_t)c * ldq + i);
+        } else {
+#pragma unroll
+          for (int e = 0; e < 4; ++e) v[e] = (i + e < m) ? Qc[(int64_t)c * ldq + i + e] : 0.0f;
+        }
+        *reinterpret_cast<f32x4*>(q + c * QP + i4) = v;
+      }
+    };
+    auto load = [&](float (&x)[32], int kc) {
+      const int i0 = kc * 64 + 32 * b;
+      if (j < n) {
+        const float* p = A + j * lda + i0;
+        if (vec && i0 + 31 < m) {
+#pragma unroll
+          for (int t = 0; t < 8; ++t) {
+            const f32x4 v = *reinterpret_cast<const f32x4*>(p + 4 * t);
+            x[4 * t] = v[0]; x[4 * t + 1] = v[1]; x[4 * t + 2] = v[2]; x[4 * t + 3] = v[3];
+          }
+        } else {
+#pragma unroll
+          for (int e = 0; e < 32; ++e) x[e] = (i0 + e < m) ? p[e] : 0.0f;
+        }
+      } else {
+#pragma unroll
+        for (int e = 0; e < 32; ++e) x[e] = 0.0f;
+      }
+    };
+    auto compute = [&](const float (&x)[32], const float* q) {
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        const float* xb = x + 8 * t;   // B operand: rows 32 b + 8 t + [0,8) of column j
+        if constexpr (AR == AR_FP32) {
+#pragma unroll
+          for (int lt = 0; lt < LT; ++lt) {
+            const float* qp = q + (32 * lt + jl) * QP + 32 * b + 8 * t;
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+              // MFMA k index = lane half b; K-step (t, u) pairs row 32 b + 8 t + u of both operands
+              acc[lt] = mfma_f32(qp[u], xb[u], acc[lt]);
+            }
+          }
+        } else if constexpr (AR == AR_FP16) {
+          f16x8 bh;
+          round8h(xb, bh);
+#pragma unroll
+          for (int lt = 0; lt < LT; ++lt) {
+            const float* qp = q + (32 * lt + jl) * QP + 32 * b + 8 * t;
+            float qv[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) qv[u] = qp[u];
+            f16x8 ah;
+            round8h(qv, ah);
+            acc[lt] = mfma_f16(ah, bh, acc[lt]);
+          }
+        } else {
+          bf16x8 bh, bl;
+          if (AR == AR_BF16X3) split8(xb, bh, bl);
+          else round8(xb, bh);
+#pragma unroll
+          for (int lt = 0; lt < LT; ++lt) {
+            const float* qp = q + (32 * lt + jl) * QP + 32 * b + 8 * t;
+            float qv[8];
+            const f32x4 u0 = *reinterpret_cast<const f32x4*>(qp);
+            const f32x4 u1 = *reinterpret_cast<const f32x4*>(qp + 4);
+            qv[0] = u0[0]; qv[1] = u0[1]; qv[2] = u0[2]; qv[3] = u0[3];
+            qv[4] = u1[0]; qv[5] = u1[1]; qv[6] = u1[2]; qv[7] = u1[3];
+            bf16x8 ah, al;
+            if (AR == AR_BF16X3) split8(qv, ah, al);
+            else round8(qv, ah);
+            acc[lt] = mfma_bf16(ah, bh, acc[lt]);
+            if (AR == AR_BF16X3) {
+              acc[lt] = mfma_bf16(al, bh, acc[lt]);
+              acc[lt] = mfma_bf16(ah, bl, acc[lt]);
+            }
+          }
+        }
+      }
+    };
+
+    float x0[32], x1[32];
+    fill(qs[0], 0);
+    load(x0, 0);
+    __syncthreads();
+    for (int kc = 0; kc < nk; kc += 2) {
+      if (kc + 1 < nk) {
+        fill(qs[1], kc + 1);
+        load(x1, kc + 1);
+      }
+      compute(x0, qs[0]);
+      __syncthreads();
+      if (kc + 1 >= nk) break;
+      if (kc + 2 < nk) {
+        fill(qs[0], kc + 2);
+        load(x0, kc + 2);
+      }
+      compute(x1, qs[1]);
+      __syncthreads();
+    }
+
+    if (j < n) {
+      if (out_mode == 0) {
+        // Z = D^T, row-major n x ldo, all kp columns
+        float* o = out + j * ldo;
+#pragma unroll
+        for (int lt = 0; lt < LT; ++lt)
+#pragma unroll
+          for (int g = 0; g < 4; ++g) {
+            f32x4 v;
+            v[0] = acc[lt][4 * g]; v[1] = acc[lt][4 * g + 1]; v[2] = acc[lt][4 * g + 2]; v[3] = acc[lt][4 * g + 3];
+            *reinterpret_cast<f32x4*>(o + 32 * lt + 8 * g + 4 * b) = v;
+          }
+      } else {
+        // next unfolding, column-major r_out x n
+        float* o = out + j * (int64_t)r_out;
+#pragma unroll
+        for (int lt = 0; lt < LT; ++lt)
+#pragma unroll
+          for (int v = 0; v < 16; ++v) {
+            const int c = 32 * lt + mfma32_row(v, lane);
+            if (c < r_out) o[c] = acc[lt][v];
+          }
+      }
+    }
+  }
+}
+
+// ------------------------------------------------------------------------------------------------
+// exact-path Gram: C = A A^T for m <= 64 (MT = number of 32-row tiles), FP64 partial per WG
+// ------------------------------------------------------------------------------------------------
+template <int MT, int AR>
+__global__ __launch_bounds__(256) void k_gram_small(const float* __restrict__ A, int64_t lda, int m, int64_t n,
+                                                    int64_t cols_per_wg, double* __restrict__ part,
+                                                    float* __restrict__ absmax) {
+  constexpr int NP = MT * (MT + 1) / 2;
+  const int lane = threadIdx.x & 63;
+  const int w = threadIdx.x >> 6;
+  const int r = lane & 31;
+  const int b = lane >> 5;
+  const int64_t j0 = (int64_t)blockIdx.x * cols_per_wg;
+  const int64_t j1 = (j0 + cols_per_wg < n) ? j0 + cols_per_wg : n;
+
+  double dacc[NP][16];
+#pragma unroll
+  for (int p = 0; p < NP; ++p)
+#pragma unroll
+    for (int v = 0; v < 16; ++v) dacc[p][v] = 0.0;
+  float amax = 0.0f;
+
+  // each wave takes 16-column steps j0 + 16 (w + 4 i); flush the fp32 MFMA accumulator to fp64
+  // every 64 steps
+  int64_t step = 0;
+  f32x16 acc[NP];
+#pragma unroll
+  for (int p = 0; p < NP; ++p) acc[p] = zero16();
+  for (int64_t jc = j0 + 16 * w; jc < j1; jc += 64) {
+    float v[MT][8];
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+      for (int t = 0; t < 8; ++t) {
+        const int64_t j = jc + 8 * b + t;
+        const int i = 32 * mt + r;
+        v[mt][t] = (j < j1 && i < m) ? A[i + j * lda] : 0.0f;
+        amax = fmaxf(amax, fabsf(v[mt][t]));
+      }
+    if constexpr (AR == AR_FP32) {
+#pragma unroll
+      for (int t = 0; t < 8; ++t) {
+        int p = 0;
+#pragma unroll
+        for (int m1 = 0; m1 < MT; ++m1)
+#pragma unroll
+          for (int m2 = m1; m2 < MT; ++m2) { acc[p] = mfma_f32(v[m1][t], v[m2][t], acc[p]); ++p; }
+      }
+    } else {
+      bf16x8 hi[MT], lo[MT];
+#pragma unroll
+      for (int mt = 0; mt < MT; ++mt) {
+        if (AR == AR_BF16X3) split8(v[mt], hi[mt], lo[mt]);
+        else round8(v[mt], hi[mt]);
+      }
+      int p = 0;
+#pragma unroll
+      for (int m1 = 0; m1 < MT; ++m1)
+#pragma unroll
+        for (int m2 = m1; m2 < MT; ++m2) {
+          acc[p] = mfma_bf16(hi[m1], hi[m2], acc[p]);
+          if (AR == AR_BF16X3) {
+            acc[p] = mfma_bf16(lo[m1], hi[m2], acc[p]);
+            acc[p] = mfma_bf16(hi[m1], lo[m2], acc[p]);
+          }
+          ++p;
+        }
+    }
+    if (++step == 64) {
+      step = 0;
+#pragma unroll
+      for (int p = 0; p < NP; ++p) {
+#pragma unroll
+        for (int q = 0; q < 16; ++q) dacc[p][q] += (double)acc[p][q];
+        acc[p] = zero16();
+      }
+    }
+  }
+#pragma unroll
+  for (int p = 0; p < NP; ++p)
+#pragma unroll
+    for (int q = 0; q < 16; ++q) dacc[p][q] += (double)acc[p][q];
+
+  // per-wave partial slab: part[4 * blockIdx.x + w][m][m]
+  double* o = part + ((int64_t)blockIdx.x * 4 + w) * m * m;
+  {
+    int p = 0;
+#pragma unroll
+    for (int m1 = 0; m1 < MT; ++m1)
+#pragma unroll
+      for (int m2 = m1; m2 < MT; ++m2) {
+#pragma unroll
+        for (int v = 0; v < 16; ++v) {
+          const int i = 32 * m1 + mfma32_row(v, lane), k = 32 * m2 + r;
+          if (i < m && k < m) {
+            o[i * m + k] = dacc[p][v];
+            if (m1 != m2) o[k * m + i] = dacc[p][v];
+          }
+        }
+        ++p;
+      }
+  }
+  // |A|max for the optional fp16 scaling
+  if (absmax) {
+    for (int off = 32; off > 0; off >>= 1) amax = fmaxf(amax, __shfl_xor(amax, off));
+    if (lane == 0) atomicMax(reinterpret_cast<unsigned int*>(absmax), __float_as_uint(amax));
+  }
+}
+
+// ------------------------------------------------------------------------------------------------
+// deterministic slab reductions
+// ------------------------------------------------------------------------------------------------
+__global__ void k_reduce_f32(const float* __restrict__ part, int64_t S, int64_t count, float* __restrict__ out) {
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < count; e += (int64_t)gridDim.x * blockDim.x) {
+    float s = 0.0f;
+    for (int64_t k = 0; k < S; ++k) s += part[k * count + e];
+    out[e] = s;
+  }
+}
+__global__ void k_reduce_f32_f64(const float* __restrict__ part, int64_t S, int64_t count, float* __restrict__ out) {
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < count; e += (int64_t)gridDim.x * blockDim.x) {
+    double s = 0.0;
+    for (int64_t k = 0; k < S; ++k) s += (double)part[k * count + e];
+    out[e] = (float)s;
+  }
+}
+__global__ void k_reduce_f64(const double* __restrict__ part, int64_t S, int64_t count, double* __restrict__ out) {
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < count; e += (int64_t)gridDim.x * blockDim.x) {
+    double s = 0.0;
+    for (int64_t k = 0; k < S; ++k) s += part[k * count + e];
+    out[e] = s;
+  }
+}
+
+// ------------------------------------------------------------------------------------------------
+// launchers
+// ------------------------------------------------------------------------------------------------
+static int g_num_cus = 0;
+int num_cus() {
+  if (g_num_cus == 0) {
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, dev) == hipSuccess) g_num_cus = prop.multiProcessorCount;
+    if (g_num_cus <= 0) g_num_cus = 256;
+  }
+  return g_num_cus;
+}
+
+SketchPlan plan_sketch(int m, int64_t n, int kp, int wgs_hint) {
+  SketchPlan p;
+  const int NW = (kp == 32) ? 8 : 4;   // waves per WG (sketch_waves)
+  const int rows = m < 128 * NW ? m : 128 * NW;
+  int rb = (rows + 127) / 128;
+  int RB = 1;
+  while (RB < rb) RB *= 2;
+  p.RB = RB;
+  p.ysplit = (m + 128 * RB - 1) / (128 * RB);
+  int CG = NW / RB;
+  const int cg_lds = (kp == 32) ? 8 : 4;   // keep the two X chunks within 64 KB of LDS
+  if (CG > cg_lds) CG = cg_lds;
+  p.CG = CG;
+  const int KC = 16 * CG;
+  const int64_t nchunks = (n + KC - 1) / KC;
+  const int waves = RB * CG;
+  // one WG per CU at full width; narrower WGs share a CU
+  int64_t target = wgs_hint > 0 ? wgs_hint : (int64_t)num_cus() * (NW / waves) / p.ysplit;
+  if (target < 1) target = 1;
+  int64_t cpw = (nchunks + target - 1) / target;
+  if (cpw < 1) cpw = 1;
+  p.chunks_per_wg = cpw;
+  p.grid_x = (int)((nchunks + cpw - 1) / cpw);
+  if (p.grid_x < 1) p.grid_x = 1;
+  p.slabs = (int64_t)p.grid_x * CG;
+  p.kp = kp;
+  p.lds_bytes = 2 * kp * (KC + 4) * (int)sizeof(float);
+  return p;
+}
+
+template <int LT, int AR, bool OM>
+static void launch_sketch_t(const SketchPlan& p, const float* A, int64_t lda, int m, int64_t n, const float* X,
+                            int64_t ldx, uint64_t seed, uint32_t stream, int ell, float* part, hipStream_t st) {
+  dim3 grid(p.grid_x, p.ysplit);
+  hipLaunchKernelGGL((k_sketch<LT, AR, OM>), grid, dim3(64 * p.RB * p.CG), p.lds_bytes, st, A, lda, m, n, X, ldx,
+                     seed, stream, ell, p.RB, p.CG, p.chunks_per_wg, part);
+}
+
+template <int LT, bool OM>
+static void launch_sketch_ar(int ar, const SketchPlan& p, const float* A, int64_t lda, int m, int64_t n, const float* X,
+                             int64_t ldx, uint64_t seed, uint32_t stream, int ell, float* part, hipStream_t st) {
+  switch (ar) {
+    case AR_FP32: launch_sketch_t<LT, AR_FP32, OM>(p, A, lda, m, n, X, ldx, seed, stream, ell, part, st); break;
+    case AR_BF16: launch_sketch_t<LT, AR_BF16, OM>(p, A, lda, m, n, X, ldx, seed, stream, ell, part, st); break;
+    case AR_FP16: launch_sketch_t<LT, AR_FP16, OM>(p, A, lda, m, n, X, ldx, seed, stream, ell, part, st); break;
+    default: launch_sketch_t<LT, AR_BF16X3, OM>(p, A, lda, m, n, X, ldx, seed, stream, ell, part, st); break;
+  }
+}
+
+void launch_sketch(int ar, const SketchPlan& p, const float* A, int64_t lda, int m, int64_t n, const float* X,
+                   int64_t ldx, uint64_t seed, uint32_t stream, int ell, float* part, hipStream_t st) {
+  const bool om = (X == nullptr);
+  if (p.kp == 32) {
+    if (om) launch_sketch_ar<1, true>(ar, p, A, lda, m, n, X, ldx, seed, stream, ell, part, st);
+    else launch_sketch_ar<1, false>(ar, p, A, lda, m, n, X, ldx, seed, stream, ell, part, st);
+  } else {
+    if (om) launch_sketch_ar<2, true>(ar, p, A, lda, m, n, X, ldx, seed, stream, ell, part, st);
+    else launch_sketch_ar<2, false>(ar, p, A, lda, m, n, X, ldx, seed, stream, ell, part, st);
+  }
+}
+
+template <int LT, int AR>
+static void launch_proj_t(const float* A, int64_t lda, int m, int64_t n, const float* Qc, int64_t ldq, int mode,
+                          float* out, int64_t ldo, int r_out, hipStream_t st) {
+  const int64_t blocks = (n + 255) / 256;
+  int64_t grid = (int64_t)num_cus() * 2;
+  if (grid > blocks) grid = blocks;
+  if (grid < 1) grid = 1;
+  hipLaunchKernelGGL((k_proj<LT, AR>), dim3((unsigned)grid), dim3(512), 0, st, A, lda, m, n, Qc, ldq, mode, out, ldo,
+                     r_out, blocks);
+}
+
+template <int LT>
+static void launch_proj_ar(int ar, const float* A, int64_t lda, int m, int64_t n, const float* Qc, int64_t ldq,
+                           int mode, float* out, int64_t ldo, int r_out, hipStream_t st) {
+  switch (ar) {
+    case AR_FP32: launch_proj_t<LT, AR_FP32>(A, lda, m, n, Qc, ldq, mode, out, ldo, r_out, st); break;
+    case AR_BF16: launch_proj_t<LT, AR_BF16>(A, lda, m, n, Qc, ldq, mode, out, ldo, r_out, st); break;
+    case AR_FP16: launch_proj_t<LT, AR_FP16>(A, lda, m, n, Qc, ldq, mode, out, ldo, r_out, st); break;
+    default: launch_proj_t<LT, AR_BF16X3>(A, lda, m, n, Qc, ldq, mode, out, ldo, r_out, st); break;
+  }
+}
+
+void launch_proj(int ar, int kp, const float* A, int64_t lda, int m, int64_t n, const float* Qc, int64_t ldq,
+                 int mode, float* out, int64_t ldo, int r_out, hipStream_t st) {
+  if (kp == 32) launch_proj_ar<1>(ar, A, lda, m, n, Qc, ldq, mode, out, ldo, r_out, st);
+  else launch_proj_ar<2>(ar, A, lda, m, n, Qc, ldq, mode, out, ldo, r_out, st);
+}
+
+int64_t gram_small_grid(int64_t n) {
+  int64_t g = (int64_t)num_cus() * 4;
+  const int64_t minc = 4096;   // at least this many columns per WG
+  if (g * minc > n) g = (n + minc - 1) / minc;
+  if (g < 1) g = 1;
+  return g;
+}
+
+void launch_gram_small(int ar, const float* A, int64_t lda, int m, int64_t n, double* part, int64_t grid,
+                       float* absmax, hipStream_t st) {
+  int64_t cpw = (n + grid - 1) / grid;
+  cpw = ((cpw + 63) / 64) * 64;
+  if (m <= 32) {
+    switch (ar) {
+      case AR_FP32: hipLaunchKernelGGL((k_gram_small<1, AR_FP32>), dim3(grid), dim3(256), 0, st, A, lda, m, n, cpw, part, absmax); break;
+      case AR_BF16: hipLaunchKernelGGL((k_gram_small<1, AR_BF16>), dim3(grid), dim3(256), 0, st, A, lda, m, n, cpw, part, absmax); break;
+      default: hipLaunchKernelGGL((k_gram_small<1, AR_BF16X3>), dim3(grid), dim3(256), 0, st, A, lda, m, n, cpw, part, absmax); break;
+    }
+  } else {
+    switch (ar) {
+      case AR_FP32: hipLaunchKernelGGL((k_gram_small<2, AR_FP32>), dim3(grid), dim3(256), 0, st, A, lda, m, n, cpw, part, absmax); break;
+      case AR_BF16: hipLaunchKernelGGL((k_gram_small<2, AR_BF16>), dim3(grid), dim3(256), 0, st, A, lda, m, n, cpw, part, absmax); break;
+      default: hipLaunchKernelGGL((k_gram_small<2, AR_BF16X3>), dim3(grid), dim3(256), 0, st, A, lda, m, n, cpw, part, absmax); break;
+    }
+  }
+}
+
+void launch_reduce_f32(const float* part, int64_t S, int64_t count, float* out, hipStream_t st) {
+  int64_t grid = (count + 255) / 256;
+  if (grid > 4096) grid = 4096;
+  if (grid < 1) grid = 1;
+  hipLaunchKernelGGL(k_reduce_f32_f64, dim3((unsigned)grid), dim3(256), 0, st, part, S, count, out);
+}
+void launch_reduce_f64(const double* part, int64_t S, int64_t count, double* out, hipStream_t st) {
+  int64_t grid = (count + 255) / 256;
+  if (grid > 4096) grid = 4096;
+  if (grid < 1) grid = 1;
+  hipLaunchKernelGGL(k_reduce_f64, dim3((unsigned)grid), dim3(256), 0, st, part, S, count, out);
+}
+
+}  // namespace ttx

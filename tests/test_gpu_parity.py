@@ -1,0 +1,182 @@
+"""Parity of the HIP engine (through the C ABI) with the FP64 CPU oracle on the same seeded inputs
+and the same Philox sketch.
+
+Tolerances (relative):
+* generators and permutations: bit-exact;
+* FP32 arithmetic: singular values 1e-5, cores / remainders 1e-4;
+* BF16 arithmetic (hi/lo split, FP32 accumulation): singular values 1e-4, cores / remainders 1e-3;
+* reconstruction errors: |err_gpu - err_oracle| <= 1e-5 + 2e-2 * err_oracle.
+"""
+import numpy as np
+import pytest
+
+from oracle import philox, ttsvd
+
+pytestmark = pytest.mark.gpu
+
+TOL = {"fp32": dict(sig=1e-5, vec=1e-4), "bf16": dict(sig=1e-4, vec=1e-3)}
+
+
+def host(t):
+    return t.detach().cpu().numpy()
+
+
+def fmat(t, rows, cols):
+    return host(t)[: rows * cols].reshape(rows, cols, order="F").astype(np.float64)
+
+
+def dev(x):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(np.asarray(x, dtype=np.float32).reshape(-1, order="F"))).cuda()
+
+
+def rel(a, b):
+    return float(np.linalg.norm(np.asarray(a) - np.asarray(b)) / max(np.linalg.norm(b), 1e-300))
+
+
+def lowrank(m, n, r, seed, decay=2.0, noise=1e-6):
+    rng = np.random.default_rng(seed)
+    U = np.linalg.qr(rng.standard_normal((m, r)))[0]
+    V = np.linalg.qr(rng.standard_normal((n, r)))[0]
+    s = np.logspace(0, -decay, r)
+    A = (U * s) @ V.T
+    A += noise * np.linalg.norm(A) / np.sqrt(m * n) * rng.standard_normal((m, n))
+    return A.astype(np.float32)
+
+
+# ------------------------------------------------------------------------------------------------
+def test_fill_normal_bit_exact(engine):
+    z = host(engine.fill_normal(10007, 1134, 0x101, 0.5))
+    ref = philox.normal_flat(1134, 0x101, 10007) * np.float32(0.5)
+    assert np.array_equal(z.view(np.uint32), ref.astype(np.float32).view(np.uint32))
+
+
+def test_omega_bit_exact(engine):
+    om = host(engine.omega(3001, 42, 1234, 3)).reshape(3001, 42)
+    ref = philox.omega(1234, 3, 3001, 42)
+    assert np.array_equal(om.view(np.uint32), ref.view(np.uint32))
+
+
+@pytest.mark.parametrize("r0,r1,n", [(4, 16, 1000), (3, 5, 7), (2, 4, 12345), (8, 3, 64)])
+def test_permute_231_bit_exact(engine, r0, r1, n):
+    import torch
+    x = torch.randn(r0 * r1 * n, device="cuda")
+    y = host(engine.permute_231(x, r0, r1, n))
+    ref = ttsvd.permute_231(host(x), r0, r1, n)
+    assert np.array_equal(y.view(np.uint32), ref.astype(np.float32).view(np.uint32))
+
+
+def test_generate_tt_matches_oracle(engine):
+    dims, ranks = [6, 7, 8, 5], [1, 3, 4, 2, 1]
+    A, cores = engine.generate(dims, ranks, std=0.5, noise_rel=1e-3, seed=1134, return_cores=True)
+    ref_cores = ttsvd.make_tt_cores(dims, ranks, std=0.5, seed=1134)
+    for c, rc in zip(cores, ref_cores):
+        assert np.array_equal(host(c), rc.reshape(-1, order="F"))
+    clean = ttsvd.tt_full(ref_cores, dims, ranks).astype(np.float32)
+    ref = ttsvd.add_noise(clean, 1e-3, seed=1134)
+    assert rel(host(A), ref) < 1e-6
+
+
+def test_generate_tr_matches_oracle(engine):
+    dims, ranks = [6, 5, 4, 3], [2, 3, 2, 4, 2]
+    A, cores = engine.generate(dims, ranks, std=0.5, noise_rel=0.0, seed=77, tr=True, return_cores=True)
+    ref_cores = ttsvd.make_tr_cores(dims, ranks, std=0.5, seed=77)
+    for c, rc in zip(cores, ref_cores):
+        assert np.array_equal(host(c), rc.reshape(-1, order="F"))
+    assert rel(host(A), ttsvd.tr_full(ref_cores, dims, ranks)) < 1e-6
+
+
+# ------------------------------------------------------------------------------------------------
+def _check_step(engine, A, r, s, q, prec, expect_path):
+    m, n = A.shape
+    core, nxt, sig, path = engine.step(dev(A), m, n, r, s=s, q=q, seed=1234, stream_id=5, precision=prec)
+    assert path == expect_path
+    G_ref, sig_ref, N_ref, _ = ttsvd.step(A.astype(np.float64), r, s, q, 1234, 5)
+    tol = TOL[prec]
+    k = min(r, len(sig_ref))
+    assert np.allclose(sig[:k], sig_ref[:k], rtol=tol["sig"], atol=tol["sig"] * sig_ref[0]), (sig[:k], sig_ref[:k])
+    G = fmat(core, m, r)
+    assert rel(G @ G.T, G_ref @ G_ref.T) < tol["vec"]
+    assert rel(G, G_ref) < tol["vec"], "core differs beyond the gauge"
+    assert rel(fmat(nxt, r, n), N_ref) < tol["vec"]
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16"])
+def test_step_exact_short(engine, prec):
+    # m <= ell: exact Gram path (ttdec6_singleGPU.cu:510-536)
+    _check_step(engine, lowrank(24, 3000, 12, 11), 8, 20, 2, prec, 0)
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16"])
+def test_step_exact_wide(engine, prec):
+    # n <= ell: orthonormal basis of the columns, then the common tail
+    _check_step(engine, lowrank(500, 20, 20, 12, decay=1.0), 12, 10, 2, prec, 1)
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16"])
+@pytest.mark.parametrize("m,n,r,s,q", [(1024, 4000, 16, 8, 1), (300, 2500, 10, 6, 2), (96, 777, 5, 3, 0),
+                                       (1024, 3000, 32, 10, 2)])
+def test_step_randomized(engine, prec, m, n, r, s, q):
+    _check_step(engine, lowrank(m, n, r, m + n, decay=1.5), r, s, q, prec, 2)
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16"])
+def test_rsvd_matches_oracle(engine, prec):
+    A = lowrank(700, 3000, 24, 3, decay=2.0)
+    U, S, SVT = engine.rsvd(dev(A), 700, 3000, 24, s=8, q=1, seed=1234, precision=prec)
+    U_ref, s_ref, SVT_ref = ttsvd.rsvd(A.astype(np.float64), 24, 8, 1, 1234, 0)
+    tol = TOL[prec]
+    assert np.allclose(host(S)[:24], s_ref[:24], rtol=tol["sig"])
+    Ug = fmat(U, 700, 32)
+    assert rel(Ug[:, :24], U_ref[:, :24]) < tol["vec"]
+    assert rel(fmat(SVT, 32, 3000)[:24], SVT_ref[:24]) < tol["vec"]
+
+
+# ------------------------------------------------------------------------------------------------
+@pytest.mark.parametrize("prec", ["fp32", "bf16"])
+def test_tt_svd_matches_oracle(engine, prec):
+    dims, true = [8, 9, 10, 11, 6], [1, 4, 6, 6, 5, 1]
+    A = engine.generate(dims, true, std=1.0, noise_rel=1e-6, seed=1134)
+    ranks = true
+    cores, sig, rep = engine.tt_svd(A, dims, ranks, s=4, q=1, precision=prec)
+    An = host(A).astype(np.float64)
+    c_ref, s_ref, p_ref, used = ttsvd.tt_svd(An, dims, ranks, s=4, q=1)
+    tol = TOL[prec]
+    assert rep["ranks"] == used
+    assert [pp == 2 for pp in rep["path"]] == [p == "rsvd" for p in p_ref]
+    for k in range(len(dims) - 1):
+        r = ranks[k + 1]
+        assert np.allclose(sig[k][:r], s_ref[k][:r], rtol=tol["sig"], atol=tol["sig"] * s_ref[k][0]), k
+    err = engine.rel_error(cores, dims, used, A)
+    err_ref = ttsvd.rel_error(An, ttsvd.tt_full(c_ref, dims, used))
+    assert abs(err - err_ref) <= 1e-5 + 2e-2 * err_ref, (err, err_ref)
+    for k in range(len(dims) - 1):
+        g, shape = cores[k]
+        assert rel(fmat(g, *shape), c_ref[k]) < 10 * tol["vec"], k
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16"])
+def test_tr_svd_matches_oracle(engine, prec):
+    dims, true = [8, 9, 10, 6], [2, 3, 5, 4, 2]
+    A = engine.generate(dims, true, std=1.0, noise_rel=1e-6, seed=99, tr=True)
+    ranks = [2, 3, 8, 6, 2]
+    cores, sig, rep = engine.tr_svd(A, dims, ranks, s=4, q=1, precision=prec)
+    An = host(A).astype(np.float64)
+    c_ref, s_ref, p_ref, used = ttsvd.tr_svd(An, dims, ranks, s=4, q=1)
+    tol = TOL[prec]
+    for k in range(len(dims) - 1):
+        r = (ranks[0] * ranks[1]) if k == 0 else ranks[k + 1]
+        r = min(r, len(s_ref[k]))
+        assert np.allclose(sig[k][:r], s_ref[k][:r], rtol=tol["sig"], atol=tol["sig"] * s_ref[k][0]), k
+    err = engine.rel_error(cores, dims, ranks, A, tr=True)
+    err_ref = ttsvd.rel_error(An, ttsvd.tr_full(c_ref, dims, ranks))
+    assert abs(err - err_ref) <= 1e-5 + 2e-2 * err_ref, (err, err_ref)
+
+
+def test_tt_rel_error_matches_oracle(engine):
+    dims, ranks = [6, 7, 8, 5], [1, 3, 4, 2, 1]
+    A, cores = engine.generate(dims, ranks, std=1.0, noise_rel=1e-2, seed=5, return_cores=True)
+    err = engine.rel_error(cores, dims, ranks, A)
+    ref_cores = [host(c) for c in cores]
+    ref = ttsvd.rel_error(host(A), ttsvd.tt_full(ref_cores, dims, ranks))
+    assert abs(err - ref) < 1e-6 + 1e-4 * ref

@@ -1,0 +1,248 @@
+"""Pins the FP64 CPU oracle (oracle/) against published known answers, numpy/LAPACK routines and
+closed forms.  CPU only."""
+import numpy as np
+import pytest
+
+from oracle import philox, ttsvd
+
+# ------------------------------------------------------------------------------------------------
+# Philox4x32-10: the Random123 known-answer vectors (Salmon et al., SC'11, kat_vectors)
+# ------------------------------------------------------------------------------------------------
+KAT = [
+    ((0, 0, 0, 0), (0, 0), (0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8)),
+    ((0xFFFFFFFF,) * 4, (0xFFFFFFFF, 0xFFFFFFFF), (0x408F276D, 0x41C83B0E, 0xA20BC7C6, 0x6D5451FD)),
+    ((0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344), (0xA4093822, 0x299F31D0),
+     (0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1)),
+]
+
+
+@pytest.mark.parametrize("ctr,key,expect", KAT)
+def test_philox_known_answers(ctr, key, expect):
+    c = tuple(np.array([x], dtype=np.uint32) for x in ctr)
+    k = tuple(np.array([x], dtype=np.uint32) for x in key)
+    out = philox.philox4x32_10(c, k)
+    assert tuple(int(o[0]) for o in out) == expect
+
+
+def test_log_matches_libm():
+    u = np.concatenate([np.linspace(2.0 ** -24, 1.0, 200001), 2.0 ** -np.arange(1, 25)]).astype(np.float32)
+    err = np.abs(philox.log_f32(u).astype(np.float64) - np.log(u.astype(np.float64)))
+    assert err.max() < 2e-6
+
+
+def test_sincos_matches_libm():
+    xb = (np.arange(0, 1 << 24, 97, dtype=np.uint64) << np.uint64(8)).astype(np.uint32)
+    c, s = philox.sincos_quadrant(xb)
+    ang = 2 * np.pi * (xb >> 8).astype(np.float64) / 2.0 ** 24
+    assert np.abs(c - np.cos(ang)).max() < 2e-6
+    assert np.abs(s - np.sin(ang)).max() < 2e-6
+
+
+def test_normals_moments():
+    z = philox.normal_flat(1134, 7, 1 << 20).astype(np.float64)
+    assert abs(z.mean()) < 5e-3
+    assert abs(z.var() - 1.0) < 5e-3
+    # fourth moment of a Gaussian is 3
+    assert abs(np.mean(z ** 4) - 3.0) < 5e-2
+    # element e comes from normal4(seed, stream, e // 4, 0)[e % 4]
+    z4 = philox.normal4(1134, 7, np.arange(3, dtype=np.uint64), np.zeros(3, dtype=np.uint32)).reshape(-1)
+    assert np.array_equal(z4.astype(np.float32), philox.normal_flat(1134, 7, 12))
+
+
+def test_round_bf16_is_round_to_nearest_even():
+    torch = pytest.importorskip("torch")
+    rng = np.random.default_rng(0)
+    x = np.concatenate([rng.standard_normal(100000).astype(np.float32),
+                        # exact ties: low 16 bits 0x8000, both parities of bit 16
+                        (np.arange(1000, dtype=np.uint32) * 0x10000 + 0x3F808000).view(np.float32)])
+    ref = torch.from_numpy(x).to(torch.bfloat16).to(torch.float32).numpy()
+    assert np.array_equal(philox.round_bf16(x), ref)
+
+
+def test_omega_entries_exact_in_low_precision():
+    om = philox.omega(1234, 3, 4096, 42)
+    assert om.shape == (4096, 42)
+    assert np.array_equal(om.astype(np.float16).astype(np.float32), om)
+    assert np.array_equal(philox.round_bf16(om), om)
+    assert np.all((om == 0) | (np.abs(om) >= 2.0 ** -17))
+    # columns beyond ell are not part of the sketch; the first ell columns do not depend on ell
+    assert np.array_equal(philox.omega(1234, 3, 4096, 40), om[:, :40])
+
+
+# ------------------------------------------------------------------------------------------------
+# dense pieces against LAPACK
+# ------------------------------------------------------------------------------------------------
+def test_gram_eig_step_matches_svd():
+    rng = np.random.default_rng(1)
+    A = rng.standard_normal((12, 300)) * np.logspace(0, -3, 12)[:, None]
+    G, sig, An = ttsvd.gram_eig_step(A, 5)
+    s_ref = np.linalg.svd(A, compute_uv=False)
+    assert np.allclose(sig[: len(s_ref)], s_ref, rtol=1e-9, atol=1e-12)
+    U = np.linalg.svd(A)[0][:, :5]
+    assert np.allclose(G @ G.T, U @ U.T, atol=1e-9)
+    assert np.allclose(An, G.T @ A)
+    # gauge: the largest |entry| of every column is positive
+    idx = np.argmax(np.abs(G), axis=0)
+    assert np.all(G[idx, np.arange(5)] > 0)
+
+
+def test_gram_eig_step_zero_pads_when_rank_exceeds_rows():
+    rng = np.random.default_rng(2)
+    A = rng.standard_normal((6, 50))
+    G, sig, An = ttsvd.gram_eig_step(A, 9)
+    assert G.shape == (6, 9) and An.shape == (9, 50)
+    assert np.allclose(G[:, 6:], 0) and np.allclose(An[6:], 0)
+    assert np.allclose(G @ An, A, atol=1e-12)
+
+
+def test_cholqr2_orthogonal_and_consistent():
+    rng = np.random.default_rng(3)
+    Y = rng.standard_normal((500, 20)) @ np.diag(np.logspace(0, -6, 20))
+    Q, R = ttsvd.cholqr2(Y)
+    assert np.abs(Q.T @ Q - np.eye(20)).max() < 1e-13
+    assert np.allclose(Q @ R, Y, atol=1e-12)
+    assert np.allclose(np.tril(R, -1), 0) and np.all(np.diag(R) > 0)
+    Qh, Rh = ttsvd.qr_pos(Y)
+    assert np.allclose(Q, Qh, atol=1e-8)
+
+
+def test_rsvd_exact_when_rank_fits_sketch():
+    rng = np.random.default_rng(4)
+    A = rng.standard_normal((80, 6)) @ rng.standard_normal((6, 500))
+    U, sig, SVT = ttsvd.rsvd(A, 6, 4, 0, seed=1234)
+    assert np.allclose(U @ SVT, A, atol=1e-10)
+    assert np.allclose(sig[:6], np.linalg.svd(A, compute_uv=False)[:6], rtol=1e-10)
+    assert np.abs(U.T @ U - np.eye(10)).max() < 1e-12
+
+
+def test_rsvd_power_iterations_reduce_error():
+    # a planted flat tail: without power iterations the randomized range misses more of the top-r
+    # subspace than with them
+    rng = np.random.default_rng(5)
+    m, n, r = 200, 3000, 10
+    U = np.linalg.qr(rng.standard_normal((m, m)))[0]
+    V = np.linalg.qr(rng.standard_normal((n, m)))[0]
+    s = np.concatenate([np.logspace(0, -1, r), np.full(m - r, 3e-2)])
+    A = (U * s) @ V.T
+    def err(q):
+        Uq, sq, SVTq = ttsvd.rsvd(A, r, 5, q, seed=99)
+        return np.linalg.norm(A - Uq[:, :r] @ SVTq[:r]) / np.linalg.norm(A)
+    best = np.sqrt(np.sum(s[r:] ** 2)) / np.linalg.norm(s)
+    assert err(2) < err(0)
+    assert err(2) < 1.01 * best
+
+
+def test_delta_rank_hand_computed():
+    lam = np.array([100.0, 10.0, 1.0, 0.1, 0.01])
+    total = lam.sum()
+    assert ttsvd.delta_rank(lam, total, 0.2, 5) == 3      # tail(3) = 0.11 <= 0.2
+    assert ttsvd.delta_rank(lam, total, 0.1, 5) == 4      # tail(3) = 0.11 > 0.1, tail(4) = 0.01
+    assert ttsvd.delta_rank(lam, total, 0.0, 5) == 5
+    assert ttsvd.delta_rank(lam, total, 1e9, 5) == 1
+    assert ttsvd.delta_rank(lam, total, 0.0, 2) == 2      # cap
+    # unknown tail beyond the captured values counts through total2
+    assert ttsvd.delta_rank(lam, total + 5.0, 1.0, 5) == 5
+
+
+# ------------------------------------------------------------------------------------------------
+# permutations: brute force of the reference formulas
+# ------------------------------------------------------------------------------------------------
+def test_permute_231_bruteforce():
+    r0, r1, n = 3, 4, 5
+    T1 = np.arange(r0 * r1 * n, dtype=np.float64)
+    T2 = ttsvd.permute_231(T1, r0, r1, n)
+    ref = np.empty_like(T1)
+    for a in range(r0):
+        for b in range(r1):
+            for j in range(n):
+                # tensorToMode231 (trdec6_singleGPU.cu:49-58)
+                ref[b + r1 * j + r1 * n * a] = T1[a + r0 * b + r0 * r1 * j]
+    assert np.array_equal(T2, ref)
+    assert np.array_equal(ttsvd.permute_312(T2, r1, n, r0), T1)
+
+
+def test_permute_312_bruteforce():
+    r1, n, r0 = 2, 3, 4
+    T1 = np.arange(r0 * r1 * n, dtype=np.float64)
+    T2 = ttsvd.permute_312(T1, r1, n, r0)
+    ref = np.empty_like(T1)
+    for i in range(r0 * r1 * n):
+        tube, rem = divmod(i, r1 * n)
+        row, col = rem % r1, rem // r1
+        # tensorToMode3_host (trdec6_singleGPU.cu:166-173): m = r1, n = n, k = r0
+        ref[r0 * col * r1 + row * r0 + tube] = T1[tube * r1 * n + col * r1 + row]
+    assert np.array_equal(T2, ref)
+
+
+# ------------------------------------------------------------------------------------------------
+# sweeps: exact-rank inputs reconstruct to roundoff; contractions against einsum
+# ------------------------------------------------------------------------------------------------
+def test_tt_full_matches_einsum():
+    dims, ranks = [3, 4, 5], [1, 2, 3, 1]
+    cores = ttsvd.make_tt_cores(dims, ranks)
+    G = [c.astype(np.float64).reshape(ranks[k], dims[k], ranks[k + 1], order="F") for k, c in enumerate(cores)]
+    ref = np.einsum("aib,bjc,ckd->ijk", *G).reshape(-1, order="F")
+    assert np.allclose(ttsvd.tt_full(cores, dims, ranks), ref, atol=1e-12)
+
+
+def test_tr_full_matches_einsum():
+    dims, ranks = [4, 5, 6], [2, 2, 3, 2]
+    cores = ttsvd.make_tr_cores(dims, ranks)
+    G1 = cores[0].astype(np.float64).reshape(4, 2, 2, order="F")    # [i, a, b], column a + r0 b
+    G2 = cores[1].astype(np.float64).reshape(2, 5, 3, order="F")    # [b, j, c]
+    G3 = cores[2].astype(np.float64).reshape(3, 6, 2, order="F")    # [c, k, a]
+    ref = np.einsum("iab,bjc,cka->ijk", G1, G2, G3).reshape(-1, order="F")
+    assert np.allclose(ttsvd.tr_full(cores, dims, ranks), ref, atol=1e-12)
+
+
+@pytest.mark.parametrize("method", ["exact", "rsvd"])
+def test_tt_svd_exact_rank_reconstructs(method):
+    dims, ranks = [6, 7, 8, 5], [1, 3, 4, 2, 1]
+    A = ttsvd.tt_full(ttsvd.make_tt_cores(dims, ranks), dims, ranks)
+    cores, sig, paths, used = ttsvd.tt_svd(A, dims, ranks, s=3, q=1, method=method)
+    assert used == ranks
+    assert ttsvd.rel_error(A, ttsvd.tt_full(cores, dims, used)) < 1e-12
+    for k in range(len(dims) - 1):          # left-orthogonal cores
+        assert np.abs(cores[k].T @ cores[k] - np.eye(ranks[k + 1])).max() < 1e-12
+
+
+def test_tt_svd_truncation_error_bound():
+    rng = np.random.default_rng(6)
+    dims = [5, 6, 7, 4]
+    A = rng.standard_normal(np.prod(dims))
+    ranks = [1, 3, 4, 3, 1]
+    cores, sig, paths, used = ttsvd.tt_svd(A, dims, ranks, method="exact")
+    err = ttsvd.rel_error(A, ttsvd.tt_full(cores, dims, used))
+    # Oseledets 2011, Thm 2.2: ||A - TT||^2 <= sum_k tail_k^2
+    bound = np.sqrt(sum(np.sum(sig[k][ranks[k + 1]:] ** 2) for k in range(3))) / np.linalg.norm(A)
+    assert err <= bound * (1 + 1e-9)
+    # step 1 singular values are those of the first unfolding
+    s1 = np.linalg.svd(A.reshape(dims[0], -1, order="F"), compute_uv=False)
+    assert np.allclose(sig[0][: len(s1)], s1, rtol=1e-10)
+
+
+def test_tt_svd_delta_ranks():
+    dims, ranks = [6, 7, 8, 5], [1, 3, 4, 2, 1]
+    A = ttsvd.tt_full(ttsvd.make_tt_cores(dims, ranks), dims, ranks)
+    cores, sig, paths, used = ttsvd.tt_svd(A, dims, [1, 30, 30, 30, 1], method="exact", rank_mode="delta",
+                                            eps=1e-6)
+    assert used == ranks
+    assert ttsvd.rel_error(A, ttsvd.tt_full(cores, dims, used)) < 1e-6
+
+
+def test_tr_svd_exact_ranks_reconstruct():
+    dims, true = [4, 5, 6], [2, 2, 3, 2]
+    T = ttsvd.tr_full(ttsvd.make_tr_cores(dims, true), dims, true)
+    cores, sig, paths, used = ttsvd.tr_svd(T, dims, [2, 2, 10, 2])
+    assert [c.shape for c in cores] == [(4, 4), (10, 10), (10, 12)]
+    assert ttsvd.rel_error(T, ttsvd.tr_full(cores, dims, used)) < 1e-12
+
+
+def test_noise_and_generator_definitions():
+    dims, ranks = [5, 6, 7], [1, 2, 3, 1]
+    cores = ttsvd.make_tt_cores(dims, ranks, std=0.5)
+    assert np.array_equal(cores[1].reshape(-1, order="F"), philox.normal_flat(ttsvd.DATA_SEED, 0x101, 36) * np.float32(0.5))
+    A = ttsvd.tt_full(cores, dims, ranks).astype(np.float32)
+    An = ttsvd.add_noise(A, 1e-3)
+    rel = np.linalg.norm(An.astype(np.float64) - A) / np.linalg.norm(A)
+    assert 0.8e-3 < rel < 1.2e-3
