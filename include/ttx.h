@@ -43,6 +43,15 @@ typedef enum { TTX_PATH_EXACT_M = 0, TTX_PATH_EXACT_N = 1, TTX_PATH_RSVD = 2 } t
 
 typedef struct ttx_handle_s* ttx_handle;
 
+/* kernel classes of the per-kernel timing in ttx_report */
+#define TTX_NUM_KCLASS 8
+#define TTX_K_SKETCH 0     /* Y = A X streaming pass */
+#define TTX_K_PROJ 1       /* Q^T A streaming pass (power pass, remainder, exact projection) */
+#define TTX_K_GRAM 2       /* exact-path A A^T streaming pass */
+#define TTX_K_PERMUTE 3    /* TR 231 relayout */
+#define TTX_K_TALL 4       /* Gram of Z and A_next = U^T Z^T */
+#define TTX_K_SMALL 5      /* FP64 CholeskyQR rounds, eigensolver, assembly, reductions */
+
 typedef struct {
   int32_t order;                    /* d, 2..TTX_MAX_ORDER */
   int64_t dims[TTX_MAX_ORDER];      /* N_1..N_d */
@@ -72,6 +81,11 @@ typedef struct {
   double step_ms[TTX_MAX_ORDER];      /* with TTX_FLAG_TIMING */
   double total_ms;                    /* with TTX_FLAG_TIMING: whole call on the device */
   int64_t bytes_alg;                  /* algorithmic HBM bytes of the passes that ran */
+  /* with TTX_FLAG_TIMING, per kernel class (TTX_K_*): summed device time of its launches (hipEvent
+   * pairs on the handle's stream), launch count and algorithmic HBM bytes of those launches */
+  double kernel_ms[TTX_NUM_KCLASS];
+  int32_t kernel_launches[TTX_NUM_KCLASS];
+  int64_t kernel_bytes[TTX_NUM_KCLASS];
 } ttx_report;
 
 /* handle ---------------------------------------------------------------------------------------- */

@@ -18,6 +18,8 @@ RANK_FIXED, RANK_DELTA, RANK_LEARNED = 0, 1, 2
 METHOD_AUTO, METHOD_EXACT, METHOD_RSVD = 0, 1, 2
 PATH_EXACT_M, PATH_EXACT_N, PATH_RSVD = 0, 1, 2
 FLAG_FAST_RANGE, FLAG_TIMING = 1, 2
+NUM_KCLASS = 8
+KCLASS = ["sketch", "proj", "gram", "permute", "tall", "small", "unused6", "unused7"]
 
 PRECISIONS = {"fp32": PREC_FP32, "bf16": PREC_BF16, "fp16": PREC_FP16}
 RANK_MODES = {"fixed": RANK_FIXED, "delta": RANK_DELTA, "learned": RANK_LEARNED}
@@ -52,6 +54,9 @@ class Report(ctypes.Structure):
         ("step_ms", ctypes.c_double * MAX_ORDER),
         ("total_ms", ctypes.c_double),
         ("bytes_alg", ctypes.c_int64),
+        ("kernel_ms", ctypes.c_double * NUM_KCLASS),
+        ("kernel_launches", ctypes.c_int32 * NUM_KCLASS),
+        ("kernel_bytes", ctypes.c_int64 * NUM_KCLASS),
     ]
 
 

@@ -213,4 +213,7 @@ def report_dict(rep, d):
         "step_ms": [float(rep.step_ms[k]) for k in range(d - 1)],
         "total_ms": float(rep.total_ms),
         "bytes_alg": int(rep.bytes_alg),
+        "kernels": {L.KCLASS[c]: {"ms": float(rep.kernel_ms[c]), "launches": int(rep.kernel_launches[c]),
+                                  "bytes": int(rep.kernel_bytes[c])}
+                    for c in range(L.NUM_KCLASS) if rep.kernel_launches[c] > 0},
     }

@@ -31,6 +31,39 @@ struct Handle {
   double* h_lam = nullptr;       // pinned
   int dev = 0;
   std::vector<float*> last_cores;  // device buffers kept for the reference-name shims
+  // per-kernel timing (TTX_FLAG_TIMING): event pairs recorded on the stream around launches
+  bool ktime = false;
+  std::vector<hipEvent_t> evpool;
+  size_t evnext = 0;
+  struct KRec { hipEvent_t a, b; int kind; int64_t bytes; };
+  std::vector<KRec> krecs;
+};
+
+static hipEvent_t next_event(Handle* h) {
+  if (h->evnext == h->evpool.size()) {
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    h->evpool.push_back(e);
+  }
+  return h->evpool[h->evnext++];
+}
+
+// Records an event pair around the launches made during its lifetime (TTX_FLAG_TIMING only).
+struct KScope {
+  Handle* h;
+  int kind;
+  int64_t bytes;
+  hipEvent_t a = nullptr;
+  KScope(Handle* h_, int kind_, int64_t bytes_) : h(h_), kind(kind_), bytes(bytes_) {
+    if (h->ktime && (a = next_event(h))) (void)hipEventRecord(a, h->stream);
+  }
+  ~KScope() {
+    if (!a) return;
+    hipEvent_t b = next_event(h);
+    if (!b) return;
+    (void)hipEventRecord(b, h->stream);
+    h->krecs.push_back({a, b, kind, bytes});
+  }
 };
 
 static ttx_status fail(Handle* h, ttx_status s, const std::string& msg) {
@@ -110,6 +143,7 @@ static ttx_status orth(Handle* h, const float* Y, int64_t kpy, int m, int k, int
   double* W1 = P<double>(h->W1);
   double* W2 = P<double>(h->W2);
   int* stat = P<int>(h->status);
+  KScope ks(h, TTX_K_SMALL, 0);
   launch_gram_tall_f32(Y, kpy, m, k, P<double>(h->gpart), gg, st);
   launch_reduce_f64(P<double>(h->gpart), gg, (int64_t)k * k, G, st);
   launch_chol_inv(G, k, m, 1, Ri, stat, st);
@@ -183,13 +217,22 @@ static ttx_status step_tail(Handle* h, const StepArgs& a, const Arith3& ar, int 
   const int64_t n = a.n;
   TTX_TRY(ensure(h, h->Z, (size_t)n * kp * sizeof(float)));
   float* Z = P<float>(h->Z);
-  launch_proj(ar.accurate, kp, a.A, a.m, m, n, P<float>(h->Qc), m, 0, Z, kp, 0, st);
+  {
+    KScope ks(h, TTX_K_PROJ, a.m * n * 4 + n * kp * 4);
+    launch_proj(ar.accurate, kp, a.A, a.m, m, n, P<float>(h->Qc), m, 0, Z, kp, 0, st);
+  }
   o.bytes += (int64_t)(a.m * n * 4);
   const int64_t gg = gram_tall_grid(n);
   TTX_TRY(ensure(h, h->gpart, (size_t)gg * k * k * sizeof(double)));
-  launch_gram_tall_f32(Z, kp, n, k, P<double>(h->gpart), gg, st);
-  launch_reduce_f64(P<double>(h->gpart), gg, (int64_t)k * k, P<double>(h->G), st);
-  launch_syevj(P<double>(h->G), k, P<double>(h->lam), P<double>(h->V), P<int>(h->status), nullptr, st);
+  {
+    KScope ks(h, TTX_K_TALL, n * kp * 4);
+    launch_gram_tall_f32(Z, kp, n, k, P<double>(h->gpart), gg, st);
+  }
+  {
+    KScope ks(h, TTX_K_SMALL, 0);
+    launch_reduce_f64(P<double>(h->gpart), gg, (int64_t)k * k, P<double>(h->G), st);
+    launch_syevj(P<double>(h->G), k, P<double>(h->lam), P<double>(h->V), P<int>(h->status), nullptr, st);
+  }
   TTX_TRY(check_launch(h, "step tail"));
   // rank decision on the host when needed
   int r = a.r;
@@ -203,10 +246,16 @@ static ttx_status step_tail(Handle* h, const StepArgs& a, const Arith3& ar, int 
   TTX_TRY(ensure(h, h->Ur, (size_t)kp * r * sizeof(float)));
   TTX_TRY(ensure(h, h->sig, TTX_MAX_ELL * sizeof(float)));
   // G = Q U_B[:, :r] with the sign gauge; sigma = sqrt(eig(B B^T)) (basicRandSVD_LU :717-763)
-  launch_assemble(P<float>(h->Qc), m, m, k, P<double>(h->lam), P<double>(h->V), r, a.core, P<float>(h->Ur), nullptr,
-                  0, P<float>(h->sig), st);
+  {
+    KScope ks(h, TTX_K_SMALL, 0);
+    launch_assemble(P<float>(h->Qc), m, m, k, P<double>(h->lam), P<double>(h->V), r, a.core, P<float>(h->Ur),
+                    nullptr, 0, P<float>(h->sig), st);
+  }
   // remainder S V^T = U_r^T B = U_r^T Z^T (:734-739)
-  launch_next_from_z(Z, kp, n, k, P<float>(h->Ur), r, a.next, st);
+  {
+    KScope ks(h, TTX_K_TALL, n * kp * 4 + (int64_t)r * n * 4);
+    launch_next_from_z(Z, kp, n, k, P<float>(h->Ur), r, a.next, st);
+  }
   o.bytes += (int64_t)(n * kp * 4 + (int64_t)r * n * 4);
   o.r = r;
   return check_launch(h, "step tail assemble");
@@ -237,9 +286,15 @@ static ttx_status run_step(Handle* h, const StepArgs& a, StepOut& o) {
     k = m;
     const int64_t grid = gram_small_grid(n);
     TTX_TRY(ensure(h, h->gpart, (size_t)grid * 4 * m * m * sizeof(double)));
-    launch_gram_small(ar.accurate, a.A, a.m, m, n, P<double>(h->gpart), grid, nullptr, st);
-    launch_reduce_f64(P<double>(h->gpart), grid * 4, (int64_t)m * m, P<double>(h->G), st);
-    launch_syevj(P<double>(h->G), m, P<double>(h->lam), P<double>(h->V), P<int>(h->status), nullptr, st);
+    {
+      KScope ks(h, TTX_K_GRAM, a.m * n * 4);
+      launch_gram_small(ar.accurate, a.A, a.m, m, n, P<double>(h->gpart), grid, nullptr, st);
+    }
+    {
+      KScope ks(h, TTX_K_SMALL, 0);
+      launch_reduce_f64(P<double>(h->gpart), grid * 4, (int64_t)m * m, P<double>(h->G), st);
+      launch_syevj(P<double>(h->G), m, P<double>(h->lam), P<double>(h->V), P<int>(h->status), nullptr, st);
+    }
     TTX_TRY(check_launch(h, "exact gram"));
     o.bytes += a.m * n * 4;
     TTX_HIP(h, hipMemcpyAsync(h->h_lam, h->lam.p, m * sizeof(double), hipMemcpyDeviceToHost, st));
@@ -255,9 +310,15 @@ static ttx_status run_step(Handle* h, const StepArgs& a, StepOut& o) {
     const int kpn = kp_for(r);
     TTX_TRY(ensure(h, h->Qn, (size_t)kpn * m * sizeof(float)));
     TTX_TRY(ensure(h, h->sig, TTX_MAX_ELL * sizeof(float)));
-    launch_assemble(nullptr, 0, m, m, P<double>(h->lam), P<double>(h->V), r, a.core, nullptr, P<float>(h->Qn), kpn,
-                    P<float>(h->sig), st);
-    launch_proj(ar.accurate, kpn, a.A, a.m, m, n, P<float>(h->Qn), m, 1, a.next, 0, r, st);
+    {
+      KScope ks(h, TTX_K_SMALL, 0);
+      launch_assemble(nullptr, 0, m, m, P<double>(h->lam), P<double>(h->V), r, a.core, nullptr, P<float>(h->Qn),
+                      kpn, P<float>(h->sig), st);
+    }
+    {
+      KScope ks(h, TTX_K_PROJ, a.m * n * 4 + (int64_t)r * n * 4);
+      launch_proj(ar.accurate, kpn, a.A, a.m, m, n, P<float>(h->Qn), m, 1, a.next, 0, r, st);
+    }
     o.bytes += a.m * n * 4 + (int64_t)r * n * 4;
     o.r = r;
     o.kept2 = kept;
@@ -270,7 +331,10 @@ static ttx_status run_step(Handle* h, const StepArgs& a, StepOut& o) {
     const int kp = kp_for(k);
     TTX_TRY(ensure(h, h->Y, (size_t)m * kp * sizeof(float)));
     TTX_TRY(ensure(h, h->Qc, (size_t)kp * m * sizeof(float)));
-    launch_copy_cols(a.A, a.m, m, k, P<float>(h->Y), kp, st);
+    {
+      KScope ks(h, TTX_K_SMALL, 0);
+      launch_copy_cols(a.A, a.m, m, k, P<float>(h->Y), kp, st);
+    }
     TTX_TRY(orth(h, P<float>(h->Y), kp, m, k, kp, P<float>(h->Qc)));
     TTX_TRY(step_tail(h, a, ar, k, kp, o));
   } else {
@@ -288,16 +352,31 @@ static ttx_status run_step(Handle* h, const StepArgs& a, StepOut& o) {
     TTX_TRY(ensure(h, h->Z, (size_t)n * kp * sizeof(float)));
     float* Y = P<float>(h->Y);
     // Y = A Omega (basicRandSVD_LU :585-620; Omega from the project's Philox stream)
-    launch_sketch(ar.range, sp, a.A, a.m, m, n, nullptr, 0, a.seed, a.stream_id, ell, P<float>(h->part), st);
-    launch_reduce_f32(P<float>(h->part), sp.slabs, (int64_t)m * kp, Y, st);
+    {
+      KScope ks(h, TTX_K_SKETCH, a.m * n * 4);
+      launch_sketch(ar.range, sp, a.A, a.m, m, n, nullptr, 0, a.seed, a.stream_id, ell, P<float>(h->part), st);
+    }
+    {
+      KScope ks(h, TTX_K_SMALL, 0);
+      launch_reduce_f32(P<float>(h->part), sp.slabs, (int64_t)m * kp, Y, st);
+    }
     TTX_TRY(check_launch(h, "sketch"));
     o.bytes += a.m * n * 4;
     TTX_TRY(orth(h, Y, kp, m, k, kp, P<float>(h->Qc)));
     // power iterations (:628-683): Z = A^T Q, Y = A Z, Q = orth(Y)
     for (int it = 0; it < a.q; ++it) {
-      launch_proj(ar.range, kp, a.A, a.m, m, n, P<float>(h->Qc), m, 0, P<float>(h->Z), kp, 0, st);
-      launch_sketch(ar.range, sp, a.A, a.m, m, n, P<float>(h->Z), kp, 0, 0, ell, P<float>(h->part), st);
-      launch_reduce_f32(P<float>(h->part), sp.slabs, (int64_t)m * kp, Y, st);
+      {
+        KScope ks(h, TTX_K_PROJ, a.m * n * 4 + n * kp * 4);
+        launch_proj(ar.range, kp, a.A, a.m, m, n, P<float>(h->Qc), m, 0, P<float>(h->Z), kp, 0, st);
+      }
+      {
+        KScope ks(h, TTX_K_SKETCH, a.m * n * 4 + n * kp * 4);
+        launch_sketch(ar.range, sp, a.A, a.m, m, n, P<float>(h->Z), kp, 0, 0, ell, P<float>(h->part), st);
+      }
+      {
+        KScope ks(h, TTX_K_SMALL, 0);
+        launch_reduce_f32(P<float>(h->part), sp.slabs, (int64_t)m * kp, Y, st);
+      }
       TTX_TRY(check_launch(h, "power iteration"));
       o.bytes += 2 * a.m * n * 4;
       TTX_TRY(orth(h, Y, kp, m, k, kp, P<float>(h->Qc)));
@@ -370,6 +449,9 @@ static ttx_status sweep(Handle* h, const float* d_A, const ttx_params* p, float*
   for (int k = 0; k < d; ++k) total *= p->dims[k];
   hipEvent_t ev[TTX_MAX_ORDER + 1];
   const bool timing = (p->flags & TTX_FLAG_TIMING) != 0;
+  h->ktime = timing;
+  h->evnext = 0;
+  h->krecs.clear();
   if (timing)
     for (int k = 0; k < d; ++k) TTX_HIP(h, hipEventCreate(&ev[k]));
   double norm2 = -1.0;
@@ -432,7 +514,10 @@ static ttx_status sweep(Handle* h, const float* d_A, const ttx_params* p, float*
       if (o.r != r0 * r1) return fail(h, TTX_ERR_BAD_ARG, "TR first step rank mismatch");
       const int pbuf = (wbuf == 0) ? 1 : 0;
       TTX_TRY(ensure(h, h->next[pbuf], (size_t)o.r * n * sizeof(float)));
-      launch_permute_231(next, P<float>(h->next[pbuf]), r0, r1, n, st);
+      {
+        KScope ks(h, TTX_K_PERMUTE, 2 * o.r * n * 4);
+        launch_permute_231(next, P<float>(h->next[pbuf]), r0, r1, n, st);
+      }
       TTX_TRY(check_launch(h, "permute_231"));
       R.bytes_alg += 2 * o.r * n * 4;
       cur = P<float>(h->next[pbuf]);
@@ -458,7 +543,16 @@ static ttx_status sweep(Handle* h, const float* d_A, const ttx_params* p, float*
     TTX_HIP(h, hipEventElapsedTime(&ms, ev[0], ev[d - 1]));
     R.total_ms = ms;
     for (int k = 0; k < d; ++k) (void)hipEventDestroy(ev[k]);
+    for (const auto& kr : h->krecs) {
+      float kms = 0.0f;
+      if (hipEventElapsedTime(&kms, kr.a, kr.b) != hipSuccess) continue;
+      R.kernel_ms[kr.kind] += kms;
+      R.kernel_launches[kr.kind] += 1;
+      R.kernel_bytes[kr.kind] += kr.bytes;
+    }
   }
+  h->ktime = false;
+  h->krecs.clear();
   if (rep) *rep = R;
   return TTX_OK;
 }
@@ -668,6 +762,7 @@ ttx_status ttx_destroy(ttx_handle hh) {
   for (auto* b : bufs)
     if (b->p) (void)hipFree(b->p);
   for (float* c : h->last_cores) (void)hipFree(c);
+  for (hipEvent_t e : h->evpool) (void)hipEventDestroy(e);
   if (h->h_status) (void)hipHostFree(h->h_status);
   if (h->h_sigma) (void)hipHostFree(h->h_sigma);
   if (h->h_lam) (void)hipHostFree(h->h_lam);
