@@ -1,79 +1,84 @@
 // FP64 small dense kernels of one decomposition step (k = sketch width <= 64):
-//   Gram of tall panels, shifted Cholesky + explicit triangular inverse (CholeskyQR rounds),
-//   cyclic Jacobi eigensolver for the projected Gram, core assembly with the sign gauge, and the
-//   remainder A_next = U_r^T Z^T.
+//   * CholeskyQR rounds on an m x k panel: Gram partials per 64-row block, shifted Cholesky with an
+//     explicit inverse of the triangular factor, and the panel update W Rinv fused with the Gram of
+//     its result (for the next round);
+//   * cyclic Jacobi eigensolver for the projected Gram;
+//   * core assembly G = Q V[:, :r] with the sign gauge, and the remainder A_next = U_r^T Z^T.
 // These replace the reference's cuSOLVER calls: basicQR (geqrf + orgqr,
 // ttdecomposition/third_order_GPUs/opt/ttdec_gpu_float_rsvd_lu_half.cu:85-159), basicEig
-// (syevd, ttdecomposition/high_order_GPUs/ttdec6_singleGPU.cu:334-385) and the descending
-// column selection matrixInvertColumn (:274-290).
+// (syevd, ttdecomposition/high_order_GPUs/ttdec6_singleGPU.cu:334-385) and the descending column
+// selection matrixInvertColumn (:274-290).
 #include "ttx_device.h"
 #include "ttx_kernels.h"
 
 namespace ttx {
 
 constexpr int KMAX = 64;
+constexpr int RB64 = 64;   // rows per WG of the panel kernels
 
 // ------------------------------------------------------------------------------------------------
-// Gram of a tall panel: part[blockIdx.x][a][b] = sum over the WG's rows of X[i][a] X[i][b]
+// Gram of a panel block: part[blockIdx.x][a][b] = sum over rows i of the block of X[i][a] X[i][b]
+// 4 x 4 register blocks over the upper triangle; rows staged through LDS in chunks of 32.
 // ------------------------------------------------------------------------------------------------
 template <typename T>
-__global__ __launch_bounds__(256) void k_gram_tall(const T* __restrict__ X, int64_t ldx, int64_t rows, int k,
-                                                   int64_t rows_per_wg, double* __restrict__ part) {
-  __shared__ double xs[32][KMAX + 1];
+__device__ __forceinline__ void block_gram(const double (*xs)[KMAX + 1], int nrows, int k, double* __restrict__ out) {
   const int kb = (k + 3) / 4;
   const int nb = kb * (kb + 1) / 2;
-  const int tid = threadIdx.x;
-  // up to 2 blocks of 4x4 per thread (nb <= 136 for k <= 64)
-  int ta[2], tb[2];
-  int nmine = 0;
-  for (int bi = tid, q = 0; bi < nb && q < 2; bi += 256, ++q) {
-    // bi -> (a, b) with a <= b, row-major over the upper triangle
+  for (int bi = threadIdx.x; bi < nb; bi += blockDim.x) {
     int a = 0, rem = bi;
     while (rem >= kb - a) { rem -= kb - a; ++a; }
-    ta[q] = a;
-    tb[q] = a + rem;
-    nmine = q + 1;
-  }
-  double acc[2][16];
+    const int ca = 4 * a, cb = 4 * (a + rem);
+    double acc[16];
 #pragma unroll
-  for (int q = 0; q < 2; ++q)
+    for (int e = 0; e < 16; ++e) acc[e] = 0.0;
+    for (int rr = 0; rr < nrows; ++rr) {
+      double xa[4], xb[4];
 #pragma unroll
-    for (int e = 0; e < 16; ++e) acc[q][e] = 0.0;
-
-  const int64_t r0 = (int64_t)blockIdx.x * rows_per_wg;
-  const int64_t r1 = (r0 + rows_per_wg < rows) ? r0 + rows_per_wg : rows;
-  for (int64_t rc = r0; rc < r1; rc += 32) {
-    for (int e = tid; e < 32 * kb * 4; e += 256) {
-      const int rr = e / (kb * 4), c = e % (kb * 4);
-      const int64_t i = rc + rr;
-      xs[rr][c] = (i < r1 && c < k) ? (double)X[i * ldx + c] : 0.0;
+      for (int u = 0; u < 4; ++u) { xa[u] = xs[rr][ca + u]; xb[u] = xs[rr][cb + u]; }
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) acc[4 * u + v] += xa[u] * xb[v];
     }
-    __syncthreads();
-    for (int q = 0; q < nmine; ++q) {
-      const int ca = 4 * ta[q], cb = 4 * tb[q];
-      for (int rr = 0; rr < 32; ++rr) {
-        double xa[4], xb[4];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) { xa[u] = xs[rr][ca + u]; xb[u] = xs[rr][cb + u]; }
-#pragma unroll
-        for (int u = 0; u < 4; ++u)
-#pragma unroll
-          for (int v = 0; v < 4; ++v) acc[q][4 * u + v] += xa[u] * xb[v];
-      }
-    }
-    __syncthreads();
-  }
-  double* o = part + (int64_t)blockIdx.x * k * k;
-  for (int q = 0; q < nmine; ++q) {
     for (int u = 0; u < 4; ++u)
+#pragma unroll
       for (int v = 0; v < 4; ++v) {
-        const int a = 4 * ta[q] + u, b = 4 * tb[q] + v;
-        if (a < k && b < k) {
-          o[a * k + b] = acc[q][4 * u + v];
-          o[b * k + a] = acc[q][4 * u + v];
+        const int i = ca + u, j = cb + v;
+        if (i < k && j < k) {
+          out[i * k + j] = acc[4 * u + v];
+          out[j * k + i] = acc[4 * u + v];
         }
       }
   }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_gram_rows(const T* __restrict__ X, int64_t ldx, int64_t rows, int k,
+                                                   int64_t rows_per_wg, double* __restrict__ part) {
+  __shared__ double xs[RB64][KMAX + 1];
+  const int64_t r0 = (int64_t)blockIdx.x * rows_per_wg;
+  const int64_t r1 = (r0 + rows_per_wg < rows) ? r0 + rows_per_wg : rows;
+  const int kb4 = ((k + 3) / 4) * 4;
+  double* out = part + (int64_t)blockIdx.x * k * k;
+  // accumulate chunk Grams through a small LDS buffer
+  __shared__ double gacc[KMAX * KMAX];
+  for (int e = threadIdx.x; e < k * k; e += 256) gacc[e] = 0.0;
+  __shared__ double gchunk[KMAX * KMAX];
+  for (int64_t rc = r0; rc < r1; rc += RB64) {
+    const int nr = (int)((r1 - rc) < RB64 ? (r1 - rc) : RB64);
+    __syncthreads();
+    for (int e = threadIdx.x; e < RB64 * kb4; e += 256) {
+      const int rr = e / kb4, c = e % kb4;
+      xs[rr][c] = (rr < nr && c < k) ? (double)X[(rc + rr) * ldx + c] : 0.0;
+    }
+    __syncthreads();
+    block_gram<T>(xs, nr, k, gchunk);
+    __syncthreads();
+    for (int e = threadIdx.x; e < k * k; e += 256) gacc[e] += gchunk[e];
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < k * k; e += 256) out[e] = gacc[e];
 }
 
 int64_t gram_tall_grid(int64_t rows) {
@@ -87,31 +92,71 @@ int64_t gram_tall_grid(int64_t rows) {
 void launch_gram_tall_f32(const float* X, int64_t ldx, int64_t rows, int k, double* part, int64_t grid,
                           hipStream_t st) {
   int64_t rpw = (rows + grid - 1) / grid;
-  rpw = ((rpw + 31) / 32) * 32;
-  hipLaunchKernelGGL(k_gram_tall<float>, dim3((unsigned)grid), dim3(256), 0, st, X, ldx, rows, k, rpw, part);
+  rpw = ((rpw + RB64 - 1) / RB64) * RB64;
+  hipLaunchKernelGGL(k_gram_rows<float>, dim3((unsigned)grid), dim3(256), 0, st, X, ldx, rows, k, rpw, part);
 }
 void launch_gram_tall_f64(const double* X, int64_t ldx, int64_t rows, int k, double* part, int64_t grid,
                           hipStream_t st) {
   int64_t rpw = (rows + grid - 1) / grid;
-  rpw = ((rpw + 31) / 32) * 32;
-  hipLaunchKernelGGL(k_gram_tall<double>, dim3((unsigned)grid), dim3(256), 0, st, X, ldx, rows, k, rpw, part);
+  rpw = ((rpw + RB64 - 1) / RB64) * RB64;
+  hipLaunchKernelGGL(k_gram_rows<double>, dim3((unsigned)grid), dim3(256), 0, st, X, ldx, rows, k, rpw, part);
+}
+
+
+// dst[i][j] = sum_q part[q][i * k + j]: slabs are read whole (coalesced), two at a time, so each
+// thread keeps several independent loads in flight
+__device__ __forceinline__ void sum_slabs(const double* __restrict__ part, int64_t S, int k,
+                                          double (*dst)[KMAX + 1]) {
+  const int kk = k * k;
+  double acc[16];
+#pragma unroll
+  for (int t = 0; t < 16; ++t) acc[t] = 0.0;
+  int64_t q = 0;
+  for (; q + 1 < S; q += 2) {
+    const double* p0 = part + q * kk;
+    const double* p1 = p0 + kk;
+    double x0[16], x1[16];
+#pragma unroll
+    for (int t = 0; t < 16; ++t) {
+      const int e = threadIdx.x + t * (int)blockDim.x;
+      x0[t] = (e < kk) ? p0[e] : 0.0;
+      x1[t] = (e < kk) ? p1[e] : 0.0;
+    }
+#pragma unroll
+    for (int t = 0; t < 16; ++t) acc[t] += x0[t] + x1[t];
+  }
+  if (q < S) {
+    const double* p0 = part + q * kk;
+#pragma unroll
+    for (int t = 0; t < 16; ++t) {
+      const int e = threadIdx.x + t * (int)blockDim.x;
+      if (e < kk) acc[t] += p0[e];
+    }
+  }
+#pragma unroll
+  for (int t = 0; t < 16; ++t) {
+    const int e = threadIdx.x + t * (int)blockDim.x;
+    if (e < kk) dst[e / k][e - (e / k) * k] = acc[t];
+  }
 }
 
 // ------------------------------------------------------------------------------------------------
-// Cholesky (+ optional shift) and explicit inverse of the upper factor
+// sum S Gram partials, (shifted) Cholesky G + s I = R^T R, explicit Rinv = R^-1 (upper, row-major)
 // ------------------------------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_chol_inv(const double* __restrict__ G, int k, int64_t m, int shift_mode,
-                                                  double* __restrict__ Rinv, int* __restrict__ status) {
+__global__ __launch_bounds__(256) void k_chol_rinv(const double* __restrict__ part, int64_t S, int k, int64_t m,
+                                                   int shift_mode, double* __restrict__ Rinv,
+                                                   int* __restrict__ status) {
   __shared__ double a[KMAX][KMAX + 1];
   __shared__ double R[KMAX][KMAX + 1];
-  __shared__ double shift_s;
-  __shared__ double maxdiag;
-  __shared__ int nzero;
+  __shared__ double X[KMAX][KMAX + 1];
+  __shared__ double rinvd[KMAX];
+  __shared__ double shift_s, tiny_s;
   const int tid = threadIdx.x;
+  sum_slabs(part, S, k, a);
   for (int e = tid; e < k * k; e += 256) {
-    const int i = e / k, j = e % k;
-    a[i][j] = G[e];
+    const int i = e / k, j = e - (e / k) * k;
     R[i][j] = 0.0;
+    X[i][j] = 0.0;
   }
   __syncthreads();
   if (tid == 0) {
@@ -120,194 +165,252 @@ __global__ __launch_bounds__(256) void k_chol_inv(const double* __restrict__ G, 
     // shifted CholeskyQR (Fukaya, Kannan, Nakatsukasa, Yamamoto, Yanagisawa 2020):
     // s = 11 (m k + k (k+1)) u ||X||_2^2 with ||X||_2^2 <= trace(X^T X)
     shift_s = shift_mode ? 11.0 * ((double)m * k + (double)k * (k + 1)) * 1.1102230246251565e-16 * tr : 0.0;
-    maxdiag = md + shift_s;
-    nzero = 0;
+    tiny_s = (md + shift_s) * 1e-28;
   }
   __syncthreads();
-  if (shift_mode)
-    for (int i = tid; i < k; i += 256) a[i][i] += shift_s;
-  __syncthreads();
-  const double tiny = maxdiag * 1e-28;
-  for (int j = 0; j < k; ++j) {
-    const double d = a[j][j];
-    const bool ok = d > tiny;
-    const double rjj = ok ? sqrt(d) : 0.0;
-    for (int c = j + tid; c < k; c += 256) R[j][c] = ok ? (c == j ? rjj : a[j][c] / rjj) : 0.0;
-    if (!ok && tid == 0) nzero += 1;
-    __syncthreads();
-    // trailing update a[p][q] -= R[j][p] R[j][q], j < p <= q
-    const int t = k - j - 1;
-    for (int e = tid; e < t * t; e += 256) {
-      const int p = j + 1 + e / t, q = j + 1 + e % t;
-      if (q >= p) a[p][q] -= R[j][p] * R[j][q];
+  const double sh = shift_s, tiny = tiny_s;
+  // up-looking Cholesky: row i of R from rows 0..i-1; every thread recomputes the pivot so one
+  // barrier per row suffices
+  int nzero = 0;
+  for (int i = 0; i < k; ++i) {
+    // dot products with 4 independent partial sums so several LDS loads are in flight
+    double d0 = a[i][i] + sh, d1 = 0.0, d2 = 0.0, d3 = 0.0;
+    int l = 0;
+    for (; l + 4 <= i; l += 4) {
+      d0 -= R[l][i] * R[l][i]; d1 -= R[l + 1][i] * R[l + 1][i];
+      d2 -= R[l + 2][i] * R[l + 2][i]; d3 -= R[l + 3][i] * R[l + 3][i];
     }
-    __syncthreads();
-  }
-  // Rinv: R Rinv = I, one column per thread (column c of `a` holds x); zero pivots give zero rows /
-  // columns
-  for (int c = tid; c < k; c += 256) {
-    for (int i = 0; i < k; ++i) a[i][c] = 0.0;
-    if (R[c][c] != 0.0) {
-      a[c][c] = 1.0 / R[c][c];
-      for (int i = c - 1; i >= 0; --i) {
-        if (R[i][i] == 0.0) continue;
-        double s = 0.0;
-        for (int l = i + 1; l <= c; ++l) s += R[i][l] * a[l][c];
-        a[i][c] = -s / R[i][i];
+    for (; l < i; ++l) d0 -= R[l][i] * R[l][i];
+    const double d = (d0 + d1) + (d2 + d3);
+    const bool ok = d > tiny;
+    if (!ok) ++nzero;
+    const double rii = ok ? sqrt(d) : 0.0;
+    const double inv = ok ? 1.0 / rii : 0.0;
+    for (int j = i + tid; j < k; j += 256) {
+      if (j == i) {
+        R[i][i] = rii;
+      } else {
+        double s0 = a[i][j], s1 = 0.0, s2 = 0.0, s3 = 0.0;
+        int l2 = 0;
+        for (; l2 + 4 <= i; l2 += 4) {
+          s0 -= R[l2][i] * R[l2][j]; s1 -= R[l2 + 1][i] * R[l2 + 1][j];
+          s2 -= R[l2 + 2][i] * R[l2 + 2][j]; s3 -= R[l2 + 3][i] * R[l2 + 3][j];
+        }
+        for (; l2 < i; ++l2) s0 -= R[l2][i] * R[l2][j];
+        R[i][j] = ((s0 + s1) + (s2 + s3)) * inv;
       }
     }
+    if (tid == 0) rinvd[i] = inv;
+    __syncthreads();
   }
-  __syncthreads();
-  for (int e = tid; e < k * k; e += 256) Rinv[e] = a[e / k][e % k];
+  // Rinv by rows from the bottom: X[i][c] = (delta_ic - sum_{i<l<=c} R[i][l] X[l][c]) / R[i][i];
+  // all columns of one row in parallel
+  for (int i = k - 1; i >= 0; --i) {
+    const double ri = rinvd[i];
+    for (int c = i + tid; c < k; c += 256) {
+      double s0 = (c == i) ? 1.0 : 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+      int l = i + 1;
+      for (; l + 3 <= c; l += 4) {
+        s0 -= R[i][l] * X[l][c]; s1 -= R[i][l + 1] * X[l + 1][c];
+        s2 -= R[i][l + 2] * X[l + 2][c]; s3 -= R[i][l + 3] * X[l + 3][c];
+      }
+      for (; l <= c; ++l) s0 -= R[i][l] * X[l][c];
+      X[i][c] = ((s0 + s1) + (s2 + s3)) * ri;
+    }
+    __syncthreads();
+  }
+  for (int e = tid; e < k * k; e += 256) Rinv[e] = X[e / k][e - (e / k) * k];
   if (tid == 0 && status) status[0] += nzero;
 }
 
-void launch_chol_inv(const double* G, int k, int64_t m, int shift_mode, double* Rinv, int* status, hipStream_t st) {
-  hipLaunchKernelGGL(k_chol_inv, dim3(1), dim3(256), 0, st, G, k, m, shift_mode, Rinv, status);
+void launch_chol_inv(const double* part, int64_t S, int k, int64_t m, int shift_mode, double* Rinv, int* status,
+                     hipStream_t st) {
+  hipLaunchKernelGGL(k_chol_rinv, dim3(1), dim3(256), 0, st, part, S, k, m, shift_mode, Rinv, status);
 }
 
 // ------------------------------------------------------------------------------------------------
-// W_out = W_in Rinv (rows x k), Rinv upper triangular (row-major k x k)
+// W_out = W_in Rinv for a 64-row block, plus the Gram partial of W_out (next CholeskyQR round)
+// OUT 0: double row-major W_out (stride k); OUT 1: float column-major Qc[c * ldq + i], c < kp
 // ------------------------------------------------------------------------------------------------
-template <typename TI, int OUT>   // OUT 0: double row-major (stride k); OUT 1: float column-major Qc
-__global__ __launch_bounds__(256) void k_apply_rinv(const TI* __restrict__ Win, int64_t ldi, int64_t rows, int k,
-                                                    int kp, const double* __restrict__ Rinv, double* __restrict__ Wd,
-                                                    float* __restrict__ Qc, int64_t ldq) {
+template <typename TI, int OUT>
+__global__ __launch_bounds__(256) void k_apply_gram(const TI* __restrict__ Win, int64_t ldi, int64_t rows, int k,
+                                                    int kp, const double* __restrict__ Rinv,
+                                                    double* __restrict__ Wd, float* __restrict__ Qc, int64_t ldq,
+                                                    double* __restrict__ part, const double* __restrict__ gpart,
+                                                    int64_t S) {
   __shared__ double ri[KMAX][KMAX + 1];
-  __shared__ double ws[64][KMAX + 1];
+  __shared__ double ws[RB64][KMAX + 1];
+  __shared__ double os[RB64][KMAX + 1];
   const int tid = threadIdx.x;
-  for (int e = tid; e < k * k; e += 256) ri[e / k][e % k] = Rinv[e];
-  const int64_t i0 = (int64_t)blockIdx.x * 64;
-  for (int e = tid; e < 64 * k; e += 256) {
-    const int rr = e / k, c = e % k;
-    const int64_t i = i0 + rr;
-    ws[rr][c] = (i < rows) ? (double)Win[i * ldi + c] : 0.0;
+  if (gpart) {
+    // Newton-Schulz step towards the orthogonal polar factor: M = (3 I - W^T W) / 2 (full matrix)
+    sum_slabs(gpart, S, k, ri);
+    __syncthreads();
+    for (int e = tid; e < k * k; e += 256) {
+      const int i = e / k, j = e - (e / k) * k;
+      ri[i][j] = ((i == j) ? 1.5 : 0.0) - 0.5 * ri[i][j];
+    }
+  } else {
+    for (int e = tid; e < k * k; e += 256) ri[e / k][e - (e / k) * k] = Rinv[e];
+  }
+  const int64_t i0 = (int64_t)blockIdx.x * RB64;
+  const int nr = (int)((rows - i0) < RB64 ? (rows - i0) : RB64);
+  for (int e = tid; e < RB64 * k; e += 256) {
+    const int rr = e / k, c = e - (e / k) * k;
+    ws[rr][c] = (rr < nr) ? (double)Win[(i0 + rr) * ldi + c] : 0.0;
   }
   __syncthreads();
-  const int KPo = OUT ? kp : k;
-  for (int e = tid; e < 64 * KPo; e += 256) {
-    // OUT 1: consecutive threads take consecutive rows of one column (coalesced column-major store)
-    const int rr = OUT ? (e % 64) : (e / KPo);
-    const int c = OUT ? (e / 64) : (e % KPo);
-    const int64_t i = i0 + rr;
-    if (i >= rows) continue;
+  const int alim = gpart ? k - 1 : -1;   // full product for M, triangular for Rinv
+  for (int e = tid; e < RB64 * k; e += 256) {
+    const int rr = e % RB64, c = e / RB64;
+    const int top = alim >= 0 ? alim : c;
     double s = 0.0;
-    if (c < k)
-      for (int a2 = 0; a2 <= c; ++a2) s += ws[rr][a2] * ri[a2][c];
-    if (OUT) Qc[(int64_t)c * ldq + i] = (float)s;
-    else Wd[i * k + c] = s;
+    for (int a2 = 0; a2 <= top; ++a2) s += ws[rr][a2] * ri[a2][c];
+    os[rr][c] = (rr < nr) ? s : 0.0;
   }
+  __syncthreads();
+  if (OUT == 0) {
+    for (int e = tid; e < nr * k; e += 256) Wd[(i0 + e / k) * k + e % k] = os[e / k][e % k];
+  } else {
+    for (int e = tid; e < RB64 * kp; e += 256) {
+      const int rr = e % RB64, c = e / RB64;
+      if (rr < nr) Qc[(int64_t)c * ldq + i0 + rr] = (c < k) ? (float)os[rr][c] : 0.0f;
+    }
+  }
+  if (part) block_gram<double>(os, nr, k, part + (int64_t)blockIdx.x * k * k);
 }
 
-void launch_apply_rinv_f32(const float* Win, int64_t ldi, int64_t rows, int k, const double* Rinv, double* Wout,
+void launch_apply_gram_f32(const float* Win, int64_t ldi, int64_t rows, int k, const double* Rinv, double* Wout,
+                           double* part, hipStream_t st) {
+  const int64_t grid = (rows + RB64 - 1) / RB64;
+  hipLaunchKernelGGL((k_apply_gram<float, 0>), dim3((unsigned)grid), dim3(256), 0, st, Win, ldi, rows, k, k, Rinv,
+                     Wout, (float*)nullptr, (int64_t)0, part, (const double*)nullptr, (int64_t)0);
+}
+void launch_apply_gram_f64(const double* Win, int64_t rows, int k, const double* Rinv, double* Wout, double* part,
                            hipStream_t st) {
-  const int64_t grid = (rows + 63) / 64;
-  hipLaunchKernelGGL((k_apply_rinv<float, 0>), dim3((unsigned)grid), dim3(256), 0, st, Win, ldi, rows, k, k, Rinv,
-                     Wout, (float*)nullptr, (int64_t)0);
+  const int64_t grid = (rows + RB64 - 1) / RB64;
+  hipLaunchKernelGGL((k_apply_gram<double, 0>), dim3((unsigned)grid), dim3(256), 0, st, Win, (int64_t)k, rows, k, k,
+                     Rinv, Wout, (float*)nullptr, (int64_t)0, part, (const double*)nullptr, (int64_t)0);
 }
-void launch_apply_rinv_f64(const double* Win, int64_t rows, int k, const double* Rinv, double* Wout, hipStream_t st) {
-  const int64_t grid = (rows + 63) / 64;
-  hipLaunchKernelGGL((k_apply_rinv<double, 0>), dim3((unsigned)grid), dim3(256), 0, st, Win, (int64_t)k, rows, k, k,
-                     Rinv, Wout, (float*)nullptr, (int64_t)0);
+void launch_apply_out(const double* Win, int64_t rows, int k, int kp, const double* Rinv, float* Qc, int64_t ldq,
+                      hipStream_t st) {
+  const int64_t grid = (rows + RB64 - 1) / RB64;
+  hipLaunchKernelGGL((k_apply_gram<double, 1>), dim3((unsigned)grid), dim3(256), 0, st, Win, (int64_t)k, rows, k, kp,
+                     Rinv, (double*)nullptr, Qc, ldq, (double*)nullptr, (const double*)nullptr, (int64_t)0);
 }
-void launch_apply_rinv_out(const double* Win, int64_t rows, int k, int kp, const double* Rinv, float* Qc,
-                           int64_t ldq, hipStream_t st) {
-  const int64_t grid = (rows + 63) / 64;
-  hipLaunchKernelGGL((k_apply_rinv<double, 1>), dim3((unsigned)grid), dim3(256), 0, st, Win, (int64_t)k, rows, k, kp,
-                     Rinv, (double*)nullptr, Qc, ldq);
+void launch_apply_ns_out(const double* Win, int64_t rows, int k, int kp, const double* gpart, int64_t S, float* Qc,
+                         int64_t ldq, hipStream_t st) {
+  const int64_t grid = (rows + RB64 - 1) / RB64;
+  hipLaunchKernelGGL((k_apply_gram<double, 1>), dim3((unsigned)grid), dim3(256), 0, st, Win, (int64_t)k, rows, k, kp,
+                     (const double*)nullptr, (double*)nullptr, Qc, ldq, (double*)nullptr, gpart, S);
 }
+int64_t panel_blocks(int64_t rows) { return (rows + RB64 - 1) / RB64; }
 
 // ------------------------------------------------------------------------------------------------
 // cyclic Jacobi eigensolver (round-robin ordering), FP64, k <= 64
+// Each round rotates k2/2 disjoint pairs; A <- J^T A J and V <- V J are applied together on 2 x 2
+// blocks.  The angle is computed in FP32 and (c, s) normalised in FP64, so every applied rotation is
+// orthogonal to FP64 accuracy.  status[1] = sweeps.  keys[0..63] (optional) are zeroed for the
+// core assembly that follows.
 // ------------------------------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_syevj(const double* __restrict__ G, int k, double* __restrict__ lam,
-                                               double* __restrict__ Vout, int* __restrict__ status) {
+__global__ __launch_bounds__(1024) void k_syevj(const double* __restrict__ G, int k, double* __restrict__ lam,
+                                               double* __restrict__ Vout, int* __restrict__ status,
+                                               unsigned long long* __restrict__ keys) {
   __shared__ double a[KMAX][KMAX + 1];
   __shared__ double v[KMAX][KMAX + 1];
   __shared__ double cs[KMAX / 2], sn[KMAX / 2];
   __shared__ int pp[KMAX / 2], qq[KMAX / 2];
-  __shared__ double red[256];
+  __shared__ double red[2][1024];
   __shared__ int done;
   const int tid = threadIdx.x;
-  const int k2 = k + (k & 1);   // an odd size gets one zero row / column (its pairs rotate by identity)
-  for (int e = tid; e < k2 * k2; e += 256) {
+  if (keys && tid < 64) keys[tid] = 0ull;
+  const int k2 = k + (k & 1);   // an odd size gets one zero row / column (its rotations are identities)
+  for (int e = tid; e < k2 * k2; e += 1024) {
     const int i = e / k2, j = e % k2;
     a[i][j] = (i < k && j < k) ? G[i * k + j] : 0.0;
     v[i][j] = (i == j) ? 1.0 : 0.0;
   }
   __syncthreads();
   const int np = k2 / 2;
+  // fixed work assignment of this thread: 2 x 2 blocks (bu, bw) and V entries (vi, vw)
+  int bu[1], bw[1], vi[2], vw[2];
+  int nblk = 0, nvi = 0;
+  for (int e = tid; e < np * np && nblk < 1; e += 1024) { bu[nblk] = e / np; bw[nblk] = e % np; ++nblk; }
+  for (int e = tid; e < k2 * np && nvi < 2; e += 1024) { vi[nvi] = e / np; vw[nvi] = e % np; ++nvi; }
   int sweeps = 0;
-  for (; sweeps < 40; ++sweeps) {
-    // convergence: off-diagonal mass relative to the total
+  for (; sweeps < 30; ++sweeps) {
     double off = 0.0, tot = 0.0;
-    for (int e = tid; e < k2 * k2; e += 256) {
+    for (int e = tid; e < k2 * k2; e += 1024) {
       const int i = e / k2, j = e % k2;
       const double x = a[i][j] * a[i][j];
       tot += x;
       if (i != j) off += x;
     }
-    red[tid] = off;
+    red[0][tid] = off;
+    red[1][tid] = tot;
     __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-      if (tid < s) red[tid] += red[tid + s];
+    for (int s = 512; s > 0; s >>= 1) {
+      if (tid < s) { red[0][tid] += red[0][tid + s]; red[1][tid] += red[1][tid + s]; }
       __syncthreads();
     }
-    const double offs = red[0];
-    __syncthreads();
-    red[tid] = tot;
-    __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-      if (tid < s) red[tid] += red[tid + s];
-      __syncthreads();
-    }
-    if (tid == 0) done = (offs <= 1e-30 * red[0]) || red[0] == 0.0;
+    if (tid == 0) done = (red[0][0] <= 1e-24 * red[1][0]) || red[1][0] == 0.0;
     __syncthreads();
     if (done) break;
     for (int rnd = 0; rnd < k2 - 1; ++rnd) {
       if (tid < np) {
         int x, y;
-        if (tid == 0) { x = rnd % (k2 - 1); y = k2 - 1; }
-        else { x = (rnd + tid) % (k2 - 1); y = (rnd - tid + (k2 - 1)) % (k2 - 1); }
+        if (tid == 0) { x = rnd; y = k2 - 1; }
+        else {
+          x = rnd + tid; if (x >= k2 - 1) x -= k2 - 1;
+          y = rnd - tid; if (y < 0) y += k2 - 1;
+        }
         const int p = x < y ? x : y, q = x < y ? y : x;
         const double apq = a[p][q];
         double c = 1.0, s = 0.0;
         if (apq != 0.0) {
-          const double tau = (a[q][q] - a[p][p]) / (2.0 * apq);
-          const double t = (tau >= 0.0) ? 1.0 / (tau + sqrt(1.0 + tau * tau)) : -1.0 / (-tau + sqrt(1.0 + tau * tau));
-          c = 1.0 / sqrt(1.0 + t * t);
-          s = t * c;
+          // the angle only needs FP32 accuracy (every applied rotation is re-normalised in FP64)
+          const float tau = (float)((a[q][q] - a[p][p]) / (2.0 * apq));
+          float t;
+          if (!(fabsf(tau) <= 1e18f)) t = 0.0f;
+          else t = (tau >= 0.0f ? 1.0f : -1.0f) * __builtin_amdgcn_rcpf(fabsf(tau) + __builtin_amdgcn_sqrtf(1.0f + tau * tau));
+          const float c0 = __builtin_amdgcn_rsqf(1.0f + t * t);
+          const double cd = (double)c0, sd = (double)(t * c0);
+          const double nrm = cd * cd + sd * sd;
+          double rs = (double)__builtin_amdgcn_rsqf((float)nrm);
+          rs = rs * (1.5 - 0.5 * nrm * rs * rs);
+          rs = rs * (1.5 - 0.5 * nrm * rs * rs);
+          c = cd * rs;
+          s = sd * rs;
         }
         pp[tid] = p; qq[tid] = q; cs[tid] = c; sn[tid] = s;
       }
       __syncthreads();
-      // rows: J^T A
-      for (int e = tid; e < np * k2; e += 256) {
-        const int i = e / k2, j = e % k2;
-        const int p = pp[i], q = qq[i];
-        const double c = cs[i], s = sn[i];
-        const double ap = a[p][j], aq = a[q][j];
-        a[p][j] = c * ap - s * aq;
-        a[q][j] = s * ap + c * aq;
+      // A <- J^T A J on 2 x 2 blocks (pair u rows, pair w columns), V <- V J
+      for (int t = 0; t < nblk; ++t) {
+        const int u = bu[t], w = bw[t];
+        const int p1 = pp[u], q1 = qq[u], p2 = pp[w], q2 = qq[w];
+        const double c1 = cs[u], s1 = sn[u], c2 = cs[w], s2 = sn[w];
+        const double app = a[p1][p2], apq = a[p1][q2], aqp = a[q1][p2], aqq = a[q1][q2];
+        const double rpp = c1 * app - s1 * aqp, rpq = c1 * apq - s1 * aqq;
+        const double rqp = s1 * app + c1 * aqp, rqq = s1 * apq + c1 * aqq;
+        a[p1][p2] = c2 * rpp - s2 * rpq;
+        a[p1][q2] = s2 * rpp + c2 * rpq;
+        a[q1][p2] = c2 * rqp - s2 * rqq;
+        a[q1][q2] = s2 * rqp + c2 * rqq;
       }
-      __syncthreads();
-      // columns: (J^T A) J and V J
-      for (int e = tid; e < np * k2; e += 256) {
-        const int i = e / k2, j = e % k2;
-        const int p = pp[i], q = qq[i];
-        const double c = cs[i], s = sn[i];
-        const double ap = a[j][p], aq = a[j][q];
-        a[j][p] = c * ap - s * aq;
-        a[j][q] = s * ap + c * aq;
-        const double vp = v[j][p], vq = v[j][q];
-        v[j][p] = c * vp - s * vq;
-        v[j][q] = s * vp + c * vq;
+      for (int t = 0; t < nvi; ++t) {
+        const int i = vi[t], w = vw[t];
+        const int p2 = pp[w], q2 = qq[w];
+        const double c2 = cs[w], s2 = sn[w];
+        const double vp = v[i][p2], vq = v[i][q2];
+        v[i][p2] = c2 * vp - s2 * vq;
+        v[i][q2] = s2 * vp + c2 * vq;
       }
       __syncthreads();
     }
   }
   // sort descending: rank of each eigenvalue (ties broken by index)
-  for (int i = tid; i < k; i += 256) {
+  for (int i = tid; i < k; i += 1024) {
     const double li = a[i][i];
     int rank = 0;
     for (int j = 0; j < k; ++j) {
@@ -320,29 +423,33 @@ __global__ __launch_bounds__(256) void k_syevj(const double* __restrict__ G, int
   if (tid == 0 && status) status[1] = sweeps;
 }
 
-void launch_syevj(const double* G, int k, double* lam, double* V, int* status, double* work, hipStream_t st) {
-  (void)work;
-  hipLaunchKernelGGL(k_syevj, dim3(1), dim3(256), 0, st, G, k, lam, V, status);
+void launch_syevj(const double* G, int k, double* lam, double* V, int* status, unsigned long long* keys,
+                  hipStream_t st) {
+  hipLaunchKernelGGL(k_syevj, dim3(1), dim3(1024), 0, st, G, k, lam, V, status, keys);
 }
 
 // ------------------------------------------------------------------------------------------------
 // core assembly with the sign gauge
+// pass 1 (64-row blocks): G = Q V[:, :rk] (or V itself when Q = I), written unflipped; per column the
+//   entry of largest |value| (first index on ties) is found with a 64-bit atomicMax on the key
+//   (|v| bits << 32) | ((0x7FFFFFFF - i) << 1) | sign(v)
+// pass 2 (64-row blocks): flip columns by that entry's sign; write Qn (zero-padded copy), Ur, sigma
 // ------------------------------------------------------------------------------------------------
-__global__ __launch_bounds__(1024) void k_assemble(const float* __restrict__ Qc, int64_t ldq, int m, int k,
-                                                   const double* __restrict__ lam, const double* __restrict__ V,
-                                                   int r, float* __restrict__ Gc, float* __restrict__ Ur,
-                                                   float* __restrict__ Qn, int kpn, float* __restrict__ sigma) {
+__global__ __launch_bounds__(256) void k_assemble_g(const float* __restrict__ Qc, int64_t ldq, int m, int k,
+                                                    const double* __restrict__ V, int r, float* __restrict__ Gc,
+                                                    unsigned long long* __restrict__ keys) {
   __shared__ double vs[KMAX][KMAX + 1];
-  __shared__ float bestv[1024];
-  __shared__ int besti[1024];
-  __shared__ float sgn[128];
+  __shared__ unsigned long long kmax[KMAX];
   const int tid = threadIdx.x;
-  const int rk = r < k ? r : k;   // columns beyond k are zero
-  for (int e = tid; e < k * k; e += 1024) vs[e / k][e % k] = V[e];
+  const int rk = r < k ? r : k;
+  for (int e = tid; e < k * k; e += 256) vs[e / k][e % k] = V[e];
+  if (tid < KMAX) kmax[tid] = 0ull;
   __syncthreads();
-  // pass 1: G = Q V[:, :rk] into Gc (column-major m x r), columns rk..r-1 zero
-  for (int64_t e = tid; e < (int64_t)m * r; e += 1024) {
-    const int c = (int)(e / m), i = (int)(e % m);
+  const int64_t i0 = (int64_t)blockIdx.x * RB64;
+  for (int e = tid; e < RB64 * r; e += 256) {
+    const int rr = e % RB64, c = e / RB64;
+    const int64_t i = i0 + rr;
+    if (i >= m) continue;
     double s = 0.0;
     if (c < rk) {
       if (Qc) {
@@ -351,57 +458,59 @@ __global__ __launch_bounds__(1024) void k_assemble(const float* __restrict__ Qc,
         s = vs[i][c];
       }
     }
-    Gc[(int64_t)c * m + i] = (float)s;
+    const float g = (float)s;
+    Gc[(int64_t)c * m + i] = g;
+    if (c < rk) {
+      const unsigned long long key = ((unsigned long long)__float_as_uint(fabsf(g)) << 32) |
+                                     ((unsigned long long)(0x7FFFFFFFu - (unsigned)i) << 1) | (g < 0.0f ? 1ull : 0ull);
+      atomicMax(&kmax[c], key);
+    }
   }
   __syncthreads();
-  // pass 2: per column, the first index of the largest |entry| decides the sign
-  for (int c = 0; c < r; ++c) {
-    float bv = -1.0f;
-    int bi = 0x7fffffff;
-    for (int i = tid; i < m; i += 1024) {
-      const float x = fabsf(Gc[(int64_t)c * m + i]);
-      if (x > bv) { bv = x; bi = i; }
-    }
-    bestv[tid] = bv;
-    besti[tid] = bi;
-    __syncthreads();
-    for (int s = 512; s > 0; s >>= 1) {
-      if (tid < s) {
-        const float ov = bestv[tid + s];
-        const int oi = besti[tid + s];
-        if (ov > bestv[tid] || (ov == bestv[tid] && oi < besti[tid])) { bestv[tid] = ov; besti[tid] = oi; }
-      }
-      __syncthreads();
-    }
-    if (tid == 0) {
-      const int i = besti[0];
-      sgn[c] = (i < m && Gc[(int64_t)c * m + i] < 0.0f) ? -1.0f : 1.0f;
-    }
-    __syncthreads();
-  }
-  // pass 3: apply signs, write Ur, Qn, sigma
-  for (int64_t e = tid; e < (int64_t)m * r; e += 1024) {
-    const int c = (int)(e / m);
-    Gc[e] *= sgn[c];
-  }
+  if (tid < rk) atomicMax(&keys[tid], kmax[tid]);
+}
+
+__global__ __launch_bounds__(256) void k_assemble_fix(const unsigned long long* __restrict__ keys, int m, int k,
+                                                      const double* __restrict__ V, const double* __restrict__ lam,
+                                                      int r, float* __restrict__ Gc, float* __restrict__ Ur,
+                                                      float* __restrict__ Qn, int kpn, float* __restrict__ sigma) {
+  __shared__ float sg[KMAX];
+  const int tid = threadIdx.x;
+  const int rk = r < k ? r : k;
+  if (tid < KMAX) sg[tid] = (tid < rk && (keys[tid] & 1ull)) ? -1.0f : 1.0f;
   __syncthreads();
-  if (Ur)
-    for (int e = tid; e < k * r; e += 1024) {
-      const int a2 = e / r, c = e % r;
-      Ur[e] = (c < rk) ? (float)(vs[a2][c] * (double)sgn[c]) : 0.0f;
-    }
+  const int64_t i0 = (int64_t)blockIdx.x * RB64;
+  for (int e = tid; e < RB64 * r; e += 256) {
+    const int rr = e % RB64, c = e / RB64;
+    const int64_t i = i0 + rr;
+    if (i >= m) continue;
+    const float g = Gc[(int64_t)c * m + i] * sg[c];
+    Gc[(int64_t)c * m + i] = g;
+    if (Qn) Qn[(int64_t)c * m + i] = g;
+  }
   if (Qn)
-    for (int64_t e = tid; e < (int64_t)m * kpn; e += 1024) {
-      const int c = (int)(e / m), i = (int)(e % m);
-      Qn[e] = (c < r) ? Gc[(int64_t)c * m + i] : 0.0f;
+    for (int e = tid; e < RB64 * (kpn - r); e += 256) {
+      const int rr = e % RB64, c = r + e / RB64;
+      const int64_t i = i0 + rr;
+      if (i < m) Qn[(int64_t)c * m + i] = 0.0f;
     }
-  if (sigma)
-    for (int i = tid; i < k; i += 1024) sigma[i] = (float)sqrt(fmax(lam[i], 0.0));
+  if (blockIdx.x == 0) {
+    if (Ur)
+      for (int e = tid; e < k * r; e += 256) {
+        const int a2 = e / r, c = e % r;
+        Ur[e] = (c < rk) ? (float)(V[a2 * k + c] * (double)sg[c]) : 0.0f;
+      }
+    if (sigma)
+      for (int i = tid; i < k; i += 256) sigma[i] = (float)sqrt(fmax(lam[i], 0.0));
+  }
 }
 
 void launch_assemble(const float* Qc, int64_t ldq, int m, int k, const double* lam, const double* V, int r, float* Gc,
-                     float* Ur, float* Qn, int kpn, float* sigma, hipStream_t st) {
-  hipLaunchKernelGGL(k_assemble, dim3(1), dim3(1024), 0, st, Qc, ldq, m, k, lam, V, r, Gc, Ur, Qn, kpn, sigma);
+                     float* Ur, float* Qn, int kpn, float* sigma, unsigned long long* keys, hipStream_t st) {
+  const int64_t grid = (m + RB64 - 1) / RB64;
+  hipLaunchKernelGGL(k_assemble_g, dim3((unsigned)grid), dim3(256), 0, st, Qc, ldq, m, k, V, r, Gc, keys);
+  hipLaunchKernelGGL(k_assemble_fix, dim3((unsigned)grid), dim3(256), 0, st, keys, m, k, V, lam, r, Gc, Ur, Qn, kpn,
+                     sigma);
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -411,9 +520,10 @@ template <int RM>
 __global__ __launch_bounds__(256) void k_next_from_z(const float* __restrict__ Z, int64_t ldz, int64_t n, int k,
                                                      const float* __restrict__ Ur, int r, float* __restrict__ out) {
   __shared__ float us[KMAX * RM];
-  for (int e = threadIdx.x; e < k * r; e += 256) us[(e / r) * RM + (e % r)] = Ur[e];
-  for (int e = threadIdx.x; e < k * RM; e += 256)
-    if ((e % RM) >= r) us[e] = 0.0f;
+  for (int e = threadIdx.x; e < k * RM; e += 256) {
+    const int a2 = e / RM, c = e % RM;
+    us[e] = (c < r) ? Ur[a2 * r + c] : 0.0f;
+  }
   __syncthreads();
   for (int64_t j = blockIdx.x * 256 + threadIdx.x; j < n; j += (int64_t)gridDim.x * 256) {
     float acc[RM];
@@ -442,6 +552,71 @@ void launch_next_from_z(const float* Z, int64_t ldz, int64_t n, int k, const flo
   else if (r <= 16) hipLaunchKernelGGL((k_next_from_z<16>), dim3((unsigned)grid), dim3(256), 0, st, Z, ldz, n, k, Ur, r, out);
   else if (r <= 32) hipLaunchKernelGGL((k_next_from_z<32>), dim3((unsigned)grid), dim3(256), 0, st, Z, ldz, n, k, Ur, r, out);
   else hipLaunchKernelGGL((k_next_from_z<64>), dim3((unsigned)grid), dim3(256), 0, st, Z, ldz, n, k, Ur, r, out);
+}
+
+// ------------------------------------------------------------------------------------------------
+// pending full-rank left factor of a fused step: A = (I_N (x) G^T) S with G the previous step's
+// square core (ms x ms, column-major).  Row (b, i) of A is sum_a G[a][b] row (a, i) of S.
+//   mode 0: Y_A = P Y_S, Y row-major m x kp           (sketch results)
+//   mode 1: Q_S = P^T Q_A, Q column-major kp x m       (projection operands)
+// ------------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_blockmix(const float* __restrict__ G, int ms, int m, int kp, int mode,
+                                                  const float* __restrict__ X, float* __restrict__ Y) {
+  __shared__ float gs[KMAX * KMAX];
+  for (int e = threadIdx.x; e < ms * ms; e += 256) gs[e] = G[e];
+  __syncthreads();
+  for (int64_t e = blockIdx.x * 256 + threadIdx.x; e < (int64_t)m * kp; e += (int64_t)gridDim.x * 256) {
+    if (mode == 0) {
+      const int row = (int)(e / kp), c = (int)(e % kp);
+      const int b = row % ms, i0 = row - b;
+      float acc = 0.0f;
+      for (int a2 = 0; a2 < ms; ++a2) acc += gs[a2 + ms * b] * X[(int64_t)(i0 + a2) * kp + c];
+      Y[e] = acc;
+    } else {
+      const int c = (int)(e / m), row = (int)(e % m);
+      const int a2 = row % ms, i0 = row - a2;
+      float acc = 0.0f;
+      for (int b = 0; b < ms; ++b) acc += gs[a2 + ms * b] * X[(int64_t)c * m + i0 + b];
+      Y[e] = acc;
+    }
+  }
+}
+
+void launch_blockmix(const float* G, int ms, int m, int kp, int mode, const float* X, float* Y, hipStream_t st) {
+  int64_t grid = ((int64_t)m * kp + 255) / 256;
+  if (grid > 2048) grid = 2048;
+  if (grid < 1) grid = 1;
+  hipLaunchKernelGGL(k_blockmix, dim3((unsigned)grid), dim3(256), 0, st, G, ms, m, kp, mode, X, Y);
+}
+
+// C_A = P C_S P^T for the exact path of a fused step (m x m, FP64, m <= 64)
+__global__ __launch_bounds__(256) void k_gram_mix(const float* __restrict__ G, int ms, int m,
+                                                  const double* __restrict__ Cs, double* __restrict__ Ca) {
+  __shared__ double gs[KMAX * KMAX];
+  __shared__ double t[KMAX][KMAX + 1];
+  for (int e = threadIdx.x; e < ms * ms; e += 256) gs[e] = (double)G[e];
+  __syncthreads();
+  // t = P C_S
+  for (int e = threadIdx.x; e < m * m; e += 256) {
+    const int row = e / m, col = e % m;
+    const int b = row % ms, i0 = row - b;
+    double acc = 0.0;
+    for (int a2 = 0; a2 < ms; ++a2) acc += gs[a2 + ms * b] * Cs[(i0 + a2) * m + col];
+    t[row][col] = acc;
+  }
+  __syncthreads();
+  // C_A = t P^T
+  for (int e = threadIdx.x; e < m * m; e += 256) {
+    const int row = e / m, col = e % m;
+    const int b = col % ms, j0 = col - b;
+    double acc = 0.0;
+    for (int a2 = 0; a2 < ms; ++a2) acc += t[row][j0 + a2] * gs[a2 + ms * b];
+    Ca[row * m + col] = acc;
+  }
+}
+
+void launch_gram_mix(const float* G, int ms, int m, const double* Cs, double* Ca, hipStream_t st) {
+  hipLaunchKernelGGL(k_gram_mix, dim3(1), dim3(256), 0, st, G, ms, m, Cs, Ca);
 }
 
 __global__ void k_copy_cols(const float* __restrict__ A, int64_t lda, int m, int n, float* __restrict__ Y, int kp) {

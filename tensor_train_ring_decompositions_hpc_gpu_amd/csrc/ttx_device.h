@@ -17,9 +17,11 @@ typedef float f32x16 __attribute__((ext_vector_type(16)));
 //   FP32  : v_mfma_f32_32x32x2_f32 on fp32 operands.
 //   BF16  : one v_mfma_f32_32x32x16_bf16 per product on bf16-rounded operands (range finder).
 //   BF16X3: operands split x = hi + lo (both bf16); hi*hi + hi*lo + lo*hi with fp32 accumulation,
-//           about 16 significant bits per operand (remainder and exact-path Gram).
+//           about 16 significant bits per operand (range finder).
 //   FP16  : one v_mfma_f32_32x32x16_f16 per product on fp16-rounded operands.
-enum Arith { AR_FP32 = 0, AR_BF16 = 1, AR_BF16X3 = 2, AR_FP16 = 3 };
+//   BF16X4: as BF16X3 plus lo*lo: the product error is then the 2^-17 representation error of the
+//           split operands (remainder and exact-path Gram).
+enum Arith { AR_FP32 = 0, AR_BF16 = 1, AR_BF16X3 = 2, AR_FP16 = 3, AR_BF16X4 = 4 };
 
 // ------------------------------------------------------------------------------------------
 // MFMA 32x32 output layout (verified by a GPU test): lane l, accumulator v holds

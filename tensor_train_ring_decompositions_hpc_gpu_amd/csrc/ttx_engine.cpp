@@ -25,7 +25,7 @@ struct DevBuf {
 struct Handle {
   hipStream_t stream = nullptr;
   std::string err;
-  DevBuf part, Y, Qc, Qn, Z, W1, W2, G, Rinv, lam, V, Ur, gpart, status, next[2], sig, scratch[4];
+  DevBuf part, Y, Y2, Qc, Qp, Qn, Z, W1, W2, G, G2, Rinv, lam, V, Ur, gpart, status, next[2], sig, keys, scratch[4];
   int* h_status = nullptr;       // pinned
   float* h_sigma = nullptr;      // pinned, TTX_MAX_ELL
   double* h_lam = nullptr;       // pinned
@@ -121,8 +121,8 @@ struct Arith3 {
 static Arith3 arith_for(int precision, int flags) {
   Arith3 a;
   if (precision == TTX_PREC_FP32) { a.range = AR_FP32; a.accurate = AR_FP32; }
-  else if (precision == TTX_PREC_FP16) { a.range = AR_FP16; a.accurate = AR_BF16X3; }
-  else { a.range = (flags & TTX_FLAG_FAST_RANGE) ? AR_BF16 : AR_BF16X3; a.accurate = AR_BF16X3; }
+  else if (precision == TTX_PREC_FP16) { a.range = AR_FP16; a.accurate = AR_BF16X4; }
+  else { a.range = (flags & TTX_FLAG_FAST_RANGE) ? AR_BF16 : AR_BF16X3; a.accurate = AR_BF16X4; }
   return a;
 }
 
@@ -130,32 +130,26 @@ static Arith3 arith_for(int precision, int flags) {
 // orthonormal basis of the columns of Y (m x k, row-major stride kpy): shifted CholeskyQR then two
 // CholeskyQR rounds, all in FP64; output Qc column-major (kp x m), columns k..kp-1 zero
 // ------------------------------------------------------------------------------------------------
-static ttx_status orth(Handle* h, const float* Y, int64_t kpy, int m, int k, int kp, float* Qc) {
+static ttx_status orth(Handle* h, const float* Y, int64_t kpy, int m, int k, int kp, float* Qc, int* stat) {
   hipStream_t st = h->stream;
-  const int64_t gg = gram_tall_grid(m);
-  TTX_TRY(ensure(h, h->gpart, (size_t)gg * k * k * sizeof(double)));
-  TTX_TRY(ensure(h, h->G, (size_t)k * k * sizeof(double)));
+  const int64_t nb = panel_blocks(m);
+  TTX_TRY(ensure(h, h->gpart, (size_t)nb * k * k * sizeof(double)));
   TTX_TRY(ensure(h, h->Rinv, (size_t)k * k * sizeof(double)));
   TTX_TRY(ensure(h, h->W1, (size_t)m * k * sizeof(double)));
   TTX_TRY(ensure(h, h->W2, (size_t)m * k * sizeof(double)));
-  double* G = P<double>(h->G);
   double* Ri = P<double>(h->Rinv);
   double* W1 = P<double>(h->W1);
   double* W2 = P<double>(h->W2);
-  int* stat = P<int>(h->status);
+  double* gp = P<double>(h->gpart);
   KScope ks(h, TTX_K_SMALL, 0);
-  launch_gram_tall_f32(Y, kpy, m, k, P<double>(h->gpart), gg, st);
-  launch_reduce_f64(P<double>(h->gpart), gg, (int64_t)k * k, G, st);
-  launch_chol_inv(G, k, m, 1, Ri, stat, st);
-  launch_apply_rinv_f32(Y, kpy, m, k, Ri, W1, st);
-  launch_gram_tall_f64(W1, k, m, k, P<double>(h->gpart), gg, st);
-  launch_reduce_f64(P<double>(h->gpart), gg, (int64_t)k * k, G, st);
-  launch_chol_inv(G, k, m, 0, Ri, stat, st);
-  launch_apply_rinv_f64(W1, m, k, Ri, W2, st);
-  launch_gram_tall_f64(W2, k, m, k, P<double>(h->gpart), gg, st);
-  launch_reduce_f64(P<double>(h->gpart), gg, (int64_t)k * k, G, st);
-  launch_chol_inv(G, k, m, 0, Ri, stat, st);
-  launch_apply_rinv_out(W2, m, k, kp, Ri, Qc, m, st);
+  launch_gram_tall_f32(Y, kpy, m, k, gp, nb, st);
+  launch_chol_inv(gp, nb, k, m, 1, Ri, stat, st);
+  launch_apply_gram_f32(Y, kpy, m, k, Ri, W1, gp, st);
+  launch_chol_inv(gp, nb, k, m, 0, Ri, stat, st);
+  launch_apply_gram_f64(W1, m, k, Ri, W2, gp, st);
+  // after two CholeskyQR rounds ||W2^T W2 - I|| is at rounding level: one Newton-Schulz step
+  // (polar factor, same column span) replaces a third Cholesky
+  launch_apply_ns_out(W2, m, k, kp, gp, nb, Qc, m, st);
   return check_launch(h, "orth");
 }
 
@@ -177,14 +171,39 @@ struct StepArgs {
   double norm2 = -1.0;  // ||A||_F^2 when known
   float* core = nullptr;   // m x r_out
   float* next = nullptr;   // r_out x n
+  // fused full-rank step: A = (I_N (x) pend^T) S where S = this->A and pend is the previous step's
+  // square core (pend_ms x pend_ms); defer: compute the core only and leave A_next implicit
+  const float* pend = nullptr;
+  int pend_ms = 0;
+  bool defer = false;
+  // device slots for sigma (TTX_MAX_ELL floats) and status (4 ints) of this step; sync: copy them to
+  // the host at the end of the step (single-step entry points)
+  float* sig_dev = nullptr;
+  int* stat_dev = nullptr;
+  bool sync = true;
 };
 struct StepOut {
-  int r = 0, ell = 0, path = 0, power_iters = 0, zero_pivots = 0, sweeps = 0;
+  int r = 0, ell = 0, path = 0, power_iters = 0, zero_pivots = 0, sweeps = 0, kdim = 0;
   double kept2 = 0.0;      // sum of kept sigma^2 = ||A_next||_F^2
   double total2 = -1.0;    // ||A||_F^2 when the step knows it
   int64_t bytes = 0;
   float sigma[TTX_MAX_ELL];
 };
+
+// host copies of a step's sigma (kdim values) and status into its StepOut
+static void finish_step_out(const float* sig, const int* stat, StepOut& o) {
+  double total = 0.0, kept = 0.0;
+  for (int i = 0; i < std::min(o.kdim, TTX_MAX_ELL); ++i) {
+    o.sigma[i] = sig[i];
+    const double s2 = (double)sig[i] * sig[i];
+    total += s2;
+    if (i < o.r) kept += s2;
+  }
+  o.kept2 = kept;
+  if (o.path == TTX_PATH_EXACT_M) o.total2 = total;
+  o.zero_pivots = stat[0];
+  o.sweeps = stat[1];
+}
 
 static int choose_rank(const StepArgs& a, int k, const double* lam, double total2, double* kept2) {
   int r = a.r;
@@ -210,6 +229,16 @@ static int choose_rank(const StepArgs& a, int k, const double* lam, double total
   return r;
 }
 
+// operand of the projections: Q itself, or P^T Q for a fused step
+static const float* proj_operand(Handle* h, const StepArgs& a, const float* Qsrc, int kp) {
+  if (!a.pend) return Qsrc;
+  {
+    KScope ks(h, TTX_K_SMALL, 0);
+    launch_blockmix(a.pend, a.pend_ms, (int)a.m, kp, 1, Qsrc, P<float>(h->Qp), h->stream);
+  }
+  return P<float>(h->Qp);
+}
+
 // tail of a randomized / exact-n step: Z = A^T Q (accurate), eig of Z^T Z, core and remainder
 static ttx_status step_tail(Handle* h, const StepArgs& a, const Arith3& ar, int k, int kp, StepOut& o) {
   hipStream_t st = h->stream;
@@ -217,21 +246,20 @@ static ttx_status step_tail(Handle* h, const StepArgs& a, const Arith3& ar, int 
   const int64_t n = a.n;
   TTX_TRY(ensure(h, h->Z, (size_t)n * kp * sizeof(float)));
   float* Z = P<float>(h->Z);
+  // Z = A^T Q (FP32-accurate) and its Gram B B^T = Z^T Z in the same pass
+  const int64_t gg = proj_grid(n);
+  TTX_TRY(ensure(h, h->gpart, (size_t)gg * k * k * sizeof(double)));
+  const float* Qop = proj_operand(h, a, P<float>(h->Qc), kp);
   {
     KScope ks(h, TTX_K_PROJ, a.m * n * 4 + n * kp * 4);
-    launch_proj(ar.accurate, kp, a.A, a.m, m, n, P<float>(h->Qc), m, 0, Z, kp, 0, st);
+    launch_proj(ar.accurate, kp, a.A, a.m, m, n, Qop, m, 0, Z, kp, 0, st, P<double>(h->gpart), k);
   }
   o.bytes += (int64_t)(a.m * n * 4);
-  const int64_t gg = gram_tall_grid(n);
-  TTX_TRY(ensure(h, h->gpart, (size_t)gg * k * k * sizeof(double)));
-  {
-    KScope ks(h, TTX_K_TALL, n * kp * 4);
-    launch_gram_tall_f32(Z, kp, n, k, P<double>(h->gpart), gg, st);
-  }
   {
     KScope ks(h, TTX_K_SMALL, 0);
     launch_reduce_f64(P<double>(h->gpart), gg, (int64_t)k * k, P<double>(h->G), st);
-    launch_syevj(P<double>(h->G), k, P<double>(h->lam), P<double>(h->V), P<int>(h->status), nullptr, st);
+    launch_syevj(P<double>(h->G), k, P<double>(h->lam), P<double>(h->V), a.stat_dev,
+                 P<unsigned long long>(h->keys), st);
   }
   TTX_TRY(check_launch(h, "step tail"));
   // rank decision on the host when needed
@@ -244,12 +272,11 @@ static ttx_status step_tail(Handle* h, const StepArgs& a, const Arith3& ar, int 
   }
   if (r > k) return fail(h, TTX_ERR_BAD_ARG, "rank exceeds the sketch width");
   TTX_TRY(ensure(h, h->Ur, (size_t)kp * r * sizeof(float)));
-  TTX_TRY(ensure(h, h->sig, TTX_MAX_ELL * sizeof(float)));
   // G = Q U_B[:, :r] with the sign gauge; sigma = sqrt(eig(B B^T)) (basicRandSVD_LU :717-763)
   {
     KScope ks(h, TTX_K_SMALL, 0);
     launch_assemble(P<float>(h->Qc), m, m, k, P<double>(h->lam), P<double>(h->V), r, a.core, P<float>(h->Ur),
-                    nullptr, 0, P<float>(h->sig), st);
+                    nullptr, 0, a.sig_dev, P<unsigned long long>(h->keys), st);
   }
   // remainder S V^T = U_r^T B = U_r^T Z^T (:734-739)
   {
@@ -261,23 +288,43 @@ static ttx_status step_tail(Handle* h, const StepArgs& a, const Arith3& ar, int 
   return check_launch(h, "step tail assemble");
 }
 
-static ttx_status run_step(Handle* h, const StepArgs& a, StepOut& o) {
+// Y of a fused step: Y_A = P Y_S (in Y2); otherwise Y itself
+static float* left_mix(Handle* h, const StepArgs& a, float* Y, int kp) {
+  if (!a.pend) return Y;
+  {
+    KScope ks(h, TTX_K_SMALL, 0);
+    launch_blockmix(a.pend, a.pend_ms, (int)a.m, kp, 0, Y, P<float>(h->Y2), h->stream);
+  }
+  return P<float>(h->Y2);
+}
+
+static ttx_status run_step(Handle* h, const StepArgs& a_in, StepOut& o) {
   hipStream_t st = h->stream;
+  StepArgs a = a_in;
   const Arith3 ar = arith_for(a.precision, a.flags);
   const int m = (int)a.m;
   const int64_t n = a.n;
   if (a.m <= 0 || a.n <= 0 || a.r <= 0) return fail(h, TTX_ERR_BAD_ARG, "step: empty unfolding or rank");
   if (a.m > (int64_t)1 << 30) return fail(h, TTX_ERR_UNSUPPORTED, "step: more than 2^30 rows");
+  if (a.pend && (a.pend_ms <= 0 || a.m % a.pend_ms != 0 || a.pend_ms > TTX_MAX_ELL))
+    return fail(h, TTX_ERR_BAD_ARG, "step: bad fused left factor");
   int ell = a.r + a.s;
   const int64_t mn = std::min<int64_t>(a.m, a.n);
   const bool exact = (a.method == TTX_METHOD_EXACT) || (a.method == TTX_METHOD_AUTO && ell >= mn);
   memset(o.sigma, 0, sizeof(o.sigma));
   o.bytes = 0;
   TTX_TRY(ensure(h, h->status, 4 * sizeof(int)));
-  TTX_HIP(h, hipMemsetAsync(h->status.p, 0, 4 * sizeof(int), st));
+  TTX_TRY(ensure(h, h->sig, TTX_MAX_ELL * sizeof(float)));
+  if (!a.stat_dev) {
+    a.stat_dev = P<int>(h->status);
+    TTX_HIP(h, hipMemsetAsync(h->status.p, 0, 4 * sizeof(int), st));
+  }
+  if (!a.sig_dev) a.sig_dev = P<float>(h->sig);
   TTX_TRY(ensure(h, h->lam, TTX_MAX_ELL * sizeof(double)));
+  TTX_TRY(ensure(h, h->keys, 64 * sizeof(unsigned long long)));
   TTX_TRY(ensure(h, h->V, TTX_MAX_ELL * TTX_MAX_ELL * sizeof(double)));
   TTX_TRY(ensure(h, h->G, TTX_MAX_ELL * TTX_MAX_ELL * sizeof(double)));
+  TTX_TRY(ensure(h, h->G2, TTX_MAX_ELL * TTX_MAX_ELL * sizeof(double)));
   int k = 0;
   if (exact && a.m <= TTX_MAX_ELL) {
     // exact Gram path on the short side (ttdec6_singleGPU.cu:510-536): C = A A^T, eig, top r, G^T A
@@ -285,43 +332,51 @@ static ttx_status run_step(Handle* h, const StepArgs& a, StepOut& o) {
     o.ell = 0;
     k = m;
     const int64_t grid = gram_small_grid(n);
-    TTX_TRY(ensure(h, h->gpart, (size_t)grid * 4 * m * m * sizeof(double)));
+    TTX_TRY(ensure(h, h->gpart, (size_t)grid * m * m * sizeof(double)));
     {
       KScope ks(h, TTX_K_GRAM, a.m * n * 4);
       launch_gram_small(ar.accurate, a.A, a.m, m, n, P<double>(h->gpart), grid, nullptr, st);
     }
     {
       KScope ks(h, TTX_K_SMALL, 0);
-      launch_reduce_f64(P<double>(h->gpart), grid * 4, (int64_t)m * m, P<double>(h->G), st);
-      launch_syevj(P<double>(h->G), m, P<double>(h->lam), P<double>(h->V), P<int>(h->status), nullptr, st);
+      if (a.pend) {
+        launch_reduce_f64(P<double>(h->gpart), grid, (int64_t)m * m, P<double>(h->G2), st);
+        launch_gram_mix(a.pend, a.pend_ms, m, P<double>(h->G2), P<double>(h->G), st);
+      } else {
+        launch_reduce_f64(P<double>(h->gpart), grid, (int64_t)m * m, P<double>(h->G), st);
+      }
+      launch_syevj(P<double>(h->G), m, P<double>(h->lam), P<double>(h->V), a.stat_dev,
+                   P<unsigned long long>(h->keys), st);
     }
     TTX_TRY(check_launch(h, "exact gram"));
     o.bytes += a.m * n * 4;
-    TTX_HIP(h, hipMemcpyAsync(h->h_lam, h->lam.p, m * sizeof(double), hipMemcpyDeviceToHost, st));
-    TTX_HIP(h, hipStreamSynchronize(st));
-    double total = 0.0;
-    for (int i = 0; i < m; ++i) total += std::max(h->h_lam[i], 0.0);
-    o.total2 = total;
-    StepArgs aa = a;
-    if (aa.norm2 < 0) aa.norm2 = total;
-    double kept = 0.0;
-    const int r = choose_rank(aa, m, h->h_lam, total, &kept);
+    int r = a.r;
+    if (a.rank_mode != TTX_RANK_FIXED) {
+      TTX_HIP(h, hipMemcpyAsync(h->h_lam, h->lam.p, m * sizeof(double), hipMemcpyDeviceToHost, st));
+      TTX_HIP(h, hipStreamSynchronize(st));
+      double total = 0.0;
+      for (int i = 0; i < m; ++i) total += std::max(h->h_lam[i], 0.0);
+      StepArgs aa = a;
+      if (aa.norm2 < 0) aa.norm2 = total;
+      double kept = 0.0;
+      r = choose_rank(aa, m, h->h_lam, total, &kept);
+    }
     if (r > TTX_MAX_ELL) return fail(h, TTX_ERR_UNSUPPORTED, "rank above TTX_MAX_ELL");
     const int kpn = kp_for(r);
     TTX_TRY(ensure(h, h->Qn, (size_t)kpn * m * sizeof(float)));
-    TTX_TRY(ensure(h, h->sig, TTX_MAX_ELL * sizeof(float)));
+    TTX_TRY(ensure(h, h->Qp, (size_t)kpn * m * sizeof(float)));
     {
       KScope ks(h, TTX_K_SMALL, 0);
       launch_assemble(nullptr, 0, m, m, P<double>(h->lam), P<double>(h->V), r, a.core, nullptr, P<float>(h->Qn),
-                      kpn, P<float>(h->sig), st);
+                      kpn, a.sig_dev, P<unsigned long long>(h->keys), st);
     }
-    {
+    if (!a.defer) {
+      const float* Qop = proj_operand(h, a, P<float>(h->Qn), kpn);
       KScope ks(h, TTX_K_PROJ, a.m * n * 4 + (int64_t)r * n * 4);
-      launch_proj(ar.accurate, kpn, a.A, a.m, m, n, P<float>(h->Qn), m, 1, a.next, 0, r, st);
+      launch_proj(ar.accurate, kpn, a.A, a.m, m, n, Qop, m, 1, a.next, 0, r, st);
+      o.bytes += a.m * n * 4 + (int64_t)r * n * 4;
     }
-    o.bytes += a.m * n * 4 + (int64_t)r * n * 4;
     o.r = r;
-    o.kept2 = kept;
     TTX_TRY(check_launch(h, "exact projection"));
   } else if (exact && a.n <= TTX_MAX_ELL) {
     // short-and-wide on the other side: Q = orth(A), then the common tail
@@ -330,12 +385,15 @@ static ttx_status run_step(Handle* h, const StepArgs& a, StepOut& o) {
     k = (int)n;
     const int kp = kp_for(k);
     TTX_TRY(ensure(h, h->Y, (size_t)m * kp * sizeof(float)));
+    TTX_TRY(ensure(h, h->Y2, (size_t)m * kp * sizeof(float)));
     TTX_TRY(ensure(h, h->Qc, (size_t)kp * m * sizeof(float)));
+    TTX_TRY(ensure(h, h->Qp, (size_t)kp * m * sizeof(float)));
     {
       KScope ks(h, TTX_K_SMALL, 0);
       launch_copy_cols(a.A, a.m, m, k, P<float>(h->Y), kp, st);
     }
-    TTX_TRY(orth(h, P<float>(h->Y), kp, m, k, kp, P<float>(h->Qc)));
+    float* Ya = left_mix(h, a, P<float>(h->Y), kp);
+    TTX_TRY(orth(h, Ya, kp, m, k, kp, P<float>(h->Qc), a.stat_dev));
     TTX_TRY(step_tail(h, a, ar, k, kp, o));
   } else {
     if (exact) return fail(h, TTX_ERR_UNSUPPORTED, "exact path needs min(m, n) <= TTX_MAX_ELL");
@@ -348,7 +406,9 @@ static ttx_status run_step(Handle* h, const StepArgs& a, StepOut& o) {
     const SketchPlan sp = plan_sketch(m, n, kp, 0);
     TTX_TRY(ensure(h, h->part, (size_t)sp.slabs * m * kp * sizeof(float)));
     TTX_TRY(ensure(h, h->Y, (size_t)m * kp * sizeof(float)));
+    TTX_TRY(ensure(h, h->Y2, (size_t)m * kp * sizeof(float)));
     TTX_TRY(ensure(h, h->Qc, (size_t)kp * m * sizeof(float)));
+    TTX_TRY(ensure(h, h->Qp, (size_t)kp * m * sizeof(float)));
     TTX_TRY(ensure(h, h->Z, (size_t)n * kp * sizeof(float)));
     float* Y = P<float>(h->Y);
     // Y = A Omega (basicRandSVD_LU :585-620; Omega from the project's Philox stream)
@@ -362,12 +422,13 @@ static ttx_status run_step(Handle* h, const StepArgs& a, StepOut& o) {
     }
     TTX_TRY(check_launch(h, "sketch"));
     o.bytes += a.m * n * 4;
-    TTX_TRY(orth(h, Y, kp, m, k, kp, P<float>(h->Qc)));
+    TTX_TRY(orth(h, left_mix(h, a, Y, kp), kp, m, k, kp, P<float>(h->Qc), a.stat_dev));
     // power iterations (:628-683): Z = A^T Q, Y = A Z, Q = orth(Y)
     for (int it = 0; it < a.q; ++it) {
+      const float* Qop = proj_operand(h, a, P<float>(h->Qc), kp);
       {
         KScope ks(h, TTX_K_PROJ, a.m * n * 4 + n * kp * 4);
-        launch_proj(ar.range, kp, a.A, a.m, m, n, P<float>(h->Qc), m, 0, P<float>(h->Z), kp, 0, st);
+        launch_proj(ar.range, kp, a.A, a.m, m, n, Qop, m, 0, P<float>(h->Z), kp, 0, st);
       }
       {
         KScope ks(h, TTX_K_SKETCH, a.m * n * 4 + n * kp * 4);
@@ -379,22 +440,17 @@ static ttx_status run_step(Handle* h, const StepArgs& a, StepOut& o) {
       }
       TTX_TRY(check_launch(h, "power iteration"));
       o.bytes += 2 * a.m * n * 4;
-      TTX_TRY(orth(h, Y, kp, m, k, kp, P<float>(h->Qc)));
+      TTX_TRY(orth(h, left_mix(h, a, Y, kp), kp, m, k, kp, P<float>(h->Qc), a.stat_dev));
     }
     o.power_iters = a.q;
     TTX_TRY(step_tail(h, a, ar, k, kp, o));
   }
-  // sigma and status back to the host
-  TTX_HIP(h, hipMemcpyAsync(h->h_sigma, h->sig.p, k * sizeof(float), hipMemcpyDeviceToHost, st));
-  TTX_HIP(h, hipMemcpyAsync(h->h_status, h->status.p, 4 * sizeof(int), hipMemcpyDeviceToHost, st));
-  TTX_HIP(h, hipStreamSynchronize(st));
-  for (int i = 0; i < std::min(k, TTX_MAX_ELL); ++i) o.sigma[i] = h->h_sigma[i];
-  o.zero_pivots = h->h_status[0];
-  o.sweeps = h->h_status[1];
-  if (o.path != TTX_PATH_EXACT_M) {
-    double kept = 0.0;
-    for (int i = 0; i < o.r; ++i) kept += (double)o.sigma[i] * o.sigma[i];
-    o.kept2 = kept;
+  o.kdim = k;
+  if (a.sync) {
+    TTX_HIP(h, hipMemcpyAsync(h->h_sigma, a.sig_dev, k * sizeof(float), hipMemcpyDeviceToHost, st));
+    TTX_HIP(h, hipMemcpyAsync(h->h_status, a.stat_dev, 4 * sizeof(int), hipMemcpyDeviceToHost, st));
+    TTX_HIP(h, hipStreamSynchronize(st));
+    finish_step_out(h->h_sigma, h->h_status, o);
   }
   return TTX_OK;
 }
@@ -424,8 +480,8 @@ static ttx_status validate(Handle* h, const ttx_params* p, bool tr) {
 
 static ttx_status input_norm2(Handle* h, const float* A, int64_t count, double* out) {
   const int64_t g = sumsq_grid(count);
-  TTX_TRY(ensure(h, h->scratch[3], (size_t)(g + 1) * sizeof(double)));
-  double* part = P<double>(h->scratch[3]);
+  TTX_TRY(ensure(h, h->gpart, (size_t)(g + 1) * sizeof(double)));
+  double* part = P<double>(h->gpart);
   launch_sumsq(A, count, part + g, part, g, h->stream);
   TTX_TRY(check_launch(h, "sumsq"));
   TTX_HIP(h, hipMemcpyAsync(h->h_lam, part + g, sizeof(double), hipMemcpyDeviceToHost, h->stream));
@@ -458,10 +514,21 @@ static ttx_status sweep(Handle* h, const float* d_A, const ttx_params* p, float*
   if (p->rank_mode != TTX_RANK_FIXED) TTX_TRY(input_norm2(h, d_A, total, &norm2));
   const double delta2 = (p->rank_mode != TTX_RANK_FIXED) ? p->eps * p->eps / (double)(d - 1) * norm2 : 0.0;
 
+  // per-step sigma / status slots on the device, read back once at the end (FIXED ranks need no
+  // host round trip inside the sweep)
+  TTX_TRY(ensure(h, h->scratch[3], (size_t)TTX_MAX_ORDER * (TTX_MAX_ELL * sizeof(float) + 4 * sizeof(int))));
+  float* sig_all = P<float>(h->scratch[3]);
+  int* stat_all = reinterpret_cast<int*>(sig_all + TTX_MAX_ORDER * TTX_MAX_ELL);
+  TTX_HIP(h, hipMemsetAsync(h->scratch[3].p, 0, h->scratch[3].bytes, st));
+  const bool host_ranks = p->rank_mode != TTX_RANK_FIXED;
+  StepOut outs[TTX_MAX_ORDER];
+
   const float* cur = d_A;
   int cur_buf = -1;   // which of next[0..1] holds cur (-1: the input)
   int r_prev = 1;
   int64_t n = total;
+  const float* pend = nullptr;   // square core of a deferred full-rank step
+  int pend_ms = 0;
   R.ranks[0] = p->ranks[0];
   R.ranks[d] = p->ranks[d];
   if (timing) TTX_HIP(h, hipEventRecord(ev[0], st));
@@ -484,30 +551,48 @@ static ttx_status sweep(Handle* h, const float* d_A, const ttx_params* p, float*
     a.delta2 = delta2;
     a.norm2 = norm2;
     a.core = d_cores[k];
+    a.pend = pend;
+    a.pend_ms = pend_ms;
+    a.sig_dev = sig_all + (size_t)k * TTX_MAX_ELL;
+    a.stat_dev = stat_all + 4 * k;
+    a.sync = host_ranks;
     const bool last = (k == d - 2);
+    // A full-rank exact step (r = m, square orthogonal core) that is not the last one is not
+    // projected: the next step reads this step's input with the core folded into its small factors.
+    const int64_t mn = std::min<int64_t>(m, n);
+    const bool exact_m = (a.method != TTX_METHOD_RSVD) && ((int64_t)a.r + a.s >= mn || a.method == TTX_METHOD_EXACT) &&
+                         m <= TTX_MAX_ELL;
+    a.defer = !last && !(tr && k == 0) && !host_ranks && !pend && exact_m && a.r == m;
     float* next = nullptr;
     int wbuf = -1;
-    if (last && !tr) {
+    if (last) {
       next = d_cores[d - 1];
-    } else if (last && tr) {
-      next = d_cores[d - 1];
-    } else {
+    } else if (!a.defer) {
       wbuf = (cur_buf == 0) ? 1 : 0;
       TTX_TRY(ensure(h, h->next[wbuf], (size_t)a.r * n * sizeof(float)));
       next = P<float>(h->next[wbuf]);
     }
     a.next = next;
-    StepOut o;
+    StepOut& o = outs[k];
     TTX_TRY(run_step(h, a, o));
-    R.norm2_in[k] = (o.total2 >= 0) ? o.total2 : norm2;
     R.ell[k] = o.ell;
     R.path[k] = o.path;
     R.power_iters[k] = o.power_iters;
-    R.zero_pivots[k] = o.zero_pivots;
-    R.jacobi_sweeps[k] = o.sweeps;
     R.bytes_alg += o.bytes;
-    if (h_sigma) memcpy(h_sigma + (size_t)k * TTX_MAX_ELL, o.sigma, sizeof(float) * TTX_MAX_ELL);
-    norm2 = o.kept2;
+    if (host_ranks) {
+      R.norm2_in[k] = (o.total2 >= 0) ? o.total2 : norm2;
+      norm2 = o.kept2;
+    }
+    if (a.defer) {
+      pend = d_cores[k];
+      pend_ms = (int)m;
+      r_prev = o.r;
+      R.ranks[k + 1] = o.r;
+      if (timing) TTX_HIP(h, hipEventRecord(ev[k + 1], st));
+      continue;   // cur stays: the next step reads it with the pending factor
+    }
+    pend = nullptr;
+    pend_ms = 0;
     if (tr && k == 0) {
       // (r0, r1, N') -> (r1, N', r0) (trdec6_singleGPU.cu:865-877); the ring rank joins the columns
       const int64_t r0 = p->ranks[0], r1 = p->ranks[1];
@@ -532,6 +617,24 @@ static ttx_status sweep(Handle* h, const float* d_A, const ttx_params* p, float*
       R.ranks[k + 1] = o.r;
     }
     if (timing) TTX_HIP(h, hipEventRecord(ev[k + 1], st));
+  }
+  if (!host_ranks) {
+    std::vector<float> hs((size_t)TTX_MAX_ORDER * TTX_MAX_ELL);
+    std::vector<int> hst((size_t)TTX_MAX_ORDER * 4);
+    TTX_HIP(h, hipMemcpyAsync(hs.data(), sig_all, hs.size() * sizeof(float), hipMemcpyDeviceToHost, st));
+    TTX_HIP(h, hipMemcpyAsync(hst.data(), stat_all, hst.size() * sizeof(int), hipMemcpyDeviceToHost, st));
+    TTX_HIP(h, hipStreamSynchronize(st));
+    double n2 = -1.0;
+    for (int k = 0; k < d - 1; ++k) {
+      finish_step_out(hs.data() + (size_t)k * TTX_MAX_ELL, hst.data() + 4 * k, outs[k]);
+      R.norm2_in[k] = (outs[k].total2 >= 0) ? outs[k].total2 : n2;
+      n2 = outs[k].kept2;
+    }
+  }
+  for (int k = 0; k < d - 1; ++k) {
+    R.zero_pivots[k] = outs[k].zero_pivots;
+    R.jacobi_sweeps[k] = outs[k].sweeps;
+    if (h_sigma) memcpy(h_sigma + (size_t)k * TTX_MAX_ELL, outs[k].sigma, sizeof(float) * TTX_MAX_ELL);
   }
   TTX_HIP(h, hipStreamSynchronize(st));
   if (timing) {
@@ -756,8 +859,9 @@ ttx_status ttx_destroy(ttx_handle hh) {
   Handle* h = H(hh);
   if (!h) return TTX_OK;
   (void)hipStreamSynchronize(h->stream);
-  ttx::DevBuf* bufs[] = {&h->part, &h->Y, &h->Qc, &h->Qn, &h->Z, &h->W1, &h->W2, &h->G, &h->Rinv, &h->lam,
-                         &h->V, &h->Ur, &h->gpart, &h->status, &h->next[0], &h->next[1], &h->sig,
+  ttx::DevBuf* bufs[] = {&h->part, &h->Y, &h->Y2, &h->Qc, &h->Qp, &h->Qn, &h->Z, &h->W1, &h->W2, &h->G, &h->G2,
+                         &h->Rinv, &h->lam,
+                         &h->V, &h->Ur, &h->gpart, &h->status, &h->next[0], &h->next[1], &h->sig, &h->keys,
                          &h->scratch[0], &h->scratch[1], &h->scratch[2], &h->scratch[3]};
   for (auto* b : bufs)
     if (b->p) (void)hipFree(b->p);

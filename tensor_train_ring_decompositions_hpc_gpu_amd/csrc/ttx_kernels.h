@@ -16,11 +16,14 @@ SketchPlan plan_sketch(int m, int64_t n, int kp, int wgs_hint);
 // Y partials: part[slab][m][kp].  X == nullptr selects the in-kernel Philox sketch (seed, stream, ell).
 void launch_sketch(int ar, const SketchPlan& p, const float* A, int64_t lda, int m, int64_t n, const float* X,
                    int64_t ldx, uint64_t seed, uint32_t stream, int ell, float* part, hipStream_t st);
-// D = Q^T A; mode 0 writes Z (n x ldo row-major, kp columns), mode 1 writes r_out x n column-major
+// D = Q^T A; mode 0 writes Z (n x ldo row-major, kp columns), mode 1 writes r_out x n column-major.
+// gpart != nullptr: also the Gram D D^T, first kg rows / columns, one FP64 partial per WG
+// (proj_grid(n) partials).
+int64_t proj_grid(int64_t n);
 void launch_proj(int ar, int kp, const float* A, int64_t lda, int m, int64_t n, const float* Qc, int64_t ldq,
-                 int mode, float* out, int64_t ldo, int r_out, hipStream_t st);
+                 int mode, float* out, int64_t ldo, int r_out, hipStream_t st, double* gpart = nullptr, int kg = 0);
 int64_t gram_small_grid(int64_t n);
-// partial A A^T: part[4 * grid][m][m] (FP64); absmax (optional) receives max |A| as float bits
+// partial A A^T: part[grid][m][m] (FP64); absmax (optional) receives max |A| as float bits
 void launch_gram_small(int ar, const float* A, int64_t lda, int m, int64_t n, double* part, int64_t grid,
                        float* absmax, hipStream_t st);
 void launch_reduce_f32(const float* part, int64_t S, int64_t count, float* out, hipStream_t st);
@@ -33,29 +36,45 @@ void launch_gram_tall_f32(const float* X, int64_t ldx, int64_t rows, int k, doub
                           hipStream_t st);
 void launch_gram_tall_f64(const double* X, int64_t ldx, int64_t rows, int k, double* part, int64_t grid,
                           hipStream_t st);
-// Cholesky of G (+ shift) and the explicit inverse of the upper factor: G + s I = R^T R, Rinv = R^-1.
-// shift_mode 1 adds s = 11 (m k + k (k+1)) u ||G||_2-bound; pivots that are not positive zero
-// their row and column of Rinv.  status[0] counts the zeroed pivots.
-void launch_chol_inv(const double* G, int k, int64_t m, int shift_mode, double* Rinv, int* status, hipStream_t st);
-// W_out = W_in Rinv (rows x k); W_in float (row stride ldi) or double (row stride k); output double row-major
-// (stride k) or float column-major (Qc[c * ldq + i], columns k..kp-1 zeroed)
-void launch_apply_rinv_f32(const float* Win, int64_t ldi, int64_t rows, int k, const double* Rinv, double* Wout,
+// number of 64-row blocks of the CholeskyQR panel kernels
+int64_t panel_blocks(int64_t rows);
+// Sum S Gram partials, Cholesky of G (+ shift) and the explicit inverse of the upper factor:
+// G + s I = R^T R, Rinv = R^-1.  shift_mode 1 adds s = 11 (m k + k (k+1)) u trace(G); pivots that are
+// not positive zero their row and column of Rinv and count into status[0].
+void launch_chol_inv(const double* part, int64_t S, int k, int64_t m, int shift_mode, double* Rinv, int* status,
+                     hipStream_t st);
+// W_out = W_in Rinv (rows x k) per 64-row block with the Gram partial of W_out (part[block][k][k]).
+void launch_apply_gram_f32(const float* Win, int64_t ldi, int64_t rows, int k, const double* Rinv, double* Wout,
+                           double* part, hipStream_t st);
+void launch_apply_gram_f64(const double* Win, int64_t rows, int k, const double* Rinv, double* Wout, double* part,
                            hipStream_t st);
-void launch_apply_rinv_f64(const double* Win, int64_t rows, int k, const double* Rinv, double* Wout, hipStream_t st);
-void launch_apply_rinv_out(const double* Win, int64_t rows, int k, int kp, const double* Rinv, float* Qc,
-                           int64_t ldq, hipStream_t st);
-// symmetric eigen-decomposition (cyclic Jacobi, FP64): lam descending, V column-major k x k
-void launch_syevj(const double* G, int k, double* lam, double* V, int* status, double* work, hipStream_t st);
+// final round: Qc (column-major kp x m, FP32) = W_in Rinv, columns k..kp-1 zeroed
+void launch_apply_out(const double* Win, int64_t rows, int k, int kp, const double* Rinv, float* Qc, int64_t ldq,
+                      hipStream_t st);
+// final round as one Newton-Schulz step: Qc = W_in (3 I - G) / 2 with G the sum of S Gram partials of W_in
+void launch_apply_ns_out(const double* Win, int64_t rows, int k, int kp, const double* gpart, int64_t S, float* Qc,
+                         int64_t ldq, hipStream_t st);
+// symmetric eigen-decomposition (cyclic Jacobi, FP64): lam descending, V row-major k x k (columns are
+// eigenvectors); status[1] = sweeps; keys[0..63] are zeroed for the following assembly
+void launch_syevj(const double* G, int k, double* lam, double* V, int* status, unsigned long long* keys,
+                  hipStream_t st);
 // core assembly after the eigen-decomposition of the projected Gram:
 //   Gc = Q V[:, :r] (m x r column-major, FP32) with the sign of each column fixed so that its largest
-//   |entry| is positive (first index on ties); Ur = V[:, :r] diag(sign) (kp x r, FP32, row-major);
+//   |entry| is positive (first index on ties); Ur = V[:, :r] diag(sign) (k x r, row-major, FP32);
 //   Qn = Gc padded with zero columns to kpn (column-major kpn x m) for a following projection;
 //   sigma = sqrt(max(lam, 0)) for the first k values.  Qc == nullptr means Q = I (exact path).
+//   keys: 64 zeroed 64-bit words (launch_syevj zeroes them).
 void launch_assemble(const float* Qc, int64_t ldq, int m, int k, const double* lam, const double* V, int r,
-                     float* Gc, float* Ur, float* Qn, int kpn, float* sigma, hipStream_t st);
+                     float* Gc, float* Ur, float* Qn, int kpn, float* sigma, unsigned long long* keys,
+                     hipStream_t st);
 // A_next[c + r j] = sum_a Ur[a][c] Z[j][a]  (c < r, j < n); Z row-major with stride ldz
 void launch_next_from_z(const float* Z, int64_t ldz, int64_t n, int k, const float* Ur, int r, float* out,
                         hipStream_t st);
+// pending full-rank left factor P = I_N (x) G^T of a fused step (G: ms x ms column-major core):
+// mode 0: Y = P X (X, Y row-major m x kp); mode 1: Y = P^T X (X, Y column-major kp x m)
+void launch_blockmix(const float* G, int ms, int m, int kp, int mode, const float* X, float* Y, hipStream_t st);
+// Ca = P Cs P^T (m x m, FP64)
+void launch_gram_mix(const float* G, int ms, int m, const double* Cs, double* Ca, hipStream_t st);
 // Y (m x kp row-major) <- A (m x n column-major, n <= kp), zero columns n..kp-1
 void launch_copy_cols(const float* A, int64_t lda, int m, int n, float* Y, int kp, hipStream_t st);
 

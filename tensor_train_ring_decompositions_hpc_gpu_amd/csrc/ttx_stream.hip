@@ -232,14 +232,25 @@ __global__ __launch_bounds__(64 * sketch_waves<LT>()) void k_sketch(const float*
 // K loop in chunks of 64 rows; the Q chunk (64 rows x kp) is staged in LDS and shared by the waves.
 // Lane (jl = lane%32, b = lane/32) loads rows 64 kc + 32 b + [0, 32) of column jb + 32 w + jl
 // (one 128-byte line); K-step t uses rows 32 b + 8 t + [0, 8) of the chunk for both operands.
-template <int LT, int AR>
+// GRAM: also accumulate the Gram D D^T of the written tile (the projected Gram B B^T of the final
+// pass): each wave transposes its tile through LDS and runs FP32 MFMAs over its columns; at the end
+// the 8 wave sums are added in wave order in FP64 and gpart[blockIdx.x][kg][kg] is written.
+template <int LT, int AR, bool GRAM>
 __global__ __launch_bounds__(512) void k_proj(const float* __restrict__ A, int64_t lda, int m, int64_t n,
                                               const float* __restrict__ Qc, int64_t ldq,
                                               int out_mode, float* __restrict__ out, int64_t ldo, int r_out,
-                                              int64_t blocks_total) {
+                                              int64_t blocks_total, double* __restrict__ gpart, int kg) {
   constexpr int KP = 32 * LT;
   constexpr int QP = 64 + 4;   // LDS pitch of a Q column chunk
+  constexpr int NPG = LT * (LT + 1) / 2;
   __shared__ float qs[2][KP * QP];
+  __shared__ float dt[GRAM ? 8 : 1][GRAM ? 32 * KP : 1];
+  __shared__ double gs[GRAM ? KP * KP : 1];
+  f32x16 gacc[GRAM ? NPG : 1];
+  if constexpr (GRAM) {
+#pragma unroll
+    for (int p = 0; p < NPG; ++p) gacc[p] = zero16();
+  }
   const int lane = threadIdx.x & 63;
   const int w = threadIdx.x >> 6;
   const int jl = lane & 31;
@@ -317,7 +328,7 @@ __global__ __launch_bounds__(512) void k_proj(const float* __restrict__ A, int64
           }
         } else {
           bf16x8 bh, bl;
-          if (AR == AR_BF16X3) split8(xb, bh, bl);
+          if (AR == AR_BF16X3 || AR == AR_BF16X4) split8(xb, bh, bl);
           else round8(xb, bh);
 #pragma unroll
           for (int lt = 0; lt < LT; ++lt) {
@@ -328,13 +339,14 @@ __global__ __launch_bounds__(512) void k_proj(const float* __restrict__ A, int64
             qv[0] = u0[0]; qv[1] = u0[1]; qv[2] = u0[2]; qv[3] = u0[3];
             qv[4] = u1[0]; qv[5] = u1[1]; qv[6] = u1[2]; qv[7] = u1[3];
             bf16x8 ah, al;
-            if (AR == AR_BF16X3) split8(qv, ah, al);
+            if (AR == AR_BF16X3 || AR == AR_BF16X4) split8(qv, ah, al);
             else round8(qv, ah);
             acc[lt] = mfma_bf16(ah, bh, acc[lt]);
-            if (AR == AR_BF16X3) {
+            if (AR == AR_BF16X3 || AR == AR_BF16X4) {
               acc[lt] = mfma_bf16(al, bh, acc[lt]);
               acc[lt] = mfma_bf16(ah, bl, acc[lt]);
             }
+            if (AR == AR_BF16X4) acc[lt] = mfma_bf16(al, bl, acc[lt]);
           }
         }
       }
@@ -384,6 +396,52 @@ __global__ __launch_bounds__(512) void k_proj(const float* __restrict__ A, int64
           }
       }
     }
+    if constexpr (GRAM) {
+      // tile D (kp x 32 columns, zero for columns j >= n) -> LDS dt[j][c], then
+      // G += D D^T with MFMA 32x32x2: A = D[c][j] (lane: c = l%32, j = 2 kk + l/32), B = the same
+      float* d = dt[w];
+#pragma unroll
+      for (int lt = 0; lt < LT; ++lt)
+#pragma unroll
+        for (int v = 0; v < 16; ++v) d[jl * KP + 32 * lt + mfma32_row(v, lane)] = acc[lt][v];
+      __syncthreads();
+#pragma unroll
+      for (int kk = 0; kk < 16; ++kk) {
+        const int jr = 2 * kk + b;
+        float x[LT];
+#pragma unroll
+        for (int lt = 0; lt < LT; ++lt) x[lt] = d[jr * KP + 32 * lt + jl];
+        int pidx = 0;
+#pragma unroll
+        for (int ta = 0; ta < LT; ++ta)
+#pragma unroll
+          for (int tb = ta; tb < LT; ++tb) { gacc[pidx] = mfma_f32(x[ta], x[tb], gacc[pidx]); ++pidx; }
+      }
+    }
+  }
+  if constexpr (GRAM) {
+    for (int e = threadIdx.x; e < KP * KP; e += 512) gs[e] = 0.0;
+    __syncthreads();
+    for (int ww = 0; ww < 8; ++ww) {
+      if (w == ww) {
+        int pidx = 0;
+#pragma unroll
+        for (int ta = 0; ta < LT; ++ta)
+#pragma unroll
+          for (int tb = ta; tb < LT; ++tb) {
+#pragma unroll
+            for (int v = 0; v < 16; ++v) {
+              const int c = 32 * ta + mfma32_row(v, lane), c2 = 32 * tb + jl;
+              gs[c * KP + c2] += (double)gacc[pidx][v];
+              if (ta != tb) gs[c2 * KP + c] += (double)gacc[pidx][v];
+            }
+            ++pidx;
+          }
+      }
+      __syncthreads();
+    }
+    double* o = gpart + (int64_t)blockIdx.x * kg * kg;
+    for (int e = threadIdx.x; e < kg * kg; e += 512) o[e] = gs[(e / kg) * KP + (e % kg)];
   }
 }
 
@@ -439,7 +497,7 @@ __global__ __launch_bounds__(256) void k_gram_small(const float* __restrict__ A,
       bf16x8 hi[MT], lo[MT];
 #pragma unroll
       for (int mt = 0; mt < MT; ++mt) {
-        if (AR == AR_BF16X3) split8(v[mt], hi[mt], lo[mt]);
+        if (AR == AR_BF16X3 || AR == AR_BF16X4) split8(v[mt], hi[mt], lo[mt]);
         else round8(v[mt], hi[mt]);
       }
       int p = 0;
@@ -448,10 +506,11 @@ __global__ __launch_bounds__(256) void k_gram_small(const float* __restrict__ A,
 #pragma unroll
         for (int m2 = m1; m2 < MT; ++m2) {
           acc[p] = mfma_bf16(hi[m1], hi[m2], acc[p]);
-          if (AR == AR_BF16X3) {
+          if (AR == AR_BF16X3 || AR == AR_BF16X4) {
             acc[p] = mfma_bf16(lo[m1], hi[m2], acc[p]);
             acc[p] = mfma_bf16(hi[m1], lo[m2], acc[p]);
           }
+          if (AR == AR_BF16X4) acc[p] = mfma_bf16(lo[m1], lo[m2], acc[p]);
           ++p;
         }
     }
@@ -470,24 +529,24 @@ __global__ __launch_bounds__(256) void k_gram_small(const float* __restrict__ A,
 #pragma unroll
     for (int q = 0; q < 16; ++q) dacc[p][q] += (double)acc[p][q];
 
-  // per-wave partial slab: part[4 * blockIdx.x + w][m][m]
-  double* o = part + ((int64_t)blockIdx.x * 4 + w) * m * m;
-  {
-    int p = 0;
+  // combine the 4 waves through LDS; one slab per WG: part[blockIdx.x][m][m]
+  __shared__ double red[4][NP][1024];
 #pragma unroll
-    for (int m1 = 0; m1 < MT; ++m1)
+  for (int p = 0; p < NP; ++p)
 #pragma unroll
-      for (int m2 = m1; m2 < MT; ++m2) {
-#pragma unroll
-        for (int v = 0; v < 16; ++v) {
-          const int i = 32 * m1 + mfma32_row(v, lane), k = 32 * m2 + r;
-          if (i < m && k < m) {
-            o[i * m + k] = dacc[p][v];
-            if (m1 != m2) o[k * m + i] = dacc[p][v];
-          }
-        }
-        ++p;
-      }
+    for (int v = 0; v < 16; ++v) red[w][p][mfma32_row(v, lane) * 32 + r] = dacc[p][v];
+  __syncthreads();
+  double* o = part + (int64_t)blockIdx.x * m * m;
+  for (int e = threadIdx.x; e < NP * 1024; e += 256) {
+    const int p = e >> 10, rc = e & 1023;
+    const double sum = red[0][p][rc] + red[1][p][rc] + red[2][p][rc] + red[3][p][rc];
+    int m1 = 0, m2 = 0;
+    if (MT == 2) { m1 = (p == 2) ? 1 : 0; m2 = (p == 0) ? 0 : 1; }
+    const int i = 32 * m1 + (rc >> 5), k = 32 * m2 + (rc & 31);
+    if (i < m && k < m) {
+      o[i * m + k] = sum;
+      if (m1 != m2) o[k * m + i] = sum;
+    }
   }
   // |A|max for the optional fp16 scaling
   if (absmax) {
@@ -499,25 +558,24 @@ __global__ __launch_bounds__(256) void k_gram_small(const float* __restrict__ A,
 // ------------------------------------------------------------------------------------------------
 // deterministic slab reductions
 // ------------------------------------------------------------------------------------------------
-__global__ void k_reduce_f32(const float* __restrict__ part, int64_t S, int64_t count, float* __restrict__ out) {
-  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < count; e += (int64_t)gridDim.x * blockDim.x) {
-    float s = 0.0f;
-    for (int64_t k = 0; k < S; ++k) s += part[k * count + e];
-    out[e] = s;
-  }
-}
-__global__ void k_reduce_f32_f64(const float* __restrict__ part, int64_t S, int64_t count, float* __restrict__ out) {
-  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < count; e += (int64_t)gridDim.x * blockDim.x) {
-    double s = 0.0;
-    for (int64_t k = 0; k < S; ++k) s += (double)part[k * count + e];
-    out[e] = (float)s;
-  }
-}
-__global__ void k_reduce_f64(const double* __restrict__ part, int64_t S, int64_t count, double* __restrict__ out) {
-  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < count; e += (int64_t)gridDim.x * blockDim.x) {
-    double s = 0.0;
-    for (int64_t k = 0; k < S; ++k) s += part[k * count + e];
-    out[e] = s;
+// out[e] = sum_s part[s * count + e] in a fixed order: WG handles 32 elements; its 8 thread groups
+// take slabs g, g + 8, ...; the 8 group sums are added in group order.
+template <typename TI, typename TO>
+__global__ __launch_bounds__(256) void k_reduce(const TI* __restrict__ part, int64_t S, int64_t count,
+                                                TO* __restrict__ out) {
+  __shared__ double red[8][33];
+  const int el = threadIdx.x & 31, g = threadIdx.x >> 5;
+  const int64_t e = (int64_t)blockIdx.x * 32 + el;
+  double acc = 0.0;
+  if (e < count)
+    for (int64_t k = g; k < S; k += 8) acc += (double)part[k * count + e];
+  red[g][el] = acc;
+  __syncthreads();
+  if (g == 0 && e < count) {
+    double t = 0.0;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) t += red[i][el];
+    out[e] = (TO)t;
   }
 }
 
@@ -597,36 +655,47 @@ void launch_sketch(int ar, const SketchPlan& p, const float* A, int64_t lda, int
   }
 }
 
-template <int LT, int AR>
-static void launch_proj_t(const float* A, int64_t lda, int m, int64_t n, const float* Qc, int64_t ldq, int mode,
-                          float* out, int64_t ldo, int r_out, hipStream_t st) {
+int64_t proj_grid(int64_t n) {
   const int64_t blocks = (n + 255) / 256;
   int64_t grid = (int64_t)num_cus() * 2;
   if (grid > blocks) grid = blocks;
   if (grid < 1) grid = 1;
-  hipLaunchKernelGGL((k_proj<LT, AR>), dim3((unsigned)grid), dim3(512), 0, st, A, lda, m, n, Qc, ldq, mode, out, ldo,
-                     r_out, blocks);
+  return grid;
+}
+
+template <int LT, int AR>
+static void launch_proj_t(const float* A, int64_t lda, int m, int64_t n, const float* Qc, int64_t ldq, int mode,
+                          float* out, int64_t ldo, int r_out, double* gpart, int kg, hipStream_t st) {
+  const int64_t blocks = (n + 255) / 256;
+  const int64_t grid = proj_grid(n);
+  if (gpart)
+    hipLaunchKernelGGL((k_proj<LT, AR, true>), dim3((unsigned)grid), dim3(512), 0, st, A, lda, m, n, Qc, ldq, mode,
+                       out, ldo, r_out, blocks, gpart, kg);
+  else
+    hipLaunchKernelGGL((k_proj<LT, AR, false>), dim3((unsigned)grid), dim3(512), 0, st, A, lda, m, n, Qc, ldq, mode,
+                       out, ldo, r_out, blocks, gpart, kg);
 }
 
 template <int LT>
 static void launch_proj_ar(int ar, const float* A, int64_t lda, int m, int64_t n, const float* Qc, int64_t ldq,
-                           int mode, float* out, int64_t ldo, int r_out, hipStream_t st) {
+                           int mode, float* out, int64_t ldo, int r_out, double* gpart, int kg, hipStream_t st) {
   switch (ar) {
-    case AR_FP32: launch_proj_t<LT, AR_FP32>(A, lda, m, n, Qc, ldq, mode, out, ldo, r_out, st); break;
-    case AR_BF16: launch_proj_t<LT, AR_BF16>(A, lda, m, n, Qc, ldq, mode, out, ldo, r_out, st); break;
-    case AR_FP16: launch_proj_t<LT, AR_FP16>(A, lda, m, n, Qc, ldq, mode, out, ldo, r_out, st); break;
-    default: launch_proj_t<LT, AR_BF16X3>(A, lda, m, n, Qc, ldq, mode, out, ldo, r_out, st); break;
+    case AR_FP32: launch_proj_t<LT, AR_FP32>(A, lda, m, n, Qc, ldq, mode, out, ldo, r_out, gpart, kg, st); break;
+    case AR_BF16: launch_proj_t<LT, AR_BF16>(A, lda, m, n, Qc, ldq, mode, out, ldo, r_out, gpart, kg, st); break;
+    case AR_FP16: launch_proj_t<LT, AR_FP16>(A, lda, m, n, Qc, ldq, mode, out, ldo, r_out, gpart, kg, st); break;
+    case AR_BF16X4: launch_proj_t<LT, AR_BF16X4>(A, lda, m, n, Qc, ldq, mode, out, ldo, r_out, gpart, kg, st); break;
+    default: launch_proj_t<LT, AR_BF16X3>(A, lda, m, n, Qc, ldq, mode, out, ldo, r_out, gpart, kg, st); break;
   }
 }
 
 void launch_proj(int ar, int kp, const float* A, int64_t lda, int m, int64_t n, const float* Qc, int64_t ldq,
-                 int mode, float* out, int64_t ldo, int r_out, hipStream_t st) {
-  if (kp == 32) launch_proj_ar<1>(ar, A, lda, m, n, Qc, ldq, mode, out, ldo, r_out, st);
-  else launch_proj_ar<2>(ar, A, lda, m, n, Qc, ldq, mode, out, ldo, r_out, st);
+                 int mode, float* out, int64_t ldo, int r_out, hipStream_t st, double* gpart, int kg) {
+  if (kp == 32) launch_proj_ar<1>(ar, A, lda, m, n, Qc, ldq, mode, out, ldo, r_out, gpart, kg, st);
+  else launch_proj_ar<2>(ar, A, lda, m, n, Qc, ldq, mode, out, ldo, r_out, gpart, kg, st);
 }
 
 int64_t gram_small_grid(int64_t n) {
-  int64_t g = (int64_t)num_cus() * 4;
+  int64_t g = (int64_t)num_cus() * 2;
   const int64_t minc = 4096;   // at least this many columns per WG
   if (g * minc > n) g = (n + minc - 1) / minc;
   if (g < 1) g = 1;
@@ -641,28 +710,26 @@ void launch_gram_small(int ar, const float* A, int64_t lda, int m, int64_t n, do
     switch (ar) {
       case AR_FP32: hipLaunchKernelGGL((k_gram_small<1, AR_FP32>), dim3(grid), dim3(256), 0, st, A, lda, m, n, cpw, part, absmax); break;
       case AR_BF16: hipLaunchKernelGGL((k_gram_small<1, AR_BF16>), dim3(grid), dim3(256), 0, st, A, lda, m, n, cpw, part, absmax); break;
+      case AR_BF16X4: hipLaunchKernelGGL((k_gram_small<1, AR_BF16X4>), dim3(grid), dim3(256), 0, st, A, lda, m, n, cpw, part, absmax); break;
       default: hipLaunchKernelGGL((k_gram_small<1, AR_BF16X3>), dim3(grid), dim3(256), 0, st, A, lda, m, n, cpw, part, absmax); break;
     }
   } else {
     switch (ar) {
       case AR_FP32: hipLaunchKernelGGL((k_gram_small<2, AR_FP32>), dim3(grid), dim3(256), 0, st, A, lda, m, n, cpw, part, absmax); break;
       case AR_BF16: hipLaunchKernelGGL((k_gram_small<2, AR_BF16>), dim3(grid), dim3(256), 0, st, A, lda, m, n, cpw, part, absmax); break;
+      case AR_BF16X4: hipLaunchKernelGGL((k_gram_small<2, AR_BF16X4>), dim3(grid), dim3(256), 0, st, A, lda, m, n, cpw, part, absmax); break;
       default: hipLaunchKernelGGL((k_gram_small<2, AR_BF16X3>), dim3(grid), dim3(256), 0, st, A, lda, m, n, cpw, part, absmax); break;
     }
   }
 }
 
 void launch_reduce_f32(const float* part, int64_t S, int64_t count, float* out, hipStream_t st) {
-  int64_t grid = (count + 255) / 256;
-  if (grid > 4096) grid = 4096;
-  if (grid < 1) grid = 1;
-  hipLaunchKernelGGL(k_reduce_f32_f64, dim3((unsigned)grid), dim3(256), 0, st, part, S, count, out);
+  const int64_t grid = (count + 31) / 32;
+  hipLaunchKernelGGL((k_reduce<float, float>), dim3((unsigned)grid), dim3(256), 0, st, part, S, count, out);
 }
 void launch_reduce_f64(const double* part, int64_t S, int64_t count, double* out, hipStream_t st) {
-  int64_t grid = (count + 255) / 256;
-  if (grid > 4096) grid = 4096;
-  if (grid < 1) grid = 1;
-  hipLaunchKernelGGL(k_reduce_f64, dim3((unsigned)grid), dim3(256), 0, st, part, S, count, out);
+  const int64_t grid = (count + 31) / 32;
+  hipLaunchKernelGGL((k_reduce<double, double>), dim3((unsigned)grid), dim3(256), 0, st, part, S, count, out);
 }
 
 }  // namespace ttx

@@ -5,7 +5,9 @@ Tolerances (relative):
 * generators and permutations: bit-exact;
 * FP32 arithmetic: singular values 1e-5, cores / remainders 1e-4;
 * BF16 arithmetic (hi/lo split, FP32 accumulation): singular values 1e-4, cores / remainders 1e-3;
-* reconstruction errors: |err_gpu - err_oracle| <= 1e-5 + 2e-2 * err_oracle.
+* reconstruction errors: FP32 |err_gpu - err_oracle| <= 1e-5 + 2e-2 * err_oracle; BF16 adds the
+  arithmetic floor of the split products, |err_gpu - err_oracle| <= 1e-4 (the accuracy target of the
+  BF16 engine).
 """
 import numpy as np
 import pytest
@@ -14,7 +16,7 @@ from oracle import philox, ttsvd
 
 pytestmark = pytest.mark.gpu
 
-TOL = {"fp32": dict(sig=1e-5, vec=1e-4), "bf16": dict(sig=1e-4, vec=1e-3)}
+TOL = {"fp32": dict(sig=1e-5, vec=1e-4, err=1e-5), "bf16": dict(sig=1e-4, vec=1e-3, err=1e-4)}
 
 
 def host(t):
@@ -149,7 +151,7 @@ def test_tt_svd_matches_oracle(engine, prec):
         assert np.allclose(sig[k][:r], s_ref[k][:r], rtol=tol["sig"], atol=tol["sig"] * s_ref[k][0]), k
     err = engine.rel_error(cores, dims, used, A)
     err_ref = ttsvd.rel_error(An, ttsvd.tt_full(c_ref, dims, used))
-    assert abs(err - err_ref) <= 1e-5 + 2e-2 * err_ref, (err, err_ref)
+    assert abs(err - err_ref) <= tol["err"] + 2e-2 * err_ref, (err, err_ref)
     for k in range(len(dims) - 1):
         g, shape = cores[k]
         assert rel(fmat(g, *shape), c_ref[k]) < 10 * tol["vec"], k
@@ -170,7 +172,7 @@ def test_tr_svd_matches_oracle(engine, prec):
         assert np.allclose(sig[k][:r], s_ref[k][:r], rtol=tol["sig"], atol=tol["sig"] * s_ref[k][0]), k
     err = engine.rel_error(cores, dims, ranks, A, tr=True)
     err_ref = ttsvd.rel_error(An, ttsvd.tr_full(c_ref, dims, ranks))
-    assert abs(err - err_ref) <= 1e-5 + 2e-2 * err_ref, (err, err_ref)
+    assert abs(err - err_ref) <= tol["err"] + 2e-2 * err_ref, (err, err_ref)
 
 
 def test_tt_rel_error_matches_oracle(engine):
@@ -180,3 +182,28 @@ def test_tt_rel_error_matches_oracle(engine):
     ref_cores = [host(c) for c in cores]
     ref = ttsvd.rel_error(host(A), ttsvd.tt_full(ref_cores, dims, ranks))
     assert abs(err - ref) < 1e-6 + 1e-4 * ref
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16"])
+@pytest.mark.parametrize("dims,ranks,s,q", [
+    ([8, 8, 8, 8, 8], [1, 8, 8, 8, 8, 1], 6, 1),      # full-rank step 1 folded into a randomized step 2
+    ([4, 4, 8, 8], [1, 4, 4, 4, 1], 12, 1),          # folded into an exact (short) step 2
+    ([4, 4, 4], [1, 4, 4, 1], 4, 1),                 # folded into an exact (wide) last step
+    ([32, 16, 16, 8], [1, 32, 16, 8, 1], 10, 2),     # cfg2's shape class at a small size
+])
+def test_tt_svd_full_rank_first_step(engine, prec, dims, ranks, s, q):
+    A = engine.generate(dims, ranks, std=1.0, noise_rel=1e-6, seed=4321)
+    cores, sig, rep = engine.tt_svd(A, dims, ranks, s=s, q=q, precision=prec)
+    An = host(A).astype(np.float64)
+    c_ref, s_ref, p_ref, used = ttsvd.tt_svd(An, dims, ranks, s=s, q=q)
+    tol = TOL[prec]
+    assert rep["ranks"] == used
+    for k in range(len(dims) - 1):
+        r = ranks[k + 1]
+        assert np.allclose(sig[k][:r], s_ref[k][:r], rtol=tol["sig"], atol=tol["sig"] * s_ref[k][0]), k
+    err = engine.rel_error(cores, dims, used, A)
+    err_ref = ttsvd.rel_error(An, ttsvd.tt_full(c_ref, dims, used))
+    assert abs(err - err_ref) <= tol["err"] + 2e-2 * err_ref, (err, err_ref)
+    for k in range(len(dims) - 1):
+        g, shape = cores[k]
+        assert rel(fmat(g, *shape), c_ref[k]) < 10 * tol["vec"], k
