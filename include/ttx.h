@@ -69,6 +69,12 @@ typedef struct {
 
 #define TTX_FLAG_FAST_RANGE 1 /* BF16 mode: single bf16 MFMA per product in the range finder (no hi/lo split) */
 #define TTX_FLAG_TIMING 2     /* record per-step times into the report */
+/* a-posteriori range check (randomized steps): the sketch carries up to 8 extra Gaussian test columns
+ * Y_t = A Omega_t that the basis never sees; mean ||(I - Q Q^T) Y_t||^2 / mean ||Y_t||^2 estimates
+ * ||(I - Q Q^T) A||_F^2 / ||A||_F^2 without bias.  Power iterations run only while the estimate is
+ * above the step's tolerance, and power_iters becomes their ceiling.  In DELTA rank mode the basis
+ * also grows (ell = 16, 32, ..., r + s) while the check fails. */
+#define TTX_FLAG_ADAPTIVE 4
 
 typedef struct {
   int32_t ranks[TTX_MAX_ORDER + 1];   /* ranks used */
@@ -86,6 +92,14 @@ typedef struct {
   double kernel_ms[TTX_NUM_KCLASS];
   int32_t kernel_launches[TTX_NUM_KCLASS];
   int64_t kernel_bytes[TTX_NUM_KCLASS];
+  /* TTX_FLAG_ADAPTIVE / LEARNED: estimated relative range residual ||(I - Q Q^T) A_k||_F / ||A_k||_F of
+   * the basis that was used (-1 when not checked), number of checks, streaming passes over A_k,
+   * and 1 where a learned prediction failed its check and the adaptive range finder took over */
+  double range_resid[TTX_MAX_ORDER];
+  int32_t range_checks[TTX_MAX_ORDER];
+  int32_t passes[TTX_MAX_ORDER];
+  int32_t fallback[TTX_MAX_ORDER];
+  int32_t predicted_rank[TTX_MAX_ORDER];  /* LEARNED: the network's rank for step k (0 otherwise) */
 } ttx_report;
 
 /* handle ---------------------------------------------------------------------------------------- */

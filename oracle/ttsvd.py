@@ -24,6 +24,10 @@ Functions and the reference code each one restates:
                             ttdec6_singleGPU.cu:134-271, trdec6_singleGPU.cu:175-329
 * ``rel_error``          -- calMSE + norm2HH, ttdec6_singleGPU.cu:122-182 (non-destructive here)
 * ``rsvd``               -- the basicRandSVD_LU interface: all ell columns of U and SVT
+* ``adaptive_range``     -- the randomized range finder with an a-posteriori check on held-out Gaussian
+                            test columns (Halko, Martinsson, Tropp 2011, sec. 4.3-4.4); the
+                            reference has none (its q and ell are fixed, rsvd_lu_half.cu:557-683), so
+                            this is the engine's own rule, pinned by closed forms in the tests
 * ``delta_rank``         -- the commented truncation rule,
                             trdecomposition/third_order_GPU/unopt_GPU/trdec_gpu_float_unopt2_rank.cu:92,222-243
 """
@@ -33,6 +37,7 @@ from . import philox
 
 EXACT = "exact"
 RSVD = "rsvd"
+MAX_ELL = 64          # widest sketch (basis + test columns) of one step
 
 
 # ----------------------------------------------------------------------------------------------
@@ -116,18 +121,9 @@ def gram_eig_step(A, r):
     return G, np.sqrt(lam), A_next
 
 
-def rsvd_step(A, k, s, q, Omega):
-    """Randomized step (EDrSVD), basicRandSVD_LU, rsvd_lu_half.cu:557-775.
-
-    Y = A Omega, Q = qr(Y); q times: Z = A^T Q, Qz = qr(Z), Y = A Qz, Q = qr(Y) (:601-683);
-    B = Q^T A (:690-709); W = B B^T, eig, reverse (:717-728); SVT = U_B^T B (:734-739);
-    U = Q U_B (:745-763).  Returns (U m x ell, sigma ell, SVT ell x n)."""
-    A = np.asarray(A, dtype=np.float64)
-    Om = np.asarray(Omega, dtype=np.float64)
-    Q, _ = cholqr2(A @ Om)
-    for _ in range(q):
-        Qz, _ = cholqr2(A.T @ Q)
-        Q, _ = cholqr2(A @ Qz)
+def rsvd_tail(A, Q):
+    """B = Q^T A (:690-709); W = B B^T, eig, reverse (:717-728); SVT = U_B^T B (:734-739);
+    U = Q U_B (:745-763) with the sign gauge.  Returns (U, sigma, SVT)."""
     B = Q.T @ A
     lam, Ub = eig_desc(B @ B.T)
     lam = np.maximum(lam, 0.0)
@@ -137,17 +133,96 @@ def rsvd_step(A, k, s, q, Omega):
     return U, np.sqrt(lam), SVT
 
 
-def step(A, r, s, q, seed, stream, method=RSVD):
+def rsvd_step(A, k, s, q, Omega):
+    """Randomized step (EDrSVD), basicRandSVD_LU, rsvd_lu_half.cu:557-775.
+
+    Y = A Omega, Q = qr(Y); q times: Z = A^T Q, Qz = qr(Z), Y = A Qz, Q = qr(Y) (:601-683);
+    then rsvd_tail.  Returns (U m x ell, sigma ell, SVT ell x n)."""
+    A = np.asarray(A, dtype=np.float64)
+    Om = np.asarray(Omega, dtype=np.float64)
+    Q, _ = cholqr2(A @ Om)
+    for _ in range(q):
+        Qz, _ = cholqr2(A.T @ Q)
+        Q, _ = cholqr2(A @ Qz)
+    return rsvd_tail(A, Q)
+
+
+def range_resid2(Q, Yt):
+    """sum ||(I - Q Q^T) y_t||^2 / sum ||y_t||^2 over the test columns y_t = A omega_t: an unbiased
+    estimate of ||(I - Q Q^T) A||_F^2 / ||A||_F^2 for Gaussian omega_t independent of Q."""
+    R = Yt - Q @ (Q.T @ Yt)
+    y2 = float(np.sum(Yt * Yt))
+    return float(np.sum(R * R)) / y2 if y2 > 0 else 0.0
+
+
+def adaptive_range(A, widths, pt, q, tol2, seed, stream):
+    """The engine's adaptive range finder.  For each basis width w in ``widths`` (ascending): sketch
+    Y = A Omega[:, :w+pt], Q = orth(Y[:, :w]), check with the test columns Y[:, w:w+pt]; stop at
+    the first width that passes.  Then power iterations (Q = orth(A A^T Q)) while the check still
+    fails, at most q.  Returns (Q, w, power_iters, rel2, checks)."""
+    A = np.asarray(A, dtype=np.float64)
+    n = A.shape[1]
+    checks = 0
+    for w in widths:
+        Om = philox.omega(seed, stream, n, w + pt).astype(np.float64)
+        Y = A @ Om
+        Q, _ = cholqr2(Y[:, :w])
+        Yt = Y[:, w:]
+        rel2 = range_resid2(Q, Yt)
+        checks += 1
+        if rel2 <= tol2:
+            break
+    it = 0
+    while it < q and rel2 > tol2:
+        Q, _ = cholqr2(A @ (A.T @ Q))
+        it += 1
+        rel2 = range_resid2(Q, Yt)
+        checks += 1
+    return Q, w, it, rel2, checks
+
+
+def adaptive_widths(ell, rank_mode, pred_ell=0):
+    """Basis widths the engine tries: the fixed width ell; in delta mode 16, 32, ... up to ell; with a
+    learned prediction, the prediction first and then the wider widths of that ladder."""
+    if rank_mode == "fixed" and pred_ell <= 0:
+        return [ell]
+    w0 = min(max(pred_ell, 1), ell) if pred_ell > 0 else min(16, ell)
+    out = [w0]
+    w = 16
+    while w < ell and len(out) < 7:
+        if w > w0:
+            out.append(w)
+        w *= 2
+    if out[-1] < ell:
+        out.append(ell)
+    return out
+
+
+def step(A, r, s, q, seed, stream, method=RSVD, adaptive=None):
     """One sweep step with the engine's path rule: the exact Gram step when ell = r + s reaches
-    min(m, n) or when ``method`` is EXACT; the randomized step otherwise."""
+    min(m, n) or when ``method`` is EXACT; the randomized step otherwise.
+
+    adaptive (optional dict: tol2, rank_mode, pred_ell): the randomized step runs
+    ``adaptive_range`` with up to 8 test columns inside the 64-column sketch; the dict receives
+    "out" = {ell, power_iters, rel2, checks}."""
     m, n = A.shape
     ell = r + s
     if method == EXACT or ell >= min(m, n):
         G, sig, A_next = gram_eig_step(A, r)
         return G, sig, A_next, EXACT
-    Om = philox.omega(seed, stream, n, ell)
-    U, sig, SVT = rsvd_step(A, r, s, q, Om)
-    return U[:, :r], sig, SVT[:r], RSVD
+    ell = min(ell, MAX_ELL)
+    pt = min(8, MAX_ELL - ell) if adaptive is not None else 0
+    if pt < 2:
+        Om = philox.omega(seed, stream, n, ell)
+        U, sig, SVT = rsvd_step(A, r, s, q, Om)
+        return U[:, :r], sig, SVT[:r], RSVD
+    widths = adaptive_widths(ell, adaptive.get("rank_mode", "fixed"), adaptive.get("pred_ell", 0))
+    Q, w, it, rel2, checks = adaptive_range(np.asarray(A, dtype=np.float64), widths, pt, q, adaptive["tol2"],
+                                            seed, stream)
+    U, sig, SVT = rsvd_tail(np.asarray(A, dtype=np.float64), Q)
+    adaptive["out"] = dict(ell=w, power_iters=it, rel2=rel2, checks=checks)
+    rr = min(r, w)
+    return U[:, :rr], sig, SVT[:rr], RSVD
 
 
 def rsvd(A, k, s, q, seed, stream=0):
@@ -162,25 +237,34 @@ def rsvd(A, k, s, q, seed, stream=0):
 # sweeps
 # ----------------------------------------------------------------------------------------------
 
-def delta_rank(lam, total2, delta2, rmax):
+def delta_rank(lam, total2, delta2, rmax, rel2=None):
     """Rank of the delta rule on one step (Oseledets 2011, TT-SVD; the reference's commented rule
     trdecomposition/third_order_GPU/unopt_GPU/trdec_gpu_float_unopt2_rank.cu:92,222-243 with
-    ep = eps / sqrt(d-1)): the smallest r with tail(r) <= delta2, where
-    tail(r) = max(total2 - sum_{i<=r} lam_i, sum_{r<i} lam_i), lam = sigma^2 descending and total2 =
-    ||A_k||_F^2; capped at rmax and at least 1."""
+    ep = eps / sqrt(d-1)): the smallest r with tail(r) <= delta2, lam = sigma^2 descending, capped at
+    rmax and at least 1.
+
+    rel2 None: tail(r) = max(total2 - sum_{i<=r} lam_i, sum_{r<i} lam_i) with total2 = ||A_k||_F^2.
+    rel2 given (the basis' relative range residual, 0 for a complete basis):
+    tail(r) = sum_{r<i} lam_i + rel2 / (1 - rel2) * sum_i lam_i."""
     lam = np.maximum(np.asarray(lam, dtype=np.float64), 0.0)
     k = len(lam)
     r = k
     cum = np.cumsum(lam)
+    allv = float(cum[-1]) if k else 0.0
+    outside = 0.0 if rel2 is None else (rel2 / (1.0 - rel2) * allv if rel2 < 1.0 else np.inf)
     for i in range(k):
-        tail = max(total2 - cum[i], float(lam[i + 1:].sum()))
+        if rel2 is None:
+            tail = max(total2 - cum[i], float(lam[i + 1:].sum()))
+        else:
+            tail = float(lam[i + 1:].sum()) + outside
         if tail <= delta2:
             r = i + 1
             break
     return max(1, min(r, rmax))
 
 
-def tt_svd(A, dims, ranks, s=10, q=2, seed=1234, method=RSVD, rank_mode="fixed", eps=1e-4):
+def tt_svd(A, dims, ranks, s=10, q=2, seed=1234, method=RSVD, rank_mode="fixed", eps=1e-4, adaptive=False,
+           infos=None):
     """Order-d TT-SVD sweep (ttdec6_singleGPU.cu:799-958 generalised to order d).
 
     ranks has d+1 entries with ranks[0] = ranks[d] = 1 (fixed ranks, or ceilings for
@@ -198,13 +282,23 @@ def tt_svd(A, dims, ranks, s=10, q=2, seed=1234, method=RSVD, rank_mode="fixed",
         m = used[k] * dims[k]
         cur = cur.reshape(m, -1, order="F")
         r = ranks[k + 1]
+        ad = None
+        if adaptive or rank_mode != "fixed":
+            # a quarter of the step's share of the squared error budget, relative to ||A_k||^2
+            tol2 = 0.25 * eps * eps / (d - 1) if rank_mode == "fixed" else 0.25 * delta2 / total2
+            ad = dict(tol2=tol2, rank_mode=rank_mode)
         if rank_mode == "delta":
-            G, sig, nxt, path = step(cur, min(r, min(cur.shape)), s, q, seed, k, method)
-            rr = delta_rank(sig ** 2, total2, delta2, G.shape[1])
+            G, sig, nxt, path = step(cur, min(r, min(cur.shape)), s, q, seed, k, method, ad)
+            # exact Gram step: the spectrum is complete (rel2 = 0); checked randomized step: its
+            # residual estimate; unchecked randomized step: the carried norm
+            rel2 = 0.0 if path == EXACT else (ad["out"]["rel2"] if ad and "out" in ad else None)
+            rr = delta_rank(sig ** 2, total2, delta2, G.shape[1], rel2)
             G, nxt = G[:, :rr], nxt[:rr]
         else:
-            G, sig, nxt, path = step(cur, r, s, q, seed, k, method)
+            G, sig, nxt, path = step(cur, r, s, q, seed, k, method, ad)
             rr = r
+        if infos is not None:
+            infos.append(ad.get("out") if ad else None)
         total2 = float(np.sum(sig[:rr] ** 2))
         cores.append(G)
         sigmas.append(sig)

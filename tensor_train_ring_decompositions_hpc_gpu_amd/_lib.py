@@ -17,7 +17,7 @@ PREC_FP32, PREC_BF16, PREC_FP16 = 0, 1, 2
 RANK_FIXED, RANK_DELTA, RANK_LEARNED = 0, 1, 2
 METHOD_AUTO, METHOD_EXACT, METHOD_RSVD = 0, 1, 2
 PATH_EXACT_M, PATH_EXACT_N, PATH_RSVD = 0, 1, 2
-FLAG_FAST_RANGE, FLAG_TIMING = 1, 2
+FLAG_FAST_RANGE, FLAG_TIMING, FLAG_ADAPTIVE = 1, 2, 4
 NUM_KCLASS = 8
 KCLASS = ["sketch", "proj", "gram", "permute", "tall", "small", "unused6", "unused7"]
 
@@ -57,6 +57,11 @@ class Report(ctypes.Structure):
         ("kernel_ms", ctypes.c_double * NUM_KCLASS),
         ("kernel_launches", ctypes.c_int32 * NUM_KCLASS),
         ("kernel_bytes", ctypes.c_int64 * NUM_KCLASS),
+        ("range_resid", ctypes.c_double * MAX_ORDER),
+        ("range_checks", ctypes.c_int32 * MAX_ORDER),
+        ("passes", ctypes.c_int32 * MAX_ORDER),
+        ("fallback", ctypes.c_int32 * MAX_ORDER),
+        ("predicted_rank", ctypes.c_int32 * MAX_ORDER),
     ]
 
 

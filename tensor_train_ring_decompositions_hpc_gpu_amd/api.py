@@ -213,6 +213,11 @@ def report_dict(rep, d):
         "step_ms": [float(rep.step_ms[k]) for k in range(d - 1)],
         "total_ms": float(rep.total_ms),
         "bytes_alg": int(rep.bytes_alg),
+        "range_resid": [float(rep.range_resid[k]) for k in range(d - 1)],
+        "range_checks": [int(rep.range_checks[k]) for k in range(d - 1)],
+        "passes": [int(rep.passes[k]) for k in range(d - 1)],
+        "fallback": [int(rep.fallback[k]) for k in range(d - 1)],
+        "predicted_rank": [int(rep.predicted_rank[k]) for k in range(d - 1)],
         "kernels": {L.KCLASS[c]: {"ms": float(rep.kernel_ms[c]), "launches": int(rep.kernel_launches[c]),
                                   "bytes": int(rep.kernel_bytes[c])}
                     for c in range(L.NUM_KCLASS) if rep.kernel_launches[c] > 0},

@@ -619,6 +619,83 @@ void launch_gram_mix(const float* G, int ms, int m, const double* Cs, double* Ca
   hipLaunchKernelGGL(k_gram_mix, dim3(1), dim3(256), 0, st, G, ms, m, Cs, Ca);
 }
 
+// ------------------------------------------------------------------------------------------------
+// a-posteriori range check with held-out Gaussian test columns Yt = A Omega_t (m x p, row-major):
+// ||(I - Q Q^T) Yt||_F^2 and ||Yt||_F^2, both per 64-row block (Halko, Martinsson, Tropp 2011, 4.3).
+// The residual is formed explicitly (Yt - Q C) so that it is not lost to cancellation.
+// ------------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_test_proj(const float* __restrict__ Qc, int64_t ldq, int m, int k,
+                                                   const float* __restrict__ Yt, int p, double* __restrict__ part) {
+  const int64_t i0 = (int64_t)blockIdx.x * RB64;
+  for (int e = threadIdx.x; e < k * p; e += 256) {
+    const int c = e / p, t = e % p;
+    double s = 0.0;
+    for (int rr = 0; rr < RB64; ++rr) {
+      const int64_t i = i0 + rr;
+      if (i < m) s += (double)Qc[(int64_t)c * ldq + i] * (double)Yt[i * p + t];
+    }
+    part[(int64_t)blockIdx.x * k * p + e] = s;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_test_resid(const float* __restrict__ Qc, int64_t ldq, int m, int k,
+                                                    const float* __restrict__ Yt, int p,
+                                                    const double* __restrict__ part, int64_t S,
+                                                    double* __restrict__ out) {
+  __shared__ double C[KMAX * 16];
+  __shared__ double red[2][256];
+  for (int e = threadIdx.x; e < k * p; e += 256) {
+    double s = 0.0;
+    for (int64_t q = 0; q < S; ++q) s += part[q * k * p + e];
+    C[e] = s;
+  }
+  __syncthreads();
+  const int64_t i0 = (int64_t)blockIdx.x * RB64;
+  double r2 = 0.0, y2 = 0.0;
+  for (int e = threadIdx.x; e < RB64 * p; e += 256) {
+    const int rr = e / p, t = e % p;
+    const int64_t i = i0 + rr;
+    if (i >= m) continue;
+    const double y = (double)Yt[i * p + t];
+    double proj = 0.0;
+    for (int c = 0; c < k; ++c) proj += (double)Qc[(int64_t)c * ldq + i] * C[c * p + t];
+    const double d = y - proj;
+    r2 += d * d;
+    y2 += y * y;
+  }
+  red[0][threadIdx.x] = r2;
+  red[1][threadIdx.x] = y2;
+  __syncthreads();
+  for (int s2 = 128; s2 > 0; s2 >>= 1) {
+    if (threadIdx.x < s2) { red[0][threadIdx.x] += red[0][threadIdx.x + s2]; red[1][threadIdx.x] += red[1][threadIdx.x + s2]; }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) { out[2 * blockIdx.x] = red[0][0]; out[2 * blockIdx.x + 1] = red[1][0]; }
+}
+
+void launch_test_resid(const float* Qc, int64_t ldq, int m, int k, const float* Yt, int p, double* part, double* out,
+                       hipStream_t st) {
+  const int64_t nb = (m + RB64 - 1) / RB64;
+  hipLaunchKernelGGL(k_test_proj, dim3((unsigned)nb), dim3(256), 0, st, Qc, ldq, m, k, Yt, p, part);
+  hipLaunchKernelGGL(k_test_resid, dim3((unsigned)nb), dim3(256), 0, st, Qc, ldq, m, k, Yt, p, part, nb, out);
+}
+
+// Yt (m x p row-major) <- columns c0 .. c0+p-1 of Y (m x kpy row-major)
+__global__ void k_copy_block(const float* __restrict__ Y, int kpy, int m, int c0, int p, float* __restrict__ Yt) {
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < (int64_t)m * p; e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = e / p;
+    const int t = (int)(e % p);
+    Yt[e] = Y[i * kpy + c0 + t];
+  }
+}
+
+void launch_copy_block(const float* Y, int kpy, int m, int c0, int p, float* Yt, hipStream_t st) {
+  int64_t grid = ((int64_t)m * p + 255) / 256;
+  if (grid > 1024) grid = 1024;
+  if (grid < 1) grid = 1;
+  hipLaunchKernelGGL(k_copy_block, dim3((unsigned)grid), dim3(256), 0, st, Y, kpy, m, c0, p, Yt);
+}
+
 __global__ void k_copy_cols(const float* __restrict__ A, int64_t lda, int m, int n, float* __restrict__ Y, int kp) {
   for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < (int64_t)m * kp;
        e += (int64_t)gridDim.x * blockDim.x) {

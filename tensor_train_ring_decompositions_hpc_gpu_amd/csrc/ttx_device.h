@@ -17,7 +17,9 @@ typedef float f32x16 __attribute__((ext_vector_type(16)));
 //   FP32  : v_mfma_f32_32x32x2_f32 on fp32 operands.
 //   BF16  : one v_mfma_f32_32x32x16_bf16 per product on bf16-rounded operands (range finder).
 //   BF16X3: operands split x = hi + lo (both bf16); hi*hi + hi*lo + lo*hi with fp32 accumulation,
-//           about 16 significant bits per operand (range finder).
+//           about 16 significant bits per operand (range finder).  Against the bf16-exact Gaussian
+//           sketch the unfolding is split three ways instead (hi + mid + lo, three MFMAs), so the
+//           sketch and its held-out test columns are accurate to FP32 rounding.
 //   FP16  : one v_mfma_f32_32x32x16_f16 per product on fp16-rounded operands.
 //   BF16X4: as BF16X3 plus lo*lo: the product error is then the 2^-17 representation error of the
 //           split operands (remainder and exact-path Gram).
@@ -53,6 +55,19 @@ __device__ __forceinline__ void split8(const float* x, bf16x8& hi, bf16x8& lo) {
     const __bf16 h = (__bf16)x[i];
     hi[i] = h;
     lo[i] = (__bf16)(x[i] - (float)h);
+  }
+}
+// x = hi + mid + lo, three bf16 terms (about 24 significant bits): exact products with a bf16-exact
+// operand
+__device__ __forceinline__ void split8x3(const float* x, bf16x8& hi, bf16x8& mid, bf16x8& lo) {
+#pragma unroll
+  for (int i = 0; i < 8; ++i) {
+    const __bf16 h = (__bf16)x[i];
+    const float r1 = x[i] - (float)h;
+    const __bf16 mm = (__bf16)r1;
+    hi[i] = h;
+    mid[i] = mm;
+    lo[i] = (__bf16)(r1 - (float)mm);
   }
 }
 __device__ __forceinline__ void round8(const float* x, bf16x8& hi) {

@@ -26,9 +26,11 @@ struct Handle {
   hipStream_t stream = nullptr;
   std::string err;
   DevBuf part, Y, Y2, Qc, Qp, Qn, Z, W1, W2, G, G2, Rinv, lam, V, Ur, gpart, status, next[2], sig, keys, scratch[4];
+  DevBuf Yt, tpart, tout;         // held-out test columns of the range check and its partial sums
   int* h_status = nullptr;       // pinned
   float* h_sigma = nullptr;      // pinned, TTX_MAX_ELL
   double* h_lam = nullptr;       // pinned
+  double* h_aux = nullptr;       // pinned, 16 doubles (range-check sums)
   int dev = 0;
   std::vector<float*> last_cores;  // device buffers kept for the reference-name shims
   // per-kernel timing (TTX_FLAG_TIMING): event pairs recorded on the stream around launches
@@ -181,12 +183,18 @@ struct StepArgs {
   float* sig_dev = nullptr;
   int* stat_dev = nullptr;
   bool sync = true;
+  // TTX_FLAG_ADAPTIVE: relative squared tolerance of the range check; pred_ell > 0: the sketch width
+  // a learned model predicted (the check judges it, the adaptive growth is its fallback)
+  double range_tol2 = 0.0;
+  int pred_ell = 0;
 };
 struct StepOut {
   int r = 0, ell = 0, path = 0, power_iters = 0, zero_pivots = 0, sweeps = 0, kdim = 0;
   double kept2 = 0.0;      // sum of kept sigma^2 = ||A_next||_F^2
   double total2 = -1.0;    // ||A||_F^2 when the step knows it
   int64_t bytes = 0;
+  double resid = -1.0;     // relative range residual estimate of the basis used (-1: not checked)
+  int checks = 0, passes = 0, fallback = 0;
   float sigma[TTX_MAX_ELL];
 };
 
@@ -205,19 +213,24 @@ static void finish_step_out(const float* sig, const int* stat, StepOut& o) {
   o.sweeps = stat[1];
 }
 
-static int choose_rank(const StepArgs& a, int k, const double* lam, double total2, double* kept2) {
+// rank of the delta rule: the smallest r with tail(r) <= delta^2 (Oseledets 2011, TT-SVD; the
+// commented rule of trdec_gpu_float_unopt2_rank.cu:222-243).  rel2 < 0: tail(r) = max(total2 - sum_{i<=r}
+// lam_i, sum_{i>r} lam_i) with total2 = ||A||_F^2 carried from the previous step.  rel2 >= 0 (the
+// basis' relative range residual, 0 for a complete basis): tail(r) = sum_{i>r} lam_i + rel2 / (1 - rel2)
+// sum_i lam_i, which needs no cancellation against a norm carried across steps (whose rounding in
+// reduced-precision passes can exceed delta^2).
+static int choose_rank(const StepArgs& a, int k, const double* lam, double total2, double rel2, double* kept2) {
   int r = a.r;
-  if (a.rank_mode == TTX_RANK_DELTA) {
-    // smallest r with tail(r) <= delta^2, tail(r) = total - sum_{i<=r} lam_i (Oseledets 2011,
-    // TT-SVD; the commented rule of trdec_gpu_float_unopt2_rank.cu:222-243)
+  if (a.rank_mode != TTX_RANK_FIXED) {
+    double all = 0.0;
+    for (int i = 0; i < k; ++i) all += std::max(lam[i], 0.0);
+    const double outside = rel2 >= 0.0 ? (rel2 < 1.0 ? rel2 / (1.0 - rel2) * all : 1e300) : 0.0;
     double cum = 0.0;
     r = k;
     for (int i = 0; i < k; ++i) {
       cum += std::max(lam[i], 0.0);
-      double tail = total2 - cum;
-      double cap_tail = 0.0;
-      for (int j = i + 1; j < k; ++j) cap_tail += std::max(lam[j], 0.0);
-      tail = std::max(tail, cap_tail);
+      const double cap_tail = all - cum;
+      const double tail = rel2 >= 0.0 ? cap_tail + outside : std::max(total2 - cum, cap_tail);
       if (tail <= a.delta2) { r = i + 1; break; }
     }
     r = std::min(r, a.r);
@@ -240,7 +253,8 @@ static const float* proj_operand(Handle* h, const StepArgs& a, const float* Qsrc
 }
 
 // tail of a randomized / exact-n step: Z = A^T Q (accurate), eig of Z^T Z, core and remainder
-static ttx_status step_tail(Handle* h, const StepArgs& a, const Arith3& ar, int k, int kp, StepOut& o) {
+static ttx_status step_tail(Handle* h, const StepArgs& a, const Arith3& ar, int k, int kp, double rel2,
+                            StepOut& o) {
   hipStream_t st = h->stream;
   const int m = (int)a.m;
   const int64_t n = a.n;
@@ -255,6 +269,7 @@ static ttx_status step_tail(Handle* h, const StepArgs& a, const Arith3& ar, int 
     launch_proj(ar.accurate, kp, a.A, a.m, m, n, Qop, m, 0, Z, kp, 0, st, P<double>(h->gpart), k);
   }
   o.bytes += (int64_t)(a.m * n * 4);
+  o.passes += 1;
   {
     KScope ks(h, TTX_K_SMALL, 0);
     launch_reduce_f64(P<double>(h->gpart), gg, (int64_t)k * k, P<double>(h->G), st);
@@ -268,7 +283,7 @@ static ttx_status step_tail(Handle* h, const StepArgs& a, const Arith3& ar, int 
     TTX_HIP(h, hipMemcpyAsync(h->h_lam, h->lam.p, k * sizeof(double), hipMemcpyDeviceToHost, st));
     TTX_HIP(h, hipStreamSynchronize(st));
     double kept;
-    r = choose_rank(a, k, h->h_lam, a.norm2, &kept);
+    r = choose_rank(a, k, h->h_lam, a.norm2, rel2, &kept);
   }
   if (r > k) return fail(h, TTX_ERR_BAD_ARG, "rank exceeds the sketch width");
   TTX_TRY(ensure(h, h->Ur, (size_t)kp * r * sizeof(float)));
@@ -296,6 +311,80 @@ static float* left_mix(Handle* h, const StepArgs& a, float* Y, int kp) {
     launch_blockmix(a.pend, a.pend_ms, (int)a.m, kp, 0, Y, P<float>(h->Y2), h->stream);
   }
   return P<float>(h->Y2);
+}
+
+// Y = A Omega[:, :w + pt] (basicRandSVD_LU :585-620; Omega from the project's Philox stream), test
+// columns w .. w+pt-1 kept in Yt, Q = orth(Y[:, :w])
+static ttx_status range_sketch(Handle* h, const StepArgs& a, const Arith3& ar, int w, int pt, int kp,
+                               const SketchPlan& sp, StepOut& o) {
+  hipStream_t st = h->stream;
+  const int m = (int)a.m;
+  float* Y = P<float>(h->Y);
+  {
+    KScope ks(h, TTX_K_SKETCH, a.m * a.n * 4);
+    launch_sketch(ar.range, sp, a.A, a.m, m, a.n, nullptr, 0, a.seed, a.stream_id, w + pt, P<float>(h->part), st);
+  }
+  {
+    KScope ks(h, TTX_K_SMALL, 0);
+    launch_reduce_f32(P<float>(h->part), sp.slabs, (int64_t)m * kp, Y, st);
+  }
+  TTX_TRY(check_launch(h, "sketch"));
+  o.bytes += a.m * a.n * 4;
+  o.passes += 1;
+  float* Ya = left_mix(h, a, Y, kp);
+  if (pt) {
+    KScope ks(h, TTX_K_SMALL, 0);
+    launch_copy_block(Ya, kp, m, w, pt, P<float>(h->Yt), st);
+  }
+  return orth(h, Ya, kp, m, w, kp, P<float>(h->Qc), a.stat_dev);
+}
+
+// one power iteration on the current basis: Z = A^T Q, Y = A Z, Q = orth(Y)
+static ttx_status power_iter(Handle* h, const StepArgs& a, const Arith3& ar, int k, int kp, const SketchPlan& sp,
+                             StepOut& o) {
+  hipStream_t st = h->stream;
+  const int m = (int)a.m;
+  const int64_t n = a.n;
+  float* Y = P<float>(h->Y);
+  const float* Qop = proj_operand(h, a, P<float>(h->Qc), kp);
+  {
+    KScope ks(h, TTX_K_PROJ, a.m * n * 4 + n * kp * 4);
+    launch_proj(ar.range, kp, a.A, a.m, m, n, Qop, m, 0, P<float>(h->Z), kp, 0, st);
+  }
+  {
+    KScope ks(h, TTX_K_SKETCH, a.m * n * 4 + n * kp * 4);
+    launch_sketch(ar.range, sp, a.A, a.m, m, n, P<float>(h->Z), kp, 0, 0, k, P<float>(h->part), st);
+  }
+  {
+    KScope ks(h, TTX_K_SMALL, 0);
+    launch_reduce_f32(P<float>(h->part), sp.slabs, (int64_t)m * kp, Y, st);
+  }
+  TTX_TRY(check_launch(h, "power iteration"));
+  o.bytes += 2 * a.m * n * 4;
+  o.passes += 2;
+  return orth(h, left_mix(h, a, Y, kp), kp, m, k, kp, P<float>(h->Qc), a.stat_dev);
+}
+
+// relative squared range residual estimate sum ||(I - Q Q^T) Y_t||^2 / sum ||Y_t||^2 (host sync)
+static ttx_status range_check(Handle* h, const StepArgs& a, int k, int pt, double* rel2, StepOut& o) {
+  hipStream_t st = h->stream;
+  const int m = (int)a.m;
+  const int64_t nb = panel_blocks(m);
+  TTX_TRY(ensure(h, h->tpart, (size_t)nb * k * pt * sizeof(double)));
+  TTX_TRY(ensure(h, h->tout, (size_t)(2 * nb + 2) * sizeof(double)));
+  double* tout = P<double>(h->tout);
+  {
+    KScope ks(h, TTX_K_SMALL, 0);
+    launch_test_resid(P<float>(h->Qc), m, m, k, P<float>(h->Yt), pt, P<double>(h->tpart), tout, st);
+    launch_reduce_f64(tout, nb, 2, tout + 2 * nb, st);
+  }
+  TTX_TRY(check_launch(h, "range check"));
+  TTX_HIP(h, hipMemcpyAsync(h->h_aux, tout + 2 * nb, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
+  TTX_HIP(h, hipStreamSynchronize(st));
+  const double r2 = h->h_aux[0], y2 = h->h_aux[1];
+  *rel2 = y2 > 0.0 ? r2 / y2 : 0.0;
+  o.checks += 1;
+  return TTX_OK;
 }
 
 static ttx_status run_step(Handle* h, const StepArgs& a_in, StepOut& o) {
@@ -350,6 +439,7 @@ static ttx_status run_step(Handle* h, const StepArgs& a_in, StepOut& o) {
     }
     TTX_TRY(check_launch(h, "exact gram"));
     o.bytes += a.m * n * 4;
+    o.passes += 1;
     int r = a.r;
     if (a.rank_mode != TTX_RANK_FIXED) {
       TTX_HIP(h, hipMemcpyAsync(h->h_lam, h->lam.p, m * sizeof(double), hipMemcpyDeviceToHost, st));
@@ -359,7 +449,7 @@ static ttx_status run_step(Handle* h, const StepArgs& a_in, StepOut& o) {
       StepArgs aa = a;
       if (aa.norm2 < 0) aa.norm2 = total;
       double kept = 0.0;
-      r = choose_rank(aa, m, h->h_lam, total, &kept);
+      r = choose_rank(aa, m, h->h_lam, total, 0.0, &kept);
     }
     if (r > TTX_MAX_ELL) return fail(h, TTX_ERR_UNSUPPORTED, "rank above TTX_MAX_ELL");
     const int kpn = kp_for(r);
@@ -375,6 +465,7 @@ static ttx_status run_step(Handle* h, const StepArgs& a_in, StepOut& o) {
       KScope ks(h, TTX_K_PROJ, a.m * n * 4 + (int64_t)r * n * 4);
       launch_proj(ar.accurate, kpn, a.A, a.m, m, n, Qop, m, 1, a.next, 0, r, st);
       o.bytes += a.m * n * 4 + (int64_t)r * n * 4;
+      o.passes += 1;
     }
     o.r = r;
     TTX_TRY(check_launch(h, "exact projection"));
@@ -394,15 +485,18 @@ static ttx_status run_step(Handle* h, const StepArgs& a_in, StepOut& o) {
     }
     float* Ya = left_mix(h, a, P<float>(h->Y), kp);
     TTX_TRY(orth(h, Ya, kp, m, k, kp, P<float>(h->Qc), a.stat_dev));
-    TTX_TRY(step_tail(h, a, ar, k, kp, o));
+    TTX_TRY(step_tail(h, a, ar, k, kp, 0.0, o));   // Q spans all columns of A: no residual
   } else {
     if (exact) return fail(h, TTX_ERR_UNSUPPORTED, "exact path needs min(m, n) <= TTX_MAX_ELL");
     if (ell > TTX_MAX_ELL) ell = TTX_MAX_ELL;
     if (ell > mn) ell = (int)mn;
     o.path = TTX_PATH_RSVD;
-    o.ell = ell;
-    k = ell;
-    const int kp = kp_for(k);
+    // DELTA and LEARNED ranks always run the check: their tail estimate uses its residual
+    const bool adaptive = (a.flags & TTX_FLAG_ADAPTIVE) || a.pred_ell > 0 || a.rank_mode != TTX_RANK_FIXED;
+    // held-out test columns beyond the widest basis, inside the 64-column sketch
+    int pt = adaptive ? std::min(8, TTX_MAX_ELL - ell) : 0;
+    if (pt < 2) pt = 0;
+    const int kp = kp_for(ell + pt);
     const SketchPlan sp = plan_sketch(m, n, kp, 0);
     TTX_TRY(ensure(h, h->part, (size_t)sp.slabs * m * kp * sizeof(float)));
     TTX_TRY(ensure(h, h->Y, (size_t)m * kp * sizeof(float)));
@@ -410,40 +504,47 @@ static ttx_status run_step(Handle* h, const StepArgs& a_in, StepOut& o) {
     TTX_TRY(ensure(h, h->Qc, (size_t)kp * m * sizeof(float)));
     TTX_TRY(ensure(h, h->Qp, (size_t)kp * m * sizeof(float)));
     TTX_TRY(ensure(h, h->Z, (size_t)n * kp * sizeof(float)));
-    float* Y = P<float>(h->Y);
-    // Y = A Omega (basicRandSVD_LU :585-620; Omega from the project's Philox stream)
-    {
-      KScope ks(h, TTX_K_SKETCH, a.m * n * 4);
-      launch_sketch(ar.range, sp, a.A, a.m, m, n, nullptr, 0, a.seed, a.stream_id, ell, P<float>(h->part), st);
+    if (pt) TTX_TRY(ensure(h, h->Yt, (size_t)m * pt * sizeof(float)));
+    // basis widths tried in order: the fixed width; in DELTA mode 16, 32, ... up to r + s; a learned
+    // prediction first, then the wider ones of that ladder
+    int widths[8];
+    int nw = 0;
+    if (pt && (a.rank_mode != TTX_RANK_FIXED || a.pred_ell > 0)) {
+      int w0 = a.pred_ell > 0 ? std::min(std::max(a.pred_ell, 1), ell) : std::min(16, ell);
+      widths[nw++] = w0;
+      for (int w = 16; w < ell && nw < 7; w *= 2)
+        if (w > w0) widths[nw++] = w;
+      if (widths[nw - 1] < ell) widths[nw++] = ell;
+      if (a.pred_ell > 0) o.fallback = 0;
+    } else {
+      widths[nw++] = ell;
     }
-    {
-      KScope ks(h, TTX_K_SMALL, 0);
-      launch_reduce_f32(P<float>(h->part), sp.slabs, (int64_t)m * kp, Y, st);
+    double rel2 = -1.0;
+    int wi = 0;
+    k = widths[0];
+    TTX_TRY(range_sketch(h, a, ar, k, pt, kp, sp, o));
+    if (pt) {
+      TTX_TRY(range_check(h, a, k, pt, &rel2, o));
+      while (rel2 > a.range_tol2 && wi + 1 < nw) {
+        if (a.pred_ell > 0) o.fallback = 1;
+        k = widths[++wi];
+        TTX_TRY(range_sketch(h, a, ar, k, pt, kp, sp, o));
+        TTX_TRY(range_check(h, a, k, pt, &rel2, o));
+      }
     }
-    TTX_TRY(check_launch(h, "sketch"));
-    o.bytes += a.m * n * 4;
-    TTX_TRY(orth(h, left_mix(h, a, Y, kp), kp, m, k, kp, P<float>(h->Qc), a.stat_dev));
-    // power iterations (:628-683): Z = A^T Q, Y = A Z, Q = orth(Y)
-    for (int it = 0; it < a.q; ++it) {
-      const float* Qop = proj_operand(h, a, P<float>(h->Qc), kp);
-      {
-        KScope ks(h, TTX_K_PROJ, a.m * n * 4 + n * kp * 4);
-        launch_proj(ar.range, kp, a.A, a.m, m, n, Qop, m, 0, P<float>(h->Z), kp, 0, st);
-      }
-      {
-        KScope ks(h, TTX_K_SKETCH, a.m * n * 4 + n * kp * 4);
-        launch_sketch(ar.range, sp, a.A, a.m, m, n, P<float>(h->Z), kp, 0, 0, ell, P<float>(h->part), st);
-      }
-      {
-        KScope ks(h, TTX_K_SMALL, 0);
-        launch_reduce_f32(P<float>(h->part), sp.slabs, (int64_t)m * kp, Y, st);
-      }
-      TTX_TRY(check_launch(h, "power iteration"));
-      o.bytes += 2 * a.m * n * 4;
-      TTX_TRY(orth(h, left_mix(h, a, Y, kp), kp, m, k, kp, P<float>(h->Qc), a.stat_dev));
+    // power iterations (basicRandSVD_LU :628-683): Z = A^T Q, Y = A Z, Q = orth(Y); adaptive: only
+    // while the check fails, q at most
+    int it = 0;
+    while (it < a.q && (!pt || rel2 > a.range_tol2)) {
+      TTX_TRY(power_iter(h, a, ar, k, kp, sp, o));
+      ++it;
+      if (pt) TTX_TRY(range_check(h, a, k, pt, &rel2, o));
     }
-    o.power_iters = a.q;
-    TTX_TRY(step_tail(h, a, ar, k, kp, o));
+    if (pt && a.pred_ell > 0 && it > 0) o.fallback = 1;
+    o.power_iters = it;
+    o.ell = k;
+    o.resid = rel2 >= 0 ? std::sqrt(rel2) : -1.0;
+    TTX_TRY(step_tail(h, a, ar, k, kp, rel2, o));
   }
   o.kdim = k;
   if (a.sync) {
@@ -556,6 +657,12 @@ static ttx_status sweep(Handle* h, const float* d_A, const ttx_params* p, float*
     a.sig_dev = sig_all + (size_t)k * TTX_MAX_ELL;
     a.stat_dev = stat_all + 4 * k;
     a.sync = host_ranks;
+    // range-check tolerance: a quarter of the step's share eps^2 / (d-1) of the squared error budget
+    // (Oseledets 2011: the squared step errors add up), relative to ||A_k||_F^2
+    if (p->rank_mode == TTX_RANK_FIXED || norm2 <= 0.0)
+      a.range_tol2 = 0.25 * p->eps * p->eps / (double)(d - 1);
+    else
+      a.range_tol2 = 0.25 * delta2 / norm2;
     const bool last = (k == d - 2);
     // A full-rank exact step (r = m, square orthogonal core) that is not the last one is not
     // projected: the next step reads this step's input with the core folded into its small factors.
@@ -579,6 +686,10 @@ static ttx_status sweep(Handle* h, const float* d_A, const ttx_params* p, float*
     R.path[k] = o.path;
     R.power_iters[k] = o.power_iters;
     R.bytes_alg += o.bytes;
+    R.range_resid[k] = o.resid;
+    R.range_checks[k] = o.checks;
+    R.passes[k] = o.passes;
+    R.fallback[k] = o.fallback;
     if (host_ranks) {
       R.norm2_in[k] = (o.total2 >= 0) ? o.total2 : norm2;
       norm2 = o.kept2;
@@ -844,7 +955,8 @@ ttx_status ttx_create(ttx_handle* out, void* stream) {
   if (hipGetDevice(&h->dev) != hipSuccess ||
       hipHostMalloc((void**)&h->h_status, 16 * sizeof(int), 0) != hipSuccess ||
       hipHostMalloc((void**)&h->h_sigma, TTX_MAX_ELL * sizeof(float), 0) != hipSuccess ||
-      hipHostMalloc((void**)&h->h_lam, TTX_MAX_ELL * sizeof(double), 0) != hipSuccess) {
+      hipHostMalloc((void**)&h->h_lam, TTX_MAX_ELL * sizeof(double), 0) != hipSuccess ||
+      hipHostMalloc((void**)&h->h_aux, 16 * sizeof(double), 0) != hipSuccess) {
     (void)hipGetLastError();
     delete h;
     *out = nullptr;
@@ -862,7 +974,8 @@ ttx_status ttx_destroy(ttx_handle hh) {
   ttx::DevBuf* bufs[] = {&h->part, &h->Y, &h->Y2, &h->Qc, &h->Qp, &h->Qn, &h->Z, &h->W1, &h->W2, &h->G, &h->G2,
                          &h->Rinv, &h->lam,
                          &h->V, &h->Ur, &h->gpart, &h->status, &h->next[0], &h->next[1], &h->sig, &h->keys,
-                         &h->scratch[0], &h->scratch[1], &h->scratch[2], &h->scratch[3]};
+                         &h->scratch[0], &h->scratch[1], &h->scratch[2], &h->scratch[3], &h->Yt, &h->tpart,
+                         &h->tout};
   for (auto* b : bufs)
     if (b->p) (void)hipFree(b->p);
   for (float* c : h->last_cores) (void)hipFree(c);
@@ -870,6 +983,7 @@ ttx_status ttx_destroy(ttx_handle hh) {
   if (h->h_status) (void)hipHostFree(h->h_status);
   if (h->h_sigma) (void)hipHostFree(h->h_sigma);
   if (h->h_lam) (void)hipHostFree(h->h_lam);
+  if (h->h_aux) (void)hipHostFree(h->h_aux);
   delete h;
   return TTX_OK;
 }

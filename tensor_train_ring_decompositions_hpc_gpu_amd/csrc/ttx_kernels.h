@@ -75,6 +75,13 @@ void launch_next_from_z(const float* Z, int64_t ldz, int64_t n, int k, const flo
 void launch_blockmix(const float* G, int ms, int m, int kp, int mode, const float* X, float* Y, hipStream_t st);
 // Ca = P Cs P^T (m x m, FP64)
 void launch_gram_mix(const float* G, int ms, int m, const double* Cs, double* Ca, hipStream_t st);
+// a-posteriori range check: out[2 b] = ||(I - Q Q^T) Yt||^2 and out[2 b + 1] = ||Yt||^2 over 64-row block
+// b (panel_blocks(m) blocks); Q = first k columns of Qc (column-major, ldq), Yt m x p row-major (p <= 16);
+// part: panel_blocks(m) * k * p doubles of scratch
+void launch_test_resid(const float* Qc, int64_t ldq, int m, int k, const float* Yt, int p, double* part, double* out,
+                       hipStream_t st);
+// Yt (m x p row-major) <- columns c0 .. c0+p-1 of Y (m x kpy row-major)
+void launch_copy_block(const float* Y, int kpy, int m, int c0, int p, float* Yt, hipStream_t st);
 // Y (m x kp row-major) <- A (m x n column-major, n <= kp), zero columns n..kp-1
 void launch_copy_cols(const float* A, int64_t lda, int m, int n, float* Y, int kp, hipStream_t st);
 

@@ -98,15 +98,17 @@ __device__ __forceinline__ void sketch_compute(const float (&a)[8][4], const flo
       float v[8];
 #pragma unroll
       for (int t = 0; t < 8; ++t) v[t] = a[t][s];
-      bf16x8 ah, al;
-      if (AR == AR_BF16X3) split8(v, ah, al);
+      bf16x8 ah, am, al;
+      if (AR == AR_BF16X3 && OMEGA) split8x3(v, ah, am, al);
+      else if (AR == AR_BF16X3) split8(v, ah, al);
       else round8(v, ah);
 #pragma unroll
       for (int lt = 0; lt < LT; ++lt) {
         S.acc[s][lt] = mfma_bf16(ah, bh[lt], S.acc[s][lt]);
         if (AR == AR_BF16X3) {
           S.acc[s][lt] = mfma_bf16(al, bh[lt], S.acc[s][lt]);
-          if (!OMEGA) S.acc[s][lt] = mfma_bf16(ah, bl[lt], S.acc[s][lt]);
+          if (OMEGA) S.acc[s][lt] = mfma_bf16(am, bh[lt], S.acc[s][lt]);
+          else S.acc[s][lt] = mfma_bf16(ah, bl[lt], S.acc[s][lt]);
         }
       }
     }

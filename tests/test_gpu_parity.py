@@ -207,3 +207,96 @@ def test_tt_svd_full_rank_first_step(engine, prec, dims, ranks, s, q):
     for k in range(len(dims) - 1):
         g, shape = cores[k]
         assert rel(fmat(g, *shape), c_ref[k]) < 10 * tol["vec"], k
+
+
+# ------------------------------------------------------------------------------------------------
+# adaptive range finder: the engine takes the oracle's decisions (widths, power iterations, checks)
+# ------------------------------------------------------------------------------------------------
+def _flags():
+    from tensor_train_ring_decompositions_hpc_gpu_amd import _lib as L
+    return L.FLAG_ADAPTIVE
+
+
+def _check_adaptive(rep, infos, p_ref):
+    for k, inf in enumerate(infos):
+        if p_ref[k] != "rsvd":
+            assert rep["range_checks"][k] == 0, k
+            continue
+        assert rep["path"][k] == 2, k
+        assert rep["ell"][k] == inf["ell"], (k, rep["ell"], inf)
+        assert rep["power_iters"][k] == inf["power_iters"], (k, rep["power_iters"], inf)
+        assert rep["range_checks"][k] == inf["checks"], (k, rep["range_checks"], inf)
+        got, want = rep["range_resid"][k] ** 2, inf["rel2"]
+        assert abs(got - want) <= 0.05 * want + 1e-10, (k, got, want)
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16"])
+def test_adaptive_fixed_rank_stops_at_q0(engine, prec):
+    dims, ranks = [6, 8, 8, 8, 8], [1, 5, 6, 6, 5, 1]
+    A = engine.generate(dims, ranks, std=1.0, noise_rel=1e-6, seed=21)
+    cores, sig, rep = engine.tt_svd(A, dims, ranks, s=6, q=2, precision=prec, flags=_flags())
+    An = host(A).astype(np.float64)
+    infos = []
+    c_ref, s_ref, p_ref, used = ttsvd.tt_svd(An, dims, ranks, s=6, q=2, adaptive=True, infos=infos)
+    assert rep["ranks"] == used
+    _check_adaptive(rep, infos, p_ref)
+    for k, p in enumerate(p_ref):
+        if p == "rsvd":
+            assert rep["power_iters"][k] == 0 and rep["passes"][k] == 2, k
+    tol = TOL[prec]
+    for k in range(len(dims) - 1):
+        r = ranks[k + 1]
+        assert np.allclose(sig[k][:r], s_ref[k][:r], rtol=tol["sig"], atol=tol["sig"] * s_ref[k][0]), k
+    err = engine.rel_error(cores, dims, used, A)
+    err_ref = ttsvd.rel_error(An, ttsvd.tt_full(c_ref, dims, used))
+    assert abs(err - err_ref) <= tol["err"] + 2e-2 * err_ref, (err, err_ref)
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16"])
+def test_adaptive_delta_grows_basis(engine, prec):
+    # true ranks 20 under a ceiling of 32: the 16-column basis fails its check, 32 columns pass
+    dims, true = [4, 12, 12, 12, 4], [1, 4, 20, 20, 4, 1]
+    A = engine.generate(dims, true, std=1.0, noise_rel=1e-6, seed=22)
+    ceil = [1, 4, 32, 32, 4, 1]
+    cores, sig, rep = engine.tt_svd(A, dims, ceil, s=10, q=2, precision=prec, rank_mode="delta", eps=1e-3,
+                                    flags=_flags())
+    An = host(A).astype(np.float64)
+    infos = []
+    c_ref, s_ref, p_ref, used = ttsvd.tt_svd(An, dims, ceil, s=10, q=2, rank_mode="delta", eps=1e-3,
+                                             adaptive=True, infos=infos)
+    assert used == true
+    assert rep["ranks"] == used
+    _check_adaptive(rep, infos, p_ref)
+    assert [inf["ell"] for inf in infos if inf] == [32, 32]
+    err = engine.rel_error(cores, dims, used, A)
+    assert err < 1e-3
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16"])
+@pytest.mark.parametrize("case", ["one_iteration", "ceiling"])
+def test_adaptive_power_iterations(engine, prec, case):
+    # 8 strong directions over a flat tail (one step, d = 2): q = 0 leaves a tail-sized residual
+    rng = np.random.default_rng(9)
+    m, n = 300, 800
+    sig = np.concatenate([np.ones(8), np.full(200, 3e-3)])
+    U = np.linalg.qr(rng.standard_normal((m, len(sig))))[0]
+    V = np.linalg.qr(rng.standard_normal((n, len(sig))))[0]
+    A64 = (U * sig) @ V.T
+    A32 = A64.astype(np.float32)
+    An = A32.astype(np.float64)
+    r0 = ttsvd.adaptive_range(An, [10], 8, 0, 0.0, 1234, 0)[3]
+    r1 = ttsvd.adaptive_range(An, [10], 8, 1, 0.0, 1234, 0)[3]
+    assert r1 < 0.5 * r0
+    tol2 = np.sqrt(r0 * r1) if case == "one_iteration" else 1e-3 * r1
+    eps = float(np.sqrt(4.0 * tol2))     # d = 2: tol2 = eps^2 / 4
+    cores, s_gpu, rep = engine.tt_svd(dev(A32), [m, n], [1, 8, 1], s=2, q=3, precision=prec, eps=eps,
+                                      flags=_flags())
+    infos = []
+    c_ref, s_ref, p_ref, used = ttsvd.tt_svd(An, [m, n], [1, 8, 1], s=2, q=3, eps=eps, adaptive=True,
+                                             infos=infos)
+    want_it = 1 if case == "one_iteration" else 3
+    assert infos[0]["power_iters"] == want_it
+    _check_adaptive(rep, infos, p_ref)
+    assert rep["passes"][0] == 2 + 2 * want_it
+    tol = TOL[prec]
+    assert np.allclose(s_gpu[0][:8], s_ref[0][:8], rtol=tol["sig"], atol=tol["sig"] * s_ref[0][0])

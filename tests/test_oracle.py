@@ -246,3 +246,88 @@ def test_noise_and_generator_definitions():
     An = ttsvd.add_noise(A, 1e-3)
     rel = np.linalg.norm(An.astype(np.float64) - A) / np.linalg.norm(A)
     assert 0.8e-3 < rel < 1.2e-3
+
+
+# ------------------------------------------------------------------------------------------------
+# adaptive range finder (a-posteriori check on held-out Gaussian test columns)
+# ------------------------------------------------------------------------------------------------
+def _spectral(m, n, sig, seed):
+    rng = np.random.default_rng(seed)
+    U = np.linalg.qr(rng.standard_normal((m, len(sig))))[0]
+    V = np.linalg.qr(rng.standard_normal((n, len(sig))))[0]
+    return (U * np.asarray(sig)) @ V.T
+
+
+def test_range_resid2_is_unbiased():
+    # E ||(I - QQ^T) A w||^2 = ||(I - QQ^T) A||_F^2 for w ~ N(0, I): average over many test columns
+    rng = np.random.default_rng(3)
+    A = _spectral(60, 400, np.linspace(1.0, 0.05, 30), 4)
+    Q = np.linalg.qr(rng.standard_normal((60, 12)))[0]
+    W = rng.standard_normal((400, 20000))
+    est = ttsvd.range_resid2(Q, A @ W)
+    R = A - Q @ (Q.T @ A)
+    exact = np.sum(R * R) / np.sum(A * A)
+    assert abs(est - exact) < 0.03 * exact
+
+
+def test_range_resid2_closed_forms():
+    rng = np.random.default_rng(5)
+    Q = np.linalg.qr(rng.standard_normal((40, 6)))[0]
+    Yin = Q @ rng.standard_normal((6, 8))          # inside span(Q): zero residual
+    assert ttsvd.range_resid2(Q, Yin) < 1e-28
+    P = np.linalg.qr(np.hstack([Q, rng.standard_normal((40, 8))]))[0][:, 6:]
+    Yout = P @ rng.standard_normal((8, 8))         # orthogonal to span(Q): residual ratio 1
+    assert abs(ttsvd.range_resid2(Q, Yout) - 1.0) < 1e-12
+
+
+def test_adaptive_widths_ladder():
+    assert ttsvd.adaptive_widths(42, "fixed") == [42]
+    assert ttsvd.adaptive_widths(42, "delta") == [16, 32, 42]
+    assert ttsvd.adaptive_widths(56, "delta") == [16, 32, 56]
+    assert ttsvd.adaptive_widths(12, "delta") == [12]
+    assert ttsvd.adaptive_widths(42, "delta", pred_ell=20) == [20, 32, 42]
+    assert ttsvd.adaptive_widths(42, "fixed", pred_ell=40) == [40, 42]
+
+
+def test_adaptive_range_exact_rank_needs_no_power_iteration():
+    A = _spectral(80, 500, np.logspace(0, -3, 10), 7)
+    Q, w, it, rel2, checks = ttsvd.adaptive_range(A, [16], 8, 2, 1e-12, 1234, 1)
+    assert (w, it, checks) == (16, 0, 1) and rel2 < 1e-26
+    assert np.allclose(Q @ (Q.T @ A), A, atol=1e-12)
+
+
+def test_adaptive_range_grows_to_the_rank():
+    # rank 24: a 16-column basis misses a third of the spectrum, 32 columns capture it exactly
+    A = _spectral(80, 600, np.ones(24), 8)
+    Q, w, it, rel2, checks = ttsvd.adaptive_range(A, [16, 32, 42], 8, 2, 1e-10, 1234, 2)
+    assert (w, it, checks) == (32, 0, 2) and rel2 < 1e-24
+
+
+def test_adaptive_range_power_iteration_then_stop():
+    # 8 strong directions over a flat tail: q = 0 leaves a tail-sized residual that a power
+    # iteration removes, and the check stops after the first one
+    sig = np.concatenate([np.ones(8), np.full(200, 3e-3)])
+    A = _spectral(300, 800, sig, 9)
+    Q0, _, _, r0, _ = ttsvd.adaptive_range(A, [10], 8, 0, 0.0, 1234, 3)
+    Q1, _, _, r1, _ = ttsvd.adaptive_range(A, [10], 8, 1, 0.0, 1234, 3)
+    assert r1 < 0.3 * r0
+    best = 198 * 9e-6 / (8 + 200 * 9e-6)          # the optimal 10-dimensional range leaves 198 tail values
+    assert abs(r1 - best) < 0.05 * best
+    tol2 = np.sqrt(r0 * r1)
+    Q, w, it, rel2, checks = ttsvd.adaptive_range(A, [10], 8, 3, tol2, 1234, 3)
+    assert (it, checks) == (1, 2) and rel2 == r1
+
+
+def test_adaptive_tt_svd_matches_fixed_width_sweep():
+    # an exact-rank tensor: the adaptive sweep stops at q = 0 and equals the q = 0 sweep
+    dims, ranks = [6, 8, 8, 8, 8], [1, 5, 6, 6, 5, 1]
+    cores = ttsvd.make_tt_cores(dims, ranks, std=1.0, seed=21)
+    A = ttsvd.tt_full(cores, dims, ranks)
+    infos = []
+    c1, s1, p1, u1 = ttsvd.tt_svd(A, dims, ranks, s=6, q=2, adaptive=True, infos=infos)
+    c0, s0, p0, u0 = ttsvd.tt_svd(A, dims, ranks, s=6, q=0)
+    assert u1 == u0 and p1 == p0
+    for k, inf in enumerate(infos):
+        if p1[k] == "rsvd":
+            assert inf["power_iters"] == 0 and inf["checks"] == 1
+        assert np.allclose(s1[k][:ranks[k + 1]], s0[k][:ranks[k + 1]], rtol=1e-10)
