@@ -1,0 +1,42 @@
+/* Reference-name drop-in shims.
+ *
+ * Two libraries export the reference's six-mode entry points with their exact C++ signatures; the
+ * TT and TR versions carry the same names in the reference, so each lives in its own library:
+ *
+ *   libttx_ttdec6.so  <- ttdecomposition/high_order_GPUs/ttdec6_singleGPU.cu:656 (_host_6) and :799 (_device_6)
+ *   libttx_trdec6.so  <- trdecomposition/high_order_GPUs/trdec6_singleGPU.cu:668 (_host_6) and :827 (_device_6)
+ *
+ *   float ttdec_half_lu_device_6(float* h_A, const int* ttRanks, const int* ttDemns,
+ *                                const int& calculateTimes, const int& p, bool calError);
+ *   float ttdec_half_lu_host_6(float* h_A, int* ttRanks, const int* ttDemns,
+ *                              const int& calculateTimes, const int& p, const bool calError);
+ *
+ * Semantics kept from the reference: h_A is the caller's host tensor (column-major, 6 modes,
+ * ttDemns[0..5]); ttRanks[0..6] are the TT ranks (1, r1..r5, 1) or the TR ranks (r0, r1..r5, r0,
+ * first step of rank r0 r1); the timed loop runs calculateTimes decompositions, each including the
+ * host-to-device copy of h_A, and the function returns the mean seconds per decomposition; with
+ * calError the relative reconstruction error ||A - A_hat||_F / ||A||_F is printed per run, as the
+ * reference's calMSE line.  p (the reference's GEMM slice count) has no role on this engine and is
+ * ignored.  Differences: h_A is never overwritten; errors print a message and return -1 instead of
+ * aborting through assert; the cores are not freed but kept for ttx_last_cores.
+ *
+ * Engine settings (environment, read at each call): TTX_PRECISION = fp32 | bf16 (default) | fp16,
+ * TTX_OVERSAMPLING (default 10), TTX_POWER_ITERS (default 2, the ceiling of the adaptive range
+ * check), TTX_ADAPTIVE = 0 | 1 (default 1).
+ */
+#pragma once
+#include <stdint.h>
+
+float ttdec_half_lu_device_6(float* h_A, const int* ttRanks, const int* ttDemns, const int& calculateTimes,
+                             const int& p, bool calError);
+float ttdec_half_lu_host_6(float* h_A, int* ttRanks, const int* ttDemns, const int& calculateTimes, const int& p,
+                           const bool calError);
+
+extern "C" {
+/* Side channel for the cores the reference discards (ttdec6_singleGPU.cu:947-952): device pointers
+ * of the last call's d cores (TT or TR layouts, ttx.h) and its ranks (d + 1 entries); valid until
+ * the next shim call.  Returns the order d, 0 when no call has succeeded. */
+int32_t ttx_last_cores(float* const** d_cores, int32_t* ranks);
+/* relative reconstruction error of the last call (computed only with calError; -1 otherwise) */
+double ttx_last_rel_error(void);
+}

@@ -1,0 +1,17 @@
+// TR entry points with the reference's names and C++ signatures
+// (trdecomposition/high_order_GPUs/trdec6_singleGPU.cu:668, :827); see include/ttx_shim.h.
+#include "shim_common.h"
+
+float ttdec_half_lu_device_6(float* h_A, const int* ttRanks, const int* ttDemns, const int& calculateTimes,
+                             const int& p, bool calError) {
+  (void)p;
+  return ttx_shim::run(h_A, ttRanks, ttDemns, calculateTimes, calError, true);
+}
+
+// device-resident sweep (the reference's host-streamed variant, trdec6_singleGPU.cu:668-826, exists
+// for tensors larger than its device memory)
+float ttdec_half_lu_host_6(float* h_A, int* ttRanks, const int* ttDemns, const int& calculateTimes, const int& p,
+                           const bool calError) {
+  (void)p;
+  return ttx_shim::run(h_A, ttRanks, ttDemns, calculateTimes, calError, true);
+}
