@@ -142,84 +142,69 @@ __device__ __forceinline__ void sum_slabs(const double* __restrict__ part, int64
 
 // ------------------------------------------------------------------------------------------------
 // sum S Gram partials, (shifted) Cholesky G + s I = R^T R, explicit Rinv = R^-1 (upper, row-major)
+//
+// Right-looking elimination on the upper triangle: step i scales row i by 1/sqrt(pivot) (it becomes
+// row i of R) and subtracts R[i][l] R[i][j] from the trailing entries.  The same row operations
+// applied to B = I give B = L^-1 with L = R^T, so Rinv = B^T comes out of the same sweep: every
+// step is two parallel phases separated by barriers, with O(1) dependent latency per step.
 // ------------------------------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_chol_rinv(const double* __restrict__ part, int64_t S, int k, int64_t m,
                                                    int shift_mode, double* __restrict__ Rinv,
                                                    int* __restrict__ status) {
   __shared__ double a[KMAX][KMAX + 1];
-  __shared__ double R[KMAX][KMAX + 1];
-  __shared__ double X[KMAX][KMAX + 1];
-  __shared__ double rinvd[KMAX];
+  __shared__ double B[KMAX][KMAX + 1];
   __shared__ double shift_s, tiny_s;
   const int tid = threadIdx.x;
   sum_slabs(part, S, k, a);
-  for (int e = tid; e < k * k; e += 256) {
-    const int i = e / k, j = e - (e / k) * k;
-    R[i][j] = 0.0;
-    X[i][j] = 0.0;
+  for (int e = tid; e < KMAX * KMAX; e += 256) {
+    const int i = e >> 6, j = e & 63;
+    B[i][j] = (i == j) ? 1.0 : 0.0;
   }
   __syncthreads();
-  if (tid == 0) {
-    double tr = 0.0, md = 0.0;
-    for (int i = 0; i < k; ++i) { tr += a[i][i]; md = fmax(md, a[i][i]); }
-    // shifted CholeskyQR (Fukaya, Kannan, Nakatsukasa, Yamamoto, Yanagisawa 2020):
-    // s = 11 (m k + k (k+1)) u ||X||_2^2 with ||X||_2^2 <= trace(X^T X)
-    shift_s = shift_mode ? 11.0 * ((double)m * k + (double)k * (k + 1)) * 1.1102230246251565e-16 * tr : 0.0;
-    tiny_s = (md + shift_s) * 1e-28;
+  if (tid < 64) {
+    double tr = tid < k ? a[tid][tid] : 0.0, md = tr;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      tr += __shfl_xor(tr, o);
+      md = fmax(md, __shfl_xor(md, o));
+    }
+    if (tid == 0) {
+      // shifted CholeskyQR (Fukaya, Kannan, Nakatsukasa, Yamamoto, Yanagisawa 2020):
+      // s = 11 (m k + k (k+1)) u ||X||_2^2 with ||X||_2^2 <= trace(X^T X)
+      shift_s = shift_mode ? 11.0 * ((double)m * k + (double)k * (k + 1)) * 1.1102230246251565e-16 * tr : 0.0;
+      tiny_s = (md + shift_s) * 1e-28;
+    }
   }
   __syncthreads();
   const double sh = shift_s, tiny = tiny_s;
-  // up-looking Cholesky: row i of R from rows 0..i-1; every thread recomputes the pivot so one
-  // barrier per row suffices
+  const int tx = tid & 15, ty = tid >> 4;
   int nzero = 0;
   for (int i = 0; i < k; ++i) {
-    // dot products with 4 independent partial sums so several LDS loads are in flight
-    double d0 = a[i][i] + sh, d1 = 0.0, d2 = 0.0, d3 = 0.0;
-    int l = 0;
-    for (; l + 4 <= i; l += 4) {
-      d0 -= R[l][i] * R[l][i]; d1 -= R[l + 1][i] * R[l + 1][i];
-      d2 -= R[l + 2][i] * R[l + 2][i]; d3 -= R[l + 3][i] * R[l + 3][i];
-    }
-    for (; l < i; ++l) d0 -= R[l][i] * R[l][i];
-    const double d = (d0 + d1) + (d2 + d3);
+    const double d = a[i][i] + sh;
     const bool ok = d > tiny;
+    const double inv = ok ? 1.0 / sqrt(d) : 0.0;
     if (!ok) ++nzero;
-    const double rii = ok ? sqrt(d) : 0.0;
-    const double inv = ok ? 1.0 / rii : 0.0;
-    for (int j = i + tid; j < k; j += 256) {
-      if (j == i) {
-        R[i][i] = rii;
-      } else {
-        double s0 = a[i][j], s1 = 0.0, s2 = 0.0, s3 = 0.0;
-        int l2 = 0;
-        for (; l2 + 4 <= i; l2 += 4) {
-          s0 -= R[l2][i] * R[l2][j]; s1 -= R[l2 + 1][i] * R[l2 + 1][j];
-          s2 -= R[l2 + 2][i] * R[l2 + 2][j]; s3 -= R[l2 + 3][i] * R[l2 + 3][j];
-        }
-        for (; l2 < i; ++l2) s0 -= R[l2][i] * R[l2][j];
-        R[i][j] = ((s0 + s1) + (s2 + s3)) * inv;
-      }
+    // phase 1: row i of R (strict upper part; the diagonal is kept apart) and row i of B
+    if (tid < k) {
+      if (tid > i) a[i][tid] *= inv;
+      else B[i][tid] *= inv;
     }
-    if (tid == 0) rinvd[i] = inv;
     __syncthreads();
-  }
-  // Rinv by rows from the bottom: X[i][c] = (delta_ic - sum_{i<l<=c} R[i][l] X[l][c]) / R[i][i];
-  // all columns of one row in parallel
-  for (int i = k - 1; i >= 0; --i) {
-    const double ri = rinvd[i];
-    for (int c = i + tid; c < k; c += 256) {
-      double s0 = (c == i) ? 1.0 : 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
-      int l = i + 1;
-      for (; l + 3 <= c; l += 4) {
-        s0 -= R[i][l] * X[l][c]; s1 -= R[i][l + 1] * X[l + 1][c];
-        s2 -= R[i][l + 2] * X[l + 2][c]; s3 -= R[i][l + 3] * X[l + 3][c];
+    // phase 2: trailing rows l > i
+    for (int l = i + 1 + ty; l < k; l += 16) {
+      const double f = a[i][l];
+      for (int x = tx; x < k; x += 16) {
+        if (x <= i) B[l][x] -= f * B[i][x];
+        else if (x >= l) a[l][x] -= f * a[i][x];
       }
-      for (; l <= c; ++l) s0 -= R[i][l] * X[l][c];
-      X[i][c] = ((s0 + s1) + (s2 + s3)) * ri;
     }
     __syncthreads();
   }
-  for (int e = tid; e < k * k; e += 256) Rinv[e] = X[e / k][e - (e / k) * k];
+  // Rinv[i][c] = B[c][i] (upper triangle)
+  for (int e = tid; e < k * k; e += 256) {
+    const int i = e / k, c = e - (e / k) * k;
+    Rinv[e] = (c >= i) ? B[c][i] : 0.0;
+  }
   if (tid == 0 && status) status[0] += nzero;
 }
 
@@ -327,7 +312,8 @@ __global__ __launch_bounds__(1024) void k_syevj(const double* __restrict__ G, in
   const int k2 = k + (k & 1);   // an odd size gets one zero row / column (its rotations are identities)
   for (int e = tid; e < k2 * k2; e += 1024) {
     const int i = e / k2, j = e % k2;
-    a[i][j] = (i < k && j < k) ? G[i * k + j] : 0.0;
+    // the Gram passes accumulate (i, j) and (j, i) in different orders: use the symmetric part
+    a[i][j] = (i < k && j < k) ? 0.5 * (G[i * k + j] + G[j * k + i]) : 0.0;
     v[i][j] = (i == j) ? 1.0 : 0.0;
   }
   __syncthreads();
@@ -337,25 +323,24 @@ __global__ __launch_bounds__(1024) void k_syevj(const double* __restrict__ G, in
   int nblk = 0, nvi = 0;
   for (int e = tid; e < np * np && nblk < 1; e += 1024) { bu[nblk] = e / np; bw[nblk] = e % np; ++nblk; }
   for (int e = tid; e < k2 * np && nvi < 2; e += 1024) { vi[nvi] = e / np; vw[nvi] = e % np; ++nvi; }
-  int sweeps = 0;
-  for (; sweeps < 30; ++sweeps) {
-    double off = 0.0, tot = 0.0;
-    for (int e = tid; e < k2 * k2; e += 1024) {
-      const int i = e / k2, j = e % k2;
-      const double x = a[i][j] * a[i][j];
-      tot += x;
-      if (i != j) off += x;
-    }
-    red[0][tid] = off;
-    red[1][tid] = tot;
+  // ||A||_F^2 is invariant under the rotations: one reduction up front
+  {
+    double tot = 0.0;
+    for (int e = tid; e < k2 * k2; e += 1024) tot += a[e / k2][e % k2] * a[e / k2][e % k2];
+    red[0][tid] = tot;
     __syncthreads();
     for (int s = 512; s > 0; s >>= 1) {
-      if (tid < s) { red[0][tid] += red[0][tid + s]; red[1][tid] += red[1][tid + s]; }
+      if (tid < s) red[0][tid] += red[0][tid + s];
       __syncthreads();
     }
-    if (tid == 0) done = (red[0][0] <= 1e-24 * red[1][0]) || red[1][0] == 0.0;
-    __syncthreads();
-    if (done) break;
+  }
+  const double tot = red[0][0];
+  int sweeps = 0;
+  while (sweeps < 30 && tot > 0.0) {
+    ++sweeps;
+    // sum of the squared pivots a_pq^2 met during the sweep (each off-diagonal pair once): the
+    // sweep's off-diagonal mass, accumulated by the angle threads (all in the first wave)
+    double offacc = 0.0;
     for (int rnd = 0; rnd < k2 - 1; ++rnd) {
       if (tid < np) {
         int x, y;
@@ -366,6 +351,7 @@ __global__ __launch_bounds__(1024) void k_syevj(const double* __restrict__ G, in
         }
         const int p = x < y ? x : y, q = x < y ? y : x;
         const double apq = a[p][q];
+        offacc += apq * apq;
         double c = 1.0, s = 0.0;
         if (apq != 0.0) {
           // the angle only needs FP32 accuracy (every applied rotation is re-normalised in FP64)
@@ -408,6 +394,13 @@ __global__ __launch_bounds__(1024) void k_syevj(const double* __restrict__ G, in
       }
       __syncthreads();
     }
+    if (tid < 64) {
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) offacc += __shfl_xor(offacc, o);
+      if (tid == 0) done = offacc <= 0.5e-24 * tot;
+    }
+    __syncthreads();
+    if (done) break;
   }
   // sort descending: rank of each eigenvalue (ties broken by index)
   for (int i = tid; i < k; i += 1024) {
@@ -519,38 +512,42 @@ void launch_assemble(const float* Qc, int64_t ldq, int m, int k, const double* l
 template <int RM>
 __global__ __launch_bounds__(256) void k_next_from_z(const float* __restrict__ Z, int64_t ldz, int64_t n, int k,
                                                      const float* __restrict__ Ur, int r, float* __restrict__ out) {
-  __shared__ float us[KMAX * RM];
-  for (int e = threadIdx.x; e < k * RM; e += 256) {
+  // thread -> (row jj of this WG's 256 / RM rows, output column c): a wave writes whole consecutive
+  // output rows (coalesced); the RM lanes of one row read the same Z row with 16-byte loads
+  __shared__ float us[(KMAX + 4) * RM];
+  const int k4 = (k + 3) & ~3;
+  for (int e = threadIdx.x; e < k4 * RM; e += 256) {
     const int a2 = e / RM, c = e % RM;
-    us[e] = (c < r) ? Ur[a2 * r + c] : 0.0f;
+    us[e] = (a2 < k && c < r) ? Ur[a2 * r + c] : 0.0f;
   }
   __syncthreads();
-  for (int64_t j = blockIdx.x * 256 + threadIdx.x; j < n; j += (int64_t)gridDim.x * 256) {
-    float acc[RM];
-#pragma unroll
-    for (int c = 0; c < RM; ++c) acc[c] = 0.0f;
-    const float* z = Z + j * ldz;
-    for (int a2 = 0; a2 < k; ++a2) {
-      const float zv = z[a2];
-#pragma unroll
-      for (int c = 0; c < RM; ++c) acc[c] += us[a2 * RM + c] * zv;
+  constexpr int ROWS = 256 / RM;
+  const int jj = threadIdx.x / RM, c = threadIdx.x % RM;
+  for (int64_t j = (int64_t)blockIdx.x * ROWS + jj; j < n; j += (int64_t)gridDim.x * ROWS) {
+    const f32x4* z = reinterpret_cast<const f32x4*>(Z + j * ldz);
+    float a0 = 0.0f, a1 = 0.0f;
+    for (int q = 0; q < k4 / 4; ++q) {
+      const f32x4 v = z[q];
+      a0 += us[(4 * q) * RM + c] * v[0];
+      a1 += us[(4 * q + 1) * RM + c] * v[1];
+      a0 += us[(4 * q + 2) * RM + c] * v[2];
+      a1 += us[(4 * q + 3) * RM + c] * v[3];
     }
-    float* o = out + j * r;
-#pragma unroll
-    for (int c = 0; c < RM; ++c)
-      if (c < r) o[c] = acc[c];
+    if (c < r) out[j * r + c] = a0 + a1;
   }
 }
 
 void launch_next_from_z(const float* Z, int64_t ldz, int64_t n, int k, const float* Ur, int r, float* out,
                         hipStream_t st) {
-  int64_t grid = (n + 255) / 256;
-  const int64_t cap = (int64_t)num_cus() * 8;
+  const int RM = r <= 8 ? 8 : r <= 16 ? 16 : r <= 32 ? 32 : 64;
+  const int rows = 256 / RM;
+  int64_t grid = (n + rows - 1) / rows;
+  const int64_t cap = (int64_t)num_cus() * 16;
   if (grid > cap) grid = cap;
   if (grid < 1) grid = 1;
-  if (r <= 8) hipLaunchKernelGGL((k_next_from_z<8>), dim3((unsigned)grid), dim3(256), 0, st, Z, ldz, n, k, Ur, r, out);
-  else if (r <= 16) hipLaunchKernelGGL((k_next_from_z<16>), dim3((unsigned)grid), dim3(256), 0, st, Z, ldz, n, k, Ur, r, out);
-  else if (r <= 32) hipLaunchKernelGGL((k_next_from_z<32>), dim3((unsigned)grid), dim3(256), 0, st, Z, ldz, n, k, Ur, r, out);
+  if (RM == 8) hipLaunchKernelGGL((k_next_from_z<8>), dim3((unsigned)grid), dim3(256), 0, st, Z, ldz, n, k, Ur, r, out);
+  else if (RM == 16) hipLaunchKernelGGL((k_next_from_z<16>), dim3((unsigned)grid), dim3(256), 0, st, Z, ldz, n, k, Ur, r, out);
+  else if (RM == 32) hipLaunchKernelGGL((k_next_from_z<32>), dim3((unsigned)grid), dim3(256), 0, st, Z, ldz, n, k, Ur, r, out);
   else hipLaunchKernelGGL((k_next_from_z<64>), dim3((unsigned)grid), dim3(256), 0, st, Z, ldz, n, k, Ur, r, out);
 }
 

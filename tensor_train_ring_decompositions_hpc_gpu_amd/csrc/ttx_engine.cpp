@@ -260,13 +260,14 @@ static ttx_status step_tail(Handle* h, const StepArgs& a, const Arith3& ar, int 
   const int64_t n = a.n;
   TTX_TRY(ensure(h, h->Z, (size_t)n * kp * sizeof(float)));
   float* Z = P<float>(h->Z);
+  const int kz = (k + 3) & ~3;   // Z keeps only the basis columns (rounded to 16 bytes)
   // Z = A^T Q (FP32-accurate) and its Gram B B^T = Z^T Z in the same pass
   const int64_t gg = proj_grid(n);
   TTX_TRY(ensure(h, h->gpart, (size_t)gg * k * k * sizeof(double)));
   const float* Qop = proj_operand(h, a, P<float>(h->Qc), kp);
   {
-    KScope ks(h, TTX_K_PROJ, a.m * n * 4 + n * kp * 4);
-    launch_proj(ar.accurate, kp, a.A, a.m, m, n, Qop, m, 0, Z, kp, 0, st, P<double>(h->gpart), k);
+    KScope ks(h, TTX_K_PROJ, a.m * n * 4 + n * kz * 4);
+    launch_proj(ar.accurate, kp, a.A, a.m, m, n, Qop, m, 0, Z, kz, kz, st, P<double>(h->gpart), k);
   }
   o.bytes += (int64_t)(a.m * n * 4);
   o.passes += 1;
@@ -295,10 +296,10 @@ static ttx_status step_tail(Handle* h, const StepArgs& a, const Arith3& ar, int 
   }
   // remainder S V^T = U_r^T B = U_r^T Z^T (:734-739)
   {
-    KScope ks(h, TTX_K_TALL, n * kp * 4 + (int64_t)r * n * 4);
-    launch_next_from_z(Z, kp, n, k, P<float>(h->Ur), r, a.next, st);
+    KScope ks(h, TTX_K_TALL, n * kz * 4 + (int64_t)r * n * 4);
+    launch_next_from_z(Z, kz, n, k, P<float>(h->Ur), r, a.next, st);
   }
-  o.bytes += (int64_t)(n * kp * 4 + (int64_t)r * n * 4);
+  o.bytes += (int64_t)(n * kz * 4 + (int64_t)r * n * 4);
   o.r = r;
   return check_launch(h, "step tail assemble");
 }

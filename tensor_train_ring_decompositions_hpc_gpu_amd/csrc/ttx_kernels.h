@@ -16,7 +16,8 @@ SketchPlan plan_sketch(int m, int64_t n, int kp, int wgs_hint);
 // Y partials: part[slab][m][kp].  X == nullptr selects the in-kernel Philox sketch (seed, stream, ell).
 void launch_sketch(int ar, const SketchPlan& p, const float* A, int64_t lda, int m, int64_t n, const float* X,
                    int64_t ldx, uint64_t seed, uint32_t stream, int ell, float* part, hipStream_t st);
-// D = Q^T A; mode 0 writes Z (n x ldo row-major, kp columns), mode 1 writes r_out x n column-major.
+// D = Q^T A; mode 0 writes Z (n x ldo row-major: kp columns, or the first r_out when r_out > 0, a multiple
+// of 4), mode 1 writes r_out x n column-major.
 // gpart != nullptr: also the Gram D D^T, first kg rows / columns, one FP64 partial per WG
 // (proj_grid(n) partials).
 int64_t proj_grid(int64_t n);
@@ -67,7 +68,8 @@ void launch_syevj(const double* G, int k, double* lam, double* V, int* status, u
 void launch_assemble(const float* Qc, int64_t ldq, int m, int k, const double* lam, const double* V, int r,
                      float* Gc, float* Ur, float* Qn, int kpn, float* sigma, unsigned long long* keys,
                      hipStream_t st);
-// A_next[c + r j] = sum_a Ur[a][c] Z[j][a]  (c < r, j < n); Z row-major with stride ldz
+// A_next[c + r j] = sum_a Ur[a][c] Z[j][a]  (c < r, j < n); Z row-major with stride ldz (a multiple of 4,
+// 16-byte aligned rows, columns k .. round_up(k, 4) - 1 readable)
 void launch_next_from_z(const float* Z, int64_t ldz, int64_t n, int k, const float* Ur, int r, float* out,
                         hipStream_t st);
 // pending full-rank left factor P = I_N (x) G^T of a fused step (G: ms x ms column-major core):

@@ -376,12 +376,14 @@ __global__ __launch_bounds__(512) void k_proj(const float* __restrict__ A, int64
 
     if (j < n) {
       if (out_mode == 0) {
-        // Z = D^T, row-major n x ldo, all kp columns
+        // Z = D^T, row-major n x ldo: kp columns, or the first r_out (a multiple of 4)
         float* o = out + j * ldo;
+        const int zc = r_out > 0 ? r_out : KP;
 #pragma unroll
         for (int lt = 0; lt < LT; ++lt)
 #pragma unroll
           for (int g = 0; g < 4; ++g) {
+            if (32 * lt + 8 * g + 4 * b >= zc) continue;
             f32x4 v;
             v[0] = acc[lt][4 * g]; v[1] = acc[lt][4 * g + 1]; v[2] = acc[lt][4 * g + 2]; v[3] = acc[lt][4 * g + 3];
             *reinterpret_cast<f32x4*>(o + 32 * lt + 8 * g + 4 * b) = v;

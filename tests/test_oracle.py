@@ -238,6 +238,18 @@ def test_tr_svd_exact_ranks_reconstruct():
     assert ttsvd.rel_error(T, ttsvd.tr_full(cores, dims, used)) < 1e-12
 
 
+def test_tr_svd_order6_exact_at_r0_squared_ranks():
+    # the first step keeps an arbitrary basis of the r0 r1 column space, so the chain that follows
+    # carries the ring index twice: inner ranks r0^2 r_k reproduce a ring of ranks r_k exactly,
+    # smaller ones do not
+    dims, true = [8] * 6, [2] * 7
+    T = ttsvd.tr_full(ttsvd.make_tr_cores(dims, true, seed=1134), dims, true)
+    cores, _, _, used = ttsvd.tr_svd(T, dims, [2, 2, 8, 8, 8, 8, 2])
+    assert ttsvd.rel_error(T, ttsvd.tr_full(cores, dims, used)) < 1e-12
+    cores, _, _, used = ttsvd.tr_svd(T, dims, [2, 2, 4, 4, 4, 4, 2])
+    assert ttsvd.rel_error(T, ttsvd.tr_full(cores, dims, used)) > 1e-3
+
+
 def test_noise_and_generator_definitions():
     dims, ranks = [5, 6, 7], [1, 2, 3, 1]
     cores = ttsvd.make_tt_cores(dims, ranks, std=0.5)

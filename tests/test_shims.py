@@ -64,8 +64,10 @@ def test_shim_runs_like_reference_main(engine, kind, name):
         ranks = [1, 4, 4, 4, 4, 4, 1]
         A = engine.generate(dims, ranks, std=1.0, noise_rel=1e-6, seed=1134)
     else:
-        ranks = [2, 2, 4, 4, 4, 4, 2]
-        A = engine.generate(dims, ranks, std=1.0, noise_rel=1e-6, seed=1134, tr=True)
+        # the sequential TR-SVD reproduces a ring of ranks r_k exactly once its inner ranks reach
+        # r0^2 r_k: the first-step basis mixes the ring index into the chain index
+        ranks = [2, 2, 8, 8, 8, 8, 2]
+        A = engine.generate(dims, [2] * 7, std=1.0, noise_rel=1e-6, seed=1134, tr=True)
     hA = A.cpu().numpy()
     before = hA.copy()
     t, _ = _call(lib, name, hA, ranks, dims, times=2)
