@@ -140,6 +140,19 @@ __device__ __forceinline__ void sum_slabs(const double* __restrict__ part, int64
   }
 }
 
+// 1 / sqrt(d) for a normal d > 0 to FP64 accuracy: exact power-of-two scaling to [1, 4), the FP32
+// reciprocal square root, then two Newton steps in FP64 (each squares the relative error)
+__device__ __forceinline__ double rsqrt_f64(double d) {
+  const long long b = __double_as_longlong(d);
+  const int e = (int)((b >> 52) & 0x7FF) - 1023;
+  const int h = e >> 1;   // floor(e / 2)
+  const double ds = d * __longlong_as_double((long long)(1023 - 2 * h) << 52);
+  double r = (double)__builtin_amdgcn_rsqf((float)ds);
+  r = r * (1.5 - 0.5 * ds * r * r);
+  r = r * (1.5 - 0.5 * ds * r * r);
+  return r * __longlong_as_double((long long)(1023 - h) << 52);
+}
+
 // ------------------------------------------------------------------------------------------------
 // sum S Gram partials, (shifted) Cholesky G + s I = R^T R, explicit Rinv = R^-1 (upper, row-major)
 //
@@ -182,7 +195,7 @@ __global__ __launch_bounds__(256) void k_chol_rinv(const double* __restrict__ pa
   for (int i = 0; i < k; ++i) {
     const double d = a[i][i] + sh;
     const bool ok = d > tiny;
-    const double inv = ok ? 1.0 / sqrt(d) : 0.0;
+    const double inv = ok ? rsqrt_f64(d) : 0.0;
     if (!ok) ++nzero;
     // phase 1: row i of R (strict upper part; the diagonal is kept apart) and row i of B
     if (tid < k) {
@@ -190,13 +203,42 @@ __global__ __launch_bounds__(256) void k_chol_rinv(const double* __restrict__ pa
       else B[i][tid] *= inv;
     }
     __syncthreads();
-    // phase 2: trailing rows l > i
-    for (int l = i + 1 + ty; l < k; l += 16) {
-      const double f = a[i][l];
-      for (int x = tx; x < k; x += 16) {
-        if (x <= i) B[l][x] -= f * B[i][x];
-        else if (x >= l) a[l][x] -= f * a[i][x];
+    // phase 2: trailing rows l > i; every operand is loaded before the first store so the loads
+    // issue back to back
+    {
+      double f[4], rv[4], cur[4][4];
+      bool inB[4];
+#pragma unroll
+      for (int li = 0; li < 4; ++li) {
+        const int l = i + 1 + ty + 16 * li;
+        f[li] = (l < k) ? a[i][l] : 0.0;
       }
+#pragma unroll
+      for (int xi = 0; xi < 4; ++xi) {
+        const int x = tx + 16 * xi;
+        inB[xi] = x <= i;
+        rv[xi] = (x < k) ? (inB[xi] ? B[i][x] : a[i][x]) : 0.0;
+      }
+#pragma unroll
+      for (int li = 0; li < 4; ++li)
+#pragma unroll
+        for (int xi = 0; xi < 4; ++xi) {
+          const int l = i + 1 + ty + 16 * li, x = tx + 16 * xi;
+          const bool live = l < k && x < k && (inB[xi] || x >= l);
+          cur[li][xi] = live ? (inB[xi] ? B[l][x] : a[l][x]) : 0.0;
+        }
+#pragma unroll
+      for (int li = 0; li < 4; ++li)
+#pragma unroll
+        for (int xi = 0; xi < 4; ++xi) {
+          const int l = i + 1 + ty + 16 * li, x = tx + 16 * xi;
+          const bool live = l < k && x < k && (inB[xi] || x >= l);
+          if (live) {
+            const double nv = cur[li][xi] - f[li] * rv[xi];
+            if (inB[xi]) B[l][x] = nv;
+            else a[l][x] = nv;
+          }
+        }
     }
     __syncthreads();
   }
