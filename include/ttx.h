@@ -100,6 +100,10 @@ typedef struct {
   int32_t passes[TTX_MAX_ORDER];
   int32_t fallback[TTX_MAX_ORDER];
   int32_t predicted_rank[TTX_MAX_ORDER];  /* LEARNED: the network's rank for step k (0 otherwise) */
+  /* with TTX_FLAG_TIMING, per kernel class: the launch with the most algorithmic bytes (its bytes and
+   * device time) */
+  int64_t kernel_top_bytes[TTX_NUM_KCLASS];
+  double kernel_top_ms[TTX_NUM_KCLASS];
 } ttx_report;
 
 /* handle ---------------------------------------------------------------------------------------- */

@@ -62,6 +62,8 @@ class Report(ctypes.Structure):
         ("passes", ctypes.c_int32 * MAX_ORDER),
         ("fallback", ctypes.c_int32 * MAX_ORDER),
         ("predicted_rank", ctypes.c_int32 * MAX_ORDER),
+        ("kernel_top_bytes", ctypes.c_int64 * NUM_KCLASS),
+        ("kernel_top_ms", ctypes.c_double * NUM_KCLASS),
     ]
 
 

@@ -219,6 +219,7 @@ def report_dict(rep, d):
         "fallback": [int(rep.fallback[k]) for k in range(d - 1)],
         "predicted_rank": [int(rep.predicted_rank[k]) for k in range(d - 1)],
         "kernels": {L.KCLASS[c]: {"ms": float(rep.kernel_ms[c]), "launches": int(rep.kernel_launches[c]),
-                                  "bytes": int(rep.kernel_bytes[c])}
+                                  "bytes": int(rep.kernel_bytes[c]), "top_bytes": int(rep.kernel_top_bytes[c]),
+                                  "top_ms": float(rep.kernel_top_ms[c])}
                     for c in range(L.NUM_KCLASS) if rep.kernel_launches[c] > 0},
     }

@@ -764,6 +764,10 @@ static ttx_status sweep(Handle* h, const float* d_A, const ttx_params* p, float*
       R.kernel_ms[kr.kind] += kms;
       R.kernel_launches[kr.kind] += 1;
       R.kernel_bytes[kr.kind] += kr.bytes;
+      if (kr.bytes > R.kernel_top_bytes[kr.kind]) {
+        R.kernel_top_bytes[kr.kind] = kr.bytes;
+        R.kernel_top_ms[kr.kind] = kms;
+      }
     }
   }
   h->ktime = false;
