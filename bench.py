@@ -41,6 +41,7 @@ def parse():
     ap.add_argument("--no-adaptive", action="store_true",
                     help="headline leg with a fixed q instead of the a-posteriori range check")
     ap.add_argument("--only", default="", help="bf16 | fp32: run one leg (profiling)")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-rank workload")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run one 1 GiB device-to-device copy (scale reference for rocprof HBM counters)")
     return ap.parse_args()
@@ -61,6 +62,59 @@ def run_leg(eng, A, torch, prec, q, s, steps, warmup, flags=0, eps=1e-4):
     cores, sig, rep_t = eng.tt_svd(A, DIMS, RANKS, **dict(kw, flags=flags | L.FLAG_TIMING))
     err = eng.rel_error(cores, DIMS, rep_t["ranks"], A)
     return t, err, rep_t, sig
+
+
+def learned_leg(eng, torch, L, count=4, dims=(48, 48, 48, 48, 48), eps=1e-3):
+    """Learned-rank workload: order-5 48^5 tensors (1.02 GB) whose bonds carry weights i^-alpha,
+    alpha ~ U[0.5, 3] per bond (seeded), eps = 1e-3.  The same sweep with the trained network
+    (LEARNED) and without it (DELTA: the adaptive range finder climbs 16, 32, ... widths)."""
+    from tensor_train_ring_decompositions_hpc_gpu_amd import learned as Lr
+    if not os.path.exists(L.MODEL_PATH):
+        return None
+    eng.load_rank_model()
+    dims = list(dims)
+    res = {m: {"ms": 0.0, "bytes": 0, "passes": 0} for m in ("delta", "learned")}
+    hits = near = fallbacks = preds = 0
+    abs_err = 0
+    errs = []
+    for seed in range(count):
+        alphas = np.random.default_rng(seed).uniform(0.5, 3.0, len(dims) - 1)
+        A, r = Lr.spectral_tt(dims, 48, alphas, seed)
+        dA = torch.from_numpy(A).cuda()
+        del A
+        ceil = [1] + [min(48, x) for x in r[1:-1]] + [1]
+        reps = {}
+        for mode in ("delta", "learned"):
+            kw = dict(s=8, q=2, precision="bf16", rank_mode=mode, eps=eps)
+            eng.tt_svd(dA, dims, ceil, **kw)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(3):
+                cores, sig, rep = eng.tt_svd(dA, dims, ceil, **kw)
+            torch.cuda.synchronize()
+            res[mode]["ms"] += (time.perf_counter() - t0) / 3 * 1e3
+            res[mode]["bytes"] += rep["bytes_alg"]
+            res[mode]["passes"] += sum(rep["passes"])
+            errs.append(eng.rel_error(cores, dims, rep["ranks"], dA))
+            reps[mode] = rep
+        for k, pr in enumerate(reps["learned"]["predicted_rank"]):
+            if pr > 0:
+                preds += 1
+                ref = reps["delta"]["ranks"][k + 1]
+                hits += int(pr == ref)
+                near += int(abs(pr - ref) <= 2)
+                abs_err += abs(pr - ref)
+                fallbacks += reps["learned"]["fallback"][k]
+        del dA
+    return {"workload": "%d order-5 tensors 48^5 (1.02 GB), bond weights i^-alpha, alpha~U[0.5,3], eps=1e-3, "
+                        "BF16, s=8, q<=2" % count,
+            "learned_ms": round(res["learned"]["ms"] / count, 3), "adaptive_only_ms": round(res["delta"]["ms"] / count, 3),
+            "speedup": round(res["delta"]["ms"] / max(res["learned"]["ms"], 1e-9), 3),
+            "bytes_learned": res["learned"]["bytes"], "bytes_adaptive_only": res["delta"]["bytes"],
+            "passes_learned": res["learned"]["passes"], "passes_adaptive_only": res["delta"]["passes"],
+            "rank_hit_rate": round(hits / max(preds, 1), 3), "rank_within_2_rate": round(near / max(preds, 1), 3),
+            "rank_mae": round(abs_err / max(preds, 1), 3), "fallback_rate": round(fallbacks / max(preds, 1), 3),
+            "predictions": preds, "max_rel_error": max(errs)}
 
 
 def cpu_baseline(A_dev, torch):
@@ -158,6 +212,8 @@ def main():
         out["fp32_plain"] = {"value": round(in_bytes / t32 / 1e9, 2), "ms_per_step": round(t32 * 1e3, 4),
                              "rel_error": err32, "power_iters": 2, "oversampling": 10}
         out["speedup_vs_fp32_plain"] = round(t32 / t, 3)
+    if not args.no_learned and args.only == "":
+        out["learned_rank"] = learned_leg(eng, torch, L)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(A, torch)
     if args.copy_probe:

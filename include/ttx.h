@@ -165,6 +165,45 @@ ttx_status ttx_tt_full(ttx_handle h, int32_t order, const int64_t* dims, const i
 ttx_status ttx_tr_full(ttx_handle h, int32_t order, const int64_t* dims, const int32_t* ranks,
                        const float* const* d_cores, float* d_out);
 
+/* rank model (TTX_RANK_LEARNED) -------------------------------------------------------------------
+ * No counterpart in the reference, whose ranks are fixed inputs (ttdec6_singleGPU.cu:977).  A
+ * 2 x 128 GELU network maps 32 log-buckets of a 64-column probe sketch's spectrum (the shifted
+ * CholeskyQR pivots of A[:, :n_p] Omega, n_p = max(min(n, 512), n / 16)) plus the relative error
+ * budget and log2 m, log2 n to (rank / 64, sketch width / 64).  The predicted width seeds the
+ * adaptive range finder, whose a-posteriori check judges it; when the check fails the wider widths
+ * and power iterations take over (report: fallback).  Training runs on the GPU through the
+ * forward / backward / Adam kernels below. */
+#define TTX_MLP_IN 40
+#define TTX_MLP_HIDDEN 128
+#define TTX_MLP_OUT 2
+#define TTX_MLP_PARAMS (TTX_MLP_IN * TTX_MLP_HIDDEN + TTX_MLP_HIDDEN + TTX_MLP_HIDDEN * TTX_MLP_HIDDEN + \
+                        TTX_MLP_HIDDEN + TTX_MLP_HIDDEN * TTX_MLP_OUT + TTX_MLP_OUT)
+#define TTX_MLP_MAX_BATCH 8192
+/* He-normal weights from the Philox stream, zero biases, Adam state reset */
+ttx_status ttx_mlp_init(ttx_handle h, uint64_t seed);
+/* parameters as TTX_MLP_PARAMS host floats: W1 [IN][H], b1, W2 [H][H], b2, W3 [H][OUT], b3 */
+ttx_status ttx_mlp_set_params(ttx_handle h, const float* h_params);
+ttx_status ttx_mlp_get_params(ttx_handle h, float* h_params);
+/* read TTX_MLP_PARAMS little-endian float32 values from a file */
+ttx_status ttx_mlp_load(ttx_handle h, const char* path);
+/* Y (batch x OUT) = net(X (batch x IN)), device row-major */
+ttx_status ttx_mlp_forward(ttx_handle h, const float* d_X, int32_t batch, float* d_Y);
+/* gradient of mean((net(X) - T)^2) w.r.t. the parameters into d_grad (TTX_MLP_PARAMS floats); loss
+ * (optional, host) receives the mean squared error */
+ttx_status ttx_mlp_gradient(ttx_handle h, const float* d_X, const float* d_T, int32_t batch, float* d_grad,
+                            double* loss);
+/* one Adam step (beta 0.9 / 0.999, eps 1e-8) on that gradient; loss optional (a host sync when given) */
+ttx_status ttx_mlp_train_step(ttx_handle h, const float* d_X, const float* d_T, int32_t batch, float lr,
+                              double* loss);
+/* the model's input features from probe pivots d_pivots[0..kp-1] (device, FP64) */
+ttx_status ttx_mlp_features(ttx_handle h, const double* d_pivots, int32_t kp, double rel_tol, int64_t m, int64_t n,
+                            float* d_x);
+/* the probe of one unfolding (device A, m x n column-major): pivots R_ii^2 of the shifted Cholesky of
+ * (A[:, :n_p] Omega)^T (A[:, :n_p] Omega), Omega n_p x 64 from (seed, stream), FP32-accurate sketch;
+ * h_pivots (host) receives 64 values */
+ttx_status ttx_probe(ttx_handle h, const float* d_A, int64_t m, int64_t n, uint64_t seed, uint32_t stream,
+                     double* h_pivots);
+
 #ifdef __cplusplus
 }
 #endif

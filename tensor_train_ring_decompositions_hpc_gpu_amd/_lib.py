@@ -102,7 +102,20 @@ SIGNATURES = {
                                 ctypes.POINTER(_D)]),
     "ttx_tt_full": (_I32, [_P, _I32, ctypes.POINTER(_I64), ctypes.POINTER(_I32), _FPP, _P]),
     "ttx_tr_full": (_I32, [_P, _I32, ctypes.POINTER(_I64), ctypes.POINTER(_I32), _FPP, _P]),
+    "ttx_mlp_init": (_I32, [_P, _U64]),
+    "ttx_mlp_set_params": (_I32, [_P, ctypes.POINTER(_F)]),
+    "ttx_mlp_get_params": (_I32, [_P, ctypes.POINTER(_F)]),
+    "ttx_mlp_load": (_I32, [_P, ctypes.c_char_p]),
+    "ttx_mlp_forward": (_I32, [_P, _P, _I32, _P]),
+    "ttx_mlp_gradient": (_I32, [_P, _P, _P, _I32, _P, ctypes.POINTER(_D)]),
+    "ttx_mlp_train_step": (_I32, [_P, _P, _P, _I32, _F, ctypes.POINTER(_D)]),
+    "ttx_mlp_features": (_I32, [_P, _P, _I32, _D, _I64, _I64, _P]),
+    "ttx_probe": (_I32, [_P, _P, _I64, _I64, _U64, _U32, ctypes.POINTER(_D)]),
 }
+
+MLP_IN, MLP_HIDDEN, MLP_OUT = 40, 128, 2
+MLP_PARAMS = MLP_IN * MLP_HIDDEN + MLP_HIDDEN + MLP_HIDDEN * MLP_HIDDEN + MLP_HIDDEN + MLP_HIDDEN * MLP_OUT + MLP_OUT
+MODEL_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "rank_mlp.f32")
 
 _lib = None
 

@@ -55,6 +55,69 @@ class Engine:
         if st != L.OK:
             raise L.TtxError(st, "ttx_create failed")
         self.h = h
+        self.model_loaded = False
+
+    # ---- rank model ----------------------------------------------------------------------------
+    def load_rank_model(self, path=None):
+        """Load trained rank-model weights (raw float32, MLP_PARAMS values; default: the packaged
+        rank_mlp.f32)."""
+        path = path or L.MODEL_PATH
+        _check(self.lib, self.h, self.lib.ttx_mlp_load(self.h, path.encode()))
+        self.model_loaded = True
+
+    def mlp_init(self, seed=7):
+        _check(self.lib, self.h, self.lib.ttx_mlp_init(self.h, seed))
+        self.model_loaded = True
+
+    def mlp_set_params(self, params):
+        v = np.ascontiguousarray(params, dtype=np.float32)
+        assert v.size == L.MLP_PARAMS
+        _check(self.lib, self.h, self.lib.ttx_mlp_set_params(self.h, v.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
+        self.model_loaded = True
+
+    def mlp_get_params(self):
+        v = np.zeros(L.MLP_PARAMS, dtype=np.float32)
+        _check(self.lib, self.h, self.lib.ttx_mlp_get_params(self.h, v.ctypes.data_as(ctypes.POINTER(ctypes.c_float))))
+        return v
+
+    def mlp_forward(self, X):
+        torch = _torch()
+        Y = torch.empty(X.shape[0] * L.MLP_OUT, device=X.device, dtype=torch.float32)
+        _check(self.lib, self.h, self.lib.ttx_mlp_forward(self.h, ctypes.c_void_p(_dev_ptr(X)), X.shape[0],
+                                                          ctypes.c_void_p(Y.data_ptr())))
+        return Y.view(X.shape[0], L.MLP_OUT)
+
+    def mlp_gradient(self, X, T):
+        torch = _torch()
+        g = torch.empty(L.MLP_PARAMS, device=X.device, dtype=torch.float32)
+        loss = ctypes.c_double(0.0)
+        _check(self.lib, self.h, self.lib.ttx_mlp_gradient(self.h, ctypes.c_void_p(_dev_ptr(X)),
+                                                           ctypes.c_void_p(_dev_ptr(T)), X.shape[0],
+                                                           ctypes.c_void_p(g.data_ptr()), ctypes.byref(loss)))
+        return g, float(loss.value)
+
+    def mlp_train_step(self, X, T, lr, want_loss=False):
+        loss = ctypes.c_double(0.0)
+        _check(self.lib, self.h, self.lib.ttx_mlp_train_step(self.h, ctypes.c_void_p(_dev_ptr(X)),
+                                                             ctypes.c_void_p(_dev_ptr(T)), X.shape[0], float(lr),
+                                                             ctypes.byref(loss) if want_loss else None))
+        return float(loss.value) if want_loss else None
+
+    def mlp_features(self, pivots_dev, rel_tol, m, n):
+        torch = _torch()
+        x = torch.zeros(L.MLP_IN, device=pivots_dev.device, dtype=torch.float32)
+        if pivots_dev.dtype != torch.float64 or not pivots_dev.is_contiguous():
+            raise TypeError("ttx: pivots must be a contiguous float64 CUDA tensor")
+        _check(self.lib, self.h, self.lib.ttx_mlp_features(self.h, ctypes.c_void_p(pivots_dev.data_ptr()),
+                                                           pivots_dev.numel(), float(rel_tol), int(m), int(n),
+                                                           ctypes.c_void_p(x.data_ptr())))
+        return x
+
+    def probe(self, A, m, n, seed=1234, stream=0):
+        piv = np.zeros(64, dtype=np.float64)
+        _check(self.lib, self.h, self.lib.ttx_probe(self.h, ctypes.c_void_p(_dev_ptr(A)), int(m), int(n), seed, stream,
+                                                    piv.ctypes.data_as(ctypes.POINTER(ctypes.c_double))))
+        return piv
 
     def close(self):
         if self.h:
@@ -86,6 +149,8 @@ class Engine:
     def _sweep(self, fn, A, dims, ranks, tr, **kw):
         torch = _torch()
         p = self.params(dims, ranks, **kw)
+        if p.rank_mode == L.RANK_MODES["learned"] and not self.model_loaded:
+            self.load_rank_model()
         d = len(dims)
         cores = [torch.empty(int(self.lib.ttx_core_size(ctypes.byref(p), int(tr), k)), device=A.device,
                              dtype=torch.float32) for k in range(d)]

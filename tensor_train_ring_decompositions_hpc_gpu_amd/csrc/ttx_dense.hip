@@ -163,7 +163,7 @@ __device__ __forceinline__ double rsqrt_f64(double d) {
 // ------------------------------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_chol_rinv(const double* __restrict__ part, int64_t S, int k, int64_t m,
                                                    int shift_mode, double* __restrict__ Rinv,
-                                                   int* __restrict__ status) {
+                                                   int* __restrict__ status, double* __restrict__ pivots) {
   __shared__ double a[KMAX][KMAX + 1];
   __shared__ double B[KMAX][KMAX + 1];
   __shared__ double shift_s, tiny_s;
@@ -197,6 +197,7 @@ __global__ __launch_bounds__(256) void k_chol_rinv(const double* __restrict__ pa
     const bool ok = d > tiny;
     const double inv = ok ? rsqrt_f64(d) : 0.0;
     if (!ok) ++nzero;
+    if (pivots && tid == 0) pivots[i] = ok ? d : 0.0;
     // phase 1: row i of R (strict upper part; the diagonal is kept apart) and row i of B
     if (tid < k) {
       if (tid > i) a[i][tid] *= inv;
@@ -251,8 +252,8 @@ __global__ __launch_bounds__(256) void k_chol_rinv(const double* __restrict__ pa
 }
 
 void launch_chol_inv(const double* part, int64_t S, int k, int64_t m, int shift_mode, double* Rinv, int* status,
-                     hipStream_t st) {
-  hipLaunchKernelGGL(k_chol_rinv, dim3(1), dim3(256), 0, st, part, S, k, m, shift_mode, Rinv, status);
+                     hipStream_t st, double* pivots) {
+  hipLaunchKernelGGL(k_chol_rinv, dim3(1), dim3(256), 0, st, part, S, k, m, shift_mode, Rinv, status, pivots);
 }
 
 // ------------------------------------------------------------------------------------------------

@@ -27,6 +27,10 @@ struct Handle {
   std::string err;
   DevBuf part, Y, Y2, Qc, Qp, Qn, Z, W1, W2, G, G2, Rinv, lam, V, Ur, gpart, status, next[2], sig, keys, scratch[4];
   DevBuf Yt, tpart, tout;         // held-out test columns of the range check and its partial sums
+  // rank model: parameters, Adam moments, gradient, activations (batch-sized), probe pivots / features
+  DevBuf mP, mM1, mM2, mG, mAct, mPart, mIO;
+  bool mlp_ready = false;
+  int mlp_t = 0;
   int* h_status = nullptr;       // pinned
   float* h_sigma = nullptr;      // pinned, TTX_MAX_ELL
   double* h_lam = nullptr;       // pinned
@@ -183,6 +187,7 @@ struct StepArgs {
   float* sig_dev = nullptr;
   int* stat_dev = nullptr;
   bool sync = true;
+  double rel_budget = 0.0;   // LEARNED: delta^2 / ||A_k||^2 (a model input)
   // TTX_FLAG_ADAPTIVE: relative squared tolerance of the range check; pred_ell > 0: the sketch width
   // a learned model predicted (the check judges it, the adaptive growth is its fallback)
   double range_tol2 = 0.0;
@@ -194,7 +199,7 @@ struct StepOut {
   double total2 = -1.0;    // ||A||_F^2 when the step knows it
   int64_t bytes = 0;
   double resid = -1.0;     // relative range residual estimate of the basis used (-1: not checked)
-  int checks = 0, passes = 0, fallback = 0;
+  int checks = 0, passes = 0, fallback = 0, pred_rank = 0;
   float sigma[TTX_MAX_ELL];
 };
 
@@ -388,6 +393,97 @@ static ttx_status range_check(Handle* h, const StepArgs& a, int k, int pt, doubl
   return TTX_OK;
 }
 
+// ------------------------------------------------------------------------------------------------
+// rank model
+// ------------------------------------------------------------------------------------------------
+static constexpr int MLP_IN = TTX_MLP_IN, MLP_H = TTX_MLP_HIDDEN, MLP_OUT = TTX_MLP_OUT;
+static constexpr int PROBE_K = 64;
+
+// activation buffer for batch B: Z1, H1, Z2, H2, dH, dZ (B x H each), Y, dY (B x OUT each)
+static ttx_status mlp_buffers(Handle* h, int B) {
+  TTX_TRY(ensure(h, h->mP, TTX_MLP_PARAMS * sizeof(float)));
+  TTX_TRY(ensure(h, h->mM1, TTX_MLP_PARAMS * sizeof(float)));
+  TTX_TRY(ensure(h, h->mM2, TTX_MLP_PARAMS * sizeof(float)));
+  TTX_TRY(ensure(h, h->mG, TTX_MLP_PARAMS * sizeof(float)));
+  TTX_TRY(ensure(h, h->mAct, ((size_t)6 * B * MLP_H + (size_t)2 * B * MLP_OUT) * sizeof(float)));
+  TTX_TRY(ensure(h, h->mPart, (size_t)(mlp_loss_blocks((int64_t)B * MLP_OUT) + 1) * sizeof(double)));
+  TTX_TRY(ensure(h, h->mIO, (PROBE_K * sizeof(double) + (MLP_IN + MLP_OUT) * sizeof(float))));
+  return TTX_OK;
+}
+
+struct MlpAct {
+  float *Z1, *H1, *Z2, *H2, *dH, *dZ, *Y, *dY;
+};
+static MlpAct mlp_act(Handle* h, int B) {
+  float* b = P<float>(h->mAct);
+  const size_t bh = (size_t)B * MLP_H, bo = (size_t)B * MLP_OUT;
+  MlpAct a;
+  a.Z1 = b; a.H1 = b + bh; a.Z2 = b + 2 * bh; a.H2 = b + 3 * bh; a.dH = b + 4 * bh; a.dZ = b + 5 * bh;
+  a.Y = b + 6 * bh; a.dY = a.Y + bo;
+  return a;
+}
+
+// the probe: pivots of the shifted Cholesky of the Gram of Y_p = A[:, :n_p] Omega (64 columns)
+static ttx_status probe(Handle* h, const StepArgs& a, const Arith3& ar, double* d_piv, StepOut& o) {
+  hipStream_t st = h->stream;
+  const int m = (int)a.m;
+  int64_t np = std::max<int64_t>(std::min<int64_t>(a.n, 512), a.n / 16);
+  if (np > a.n) np = a.n;
+  const int kp = PROBE_K;
+  const SketchPlan sp = plan_sketch(m, np, kp, 0);
+  TTX_TRY(ensure(h, h->part, (size_t)sp.slabs * m * kp * sizeof(float)));
+  TTX_TRY(ensure(h, h->Y, (size_t)m * kp * sizeof(float)));
+  TTX_TRY(ensure(h, h->Y2, (size_t)m * kp * sizeof(float)));
+  const int range = (ar.range == AR_BF16) ? AR_BF16X3 : ar.range;   // FP32-accurate against the exact Omega
+  {
+    KScope ks(h, TTX_K_SKETCH, a.m * np * 4);
+    launch_sketch(range, sp, a.A, a.m, m, np, nullptr, 0, a.seed, a.stream_id | 0x10000u, kp, P<float>(h->part), st);
+  }
+  const int64_t nb = panel_blocks(m);
+  TTX_TRY(ensure(h, h->gpart, (size_t)nb * kp * kp * sizeof(double)));
+  TTX_TRY(ensure(h, h->Rinv, (size_t)kp * kp * sizeof(double)));
+  {
+    KScope ks(h, TTX_K_SMALL, 0);
+    launch_reduce_f32(P<float>(h->part), sp.slabs, (int64_t)m * kp, P<float>(h->Y), st);
+  }
+  float* Ya = left_mix(h, a, P<float>(h->Y), kp);
+  {
+    KScope ks(h, TTX_K_SMALL, 0);
+    launch_gram_tall_f32(Ya, kp, m, kp, P<double>(h->gpart), nb, st);
+    launch_chol_inv(P<double>(h->gpart), nb, kp, m, 1, P<double>(h->Rinv), nullptr, st, d_piv);
+  }
+  o.bytes += a.m * np * 4;
+  return check_launch(h, "probe");
+}
+
+// predicted (rank, width) of one randomized step; host sync
+static ttx_status predict(Handle* h, const StepArgs& a, const Arith3& ar, int ell, int* pred_r, int* pred_w,
+                          StepOut& o) {
+  if (!h->mlp_ready) return fail(h, TTX_ERR_BAD_ARG, "rank mode LEARNED needs a rank model (ttx_mlp_load)");
+  hipStream_t st = h->stream;
+  TTX_TRY(mlp_buffers(h, 1));
+  double* piv = P<double>(h->mIO);
+  float* x = reinterpret_cast<float*>(piv + PROBE_K);
+  float* y = x + MLP_IN;
+  TTX_TRY(probe(h, a, ar, piv, o));
+  const MlpAct act = mlp_act(h, 1);
+  {
+    KScope ks(h, TTX_K_SMALL, 0);
+    mlp_features(piv, PROBE_K, a.rel_budget > 0 ? a.rel_budget : 1e-12, (double)a.m, (double)a.n, x, MLP_IN, st);
+    mlp_forward(P<float>(h->mP), MLP_IN, MLP_H, MLP_OUT, x, 1, act.Z1, act.H1, act.Z2, act.H2, y, st);
+  }
+  TTX_TRY(check_launch(h, "rank model"));
+  float hy[MLP_OUT];
+  TTX_HIP(h, hipMemcpyAsync(hy, y, sizeof(hy), hipMemcpyDeviceToHost, st));
+  TTX_HIP(h, hipStreamSynchronize(st));
+  const double r = std::isfinite(hy[0]) ? 64.0 * hy[0] : 1.0;
+  const double w = std::isfinite(hy[1]) ? 64.0 * hy[1] : (double)ell;
+  *pred_r = std::max(1, std::min(a.r, (int)std::lround(r)));
+  // two columns of margin over the predicted width
+  *pred_w = std::max(1, std::min(ell, (int)std::ceil(w) + 2));
+  return TTX_OK;
+}
+
 static ttx_status run_step(Handle* h, const StepArgs& a_in, StepOut& o) {
   hipStream_t st = h->stream;
   StepArgs a = a_in;
@@ -506,6 +602,12 @@ static ttx_status run_step(Handle* h, const StepArgs& a_in, StepOut& o) {
     TTX_TRY(ensure(h, h->Qp, (size_t)kp * m * sizeof(float)));
     TTX_TRY(ensure(h, h->Z, (size_t)n * kp * sizeof(float)));
     if (pt) TTX_TRY(ensure(h, h->Yt, (size_t)m * pt * sizeof(float)));
+    if (a.rank_mode == TTX_RANK_LEARNED && pt) {
+      int pr = 0, pw = 0;
+      TTX_TRY(predict(h, a, ar, ell, &pr, &pw, o));
+      a.pred_ell = pw;
+      o.pred_rank = pr;
+    }
     // basis widths tried in order: the fixed width; in DELTA mode 16, 32, ... up to r + s; a learned
     // prediction first, then the wider ones of that ladder
     int widths[8];
@@ -664,6 +766,7 @@ static ttx_status sweep(Handle* h, const float* d_A, const ttx_params* p, float*
       a.range_tol2 = 0.25 * p->eps * p->eps / (double)(d - 1);
     else
       a.range_tol2 = 0.25 * delta2 / norm2;
+    a.rel_budget = (norm2 > 0.0) ? delta2 / norm2 : p->eps * p->eps / (double)(d - 1);
     const bool last = (k == d - 2);
     // A full-rank exact step (r = m, square orthogonal core) that is not the last one is not
     // projected: the next step reads this step's input with the core folded into its small factors.
@@ -691,6 +794,7 @@ static ttx_status sweep(Handle* h, const float* d_A, const ttx_params* p, float*
     R.range_checks[k] = o.checks;
     R.passes[k] = o.passes;
     R.fallback[k] = o.fallback;
+    R.predicted_rank[k] = o.pred_rank;
     if (host_ranks) {
       R.norm2_in[k] = (o.total2 >= 0) ? o.total2 : norm2;
       norm2 = o.kept2;
@@ -980,7 +1084,7 @@ ttx_status ttx_destroy(ttx_handle hh) {
                          &h->Rinv, &h->lam,
                          &h->V, &h->Ur, &h->gpart, &h->status, &h->next[0], &h->next[1], &h->sig, &h->keys,
                          &h->scratch[0], &h->scratch[1], &h->scratch[2], &h->scratch[3], &h->Yt, &h->tpart,
-                         &h->tout};
+                         &h->tout, &h->mP, &h->mM1, &h->mM2, &h->mG, &h->mAct, &h->mPart, &h->mIO};
   for (auto* b : bufs)
     if (b->p) (void)hipFree(b->p);
   for (float* c : h->last_cores) (void)hipFree(c);
@@ -1146,6 +1250,152 @@ ttx_status ttx_tr_full(ttx_handle h, int32_t order, const int64_t* dims, const i
                        const float* const* d_cores, float* d_out) {
   if (!h || !dims || !ranks || !d_cores || !d_out || order < 2 || order > TTX_MAX_ORDER) return TTX_ERR_BAD_ARG;
   return ttx::full(H(h), order, dims, ranks, d_cores, d_out, true);
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------------------------------------
+// rank model entry points
+// ------------------------------------------------------------------------------------------------
+extern "C" {
+
+ttx_status ttx_mlp_init(ttx_handle hh, uint64_t seed) {
+  Handle* h = H(hh);
+  if (!h) return TTX_ERR_BAD_ARG;
+  TTX_TRY(ttx::mlp_buffers(h, 1));
+  ttx::mlp_init(ttx::P<float>(h->mP), TTX_MLP_IN, TTX_MLP_HIDDEN, TTX_MLP_OUT, seed, h->stream);
+  TTX_HIP(h, hipMemsetAsync(h->mM1.p, 0, TTX_MLP_PARAMS * sizeof(float), h->stream));
+  TTX_HIP(h, hipMemsetAsync(h->mM2.p, 0, TTX_MLP_PARAMS * sizeof(float), h->stream));
+  TTX_TRY(ttx::check_launch(h, "mlp init"));
+  TTX_HIP(h, hipStreamSynchronize(h->stream));
+  h->mlp_t = 0;
+  h->mlp_ready = true;
+  return TTX_OK;
+}
+
+ttx_status ttx_mlp_set_params(ttx_handle hh, const float* hp) {
+  Handle* h = H(hh);
+  if (!h || !hp) return TTX_ERR_BAD_ARG;
+  TTX_TRY(ttx::mlp_buffers(h, 1));
+  TTX_HIP(h, hipMemcpyAsync(h->mP.p, hp, TTX_MLP_PARAMS * sizeof(float), hipMemcpyHostToDevice, h->stream));
+  TTX_HIP(h, hipMemsetAsync(h->mM1.p, 0, TTX_MLP_PARAMS * sizeof(float), h->stream));
+  TTX_HIP(h, hipMemsetAsync(h->mM2.p, 0, TTX_MLP_PARAMS * sizeof(float), h->stream));
+  TTX_HIP(h, hipStreamSynchronize(h->stream));
+  h->mlp_t = 0;
+  h->mlp_ready = true;
+  return TTX_OK;
+}
+
+ttx_status ttx_mlp_get_params(ttx_handle hh, float* hp) {
+  Handle* h = H(hh);
+  if (!h || !hp) return TTX_ERR_BAD_ARG;
+  if (!h->mlp_ready) return ttx::fail(h, TTX_ERR_BAD_ARG, "no rank model");
+  TTX_HIP(h, hipMemcpyAsync(hp, h->mP.p, TTX_MLP_PARAMS * sizeof(float), hipMemcpyDeviceToHost, h->stream));
+  TTX_HIP(h, hipStreamSynchronize(h->stream));
+  return TTX_OK;
+}
+
+ttx_status ttx_mlp_load(ttx_handle hh, const char* path) {
+  Handle* h = H(hh);
+  if (!h || !path) return TTX_ERR_BAD_ARG;
+  std::vector<float> v(TTX_MLP_PARAMS);
+  FILE* f = std::fopen(path, "rb");
+  if (!f) return ttx::fail(h, TTX_ERR_BAD_ARG, std::string("cannot open ") + path);
+  const size_t got = std::fread(v.data(), sizeof(float), v.size(), f);
+  const bool extra = std::fgetc(f) != EOF;
+  std::fclose(f);
+  if (got != v.size() || extra) return ttx::fail(h, TTX_ERR_BAD_ARG, "rank model file has the wrong size");
+  return ttx_mlp_set_params(hh, v.data());
+}
+
+static ttx_status mlp_check(Handle* h, const float* X, int32_t batch) {
+  if (!h) return TTX_ERR_BAD_ARG;
+  if (!X || batch <= 0 || batch > TTX_MLP_MAX_BATCH) return ttx::fail(h, TTX_ERR_BAD_ARG, "bad batch");
+  if (!h->mlp_ready) return ttx::fail(h, TTX_ERR_BAD_ARG, "no rank model (ttx_mlp_init / ttx_mlp_load)");
+  return ttx::mlp_buffers(h, batch);
+}
+
+ttx_status ttx_mlp_forward(ttx_handle hh, const float* X, int32_t batch, float* Y) {
+  Handle* h = H(hh);
+  TTX_TRY(mlp_check(h, X, batch));
+  if (!Y) return ttx::fail(h, TTX_ERR_BAD_ARG, "null output");
+  const ttx::MlpAct a = ttx::mlp_act(h, batch);
+  ttx::mlp_forward(ttx::P<float>(h->mP), TTX_MLP_IN, TTX_MLP_HIDDEN, TTX_MLP_OUT, X, batch, a.Z1, a.H1, a.Z2, a.H2, Y,
+                   h->stream);
+  TTX_TRY(ttx::check_launch(h, "mlp forward"));
+  TTX_HIP(h, hipStreamSynchronize(h->stream));
+  return TTX_OK;
+}
+
+static ttx_status mlp_grad(Handle* h, const float* X, const float* T, int32_t batch, float* grad, double* loss) {
+  const ttx::MlpAct a = ttx::mlp_act(h, batch);
+  const float* P = ttx::P<float>(h->mP);
+  ttx::mlp_forward(P, TTX_MLP_IN, TTX_MLP_HIDDEN, TTX_MLP_OUT, X, batch, a.Z1, a.H1, a.Z2, a.H2, a.Y, h->stream);
+  double* part = ttx::P<double>(h->mPart);
+  ttx::mlp_backward(P, TTX_MLP_IN, TTX_MLP_HIDDEN, TTX_MLP_OUT, X, T, batch, a.Z1, a.H1, a.Z2, a.H2, a.Y, a.dY, a.dH,
+                    a.dZ, grad, part, h->stream);
+  TTX_TRY(ttx::check_launch(h, "mlp backward"));
+  if (loss) {
+    const int nb = ttx::mlp_loss_blocks((int64_t)batch * TTX_MLP_OUT);
+    std::vector<double> hp(nb);
+    TTX_HIP(h, hipMemcpyAsync(hp.data(), part, nb * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    TTX_HIP(h, hipStreamSynchronize(h->stream));
+    double s = 0.0;
+    for (double v : hp) s += v;
+    *loss = s / ((double)batch * TTX_MLP_OUT);
+  }
+  return TTX_OK;
+}
+
+ttx_status ttx_mlp_gradient(ttx_handle hh, const float* X, const float* T, int32_t batch, float* grad, double* loss) {
+  Handle* h = H(hh);
+  TTX_TRY(mlp_check(h, X, batch));
+  if (!T || !grad) return ttx::fail(h, TTX_ERR_BAD_ARG, "null target or gradient");
+  TTX_TRY(mlp_grad(h, X, T, batch, grad, loss));
+  TTX_HIP(h, hipStreamSynchronize(h->stream));
+  return TTX_OK;
+}
+
+ttx_status ttx_mlp_train_step(ttx_handle hh, const float* X, const float* T, int32_t batch, float lr, double* loss) {
+  Handle* h = H(hh);
+  TTX_TRY(mlp_check(h, X, batch));
+  if (!T || !(lr > 0.0f)) return ttx::fail(h, TTX_ERR_BAD_ARG, "null target or non-positive learning rate");
+  TTX_TRY(mlp_grad(h, X, T, batch, ttx::P<float>(h->mG), loss));
+  h->mlp_t += 1;
+  ttx::mlp_adam(ttx::P<float>(h->mP), ttx::P<float>(h->mG), ttx::P<float>(h->mM1), ttx::P<float>(h->mM2),
+                TTX_MLP_PARAMS, lr, h->mlp_t, h->stream);
+  return ttx::check_launch(h, "adam");
+}
+
+ttx_status ttx_mlp_features(ttx_handle hh, const double* piv, int32_t kp, double rel_tol, int64_t m, int64_t n,
+                            float* x) {
+  Handle* h = H(hh);
+  if (!h || !piv || !x || kp <= 0 || kp > 64 || !(rel_tol > 0.0) || m <= 0 || n <= 0) return TTX_ERR_BAD_ARG;
+  ttx::mlp_features(piv, kp, rel_tol, (double)m, (double)n, x, TTX_MLP_IN, h->stream);
+  TTX_TRY(ttx::check_launch(h, "features"));
+  TTX_HIP(h, hipStreamSynchronize(h->stream));
+  return TTX_OK;
+}
+
+ttx_status ttx_probe(ttx_handle hh, const float* d_A, int64_t m, int64_t n, uint64_t seed, uint32_t stream,
+                     double* h_piv) {
+  Handle* h = H(hh);
+  if (!h || !d_A || !h_piv || m <= 0 || n <= 0) return TTX_ERR_BAD_ARG;
+  if (m > ((int64_t)1 << 30)) return ttx::fail(h, TTX_ERR_UNSUPPORTED, "probe: more than 2^30 rows");
+  ttx::StepArgs a;
+  a.A = d_A;
+  a.m = m;
+  a.n = n;
+  a.seed = seed;
+  a.stream_id = stream;
+  const ttx::Arith3 ar = ttx::arith_for(TTX_PREC_BF16, 0);
+  ttx::StepOut o;
+  TTX_TRY(ttx::mlp_buffers(h, 1));
+  double* piv = ttx::P<double>(h->mIO);
+  TTX_TRY(ttx::probe(h, a, ar, piv, o));
+  TTX_HIP(h, hipMemcpyAsync(h_piv, piv, 64 * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  TTX_HIP(h, hipStreamSynchronize(h->stream));
+  return TTX_OK;
 }
 
 }  // extern "C"

@@ -41,9 +41,10 @@ void launch_gram_tall_f64(const double* X, int64_t ldx, int64_t rows, int k, dou
 int64_t panel_blocks(int64_t rows);
 // Sum S Gram partials, Cholesky of G (+ shift) and the explicit inverse of the upper factor:
 // G + s I = R^T R, Rinv = R^-1.  shift_mode 1 adds s = 11 (m k + k (k+1)) u trace(G); pivots that are
-// not positive zero their row and column of Rinv and count into status[0].
+// not positive zero their row and column of Rinv and count into status[0].  pivots (optional, k doubles)
+// receives the shifted pivots R_ii^2.
 void launch_chol_inv(const double* part, int64_t S, int k, int64_t m, int shift_mode, double* Rinv, int* status,
-                     hipStream_t st);
+                     hipStream_t st, double* pivots = nullptr);
 // W_out = W_in Rinv (rows x k) per 64-row block with the Gram partial of W_out (part[block][k][k]).
 void launch_apply_gram_f32(const float* Win, int64_t ldi, int64_t rows, int k, const double* Rinv, double* Wout,
                            double* part, hipStream_t st);
@@ -104,5 +105,17 @@ void launch_gemm_resid(int64_t M, int64_t N, int64_t K, const float* A, int64_t 
 void launch_permute(const float* in, float* out, int nd, const int64_t* dims, const int* perm, hipStream_t st);
 // (r0, r1, n) -> (r1, n, r0), column-major (trdec6_singleGPU.cu:49-58)
 void launch_permute_231(const float* in, float* out, int64_t r0, int64_t r1, int64_t n, hipStream_t st);
+
+// ---- rank model (ttx_mlp.hip) -----------------------------------------------------------------
+void mlp_init(float* params, int in, int hid, int out, uint64_t seed, hipStream_t st);
+// features from the probe's shifted Cholesky pivots d[0..kp-1] (= R_ii^2)
+void mlp_features(const double* d, int kp, double rel_tol, double m, double n, float* x, int in, hipStream_t st);
+void mlp_forward(const float* P, int in, int hid, int out, const float* X, int B, float* Z1, float* H1, float* Z2,
+                 float* H2, float* Y, hipStream_t st);
+void mlp_backward(const float* P, int in, int hid, int out, const float* X, const float* T, int B, const float* Z1,
+                  const float* H1, const float* Z2, const float* H2, const float* Y, float* dY, float* dH,
+                  float* dZ, float* grad, double* part, hipStream_t st);
+int mlp_loss_blocks(int64_t count);
+void mlp_adam(float* P, const float* g, float* m1, float* m2, int count, float lr, int t, hipStream_t st);
 
 }  // namespace ttx
