@@ -40,6 +40,8 @@ def parse():
     ap.add_argument("--fast-range", action="store_true", help="single bf16 MFMA in the range finder")
     ap.add_argument("--no-adaptive", action="store_true",
                     help="headline leg with a fixed q instead of the a-posteriori range check")
+    ap.add_argument("--rank-mode", default="learned", choices=["learned", "fixed"],
+                    help="headline leg: learned ranks (eps 1e-4, ranks 32 as ceilings) or the fixed ranks 32")
     ap.add_argument("--only", default="", help="bf16 | fp32: run one leg (profiling)")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-rank workload")
     ap.add_argument("--copy-probe", action="store_true",
@@ -47,9 +49,9 @@ def parse():
     return ap.parse_args()
 
 
-def run_leg(eng, A, torch, prec, q, s, steps, warmup, flags=0, eps=1e-4):
+def run_leg(eng, A, torch, prec, q, s, steps, warmup, flags=0, eps=1e-4, rank_mode="fixed"):
     from tensor_train_ring_decompositions_hpc_gpu_amd import _lib as L
-    kw = dict(s=s, q=q, seed=OMEGA_SEED, precision=prec, flags=flags, eps=eps)
+    kw = dict(s=s, q=q, seed=OMEGA_SEED, precision=prec, flags=flags, eps=eps, rank_mode=rank_mode)
     for _ in range(warmup):
         eng.tt_svd(A, DIMS, RANKS, **kw)
     torch.cuda.synchronize()
@@ -160,7 +162,10 @@ def main():
 
     legs = {}
     if args.only in ("", "bf16"):
-        legs["bf16"] = run_leg(eng, A, torch, "bf16", args.q, args.s, args.steps, args.warmup, flags)
+        legs["bf16"] = run_leg(eng, A, torch, "bf16", args.q, args.s, args.steps, args.warmup, flags,
+                               rank_mode=args.rank_mode)
+    if args.only == "" and args.rank_mode == "learned":
+        legs["bf16_fixed"] = run_leg(eng, A, torch, "bf16", args.q, args.s, max(1, args.steps // 2), 1, flags)
     if args.only in ("", "fp32"):
         legs["fp32"] = run_leg(eng, A, torch, "fp32", 2, 10, max(1, args.steps // 2), 1)
     head = legs.get("bf16") or legs["fp32"]
@@ -194,9 +199,13 @@ def main():
         "warmup": args.warmup, "ms_per_step": round(t * 1e3, 4), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "bf16" if "bf16" in legs else "fp32",
         "data": "synthetic: seeded TT cores N(0,1/32) ranks (1,32,32,32,32,32,1) contracted + 1e-6 relative noise",
-        "config": {"workload": "order-6 TT-SVD 32^6 (4.295 GB FP32), ranks 32, s=%d, q%s%d" % (args.s, "=" if args.no_adaptive else "<=", args.q),
+        "config": {"workload": "order-6 TT-SVD 32^6 (4.295 GB FP32), %s, s=%d, q%s%d" % (
+                       "learned ranks (ceiling 32, eps=1e-4)" if args.rank_mode == "learned" and "bf16" in legs
+                       else "ranks 32", args.s, "=" if args.no_adaptive else "<=", args.q),
                    "dims": DIMS, "ranks": RANKS, "oversampling": args.s, "power_iters": args.q,
-                   "rank_mode": "fixed", "range_arith": "bf16" if args.fast_range else "bf16x3",
+                   "rank_mode": args.rank_mode if "bf16" in legs else "fixed",
+                   "eps": 1e-4, "ranks_used": rep["ranks"], "predicted_rank": rep["predicted_rank"],
+                   "range_arith": "bf16" if args.fast_range else "bf16x3",
                    "range_check": None if args.no_adaptive else "a-posteriori, 8 test columns, eps=1e-4",
                    "paths": rep["path"], "ell": rep["ell"], "power_iters_run": rep["power_iters"],
                    "passes": rep["passes"], "range_resid": rep["range_resid"]},
@@ -207,6 +216,10 @@ def main():
                         "GBs": round(v["bytes"] / max(v["ms"], 1e-9) / 1e6, 1)} for k, v in ks.items()},
         "roofline": roof,
     }
+    if "bf16_fixed" in legs:
+        tf_, ef_, rf_, _ = legs["bf16_fixed"]
+        out["bf16_fixed_ranks"] = {"value": round(in_bytes / tf_ / 1e9, 2), "ms_per_step": round(tf_ * 1e3, 4),
+                                   "rel_error": ef_, "passes": rf_["passes"]}
     if "fp32" in legs and "bf16" in legs:
         t32, err32, rep32, _ = legs["fp32"]
         out["fp32_plain"] = {"value": round(in_bytes / t32 / 1e9, 2), "ms_per_step": round(t32 * 1e3, 4),

@@ -173,7 +173,10 @@ struct StepArgs {
   int method = TTX_METHOD_AUTO;
   int flags = 0;
   int rank_mode = TTX_RANK_FIXED;
-  double delta2 = 0.0;  // absolute squared truncation threshold for DELTA
+  double delta2 = 0.0;  // absolute squared truncation threshold for DELTA (< 0: delta_rel times this
+                        // step's total, known only after its exact Gram)
+  double delta_rel = 0.0;   // eps^2 / (d - 1)
+  bool allow_defer = false; // DELTA / LEARNED: leave a full-rank exact step unprojected when r = m
   double norm2 = -1.0;  // ||A||_F^2 when known
   float* core = nullptr;   // m x r_out
   float* next = nullptr;   // r_out x n
@@ -200,6 +203,7 @@ struct StepOut {
   int64_t bytes = 0;
   double resid = -1.0;     // relative range residual estimate of the basis used (-1: not checked)
   int checks = 0, passes = 0, fallback = 0, pred_rank = 0;
+  bool deferred = false;   // the step left A_next implicit (full-rank exact step)
   float sigma[TTX_MAX_ELL];
 };
 
@@ -545,6 +549,7 @@ static ttx_status run_step(Handle* h, const StepArgs& a_in, StepOut& o) {
       for (int i = 0; i < m; ++i) total += std::max(h->h_lam[i], 0.0);
       StepArgs aa = a;
       if (aa.norm2 < 0) aa.norm2 = total;
+      if (aa.delta2 < 0) aa.delta2 = a.delta_rel * total;
       double kept = 0.0;
       r = choose_rank(aa, m, h->h_lam, total, 0.0, &kept);
     }
@@ -557,7 +562,8 @@ static ttx_status run_step(Handle* h, const StepArgs& a_in, StepOut& o) {
       launch_assemble(nullptr, 0, m, m, P<double>(h->lam), P<double>(h->V), r, a.core, nullptr, P<float>(h->Qn),
                       kpn, a.sig_dev, P<unsigned long long>(h->keys), st);
     }
-    if (!a.defer) {
+    o.deferred = a.defer || (a.allow_defer && r == m);
+    if (!o.deferred) {
       const float* Qop = proj_operand(h, a, P<float>(h->Qn), kpn);
       KScope ks(h, TTX_K_PROJ, a.m * n * 4 + (int64_t)r * n * 4);
       launch_proj(ar.accurate, kpn, a.A, a.m, m, n, Qop, m, 1, a.next, 0, r, st);
@@ -715,8 +721,21 @@ static ttx_status sweep(Handle* h, const float* d_A, const ttx_params* p, float*
   if (timing)
     for (int k = 0; k < d; ++k) TTX_HIP(h, hipEventCreate(&ev[k]));
   double norm2 = -1.0;
-  if (p->rank_mode != TTX_RANK_FIXED) TTX_TRY(input_norm2(h, d_A, total, &norm2));
-  const double delta2 = (p->rank_mode != TTX_RANK_FIXED) ? p->eps * p->eps / (double)(d - 1) * norm2 : 0.0;
+  const double delta_rel = p->eps * p->eps / (double)(d - 1);
+  double delta2 = 0.0;
+  if (p->rank_mode != TTX_RANK_FIXED) {
+    // an exact first step measures ||A||_F^2 as the trace of its Gram: no separate pass
+    const int64_t m0 = (int64_t)p->ranks[0] * p->dims[0], n0 = total / p->dims[0];
+    const int r0 = tr ? p->ranks[0] * p->ranks[1] : p->ranks[1];
+    const bool exact0 = (p->method != TTX_METHOD_RSVD) && m0 <= TTX_MAX_ELL &&
+                        ((int64_t)r0 + p->oversampling >= std::min(m0, n0) || p->method == TTX_METHOD_EXACT);
+    if (exact0) {
+      delta2 = -1.0;
+    } else {
+      TTX_TRY(input_norm2(h, d_A, total, &norm2));
+      delta2 = delta_rel * norm2;
+    }
+  }
 
   // per-step sigma / status slots on the device, read back once at the end (FIXED ranks need no
   // host round trip inside the sweep)
@@ -753,6 +772,7 @@ static ttx_status sweep(Handle* h, const float* d_A, const ttx_params* p, float*
     a.flags = p->flags;
     a.rank_mode = p->rank_mode;
     a.delta2 = delta2;
+    a.delta_rel = delta_rel;
     a.norm2 = norm2;
     a.core = d_cores[k];
     a.pend = pend;
@@ -773,7 +793,9 @@ static ttx_status sweep(Handle* h, const float* d_A, const ttx_params* p, float*
     const int64_t mn = std::min<int64_t>(m, n);
     const bool exact_m = (a.method != TTX_METHOD_RSVD) && ((int64_t)a.r + a.s >= mn || a.method == TTX_METHOD_EXACT) &&
                          m <= TTX_MAX_ELL;
-    a.defer = !last && !(tr && k == 0) && !host_ranks && !pend && exact_m && a.r == m;
+    const bool can_defer = !last && !(tr && k == 0) && !pend && exact_m;
+    a.defer = can_defer && !host_ranks && a.r == m;
+    a.allow_defer = can_defer && host_ranks;
     float* next = nullptr;
     int wbuf = -1;
     if (last) {
@@ -795,11 +817,15 @@ static ttx_status sweep(Handle* h, const float* d_A, const ttx_params* p, float*
     R.passes[k] = o.passes;
     R.fallback[k] = o.fallback;
     R.predicted_rank[k] = o.pred_rank;
+    if (host_ranks && delta2 < 0.0 && o.total2 >= 0.0) {
+      norm2 = o.total2;
+      delta2 = delta_rel * norm2;
+    }
     if (host_ranks) {
       R.norm2_in[k] = (o.total2 >= 0) ? o.total2 : norm2;
       norm2 = o.kept2;
     }
-    if (a.defer) {
+    if (a.defer || o.deferred) {
       pend = d_cores[k];
       pend_ms = (int)m;
       r_prev = o.r;
