@@ -44,6 +44,7 @@ def parse():
                     help="headline leg: learned ranks (eps 1e-4, ranks 32 as ceilings) or the fixed ranks 32")
     ap.add_argument("--only", default="", help="bf16 | fp32: run one leg (profiling)")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-rank workload")
+    ap.add_argument("--no-tr", action="store_true", help="skip the TR-SVD workload")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run one 1 GiB device-to-device copy (scale reference for rocprof HBM counters)")
     return ap.parse_args()
@@ -117,6 +118,38 @@ def learned_leg(eng, torch, L, count=4, dims=(48, 48, 48, 48, 48), eps=1e-3):
             "rank_hit_rate": round(hits / max(preds, 1), 3), "rank_within_2_rate": round(near / max(preds, 1), 3),
             "rank_mae": round(abs_err / max(preds, 1), 3), "fallback_rate": round(fallbacks / max(preds, 1), 3),
             "predictions": preds, "max_rel_error": max(errs)}
+
+
+def tr_leg(eng, torch, L, steps=3):
+    """TR workload: order-6 32^6 tensor (4.295 GB) from ring cores 16 x 32 x 16 with N(0, 1/4) entries;
+    TR-SVD with first-step split r0 r1 = 4 x 16 (64 columns on a 32-row unfolding: the core is zero-padded),
+    inner ranks 16, FP16 range finder with power-of-two operand scaling, s = 10, q <= 2 (adaptive).
+    The 231 relayout moves 2^31 elements."""
+    dims = [32] * 6
+    A = eng.generate(dims, [16] * 7, std=0.5, noise_rel=0.0, seed=DATA_SEED, tr=True)
+    ranks = [4, 16, 16, 16, 16, 16, 4]
+    kw = dict(s=10, q=2, seed=OMEGA_SEED, precision="fp16", flags=L.FLAG_ADAPTIVE)
+    eng.tr_svd(A, dims, ranks, **kw)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        cores, sig, rep = eng.tr_svd(A, dims, ranks, **kw)
+    torch.cuda.synchronize()
+    t = (time.perf_counter() - t0) / steps
+    cores, sig, rep = eng.tr_svd(A, dims, ranks, **dict(kw, flags=kw["flags"] | L.FLAG_TIMING))
+    err = eng.rel_error(cores, dims, ranks, A, tr=True)
+    c32, _, _ = eng.tr_svd(A, dims, ranks, **dict(kw, precision="fp32"))
+    err32 = eng.rel_error(c32, dims, ranks, A, tr=True)
+    pk = rep["kernels"].get("permute", {})
+    out = {"workload": "order-6 TR-SVD 32^6 (4.295 GB), ring cores 16x32x16 N(0,1/4), split r0 r1 = 4 x 16, "
+                       "inner ranks 16, FP16 range finder, s=10, q<=2",
+           "value": round(A.numel() * 4 / t / 1e9, 2), "unit": "GB/s", "ms_per_step": round(t * 1e3, 3),
+           "rel_error_fp16": err, "rel_error_fp32_same_schedule": err32,
+           "paths": rep["path"], "power_iters_run": rep["power_iters"],
+           "permute": {"ms": round(pk.get("top_ms", 0.0), 4), "bytes": pk.get("top_bytes", 0),
+                       "GBs": round(pk.get("top_bytes", 0) / max(pk.get("top_ms", 0.0), 1e-9) / 1e6, 1)}}
+    del A, cores, c32
+    return out
 
 
 def cpu_baseline(A_dev, torch):
@@ -227,6 +260,8 @@ def main():
         out["speedup_vs_fp32_plain"] = round(t32 / t, 3)
     if not args.no_learned and args.only == "":
         out["learned_rank"] = learned_leg(eng, torch, L)
+    if not args.no_tr and args.only == "":
+        out["tr_fp16"] = tr_leg(eng, torch, L)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(A, torch)
     if args.copy_probe:

@@ -29,6 +29,7 @@ struct Handle {
   DevBuf Yt, tpart, tout;         // held-out test columns of the range check and its partial sums
   // rank model: parameters, Adam moments, gradient, activations (batch-sized), probe pivots / features
   DevBuf mP, mM1, mM2, mG, mAct, mPart, mIO;
+  DevBuf f16s;                    // FP16 operand scales (65 floats) and column-max words (64)
   bool mlp_ready = false;
   int mlp_t = 0;
   int* h_status = nullptr;       // pinned
@@ -191,6 +192,9 @@ struct StepArgs {
   int* stat_dev = nullptr;
   bool sync = true;
   double rel_budget = 0.0;   // LEARNED: delta^2 / ||A_k||^2 (a model input)
+  // previous step's kept singular values on the device (FP16 operand scale of this step's input)
+  const float* sig_prev = nullptr;
+  int r_prev = 0;
   // TTX_FLAG_ADAPTIVE: relative squared tolerance of the range check; pred_ell > 0: the sketch width
   // a learned model predicted (the check judges it, the adaptive growth is its fallback)
   double range_tol2 = 0.0;
@@ -323,6 +327,18 @@ static float* left_mix(Handle* h, const StepArgs& a, float* Y, int kp) {
   return P<float>(h->Y2);
 }
 
+// FP16 range finder: power-of-two operand scales (A from the previous step's kept energy)
+static float* fp16_scales(Handle* h, const StepArgs& a, const Arith3& ar, bool set_a) {
+  if (ar.range != AR_FP16) return nullptr;
+  if (!h->f16s.p) {
+    if (ensure(h, h->f16s, 129 * sizeof(float)) != TTX_OK) return nullptr;
+    (void)hipMemsetAsync(h->f16s.p, 0, 129 * sizeof(float), h->stream);
+  }
+  float* sc = P<float>(h->f16s);
+  if (set_a) launch_fp16_ascale(a.sig_prev, a.r_prev, (double)a.m * (double)a.n, sc, h->stream);
+  return sc;
+}
+
 // Y = A Omega[:, :w + pt] (basicRandSVD_LU :585-620; Omega from the project's Philox stream), test
 // columns w .. w+pt-1 kept in Yt, Q = orth(Y[:, :w])
 static ttx_status range_sketch(Handle* h, const StepArgs& a, const Arith3& ar, int w, int pt, int kp,
@@ -330,9 +346,11 @@ static ttx_status range_sketch(Handle* h, const StepArgs& a, const Arith3& ar, i
   hipStream_t st = h->stream;
   const int m = (int)a.m;
   float* Y = P<float>(h->Y);
+  float* sc = fp16_scales(h, a, ar, true);
   {
     KScope ks(h, TTX_K_SKETCH, a.m * a.n * 4);
-    launch_sketch(ar.range, sp, a.A, a.m, m, a.n, nullptr, 0, a.seed, a.stream_id, w + pt, P<float>(h->part), st);
+    launch_sketch(ar.range, sp, a.A, a.m, m, a.n, nullptr, 0, a.seed, a.stream_id, w + pt, P<float>(h->part), st,
+                  sc);
   }
   {
     KScope ks(h, TTX_K_SMALL, 0);
@@ -357,13 +375,18 @@ static ttx_status power_iter(Handle* h, const StepArgs& a, const Arith3& ar, int
   const int64_t n = a.n;
   float* Y = P<float>(h->Y);
   const float* Qop = proj_operand(h, a, P<float>(h->Qc), kp);
+  float* sc = fp16_scales(h, a, ar, false);
   {
     KScope ks(h, TTX_K_PROJ, a.m * n * 4 + n * kp * 4);
-    launch_proj(ar.range, kp, a.A, a.m, m, n, Qop, m, 0, P<float>(h->Z), kp, 0, st);
+    launch_proj(ar.range, kp, a.A, a.m, m, n, Qop, m, 0, P<float>(h->Z), kp, 0, st, nullptr, 0, sc);
+  }
+  if (sc) {
+    KScope ks(h, TTX_K_SMALL, 0);
+    launch_fp16_xscale(P<float>(h->Z), n, kp, kp, reinterpret_cast<unsigned*>(sc + 65), sc, st);
   }
   {
     KScope ks(h, TTX_K_SKETCH, a.m * n * 4 + n * kp * 4);
-    launch_sketch(ar.range, sp, a.A, a.m, m, n, P<float>(h->Z), kp, 0, 0, k, P<float>(h->part), st);
+    launch_sketch(ar.range, sp, a.A, a.m, m, n, P<float>(h->Z), kp, 0, 0, k, P<float>(h->part), st, sc);
   }
   {
     KScope ks(h, TTX_K_SMALL, 0);
@@ -438,7 +461,8 @@ static ttx_status probe(Handle* h, const StepArgs& a, const Arith3& ar, double* 
   TTX_TRY(ensure(h, h->part, (size_t)sp.slabs * m * kp * sizeof(float)));
   TTX_TRY(ensure(h, h->Y, (size_t)m * kp * sizeof(float)));
   TTX_TRY(ensure(h, h->Y2, (size_t)m * kp * sizeof(float)));
-  const int range = (ar.range == AR_BF16) ? AR_BF16X3 : ar.range;   // FP32-accurate against the exact Omega
+  // FP32-accurate against the exact Omega in every precision mode
+  const int range = (ar.range == AR_FP32) ? AR_FP32 : AR_BF16X3;
   {
     KScope ks(h, TTX_K_SKETCH, a.m * np * 4);
     launch_sketch(range, sp, a.A, a.m, m, np, nullptr, 0, a.seed, a.stream_id | 0x10000u, kp, P<float>(h->part), st);
@@ -751,6 +775,7 @@ static ttx_status sweep(Handle* h, const float* d_A, const ttx_params* p, float*
   int r_prev = 1;
   int64_t n = total;
   const float* pend = nullptr;   // square core of a deferred full-rank step
+  int prev_kept = 0;             // singular values kept by the previous step
   int pend_ms = 0;
   R.ranks[0] = p->ranks[0];
   R.ranks[d] = p->ranks[d];
@@ -778,6 +803,8 @@ static ttx_status sweep(Handle* h, const float* d_A, const ttx_params* p, float*
     a.pend = pend;
     a.pend_ms = pend_ms;
     a.sig_dev = sig_all + (size_t)k * TTX_MAX_ELL;
+    a.sig_prev = (k > 0) ? sig_all + (size_t)(k - 1) * TTX_MAX_ELL : nullptr;
+    a.r_prev = prev_kept;
     a.stat_dev = stat_all + 4 * k;
     a.sync = host_ranks;
     // range-check tolerance: a quarter of the step's share eps^2 / (d-1) of the squared error budget
@@ -817,6 +844,7 @@ static ttx_status sweep(Handle* h, const float* d_A, const ttx_params* p, float*
     R.passes[k] = o.passes;
     R.fallback[k] = o.fallback;
     R.predicted_rank[k] = o.pred_rank;
+    prev_kept = std::min(o.r, TTX_MAX_ELL);
     if (host_ranks && delta2 < 0.0 && o.total2 >= 0.0) {
       norm2 = o.total2;
       delta2 = delta_rel * norm2;
@@ -1110,7 +1138,8 @@ ttx_status ttx_destroy(ttx_handle hh) {
                          &h->Rinv, &h->lam,
                          &h->V, &h->Ur, &h->gpart, &h->status, &h->next[0], &h->next[1], &h->sig, &h->keys,
                          &h->scratch[0], &h->scratch[1], &h->scratch[2], &h->scratch[3], &h->Yt, &h->tpart,
-                         &h->tout, &h->mP, &h->mM1, &h->mM2, &h->mG, &h->mAct, &h->mPart, &h->mIO};
+                         &h->tout, &h->mP, &h->mM1, &h->mM2, &h->mG, &h->mAct, &h->mPart, &h->mIO,
+                         &h->f16s};
   for (auto* b : bufs)
     if (b->p) (void)hipFree(b->p);
   for (float* c : h->last_cores) (void)hipFree(c);

@@ -14,15 +14,24 @@ struct SketchPlan {
 };
 SketchPlan plan_sketch(int m, int64_t n, int kp, int wgs_hint);
 // Y partials: part[slab][m][kp].  X == nullptr selects the in-kernel Philox sketch (seed, stream, ell).
+// FP16 arithmetic: scales (optional) holds power-of-two operand scales, [0] for A and [1 + c] for
+// column c of X, undone on the partial sums.
 void launch_sketch(int ar, const SketchPlan& p, const float* A, int64_t lda, int m, int64_t n, const float* X,
-                   int64_t ldx, uint64_t seed, uint32_t stream, int ell, float* part, hipStream_t st);
+                   int64_t ldx, uint64_t seed, uint32_t stream, int ell, float* part, hipStream_t st,
+                   const float* scales = nullptr);
+// scales[0] from sum_i<r sig_prev[i]^2 / numel (RMS entry to [8, 16)); sig_prev == nullptr gives 1
+void launch_fp16_ascale(const float* sig_prev, int r_prev, double numel, float* scales, hipStream_t st);
+// scales[1 + c] from max_j |X[j][c]| (to [128, 256)); mx: kp zeroed words, zeroed again afterwards
+void launch_fp16_xscale(const float* X, int64_t n, int64_t ldx, int kp, unsigned* mx, float* scales, hipStream_t st);
 // D = Q^T A; mode 0 writes Z (n x ldo row-major: kp columns, or the first r_out when r_out > 0, a multiple
 // of 4), mode 1 writes r_out x n column-major.
 // gpart != nullptr: also the Gram D D^T, first kg rows / columns, one FP64 partial per WG
 // (proj_grid(n) partials).
 int64_t proj_grid(int64_t n);
+// FP16 arithmetic: scales[0] (optional) is a power-of-two scale of A, undone on the products.
 void launch_proj(int ar, int kp, const float* A, int64_t lda, int m, int64_t n, const float* Qc, int64_t ldq,
-                 int mode, float* out, int64_t ldo, int r_out, hipStream_t st, double* gpart = nullptr, int kg = 0);
+                 int mode, float* out, int64_t ldo, int r_out, hipStream_t st, double* gpart = nullptr, int kg = 0,
+                 const float* scales = nullptr);
 int64_t gram_small_grid(int64_t n);
 // partial A A^T: part[grid][m][m] (FP64); absmax (optional) receives max |A| as float bits
 void launch_gram_small(int ar, const float* A, int64_t lda, int m, int64_t n, double* part, int64_t grid,

@@ -52,7 +52,8 @@ __device__ __forceinline__ void sketch_load(const float* __restrict__ A, int64_t
 
 template <int LT, int AR, bool OMEGA>
 __device__ __forceinline__ void sketch_compute(const float (&a)[8][4], const float* __restrict__ xs,
-                                               int pitch, int kofs, int lane, SketchAcc<LT>& S) {
+                                               int pitch, int kofs, int lane, SketchAcc<LT>& S, float sa,
+                                               const float (&sx)[LT]) {
   const int c = lane & 31;
   const int b = lane >> 5;
   // B operands: X[k = kofs + 8b + t][col = 32 lt + c]
@@ -73,14 +74,19 @@ __device__ __forceinline__ void sketch_compute(const float (&a)[8][4], const flo
 #pragma unroll
         for (int lt = 0; lt < LT; ++lt) S.acc[s][lt] = mfma_f32(a[t][s], xv[lt][t], S.acc[s][lt]);
   } else if constexpr (AR == AR_FP16) {
+    // operands scaled by powers of two (A by sa, column c of X by sx) before rounding to fp16
     f16x8 bh[LT];
 #pragma unroll
-    for (int lt = 0; lt < LT; ++lt) round8h(xv[lt], bh[lt]);
+    for (int lt = 0; lt < LT; ++lt) {
+#pragma unroll
+      for (int t = 0; t < 8; ++t) xv[lt][t] *= sx[lt];
+      round8h(xv[lt], bh[lt]);
+    }
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
       float v[8];
 #pragma unroll
-      for (int t = 0; t < 8; ++t) v[t] = a[t][s];
+      for (int t = 0; t < 8; ++t) v[t] = a[t][s] * sa;
       f16x8 ah;
       round8h(v, ah);
 #pragma unroll
@@ -155,7 +161,7 @@ template <int LT, int AR, bool OMEGA>
 __global__ __launch_bounds__(64 * sketch_waves<LT>()) void k_sketch(const float* __restrict__ A, int64_t lda, int m, int64_t n,
                                                 const float* __restrict__ X, int64_t ldx, uint64_t seed,
                                                 uint32_t stream, int ell, int RB, int CG, int64_t chunks_per_wg,
-                                                float* __restrict__ part) {
+                                                float* __restrict__ part, const float* __restrict__ scales) {
   constexpr int KP = 32 * LT;
   const int KC = 16 * CG;
   const int pitch = KC + 4;
@@ -182,6 +188,17 @@ __global__ __launch_bounds__(64 * sketch_waves<LT>()) void k_sketch(const float*
 #pragma unroll
     for (int lt = 0; lt < LT; ++lt) S.acc[s][lt] = zero16();
 
+  // FP16: power-of-two operand scales (scales[0] for A, scales[1 + c] for column c of X)
+  float sa = 1.0f, sx[LT];
+#pragma unroll
+  for (int lt = 0; lt < LT; ++lt) sx[lt] = 1.0f;
+  if (AR == AR_FP16 && scales) {
+    sa = scales[0];
+    if (!OMEGA)
+#pragma unroll
+      for (int lt = 0; lt < LT; ++lt) sx[lt] = scales[1 + 32 * lt + r];
+  }
+
   float a0[8][4], a1[8][4];
   const int kofs = 16 * cg;
   if (c0 < c1) {
@@ -199,7 +216,7 @@ __global__ __launch_bounds__(64 * sketch_waves<LT>()) void k_sketch(const float*
       if (vec) sketch_load<true>(A, lda, m, n, row, col, a1);
       else sketch_load<false>(A, lda, m, n, row, col, a1);
     }
-    sketch_compute<LT, AR, OMEGA>(a0, xs0, pitch, kofs, lane, S);
+    sketch_compute<LT, AR, OMEGA>(a0, xs0, pitch, kofs, lane, S, sa, sx);
     __syncthreads();
     if (c + 1 >= c1) break;
     // odd step: compute chunk c+1 from a1 / xs1, prefetch chunk c+2 into a0 / xs0
@@ -209,13 +226,16 @@ __global__ __launch_bounds__(64 * sketch_waves<LT>()) void k_sketch(const float*
       if (vec) sketch_load<true>(A, lda, m, n, row, col, a0);
       else sketch_load<false>(A, lda, m, n, row, col, a0);
     }
-    sketch_compute<LT, AR, OMEGA>(a1, xs1, pitch, kofs, lane, S);
+    sketch_compute<LT, AR, OMEGA>(a1, xs1, pitch, kofs, lane, S, sa, sx);
     __syncthreads();
   }
 
-  // partial output (row-major, kp columns)
+  // partial output (row-major, kp columns), the FP16 operand scales undone (exact powers of two)
   const int64_t slab = (int64_t)blockIdx.x * CG + cg;
   float* out = part + slab * (int64_t)m * KP;
+  float unscale[LT];
+#pragma unroll
+  for (int lt = 0; lt < LT; ++lt) unscale[lt] = 1.0f / (sa * sx[lt]);
 #pragma unroll
   for (int s = 0; s < 4; ++s)
 #pragma unroll
@@ -223,7 +243,7 @@ __global__ __launch_bounds__(64 * sketch_waves<LT>()) void k_sketch(const float*
 #pragma unroll
       for (int v = 0; v < 16; ++v) {
         const int prow = blockIdx.y * 128 * RB + 128 * rb + 4 * mfma32_row(v, lane) + s;
-        if (prow < m) out[(int64_t)prow * KP + 32 * lt + r] = S.acc[s][lt][v];
+        if (prow < m) out[(int64_t)prow * KP + 32 * lt + r] = S.acc[s][lt][v] * unscale[lt];
       }
 }
 
@@ -241,9 +261,12 @@ template <int LT, int AR, bool GRAM>
 __global__ __launch_bounds__(512) void k_proj(const float* __restrict__ A, int64_t lda, int m, int64_t n,
                                               const float* __restrict__ Qc, int64_t ldq,
                                               int out_mode, float* __restrict__ out, int64_t ldo, int r_out,
-                                              int64_t blocks_total, double* __restrict__ gpart, int kg) {
+                                              int64_t blocks_total, double* __restrict__ gpart, int kg,
+                                              const float* __restrict__ scales) {
   constexpr int KP = 32 * LT;
   constexpr int QP = 64 + 4;   // LDS pitch of a Q column chunk
+  // FP16: A is scaled by the power of two scales[0] before rounding, the products are unscaled
+  const float sa = (AR == AR_FP16 && scales) ? scales[0] : 1.0f;
   constexpr int NPG = LT * (LT + 1) / 2;
   __shared__ float qs[2][KP * QP];
   __shared__ float dt[GRAM ? 8 : 1][GRAM ? 32 * KP : 1];
@@ -317,7 +340,10 @@ __global__ __launch_bounds__(512) void k_proj(const float* __restrict__ A, int64
           }
         } else if constexpr (AR == AR_FP16) {
           f16x8 bh;
-          round8h(xb, bh);
+          float xsc[8];
+#pragma unroll
+          for (int u = 0; u < 8; ++u) xsc[u] = xb[u] * sa;
+          round8h(xsc, bh);
 #pragma unroll
           for (int lt = 0; lt < LT; ++lt) {
             const float* qp = q + (32 * lt + jl) * QP + 32 * b + 8 * t;
@@ -374,6 +400,13 @@ __global__ __launch_bounds__(512) void k_proj(const float* __restrict__ A, int64
       __syncthreads();
     }
 
+    if (AR == AR_FP16 && scales) {
+      const float inv = 1.0f / sa;
+#pragma unroll
+      for (int lt = 0; lt < LT; ++lt)
+#pragma unroll
+        for (int v = 0; v < 16; ++v) acc[lt][v] *= inv;
+    }
     if (j < n) {
       if (out_mode == 0) {
         // Z = D^T, row-major n x ldo: kp columns, or the first r_out (a multiple of 4)
@@ -630,33 +663,95 @@ SketchPlan plan_sketch(int m, int64_t n, int kp, int wgs_hint) {
 
 template <int LT, int AR, bool OM>
 static void launch_sketch_t(const SketchPlan& p, const float* A, int64_t lda, int m, int64_t n, const float* X,
-                            int64_t ldx, uint64_t seed, uint32_t stream, int ell, float* part, hipStream_t st) {
+                            int64_t ldx, uint64_t seed, uint32_t stream, int ell, float* part, hipStream_t st,
+                            const float* scales) {
   dim3 grid(p.grid_x, p.ysplit);
   hipLaunchKernelGGL((k_sketch<LT, AR, OM>), grid, dim3(64 * p.RB * p.CG), p.lds_bytes, st, A, lda, m, n, X, ldx,
-                     seed, stream, ell, p.RB, p.CG, p.chunks_per_wg, part);
+                     seed, stream, ell, p.RB, p.CG, p.chunks_per_wg, part, scales);
 }
 
 template <int LT, bool OM>
 static void launch_sketch_ar(int ar, const SketchPlan& p, const float* A, int64_t lda, int m, int64_t n, const float* X,
-                             int64_t ldx, uint64_t seed, uint32_t stream, int ell, float* part, hipStream_t st) {
+                             int64_t ldx, uint64_t seed, uint32_t stream, int ell, float* part, hipStream_t st,
+                             const float* sc) {
   switch (ar) {
-    case AR_FP32: launch_sketch_t<LT, AR_FP32, OM>(p, A, lda, m, n, X, ldx, seed, stream, ell, part, st); break;
-    case AR_BF16: launch_sketch_t<LT, AR_BF16, OM>(p, A, lda, m, n, X, ldx, seed, stream, ell, part, st); break;
-    case AR_FP16: launch_sketch_t<LT, AR_FP16, OM>(p, A, lda, m, n, X, ldx, seed, stream, ell, part, st); break;
-    default: launch_sketch_t<LT, AR_BF16X3, OM>(p, A, lda, m, n, X, ldx, seed, stream, ell, part, st); break;
+    case AR_FP32: launch_sketch_t<LT, AR_FP32, OM>(p, A, lda, m, n, X, ldx, seed, stream, ell, part, st, sc); break;
+    case AR_BF16: launch_sketch_t<LT, AR_BF16, OM>(p, A, lda, m, n, X, ldx, seed, stream, ell, part, st, sc); break;
+    case AR_FP16: launch_sketch_t<LT, AR_FP16, OM>(p, A, lda, m, n, X, ldx, seed, stream, ell, part, st, sc); break;
+    default: launch_sketch_t<LT, AR_BF16X3, OM>(p, A, lda, m, n, X, ldx, seed, stream, ell, part, st, sc); break;
   }
 }
 
 void launch_sketch(int ar, const SketchPlan& p, const float* A, int64_t lda, int m, int64_t n, const float* X,
-                   int64_t ldx, uint64_t seed, uint32_t stream, int ell, float* part, hipStream_t st) {
+                   int64_t ldx, uint64_t seed, uint32_t stream, int ell, float* part, hipStream_t st,
+                   const float* scales) {
   const bool om = (X == nullptr);
   if (p.kp == 32) {
-    if (om) launch_sketch_ar<1, true>(ar, p, A, lda, m, n, X, ldx, seed, stream, ell, part, st);
-    else launch_sketch_ar<1, false>(ar, p, A, lda, m, n, X, ldx, seed, stream, ell, part, st);
+    if (om) launch_sketch_ar<1, true>(ar, p, A, lda, m, n, X, ldx, seed, stream, ell, part, st, scales);
+    else launch_sketch_ar<1, false>(ar, p, A, lda, m, n, X, ldx, seed, stream, ell, part, st, scales);
   } else {
-    if (om) launch_sketch_ar<2, true>(ar, p, A, lda, m, n, X, ldx, seed, stream, ell, part, st);
-    else launch_sketch_ar<2, false>(ar, p, A, lda, m, n, X, ldx, seed, stream, ell, part, st);
+    if (om) launch_sketch_ar<2, true>(ar, p, A, lda, m, n, X, ldx, seed, stream, ell, part, st, scales);
+    else launch_sketch_ar<2, false>(ar, p, A, lda, m, n, X, ldx, seed, stream, ell, part, st, scales);
   }
+}
+
+// ------------------------------------------------------------------------------------------------
+// FP16 operand scales (powers of two): scales[0] for A from ||A||_F^2 = sum of the previous step's
+// kept sigma^2 (RMS entry scaled to [8, 16)); scales[1 + c] for column c of X from max_j |X[j][c]|
+// (scaled to [128, 256)).  Zero or non-finite inputs give scale 1.
+// ------------------------------------------------------------------------------------------------
+__device__ __forceinline__ float pow2_scale_to(float v, int target_exp) {
+  if (!(v > 0.0f) || !isfinite(v)) return 1.0f;
+  const int e = (int)((__float_as_uint(v) >> 23) & 0xFF) - 127;     // floor(log2 v) for normal v
+  int t = target_exp - e;
+  t = t < -100 ? -100 : (t > 100 ? 100 : t);
+  return ldexpf(1.0f, t);
+}
+
+__global__ void k_fp16_ascale(const float* __restrict__ sig, int r, double numel, float* __restrict__ scales) {
+  if (threadIdx.x != 0) return;
+  double e2 = 0.0;
+  if (sig)
+    for (int i = 0; i < r; ++i) e2 += (double)sig[i] * sig[i];
+  const float rms = (float)sqrt(e2 / numel);
+  scales[0] = sig ? pow2_scale_to(rms, 3) : 1.0f;
+}
+
+__global__ void k_colmax(const float* __restrict__ X, int64_t n, int64_t ldx, int kp, unsigned* __restrict__ mx) {
+  __shared__ unsigned smax[64];
+  if (threadIdx.x < 64) smax[threadIdx.x] = 0u;
+  __syncthreads();
+  const int c = threadIdx.x % kp;
+  const int rstep = blockDim.x / kp;
+  unsigned local = 0u;
+  for (int64_t j = (int64_t)blockIdx.x * rstep + threadIdx.x / kp; j < n; j += (int64_t)gridDim.x * rstep) {
+    const unsigned b = __float_as_uint(fabsf(X[j * ldx + c]));
+    local = b > local ? b : local;
+  }
+  atomicMax(&smax[c], local);
+  __syncthreads();
+  if (threadIdx.x < kp) atomicMax(&mx[threadIdx.x], smax[threadIdx.x]);
+}
+
+__global__ void k_colscale(unsigned* __restrict__ mx, int kp, float* __restrict__ scales) {
+  const int c = threadIdx.x;
+  if (c < kp) {
+    scales[1 + c] = pow2_scale_to(__uint_as_float(mx[c]), 7);
+    mx[c] = 0u;   // ready for the next pass
+  }
+}
+
+void launch_fp16_ascale(const float* sig_prev, int r_prev, double numel, float* scales, hipStream_t st) {
+  hipLaunchKernelGGL(k_fp16_ascale, dim3(1), dim3(64), 0, st, sig_prev, r_prev, numel, scales);
+}
+
+// mx: kp zeroed words (left zeroed again afterwards)
+void launch_fp16_xscale(const float* X, int64_t n, int64_t ldx, int kp, unsigned* mx, float* scales, hipStream_t st) {
+  int64_t grid = (n * kp + 255) / 256;
+  if (grid > 1024) grid = 1024;
+  if (grid < 1) grid = 1;
+  hipLaunchKernelGGL(k_colmax, dim3((unsigned)grid), dim3(256), 0, st, X, n, ldx, kp, mx);
+  hipLaunchKernelGGL(k_colscale, dim3(1), dim3(64), 0, st, mx, kp, scales);
 }
 
 int64_t proj_grid(int64_t n) {
@@ -669,33 +764,36 @@ int64_t proj_grid(int64_t n) {
 
 template <int LT, int AR>
 static void launch_proj_t(const float* A, int64_t lda, int m, int64_t n, const float* Qc, int64_t ldq, int mode,
-                          float* out, int64_t ldo, int r_out, double* gpart, int kg, hipStream_t st) {
+                          float* out, int64_t ldo, int r_out, double* gpart, int kg, hipStream_t st,
+                          const float* sc) {
   const int64_t blocks = (n + 255) / 256;
   const int64_t grid = proj_grid(n);
   if (gpart)
     hipLaunchKernelGGL((k_proj<LT, AR, true>), dim3((unsigned)grid), dim3(512), 0, st, A, lda, m, n, Qc, ldq, mode,
-                       out, ldo, r_out, blocks, gpart, kg);
+                       out, ldo, r_out, blocks, gpart, kg, sc);
   else
     hipLaunchKernelGGL((k_proj<LT, AR, false>), dim3((unsigned)grid), dim3(512), 0, st, A, lda, m, n, Qc, ldq, mode,
-                       out, ldo, r_out, blocks, gpart, kg);
+                       out, ldo, r_out, blocks, gpart, kg, sc);
 }
 
 template <int LT>
 static void launch_proj_ar(int ar, const float* A, int64_t lda, int m, int64_t n, const float* Qc, int64_t ldq,
-                           int mode, float* out, int64_t ldo, int r_out, double* gpart, int kg, hipStream_t st) {
+                           int mode, float* out, int64_t ldo, int r_out, double* gpart, int kg, hipStream_t st,
+                           const float* sc) {
   switch (ar) {
-    case AR_FP32: launch_proj_t<LT, AR_FP32>(A, lda, m, n, Qc, ldq, mode, out, ldo, r_out, gpart, kg, st); break;
-    case AR_BF16: launch_proj_t<LT, AR_BF16>(A, lda, m, n, Qc, ldq, mode, out, ldo, r_out, gpart, kg, st); break;
-    case AR_FP16: launch_proj_t<LT, AR_FP16>(A, lda, m, n, Qc, ldq, mode, out, ldo, r_out, gpart, kg, st); break;
-    case AR_BF16X4: launch_proj_t<LT, AR_BF16X4>(A, lda, m, n, Qc, ldq, mode, out, ldo, r_out, gpart, kg, st); break;
-    default: launch_proj_t<LT, AR_BF16X3>(A, lda, m, n, Qc, ldq, mode, out, ldo, r_out, gpart, kg, st); break;
+    case AR_FP32: launch_proj_t<LT, AR_FP32>(A, lda, m, n, Qc, ldq, mode, out, ldo, r_out, gpart, kg, st, sc); break;
+    case AR_BF16: launch_proj_t<LT, AR_BF16>(A, lda, m, n, Qc, ldq, mode, out, ldo, r_out, gpart, kg, st, sc); break;
+    case AR_FP16: launch_proj_t<LT, AR_FP16>(A, lda, m, n, Qc, ldq, mode, out, ldo, r_out, gpart, kg, st, sc); break;
+    case AR_BF16X4: launch_proj_t<LT, AR_BF16X4>(A, lda, m, n, Qc, ldq, mode, out, ldo, r_out, gpart, kg, st, sc); break;
+    default: launch_proj_t<LT, AR_BF16X3>(A, lda, m, n, Qc, ldq, mode, out, ldo, r_out, gpart, kg, st, sc); break;
   }
 }
 
 void launch_proj(int ar, int kp, const float* A, int64_t lda, int m, int64_t n, const float* Qc, int64_t ldq,
-                 int mode, float* out, int64_t ldo, int r_out, hipStream_t st, double* gpart, int kg) {
-  if (kp == 32) launch_proj_ar<1>(ar, A, lda, m, n, Qc, ldq, mode, out, ldo, r_out, gpart, kg, st);
-  else launch_proj_ar<2>(ar, A, lda, m, n, Qc, ldq, mode, out, ldo, r_out, gpart, kg, st);
+                 int mode, float* out, int64_t ldo, int r_out, hipStream_t st, double* gpart, int kg,
+                 const float* scales) {
+  if (kp == 32) launch_proj_ar<1>(ar, A, lda, m, n, Qc, ldq, mode, out, ldo, r_out, gpart, kg, st, scales);
+  else launch_proj_ar<2>(ar, A, lda, m, n, Qc, ldq, mode, out, ldo, r_out, gpart, kg, st, scales);
 }
 
 int64_t gram_small_grid(int64_t n) {

@@ -300,3 +300,73 @@ def test_adaptive_power_iterations(engine, prec, case):
     assert rep["passes"][0] == 2 + 2 * want_it
     tol = TOL[prec]
     assert np.allclose(s_gpu[0][:8], s_ref[0][:8], rtol=tol["sig"], atol=tol["sig"] * s_ref[0][0])
+
+
+# ------------------------------------------------------------------------------------------------
+# FP16 range finder with power-of-two operand scaling
+# ------------------------------------------------------------------------------------------------
+def test_fp16_tt_svd_close_to_oracle(engine):
+    # binary16 sketch operands (11 significant bits) perturb the range basis by about 2^-11 relative;
+    # the projection and remainder stay 4-term bf16 accurate, so the reconstruction error stays within
+    # 2e-3 of the FP64 oracle's
+    dims, true = [8, 9, 10, 11, 6], [1, 4, 6, 6, 5, 1]
+    A = engine.generate(dims, true, std=1.0, noise_rel=1e-6, seed=1134)
+    cores, sig, rep = engine.tt_svd(A, dims, true, s=4, q=1, precision="fp16")
+    An = host(A).astype(np.float64)
+    c_ref, s_ref, p_ref, used = ttsvd.tt_svd(An, dims, true, s=4, q=1)
+    assert rep["ranks"] == used
+    err = engine.rel_error(cores, dims, used, A)
+    err_ref = ttsvd.rel_error(An, ttsvd.tt_full(c_ref, dims, used))
+    assert err <= err_ref + 2e-3, (err, err_ref)
+    for k in range(len(dims) - 1):
+        r = true[k + 1]
+        assert np.allclose(sig[k][:r], s_ref[k][:r], rtol=1e-3, atol=1e-3 * s_ref[k][0]), k
+
+
+def test_fp16_power_of_two_scaling_is_exact(engine):
+    # entries of 1e6 would overflow binary16 (max 65504); with the power-of-two operand scales the
+    # decomposition of 2^20 A is that of A scaled by 2^20 (every other step is scale-invariant)
+    import torch
+    dims, true = [6, 20, 20, 20], [1, 6, 8, 6, 1]
+    A = engine.generate(dims, true, std=1.0, noise_rel=1e-4, seed=77)
+    big = (A * float(2 ** 20)).contiguous()
+    assert float(big.abs().max()) > 65504.0
+    c1, s1, r1 = engine.tt_svd(A, dims, true, s=10, q=1, precision="fp16")
+    c2, s2, r2 = engine.tt_svd(big, dims, true, s=10, q=1, precision="fp16")
+    assert r1["path"][1] == 2    # the middle step is randomized (fp16 range finder)
+    assert np.all(np.isfinite(s2))
+    assert np.allclose(s2, s1 * 2.0 ** 20, rtol=1e-6, atol=0)
+    for k in range(len(dims)):
+        g1, g2 = host(c1[k][0]), host(c2[k][0])
+        if k == len(dims) - 1:
+            g2 = g2 / 2.0 ** 20
+        assert np.max(np.abs(g1 - g2)) <= 1e-6 * np.max(np.abs(g1)), k
+    err = engine.rel_error(c2, dims, true, big)
+    assert err < 1e-3
+
+
+def test_permute_231_bit_exact_at_2_pow_31(engine):
+    """The TR first-step relayout at 32^6 with r0 r1 = 4 x 16: 64 x 32^5 = 2^31 elements (8.6 GB in, 8.6 GB
+    out), past the int32 index range where the reference's tensorToMode231 overflows
+    (trdec6_singleGPU.cu:51).  Every element carries its own source index as its bit pattern."""
+    import torch
+    r0, r1, n = 4, 16, 32 ** 5
+    total = r0 * r1 * n
+    assert total == 2 ** 31
+    x = torch.empty(total, dtype=torch.int32, device="cuda")
+    step = 1 << 28
+    for c in range(0, total, step):
+        x[c:c + step] = torch.arange(c, c + step, dtype=torch.int64, device="cuda").to(torch.int32)
+    out = engine.permute_231(x.view(torch.float32), r0, r1, n)
+    del x
+    o = out.view(torch.int32)
+    # out position beta + r1 (j + n alpha) holds source index alpha + r0 (beta + r1 j)
+    beta = torch.arange(r1, dtype=torch.int64, device="cuda")
+    rows = 1 << 22
+    for alpha in range(r0):
+        sl = o[alpha * r1 * n:(alpha + 1) * r1 * n].view(n, r1)
+        for j0 in range(0, n, rows):
+            j = torch.arange(j0, min(n, j0 + rows), dtype=torch.int64, device="cuda")
+            want = alpha + r0 * beta[None, :] + (r0 * r1) * j[:, None]
+            assert torch.equal(sl[j0:j0 + rows].to(torch.int64), want), (alpha, j0)
+    torch.cuda.synchronize()
