@@ -165,6 +165,32 @@ ttx_status ttx_tt_full(ttx_handle h, int32_t order, const int64_t* dims, const i
 ttx_status ttx_tr_full(ttx_handle h, int32_t order, const int64_t* dims, const int32_t* ranks,
                        const float* const* d_cores, float* d_out);
 
+/* batched TT-SVD -----------------------------------------------------------------------------------
+ * Many tensors of uneven shapes and ranks (the reference decomposes one tensor per call,
+ * ttdec6_singleGPU.cu:799).  The tensors are spread round-robin over `streams` worker streams, each
+ * with its own workspace; with TTX_BATCH_GRAPH the whole batch is captured once into a hipGraph
+ * (fork / join over the worker streams) and every call with the same tensors, shapes and parameters
+ * relaunches that graph as one launch.  TTX_BATCH_AUTOTUNE times candidate sketch grids per shape
+ * class (ceil log2 m, ceil log2 n, sketch width, arithmetic) on the first unfolding of that class and
+ * keeps the fastest for the handle's lifetime.  Fixed ranks only (no host round trips inside).
+ * dims: count x order, ranks: count x (order + 1) (r_0 = r_d = 1), d_cores: count x order device
+ * pointers (ttx_core_size of each tensor's parameters). */
+#define TTX_BATCH_GRAPH 1
+#define TTX_BATCH_PER_TENSOR 2   /* also time each tensor alone: p50 / p99 */
+#define TTX_BATCH_AUTOTUNE 4
+typedef struct {
+  double total_ms;      /* device time of the batch */
+  double tensors_per_s;
+  double p50_ms, p99_ms; /* TTX_BATCH_PER_TENSOR: device time of one decomposition alone, else -1 */
+  int32_t streams;
+  int32_t graph;        /* 1: the batch ran as one graph launch */
+  int32_t graph_reused; /* 1: an identical earlier batch's graph was relaunched */
+  int32_t tuned_shapes; /* shape classes in the autotuner's table */
+} ttx_batch_report;
+ttx_status ttx_batch_tt_svd(ttx_handle h, int32_t count, int32_t order, const int64_t* dims, const int32_t* ranks,
+                            const float* const* d_tensors, float* const* d_cores, const ttx_params* common,
+                            int32_t streams, int32_t batch_flags, ttx_batch_report* rep);
+
 /* rank model (TTX_RANK_LEARNED) -------------------------------------------------------------------
  * No counterpart in the reference, whose ranks are fixed inputs (ttdec6_singleGPU.cu:977).  A
  * 2 x 128 GELU network maps 32 log-buckets of a 64-column probe sketch's spectrum (the shifted

@@ -281,7 +281,7 @@ def tt_svd(A, dims, ranks, s=10, q=2, seed=1234, method=RSVD, rank_mode="fixed",
     for k in range(d - 1):
         m = used[k] * dims[k]
         cur = cur.reshape(m, -1, order="F")
-        r = ranks[k + 1]
+        r = min(ranks[k + 1], cur.shape[0], cur.shape[1])    # min(m, n) at most, as the engine
         ad = None
         if adaptive or rank_mode != "fixed":
             # a quarter of the step's share of the squared error budget, relative to ||A_k||^2

@@ -67,6 +67,21 @@ class Report(ctypes.Structure):
     ]
 
 
+class BatchReport(ctypes.Structure):
+    _fields_ = [
+        ("total_ms", ctypes.c_double),
+        ("tensors_per_s", ctypes.c_double),
+        ("p50_ms", ctypes.c_double),
+        ("p99_ms", ctypes.c_double),
+        ("streams", ctypes.c_int32),
+        ("graph", ctypes.c_int32),
+        ("graph_reused", ctypes.c_int32),
+        ("tuned_shapes", ctypes.c_int32),
+    ]
+
+
+BATCH_GRAPH, BATCH_PER_TENSOR, BATCH_AUTOTUNE = 1, 2, 4
+
 _P = ctypes.c_void_p
 _I32 = ctypes.c_int32
 _I64 = ctypes.c_int64
@@ -102,6 +117,8 @@ SIGNATURES = {
                                 ctypes.POINTER(_D)]),
     "ttx_tt_full": (_I32, [_P, _I32, ctypes.POINTER(_I64), ctypes.POINTER(_I32), _FPP, _P]),
     "ttx_tr_full": (_I32, [_P, _I32, ctypes.POINTER(_I64), ctypes.POINTER(_I32), _FPP, _P]),
+    "ttx_batch_tt_svd": (_I32, [_P, _I32, _I32, ctypes.POINTER(_I64), ctypes.POINTER(_I32), _FPP, _FPP,
+                                ctypes.POINTER(Params), _I32, _I32, ctypes.POINTER(BatchReport)]),
     "ttx_mlp_init": (_I32, [_P, _U64]),
     "ttx_mlp_set_params": (_I32, [_P, ctypes.POINTER(_F)]),
     "ttx_mlp_get_params": (_I32, [_P, ctypes.POINTER(_F)]),

@@ -180,6 +180,42 @@ class Engine:
         """TR-SVD (first step rank ranks[0] * ranks[1]); returns ([(core, shape)], sigma, report)."""
         return self._sweep(self.lib.ttx_tr_svd, A, dims, ranks, True, **kw)
 
+    def batch_tt_svd(self, tensors, dims_list, ranks_list, s=10, q=2, seed=1234, precision="bf16", flags=0,
+                     streams=4, graph=True, autotune=True, per_tensor=False, cores=None):
+        """TT-SVD of many tensors with fixed ranks (ttx_batch_tt_svd).  tensors: list of contiguous float32
+        CUDA tensors; all of one order.  Returns (cores per tensor [(core, shape)], report dict).  Pass the
+        returned core buffers back as ``cores`` to relaunch the same captured graph."""
+        torch = _torch()
+        count = len(tensors)
+        d = len(dims_list[0])
+        if any(len(x) != d for x in dims_list) or any(len(r) != d + 1 for r in ranks_list):
+            raise ValueError("ttx: one order for the whole batch")
+        if cores is None:
+            cores = []
+            for dims, ranks in zip(dims_list, ranks_list):
+                p = self.params(dims, ranks)
+                cores.append([torch.empty(int(self.lib.ttx_core_size(ctypes.byref(p), 0, k)), device="cuda",
+                                          dtype=torch.float32) for k in range(d)])
+        dims_flat = L.i64_array([x for dims in dims_list for x in dims])
+        ranks_flat = L.i32_array([x for ranks in ranks_list for x in ranks])
+        tptr = L.ptr_array([_dev_ptr(t) for t in tensors])
+        cptr = L.ptr_array([c.data_ptr() for cs in cores for c in cs])
+        common = self.params(dims_list[0], ranks_list[0], s=s, q=q, seed=seed, precision=precision, flags=flags)
+        bflags = (L.BATCH_GRAPH if graph else 0) | (L.BATCH_AUTOTUNE if autotune else 0) | \
+                 (L.BATCH_PER_TENSOR if per_tensor else 0)
+        rep = L.BatchReport()
+        st = self.lib.ttx_batch_tt_svd(self.h, count, d, dims_flat, ranks_flat, tptr, cptr, ctypes.byref(common),
+                                       streams, bflags, ctypes.byref(rep))
+        _check(self.lib, self.h, st)
+        out = []
+        for cs, dims, ranks in zip(cores, dims_list, ranks_list):
+            shapes = [(ranks[k] * dims[k], ranks[k + 1]) for k in range(d - 1)] + [(ranks[d - 1], dims[d - 1])]
+            out.append([(c[: a * b], (a, b)) for c, (a, b) in zip(cs, shapes)])
+        info = {"total_ms": rep.total_ms, "tensors_per_s": rep.tensors_per_s, "p50_ms": rep.p50_ms,
+                "p99_ms": rep.p99_ms, "streams": rep.streams, "graph": rep.graph, "graph_reused": rep.graph_reused,
+                "tuned_shapes": rep.tuned_shapes, "core_buffers": cores}
+        return out, info
+
     def rsvd(self, A, m, n, k, s=10, q=2, seed=1234, precision="bf16"):
         torch = _torch()
         ell = k + s
@@ -264,6 +300,17 @@ class Engine:
         st = fn(self.h, len(dims), L.i64_array(dims), L.i32_array(ranks), ptrs, ctypes.c_void_p(out.data_ptr()))
         _check(self.lib, self.h, st)
         return out
+
+
+def reachable_ranks(dims, ranks):
+    """TT ranks the sweep keeps: r_k = min(r_k, r_{k-1} N_k, N_{k+1} ... N_d), left to right"""
+    out = [1]
+    for k in range(1, len(dims)):
+        m = out[k - 1] * dims[k - 1]
+        n = int(np.prod(dims[k:]))
+        out.append(int(min(ranks[k], m, n)))
+    out.append(1)
+    return out
 
 
 def report_dict(rep, d):

@@ -8,6 +8,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -44,6 +45,18 @@ struct Handle {
   size_t evnext = 0;
   struct KRec { hipEvent_t a, b; int kind; int64_t bytes; };
   std::vector<KRec> krecs;
+  // batches: capture = queue only (no host synchronisation, no readback); strict = under graph
+  // capture (no allocation either); sub-handles run the tensors on their own streams
+  bool capture = false;
+  bool strict = false;
+  bool autotune = false;
+  std::map<uint64_t, int> tune_own;          // autotuned sketch WG counts per shape class
+  std::map<uint64_t, int>* tune = nullptr;   // the root handle's table (sub-handles share it)
+  std::vector<Handle*> subs;
+  hipStream_t bstream = nullptr;             // capture origin of the batch graph (never the null stream)
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  uint64_t gsig = 0;
 };
 
 static hipEvent_t next_event(Handle* h) {
@@ -86,6 +99,7 @@ static ttx_status fail(Handle* h, ttx_status s, const std::string& msg) {
 
 static ttx_status ensure(Handle* h, DevBuf& b, size_t bytes) {
   if (b.bytes >= bytes && b.p) return TTX_OK;
+  if (h->strict) return fail(h, TTX_ERR_HIP, "workspace growth under graph capture");
   if (b.p) {
     (void)hipStreamSynchronize(h->stream);
     (void)hipFree(b.p);
@@ -325,6 +339,71 @@ static float* left_mix(Handle* h, const StepArgs& a, float* Y, int kp) {
     launch_blockmix(a.pend, a.pend_ms, (int)a.m, kp, 0, Y, P<float>(h->Y2), h->stream);
   }
   return P<float>(h->Y2);
+}
+
+// ------------------------------------------------------------------------------------------------
+// autotuner for the sketch's grid: per shape class (ceil log2 m, ceil log2 n, kp, arithmetic) the
+// number of WGs along the columns, chosen by timing candidate grids on the first unfolding of that
+// class (outputs discarded: the real launch follows on the same buffers)
+// ------------------------------------------------------------------------------------------------
+static int ceil_log2(int64_t v) {
+  int e = 0;
+  while (((int64_t)1 << e) < v) ++e;
+  return e;
+}
+static uint64_t tune_key(int m, int64_t n, int kp, int ar) {
+  return ((uint64_t)ceil_log2(m) << 32) | ((uint64_t)ceil_log2(n) << 16) | ((uint64_t)kp << 4) | (uint64_t)ar;
+}
+static std::map<uint64_t, int>& tune_table(Handle* h) { return h->tune ? *h->tune : h->tune_own; }
+
+static ttx_status autotune_sketch(Handle* h, int ar, const float* A, int m, int64_t n, int kp, int* best) {
+  const int cus = num_cus();
+  const int cand[] = {cus / 4, cus / 2, cus, 2 * cus, 4 * cus};
+  hipEvent_t e0, e1;
+  TTX_HIP(h, hipEventCreate(&e0));
+  TTX_HIP(h, hipEventCreate(&e1));
+  float best_ms = 1e30f;
+  *best = 0;
+  int last_gx = -1;
+  for (int c : cand) {
+    if (c < 1) continue;
+    const SketchPlan p = plan_sketch(m, n, kp, c);
+    if (p.grid_x == last_gx) continue;   // capped by the number of chunks
+    last_gx = p.grid_x;
+    TTX_TRY(ensure(h, h->part, (size_t)p.slabs * m * kp * sizeof(float)));
+    float total = 0.0f;
+    for (int rep = 0; rep < 3; ++rep) {
+      TTX_HIP(h, hipEventRecord(e0, h->stream));
+      launch_sketch(ar, p, A, m, m, n, nullptr, 0, 1, 0x7E57u, kp, P<float>(h->part), h->stream);
+      TTX_HIP(h, hipEventRecord(e1, h->stream));
+      TTX_HIP(h, hipEventSynchronize(e1));
+      float ms = 0.0f;
+      TTX_HIP(h, hipEventElapsedTime(&ms, e0, e1));
+      if (rep > 0) total += ms;    // the first launch warms the caches
+    }
+    if (total < best_ms) {
+      best_ms = total;
+      *best = c;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return check_launch(h, "autotune");
+}
+
+static ttx_status plan_for(Handle* h, int ar, const float* A, int m, int64_t n, int kp, SketchPlan* out) {
+  auto& T = tune_table(h);
+  const uint64_t key = tune_key(m, n, kp, ar);
+  auto it = T.find(key);
+  int hint = 0;
+  if (it != T.end()) {
+    hint = it->second;
+  } else if (h->autotune && !h->strict) {
+    TTX_TRY(autotune_sketch(h, ar, A, m, n, kp, &hint));
+    T[key] = hint;
+  }
+  *out = plan_sketch(m, n, kp, hint);
+  return TTX_OK;
 }
 
 // FP16 range finder: power-of-two operand scales (A from the previous step's kept energy)
@@ -624,7 +703,8 @@ static ttx_status run_step(Handle* h, const StepArgs& a_in, StepOut& o) {
     int pt = adaptive ? std::min(8, TTX_MAX_ELL - ell) : 0;
     if (pt < 2) pt = 0;
     const int kp = kp_for(ell + pt);
-    const SketchPlan sp = plan_sketch(m, n, kp, 0);
+    SketchPlan sp;
+    TTX_TRY(plan_for(h, ar.range, a.A, m, n, kp, &sp));
     TTX_TRY(ensure(h, h->part, (size_t)sp.slabs * m * kp * sizeof(float)));
     TTX_TRY(ensure(h, h->Y, (size_t)m * kp * sizeof(float)));
     TTX_TRY(ensure(h, h->Y2, (size_t)m * kp * sizeof(float)));
@@ -766,7 +846,7 @@ static ttx_status sweep(Handle* h, const float* d_A, const ttx_params* p, float*
   TTX_TRY(ensure(h, h->scratch[3], (size_t)TTX_MAX_ORDER * (TTX_MAX_ELL * sizeof(float) + 4 * sizeof(int))));
   float* sig_all = P<float>(h->scratch[3]);
   int* stat_all = reinterpret_cast<int*>(sig_all + TTX_MAX_ORDER * TTX_MAX_ELL);
-  TTX_HIP(h, hipMemsetAsync(h->scratch[3].p, 0, h->scratch[3].bytes, st));
+  launch_zero(h->scratch[3].p, (int64_t)h->scratch[3].bytes, st);
   const bool host_ranks = p->rank_mode != TTX_RANK_FIXED;
   StepOut outs[TTX_MAX_ORDER];
 
@@ -788,6 +868,9 @@ static ttx_status sweep(Handle* h, const float* d_A, const ttx_params* p, float*
     a.m = m;
     a.n = n;
     a.r = (tr && k == 0) ? p->ranks[0] * p->ranks[1] : p->ranks[k + 1];
+    // a rank above min(m, n) cannot be reached: the step keeps min(m, n) (the TR first step keeps its
+    // r0 r1 columns, zero-padded, because the relayout needs them)
+    if (!(tr && k == 0)) a.r = (int)std::min<int64_t>(a.r, std::min<int64_t>(m, n));
     a.s = p->oversampling;
     a.q = p->power_iters;
     a.seed = p->omega_seed;
@@ -887,6 +970,10 @@ static ttx_status sweep(Handle* h, const float* d_A, const ttx_params* p, float*
       R.ranks[k + 1] = o.r;
     }
     if (timing) TTX_HIP(h, hipEventRecord(ev[k + 1], st));
+  }
+  if (h->capture) {
+    if (rep) *rep = R;
+    return check_launch(h, "batch sweep");
   }
   if (!host_ranks) {
     std::vector<float> hs((size_t)TTX_MAX_ORDER * TTX_MAX_ELL);
@@ -1130,10 +1217,13 @@ ttx_status ttx_create(ttx_handle* out, void* stream) {
   return TTX_OK;
 }
 
-ttx_status ttx_destroy(ttx_handle hh) {
-  Handle* h = H(hh);
-  if (!h) return TTX_OK;
+static void free_handle(Handle* h, bool own_stream) {
   (void)hipStreamSynchronize(h->stream);
+  for (Handle* sub : h->subs) free_handle(sub, true);
+  h->subs.clear();
+  if (h->gexec) (void)hipGraphExecDestroy(h->gexec);
+  if (h->graph) (void)hipGraphDestroy(h->graph);
+  if (h->bstream) (void)hipStreamDestroy(h->bstream);
   ttx::DevBuf* bufs[] = {&h->part, &h->Y, &h->Y2, &h->Qc, &h->Qp, &h->Qn, &h->Z, &h->W1, &h->W2, &h->G, &h->G2,
                          &h->Rinv, &h->lam,
                          &h->V, &h->Ur, &h->gpart, &h->status, &h->next[0], &h->next[1], &h->sig, &h->keys,
@@ -1148,7 +1238,14 @@ ttx_status ttx_destroy(ttx_handle hh) {
   if (h->h_sigma) (void)hipHostFree(h->h_sigma);
   if (h->h_lam) (void)hipHostFree(h->h_lam);
   if (h->h_aux) (void)hipHostFree(h->h_aux);
+  if (own_stream && h->stream) (void)hipStreamDestroy(h->stream);
   delete h;
+}
+
+ttx_status ttx_destroy(ttx_handle hh) {
+  Handle* h = H(hh);
+  if (!h) return TTX_OK;
+  free_handle(h, false);
   return TTX_OK;
 }
 
@@ -1454,3 +1551,225 @@ ttx_status ttx_probe(ttx_handle hh, const float* d_A, int64_t m, int64_t n, uint
 }
 
 }  // extern "C"
+
+// ------------------------------------------------------------------------------------------------
+// batched TT-SVD
+// ------------------------------------------------------------------------------------------------
+namespace ttx {
+
+static ttx_status make_subs(Handle* h, int S) {
+  if (!h->bstream && hipStreamCreateWithFlags(&h->bstream, hipStreamNonBlocking) != hipSuccess) {
+    h->bstream = nullptr;
+    return fail(h, TTX_ERR_HIP, "cannot create the batch stream");
+  }
+  while ((int)h->subs.size() < S) {
+    Handle* s = new Handle();
+    s->dev = h->dev;
+    if (hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipHostMalloc((void**)&s->h_status, 16 * sizeof(int), 0) != hipSuccess ||
+        hipHostMalloc((void**)&s->h_sigma, TTX_MAX_ELL * sizeof(float), 0) != hipSuccess ||
+        hipHostMalloc((void**)&s->h_lam, TTX_MAX_ELL * sizeof(double), 0) != hipSuccess ||
+        hipHostMalloc((void**)&s->h_aux, 16 * sizeof(double), 0) != hipSuccess) {
+      (void)hipGetLastError();
+      free_handle(s, s->stream != nullptr);
+      return fail(h, TTX_ERR_HIP, "cannot create a batch worker stream");
+    }
+    s->tune = &tune_table(h);
+    h->subs.push_back(s);
+  }
+  return TTX_OK;
+}
+
+static uint64_t fnv(uint64_t hsh, const void* p, size_t bytes) {
+  const unsigned char* c = static_cast<const unsigned char*>(p);
+  for (size_t i = 0; i < bytes; ++i) hsh = (hsh ^ c[i]) * 0x100000001B3ull;
+  return hsh;
+}
+
+// queue every tensor's sweep on its worker stream (no host synchronisation)
+static ttx_status batch_queue(Handle* h, int S, const std::vector<ttx_params>& P, const float* const* tensors,
+                              float* const* cores, int order) {
+  for (size_t i = 0; i < P.size(); ++i) {
+    Handle* sub = h->subs[i % S];
+    const ttx_status st = sweep(sub, tensors[i], &P[i], cores + i * order, nullptr, nullptr, false);
+    if (st != TTX_OK) return fail(h, st, "tensor " + std::to_string(i) + ": " + sub->err);
+  }
+  return TTX_OK;
+}
+
+static ttx_status batch_fork(Handle* h, hipStream_t origin, int S, hipEvent_t fork) {
+  TTX_HIP(h, hipEventRecord(fork, origin));
+  for (int s = 0; s < S; ++s) TTX_HIP(h, hipStreamWaitEvent(h->subs[s]->stream, fork, 0));
+  return TTX_OK;
+}
+static ttx_status batch_join(Handle* h, hipStream_t origin, int S, std::vector<hipEvent_t>& joins) {
+  for (int s = 0; s < S; ++s) {
+    TTX_HIP(h, hipEventRecord(joins[s], h->subs[s]->stream));
+    TTX_HIP(h, hipStreamWaitEvent(origin, joins[s], 0));
+  }
+  return TTX_OK;
+}
+
+static ttx_status batch_run(Handle* h, int count, int order, const int64_t* dims, const int32_t* ranks,
+                            const float* const* tensors, float* const* cores, const ttx_params* common, int S,
+                            int bflags, ttx_batch_report* rep) {
+  if (count <= 0 || order < 2 || order > TTX_MAX_ORDER || !dims || !ranks || !tensors || !cores || !common)
+    return fail(h, TTX_ERR_BAD_ARG, "batch: bad arguments");
+  if (S < 1 || S > 16) return fail(h, TTX_ERR_BAD_ARG, "batch: 1..16 streams");
+  if (common->rank_mode != TTX_RANK_FIXED) return fail(h, TTX_ERR_UNSUPPORTED, "batch: fixed ranks only");
+  if (common->flags & TTX_FLAG_TIMING) return fail(h, TTX_ERR_UNSUPPORTED, "batch: no per-kernel timing");
+  std::vector<ttx_params> P(count);
+  for (int i = 0; i < count; ++i) {
+    ttx_params_init(&P[i], order, dims + (size_t)i * order, ranks + (size_t)i * (order + 1));
+    P[i].oversampling = common->oversampling;
+    P[i].power_iters = common->power_iters;
+    P[i].omega_seed = common->omega_seed;
+    P[i].precision = common->precision;
+    P[i].method = common->method;
+    P[i].rank_mode = TTX_RANK_FIXED;
+    P[i].eps = common->eps;
+    P[i].flags = common->flags;
+    TTX_TRY(validate(h, &P[i], false));
+    if (!tensors[i]) return fail(h, TTX_ERR_BAD_ARG, "batch: null tensor");
+    for (int k = 0; k < order; ++k)
+      if (!cores[(size_t)i * order + k]) return fail(h, TTX_ERR_BAD_ARG, "batch: null core");
+  }
+  TTX_TRY(make_subs(h, S));
+  const bool graph = (bflags & TTX_BATCH_GRAPH) != 0;
+  uint64_t sig = 0xCBF29CE484222325ull;
+  sig = fnv(sig, &count, sizeof(count));
+  sig = fnv(sig, &order, sizeof(order));
+  sig = fnv(sig, &S, sizeof(S));
+  sig = fnv(sig, dims, sizeof(int64_t) * count * order);
+  sig = fnv(sig, ranks, sizeof(int32_t) * count * (order + 1));
+  sig = fnv(sig, tensors, sizeof(void*) * count);
+  sig = fnv(sig, cores, sizeof(void*) * count * order);
+  sig = fnv(sig, &P[0].oversampling, sizeof(int32_t) * 2);
+  sig = fnv(sig, &P[0].omega_seed, sizeof(uint64_t));
+  sig = fnv(sig, &P[0].precision, sizeof(int32_t) * 2);
+  sig = fnv(sig, &P[0].flags, sizeof(int32_t));
+  const bool reuse = graph && h->gexec && h->gsig == sig;
+
+  hipEvent_t fork = nullptr, e0 = nullptr, e1 = nullptr;
+  std::vector<hipEvent_t> joins(S, nullptr);
+  auto cleanup = [&]() {
+    for (Handle* sub : h->subs) { sub->capture = false; sub->strict = false; sub->autotune = false; }
+    if (fork) (void)hipEventDestroy(fork);
+    for (hipEvent_t e : joins) if (e) (void)hipEventDestroy(e);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+  };
+  ttx_status st = TTX_OK;
+  do {
+    if (hipEventCreateWithFlags(&fork, hipEventDisableTiming) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
+        hipEventCreate(&e1) != hipSuccess) { st = fail(h, TTX_ERR_HIP, "batch: event creation"); break; }
+    for (int s = 0; s < S; ++s)
+      if (hipEventCreateWithFlags(&joins[s], hipEventDisableTiming) != hipSuccess) {
+        st = fail(h, TTX_ERR_HIP, "batch: event creation");
+        break;
+      }
+    if (st != TTX_OK) break;
+    for (Handle* sub : h->subs) { sub->capture = true; sub->autotune = (bflags & TTX_BATCH_AUTOTUNE) != 0; }
+    if (!reuse) {
+      // eager pass: sizes every worker's workspace and runs the autotuner
+      if ((st = batch_fork(h, h->stream, S, fork)) != TTX_OK) break;
+      if ((st = batch_queue(h, S, P, tensors, cores, order)) != TTX_OK) break;
+      if ((st = batch_join(h, h->stream, S, joins)) != TTX_OK) break;
+      if (hipStreamSynchronize(h->stream) != hipSuccess) { st = fail(h, TTX_ERR_HIP, "batch: eager pass"); break; }
+      if (graph) {
+        if (h->gexec) { (void)hipGraphExecDestroy(h->gexec); h->gexec = nullptr; }
+        if (h->graph) { (void)hipGraphDestroy(h->graph); h->graph = nullptr; }
+        for (Handle* sub : h->subs) sub->strict = true;
+        if (hipStreamBeginCapture(h->bstream, hipStreamCaptureModeRelaxed) != hipSuccess) {
+          (void)hipGetLastError();
+          st = fail(h, TTX_ERR_HIP, "batch: begin capture");
+          break;
+        }
+        ttx_status qs = batch_fork(h, h->bstream, S, fork);
+        if (qs == TTX_OK) qs = batch_queue(h, S, P, tensors, cores, order);
+        if (qs == TTX_OK) qs = batch_join(h, h->bstream, S, joins);
+        hipGraph_t g = nullptr;
+        const hipError_t ce = hipStreamEndCapture(h->bstream, &g);
+        for (Handle* sub : h->subs) sub->strict = false;
+        if (qs != TTX_OK || ce != hipSuccess || !g) {
+          if (g) (void)hipGraphDestroy(g);
+          (void)hipGetLastError();
+          st = (qs != TTX_OK) ? qs : fail(h, TTX_ERR_HIP, "batch: end capture");
+          break;
+        }
+        h->graph = g;
+        if (hipGraphInstantiate(&h->gexec, g, nullptr, nullptr, 0) != hipSuccess) {
+          h->gexec = nullptr;
+          st = fail(h, TTX_ERR_HIP, "batch: graph instantiation");
+          break;
+        }
+        h->gsig = sig;
+      }
+    }
+    // the measured run
+    if (hipEventRecord(e0, h->stream) != hipSuccess) { st = fail(h, TTX_ERR_HIP, "batch: event"); break; }
+    if (graph) {
+      // the caller's stream -> the graph on the batch stream -> the caller's stream
+      if (hipEventRecord(fork, h->stream) != hipSuccess || hipStreamWaitEvent(h->bstream, fork, 0) != hipSuccess ||
+          hipGraphLaunch(h->gexec, h->bstream) != hipSuccess || hipEventRecord(joins[0], h->bstream) != hipSuccess ||
+          hipStreamWaitEvent(h->stream, joins[0], 0) != hipSuccess) {
+        st = fail(h, TTX_ERR_HIP, "batch: graph launch");
+        break;
+      }
+    } else {
+      if ((st = batch_fork(h, h->stream, S, fork)) != TTX_OK) break;
+      if ((st = batch_queue(h, S, P, tensors, cores, order)) != TTX_OK) break;
+      if ((st = batch_join(h, h->stream, S, joins)) != TTX_OK) break;
+    }
+    if (hipEventRecord(e1, h->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) {
+      st = fail(h, TTX_ERR_HIP, "batch: run");
+      break;
+    }
+    float ms = 0.0f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    ttx_batch_report R;
+    memset(&R, 0, sizeof(R));
+    R.total_ms = ms;
+    R.tensors_per_s = ms > 0 ? count / (ms * 1e-3) : 0.0;
+    R.p50_ms = R.p99_ms = -1.0;
+    R.streams = S;
+    R.graph = graph ? 1 : 0;
+    R.graph_reused = reuse ? 1 : 0;
+    R.tuned_shapes = (int)tune_table(h).size();
+    if (bflags & TTX_BATCH_PER_TENSOR) {
+      // each tensor alone on one worker stream (queued work only, as in the batch)
+      Handle* sub = h->subs[0];
+      std::vector<double> t(count);
+      for (int i = 0; i < count && st == TTX_OK; ++i) {
+        if (hipEventRecord(e0, sub->stream) != hipSuccess) { st = fail(h, TTX_ERR_HIP, "batch: event"); break; }
+        const ttx_status s2 = sweep(sub, tensors[i], &P[i], cores + (size_t)i * order, nullptr, nullptr, false);
+        if (s2 != TTX_OK) { st = fail(h, s2, sub->err); break; }
+        if (hipEventRecord(e1, sub->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) {
+          st = fail(h, TTX_ERR_HIP, "batch: per-tensor run");
+          break;
+        }
+        float ti = 0.0f;
+        (void)hipEventElapsedTime(&ti, e0, e1);
+        t[i] = ti;
+      }
+      if (st != TTX_OK) break;
+      std::sort(t.begin(), t.end());
+      R.p50_ms = t[(size_t)((count - 1) * 0.50)];
+      R.p99_ms = t[(size_t)((count - 1) * 0.99)];
+    }
+    if (rep) *rep = R;
+  } while (false);
+  cleanup();
+  return st;
+}
+
+}  // namespace ttx
+
+extern "C" ttx_status ttx_batch_tt_svd(ttx_handle hh, int32_t count, int32_t order, const int64_t* dims,
+                                       const int32_t* ranks, const float* const* d_tensors, float* const* d_cores,
+                                       const ttx_params* common, int32_t streams, int32_t batch_flags,
+                                       ttx_batch_report* rep) {
+  Handle* h = H(hh);
+  if (!h) return TTX_ERR_BAD_ARG;
+  return ttx::batch_run(h, count, order, dims, ranks, d_tensors, d_cores, common, streams, batch_flags, rep);
+}

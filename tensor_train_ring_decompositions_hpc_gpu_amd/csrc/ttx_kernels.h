@@ -99,6 +99,8 @@ void launch_copy_cols(const float* A, int64_t lda, int m, int n, float* Y, int k
 
 // ---- generators / reconstruction (ttx_rng.hip, ttx_gemm.hip) -----------------------------------
 void launch_fill_normal(float* out, int64_t count, uint64_t seed, uint32_t stream, float scale, hipStream_t st);
+// zero bytes (a multiple of 4) with a kernel
+void launch_zero(void* p, int64_t bytes, hipStream_t st);
 void launch_add_noise(float* A, int64_t count, uint64_t seed, uint32_t stream, float coef, hipStream_t st);
 void launch_sumsq(const float* A, int64_t count, double* out2 /* [1] */, double* part, int64_t grid, hipStream_t st);
 int64_t sumsq_grid(int64_t count);

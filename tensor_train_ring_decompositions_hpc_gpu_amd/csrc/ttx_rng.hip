@@ -101,4 +101,18 @@ void launch_omega(float* out, int64_t n, int ell, uint64_t seed, uint32_t stream
   hipLaunchKernelGGL(k_omega, dim3(g), dim3(256), 0, st, out, n, ell, seed, stream);
 }
 
+// zero-fill (a kernel, so that captured graphs hold no memset nodes)
+__global__ void k_zero_words(uint32_t* __restrict__ p, int64_t words) {
+  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < words; e += (int64_t)gridDim.x * blockDim.x)
+    p[e] = 0u;
+}
+
+void launch_zero(void* p, int64_t bytes, hipStream_t st) {
+  const int64_t words = bytes / 4;
+  if (words <= 0) return;
+  int64_t g = (words + 255) / 256;
+  if (g > 1024) g = 1024;
+  hipLaunchKernelGGL(k_zero_words, dim3((unsigned)g), dim3(256), 0, st, reinterpret_cast<uint32_t*>(p), words);
+}
+
 }  // namespace ttx
