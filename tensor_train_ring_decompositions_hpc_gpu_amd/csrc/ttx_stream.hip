@@ -289,12 +289,16 @@ __global__ __launch_bounds__(512) void k_proj(const float* __restrict__ A, int64
 #pragma unroll
     for (int lt = 0; lt < LT; ++lt) acc[lt] = zero16();
 
-    auto fill = [&](float* q, int kc) {
-      // 64 rows x KP columns; thread handles float4 pieces along rows
-      for (int idx = threadIdx.x; idx < KP * 16; idx += 512) {
+    // Q chunk kc (64 rows x KP columns): QF float4 pieces per thread, loaded to registers one chunk
+    // ahead (qload) and written to LDS after the MFMAs of the current chunk (qstore), so the A loads
+    // and the MFMAs never wait on these (L2) loads
+    constexpr int QF = KP * 16 / 512;
+    auto qload = [&](f32x4 (&qr)[QF], int kc) {
+#pragma unroll
+      for (int u = 0; u < QF; ++u) {
+        const int idx = threadIdx.x + 512 * u;
         const int c = idx / 16;
-        const int i4 = (idx % 16) * 4;
-        const int i = kc * 64 + i4;
+        const int i = kc * 64 + (idx % 16) * 4;
         f32x4 v;
         if (((ldq & 3) == 0) && i + 3 < m) {
           v = *reinterpret_cast<const f32x4*>(Qc + (int64_t)c * ldq + i);
@@ -302,7 +306,14 @@ __global__ __launch_bounds__(512) void k_proj(const float* __restrict__ A, int64
 #pragma unroll
           for (int e = 0; e < 4; ++e) v[e] = (i + e < m) ? Qc[(int64_t)c * ldq + i + e] : 0.0f;
         }
-        *reinterpret_cast<f32x4*>(q + c * QP + i4) = v;
+        qr[u] = v;
+      }
+    };
+    auto qstore = [&](const f32x4 (&qr)[QF], float* q) {
+#pragma unroll
+      for (int u = 0; u < QF; ++u) {
+        const int idx = threadIdx.x + 512 * u;
+        *reinterpret_cast<f32x4*>(q + (idx / 16) * QP + (idx % 16) * 4) = qr[u];
       }
     };
     auto load = [&](float (&x)[32], int kc) {
@@ -381,22 +392,28 @@ __global__ __launch_bounds__(512) void k_proj(const float* __restrict__ A, int64
     };
 
     float x0[32], x1[32];
-    fill(qs[0], 0);
+    f32x4 qr[QF];
+    qload(qr, 0);
     load(x0, 0);
+    qstore(qr, qs[0]);
     __syncthreads();
     for (int kc = 0; kc < nk; kc += 2) {
       if (kc + 1 < nk) {
-        fill(qs[1], kc + 1);
         load(x1, kc + 1);
+        qload(qr, kc + 1);
       }
       compute(x0, qs[0]);
+      __builtin_amdgcn_sched_barrier(0);
+      if (kc + 1 < nk) qstore(qr, qs[1]);
       __syncthreads();
       if (kc + 1 >= nk) break;
       if (kc + 2 < nk) {
-        fill(qs[0], kc + 2);
         load(x0, kc + 2);
+        qload(qr, kc + 2);
       }
       compute(x1, qs[1]);
+      __builtin_amdgcn_sched_barrier(0);
+      if (kc + 2 < nk) qstore(qr, qs[0]);
       __syncthreads();
     }
 
@@ -510,8 +527,8 @@ __global__ __launch_bounds__(256) void k_gram_small(const float* __restrict__ A,
   f32x16 acc[NP];
 #pragma unroll
   for (int p = 0; p < NP; ++p) acc[p] = zero16();
-  for (int64_t jc = j0 + 16 * w; jc < j1; jc += 64) {
-    float v[MT][8];
+  // the loads of step jc + 64 are issued before the MFMAs of step jc (register double buffer)
+  auto loadv = [&](float (&v)[MT][8], int64_t jc) {
 #pragma unroll
     for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
@@ -519,8 +536,20 @@ __global__ __launch_bounds__(256) void k_gram_small(const float* __restrict__ A,
         const int64_t j = jc + 8 * b + t;
         const int i = 32 * mt + r;
         v[mt][t] = (j < j1 && i < m) ? A[i + j * lda] : 0.0f;
+      }
+  };
+  float vn[MT][8];
+  loadv(vn, j0 + 16 * w);
+  for (int64_t jc = j0 + 16 * w; jc < j1; jc += 64) {
+    float v[MT][8];
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+      for (int t = 0; t < 8; ++t) {
+        v[mt][t] = vn[mt][t];
         amax = fmaxf(amax, fabsf(v[mt][t]));
       }
+    if (jc + 64 < j1) loadv(vn, jc + 64);
     if constexpr (AR == AR_FP32) {
 #pragma unroll
       for (int t = 0; t < 8; ++t) {
@@ -797,7 +826,8 @@ void launch_proj(int ar, int kp, const float* A, int64_t lda, int m, int64_t n, 
 }
 
 int64_t gram_small_grid(int64_t n) {
-  int64_t g = (int64_t)num_cus() * 2;
+  // four 4-wave WGs per CU (the kernel's registers allow five)
+  int64_t g = (int64_t)num_cus() * 4;
   const int64_t minc = 4096;   // at least this many columns per WG
   if (g * minc > n) g = (n + minc - 1) / minc;
   if (g < 1) g = 1;
