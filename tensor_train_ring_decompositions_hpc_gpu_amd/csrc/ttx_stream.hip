@@ -264,12 +264,17 @@ __global__ __launch_bounds__(512) void k_proj(const float* __restrict__ A, int64
                                               int64_t blocks_total, double* __restrict__ gpart, int kg,
                                               const float* __restrict__ scales) {
   constexpr int KP = 32 * LT;
-  constexpr int QP = 64 + 4;   // LDS pitch of a Q column chunk
+  constexpr int QP = 64 + 8;   // LDS pitch of a Q column chunk (floats; bf16 halves use the same pitch)
+  // bf16 arithmetic keeps the Q chunk in LDS already split: hi in the first half of a buffer, lo in
+  // the second (pitch QP bf16), so no wave re-splits Q
+  constexpr bool QSPLIT = (AR == AR_BF16 || AR == AR_BF16X3 || AR == AR_BF16X4);
   // FP16: A is scaled by the power of two scales[0] before rounding, the products are unscaled
   const float sa = (AR == AR_FP16 && scales) ? scales[0] : 1.0f;
   constexpr int NPG = LT * (LT + 1) / 2;
+  constexpr int SP = 64 + 4;   // LDS pitch of a staged A column chunk
   __shared__ float qs[2][KP * QP];
-  __shared__ float dt[GRAM ? 8 : 1][GRAM ? 32 * KP : 1];
+  // per-wave staging of its A tile (32 columns x 64 rows); the Gram tile reuses it after the K loop
+  __shared__ float stg[8][32 * SP];
   __shared__ double gs[GRAM ? KP * KP : 1];
   f32x16 gacc[GRAM ? NPG : 1];
   if constexpr (GRAM) {
@@ -313,32 +318,76 @@ __global__ __launch_bounds__(512) void k_proj(const float* __restrict__ A, int64
 #pragma unroll
       for (int u = 0; u < QF; ++u) {
         const int idx = threadIdx.x + 512 * u;
-        *reinterpret_cast<f32x4*>(q + (idx / 16) * QP + (idx % 16) * 4) = qr[u];
+        if constexpr (QSPLIT) {
+          __bf16* qh = reinterpret_cast<__bf16*>(q);
+          __bf16* ql = qh + KP * QP;
+          typedef __attribute__((__vector_size__(4 * sizeof(__bf16)))) __bf16 bf16x4;
+          bf16x4 h, l;
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            const __bf16 hv = (__bf16)qr[u][e];
+            h[e] = hv;
+            l[e] = (__bf16)(qr[u][e] - (float)hv);
+          }
+          const int o = (idx / 16) * QP + (idx % 16) * 4;
+          *reinterpret_cast<bf16x4*>(qh + o) = h;
+          *reinterpret_cast<bf16x4*>(ql + o) = l;
+        } else {
+          *reinterpret_cast<f32x4*>(q + (idx / 16) * QP + (idx % 16) * 4) = qr[u];
+        }
       }
     };
+    // coalesced loads of the wave's tile: load u, lane l takes rows 4 (l % 16) .. +3 of tile column
+    // 4 u + l / 16 (16 lanes read one column's 256 bytes); the tile is transposed to the MFMA layout
+    // through the wave's LDS staging area
+    const int lc = lane >> 4, lr = (lane & 15) * 4;
     auto load = [&](float (&x)[32], int kc) {
-      const int i0 = kc * 64 + 32 * b;
-      if (j < n) {
-        const float* p = A + j * lda + i0;
-        if (vec && i0 + 31 < m) {
+      const int i = kc * 64 + lr;
 #pragma unroll
-          for (int t = 0; t < 8; ++t) {
-            const f32x4 v = *reinterpret_cast<const f32x4*>(p + 4 * t);
-            x[4 * t] = v[0]; x[4 * t + 1] = v[1]; x[4 * t + 2] = v[2]; x[4 * t + 3] = v[3];
+      for (int u = 0; u < 8; ++u) {
+        const int64_t jj = blk * 256 + 32 * w + 4 * u + lc;
+        if (jj < n) {
+          const float* p = A + jj * lda + i;
+          if (vec && i + 3 < m) {
+            const f32x4 v = *reinterpret_cast<const f32x4*>(p);
+            x[4 * u] = v[0]; x[4 * u + 1] = v[1]; x[4 * u + 2] = v[2]; x[4 * u + 3] = v[3];
+          } else {
+#pragma unroll
+            for (int e = 0; e < 4; ++e) x[4 * u + e] = (i + e < m) ? p[e] : 0.0f;
           }
         } else {
 #pragma unroll
-          for (int e = 0; e < 32; ++e) x[e] = (i0 + e < m) ? p[e] : 0.0f;
+          for (int e = 0; e < 4; ++e) x[4 * u + e] = 0.0f;
         }
-      } else {
-#pragma unroll
-        for (int e = 0; e < 32; ++e) x[e] = 0.0f;
       }
     };
-    auto compute = [&](const float (&x)[32], const float* q) {
+    float* sw = stg[w];
+    auto stage = [&](const float (&x)[32]) {
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        f32x4 v;
+        v[0] = x[4 * u]; v[1] = x[4 * u + 1]; v[2] = x[4 * u + 2]; v[3] = x[4 * u + 3];
+        *reinterpret_cast<f32x4*>(sw + (4 * u + lc) * SP + lr) = v;
+      }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    };
+    auto compute = [&](const float (&xin)[32], const float* q) {
+      stage(xin);
 #pragma unroll
       for (int t = 0; t < 4; ++t) {
-        const float* xb = x + 8 * t;   // B operand: rows 32 b + 8 t + [0,8) of column j
+        // B operand: rows 32 b + 8 t + [0,8) of column j, from the staged tile
+        float xb[8];
+        {
+          const float* sp = sw + jl * SP + 32 * b + 8 * t;
+          const f32x4 u0 = *reinterpret_cast<const f32x4*>(sp);
+          const f32x4 u1 = *reinterpret_cast<const f32x4*>(sp + 4);
+          xb[0] = u0[0]; xb[1] = u0[1]; xb[2] = u0[2]; xb[3] = u0[3];
+          xb[4] = u1[0]; xb[5] = u1[1]; xb[6] = u1[2]; xb[7] = u1[3];
+        }
         if constexpr (AR == AR_FP32) {
 #pragma unroll
           for (int lt = 0; lt < LT; ++lt) {
@@ -371,15 +420,11 @@ __global__ __launch_bounds__(512) void k_proj(const float* __restrict__ A, int64
           else round8(xb, bh);
 #pragma unroll
           for (int lt = 0; lt < LT; ++lt) {
-            const float* qp = q + (32 * lt + jl) * QP + 32 * b + 8 * t;
-            float qv[8];
-            const f32x4 u0 = *reinterpret_cast<const f32x4*>(qp);
-            const f32x4 u1 = *reinterpret_cast<const f32x4*>(qp + 4);
-            qv[0] = u0[0]; qv[1] = u0[1]; qv[2] = u0[2]; qv[3] = u0[3];
-            qv[4] = u1[0]; qv[5] = u1[1]; qv[6] = u1[2]; qv[7] = u1[3];
-            bf16x8 ah, al;
-            if (AR == AR_BF16X3 || AR == AR_BF16X4) split8(qv, ah, al);
-            else round8(qv, ah);
+            const __bf16* qh = reinterpret_cast<const __bf16*>(q);
+            const int o = (32 * lt + jl) * QP + 32 * b + 8 * t;
+            const bf16x8 ah = *reinterpret_cast<const bf16x8*>(qh + o);
+            bf16x8 al;
+            if (AR == AR_BF16X3 || AR == AR_BF16X4) al = *reinterpret_cast<const bf16x8*>(qh + KP * QP + o);
             acc[lt] = mfma_bf16(ah, bh, acc[lt]);
             if (AR == AR_BF16X3 || AR == AR_BF16X4) {
               acc[lt] = mfma_bf16(al, bh, acc[lt]);
@@ -453,7 +498,7 @@ __global__ __launch_bounds__(512) void k_proj(const float* __restrict__ A, int64
     if constexpr (GRAM) {
       // tile D (kp x 32 columns, zero for columns j >= n) -> LDS dt[j][c], then
       // G += D D^T with MFMA 32x32x2: A = D[c][j] (lane: c = l%32, j = 2 kk + l/32), B = the same
-      float* d = dt[w];
+      float* d = stg[w];
 #pragma unroll
       for (int lt = 0; lt < LT; ++lt)
 #pragma unroll
