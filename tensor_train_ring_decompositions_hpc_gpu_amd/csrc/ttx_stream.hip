@@ -19,41 +19,50 @@ namespace ttx {
 // ------------------------------------------------------------------------------------------------
 // sketch: Y = A X
 // ------------------------------------------------------------------------------------------------
-// WG = 8 waves.  RB = row blocks of 128 rows per WG (1, 2, 4 or 8), CG = 8 / RB column groups.
-// Wave w: rb = w % RB, cg = w / RB; it accumulates rows [y*128*RB + 128 rb, +128) over columns
-// jc + 16 cg + [0, 16) of every chunk (chunk width KC = 16 CG).
+// WG = 8 waves (2 per SIMD); a wave holds NS = 4 / LT row sub-tiles of 32 rows, so 64 accumulators
+// per lane for either sketch width.  RB = row blocks of RW = 32 NS rows per WG, CG = 8 / RB column
+// groups.  Wave w: rb = w % RB, cg = w / RB; it accumulates rows [y RW RB + RW rb, +RW) over columns
+// jc + 16 cg + [0, 16) of every chunk (chunk width KC = 16 CG); lane r holds rows NS r .. NS r + NS-1.
 // Partial output: part[slab][row][kp], slab = blockIdx.x * CG + cg.
 
 template <int LT>
 struct SketchAcc {
-  f32x16 acc[4][LT];
+  f32x16 acc[4 / LT][LT];
 };
 
-template <bool VEC>
+template <bool VEC, int NS>
 __device__ __forceinline__ void sketch_load(const float* __restrict__ A, int64_t lda, int m, int64_t n,
-                                            int row, int64_t col0, float (&a)[8][4]) {
+                                            int row, int64_t col0, float (&a)[8][NS]) {
 #pragma unroll
   for (int t = 0; t < 8; ++t) {
     const int64_t j = col0 + t;
     if (VEC) {
-      // full rows: row + 3 < m guaranteed by the caller
+      // full rows: row + NS - 1 < m guaranteed by the caller
       if (j < n) {
-        const f32x4 v = *reinterpret_cast<const f32x4*>(A + row + j * lda);
-        a[t][0] = v[0]; a[t][1] = v[1]; a[t][2] = v[2]; a[t][3] = v[3];
+        if constexpr (NS == 4) {
+          const f32x4 v = *reinterpret_cast<const f32x4*>(A + row + j * lda);
+          a[t][0] = v[0]; a[t][1] = v[1]; a[t][2] = v[2]; a[t][3] = v[3];
+        } else {
+          typedef float f32x2 __attribute__((ext_vector_type(2)));
+          const f32x2 v = *reinterpret_cast<const f32x2*>(A + row + j * lda);
+          a[t][0] = v[0]; a[t][1] = v[1];
+        }
       } else {
-        a[t][0] = a[t][1] = a[t][2] = a[t][3] = 0.0f;
+#pragma unroll
+        for (int s = 0; s < NS; ++s) a[t][s] = 0.0f;
       }
     } else {
 #pragma unroll
-      for (int s = 0; s < 4; ++s) a[t][s] = (j < n && row + s < m) ? A[row + s + j * lda] : 0.0f;
+      for (int s = 0; s < NS; ++s) a[t][s] = (j < n && row + s < m) ? A[row + s + j * lda] : 0.0f;
     }
   }
 }
 
 template <int LT, int AR, bool OMEGA>
-__device__ __forceinline__ void sketch_compute(const float (&a)[8][4], const float* __restrict__ xs,
+__device__ __forceinline__ void sketch_compute(const float (&a)[8][4 / LT], const float* __restrict__ xs,
                                                int pitch, int kofs, int lane, SketchAcc<LT>& S, float sa,
                                                const float (&sx)[LT]) {
+  constexpr int NS = 4 / LT;
   const int c = lane & 31;
   const int b = lane >> 5;
   // B operands: X[k = kofs + 8b + t][col = 32 lt + c]
@@ -70,7 +79,7 @@ __device__ __forceinline__ void sketch_compute(const float (&a)[8][4], const flo
 #pragma unroll
     for (int t = 0; t < 8; ++t)
 #pragma unroll
-      for (int s = 0; s < 4; ++s)
+      for (int s = 0; s < NS; ++s)
 #pragma unroll
         for (int lt = 0; lt < LT; ++lt) S.acc[s][lt] = mfma_f32(a[t][s], xv[lt][t], S.acc[s][lt]);
   } else if constexpr (AR == AR_FP16) {
@@ -83,7 +92,7 @@ __device__ __forceinline__ void sketch_compute(const float (&a)[8][4], const flo
       round8h(xv[lt], bh[lt]);
     }
 #pragma unroll
-    for (int s = 0; s < 4; ++s) {
+    for (int s = 0; s < NS; ++s) {
       float v[8];
 #pragma unroll
       for (int t = 0; t < 8; ++t) v[t] = a[t][s] * sa;
@@ -100,7 +109,7 @@ __device__ __forceinline__ void sketch_compute(const float (&a)[8][4], const flo
       else round8(xv[lt], bh[lt]);   // Omega entries are exact in bf16
     }
 #pragma unroll
-    for (int s = 0; s < 4; ++s) {
+    for (int s = 0; s < NS; ++s) {
       float v[8];
 #pragma unroll
       for (int t = 0; t < 8; ++t) v[t] = a[t][s];
@@ -152,13 +161,8 @@ __device__ __forceinline__ void sketch_fill(float* __restrict__ xs, int pitch, i
   }
 }
 
-// LT = 1 runs 8 waves per WG (2 per SIMD); LT = 2 runs 4 waves per WG so that one wave per SIMD
-// can hold its 128 accumulators in AGPRs.
-template <int LT>
-constexpr int sketch_waves() { return LT == 1 ? 8 : 4; }
-
 template <int LT, int AR, bool OMEGA>
-__global__ __launch_bounds__(64 * sketch_waves<LT>()) void k_sketch(const float* __restrict__ A, int64_t lda, int m, int64_t n,
+__global__ __launch_bounds__(512) void k_sketch(const float* __restrict__ A, int64_t lda, int m, int64_t n,
                                                 const float* __restrict__ X, int64_t ldx, uint64_t seed,
                                                 uint32_t stream, int ell, int RB, int CG, int64_t chunks_per_wg,
                                                 float* __restrict__ part, const float* __restrict__ scales) {
@@ -169,14 +173,16 @@ __global__ __launch_bounds__(64 * sketch_waves<LT>()) void k_sketch(const float*
   float* xs0 = smem;
   float* xs1 = smem + KP * pitch;
 
+  constexpr int NS = 4 / LT;
+  constexpr int RW = 32 * NS;
   const int lane = threadIdx.x & 63;
   const int w = threadIdx.x >> 6;
   const int rb = w % RB;
   const int cg = w / RB;
   const int r = lane & 31;
   const int b = lane >> 5;
-  const int row = blockIdx.y * 128 * RB + 128 * rb + 4 * r;
-  const bool vec = ((lda & 3) == 0) && (row + 3 < m);
+  const int row = blockIdx.y * RW * RB + RW * rb + NS * r;
+  const bool vec = ((lda & (NS - 1)) == 0) && (row + NS - 1 < m);
 
   const int64_t nchunks = (n + KC - 1) / KC;
   const int64_t c0 = (int64_t)blockIdx.x * chunks_per_wg;
@@ -184,7 +190,7 @@ __global__ __launch_bounds__(64 * sketch_waves<LT>()) void k_sketch(const float*
 
   SketchAcc<LT> S;
 #pragma unroll
-  for (int s = 0; s < 4; ++s)
+  for (int s = 0; s < NS; ++s)
 #pragma unroll
     for (int lt = 0; lt < LT; ++lt) S.acc[s][lt] = zero16();
 
@@ -199,13 +205,13 @@ __global__ __launch_bounds__(64 * sketch_waves<LT>()) void k_sketch(const float*
       for (int lt = 0; lt < LT; ++lt) sx[lt] = scales[1 + 32 * lt + r];
   }
 
-  float a0[8][4], a1[8][4];
+  float a0[8][NS], a1[8][NS];
   const int kofs = 16 * cg;
   if (c0 < c1) {
     sketch_fill<LT, OMEGA>(xs0, pitch, KC, c0 * KC, n, X, ldx, seed, stream, ell);
     const int64_t col = c0 * KC + kofs + 8 * b;
-    if (vec) sketch_load<true>(A, lda, m, n, row, col, a0);
-    else sketch_load<false>(A, lda, m, n, row, col, a0);
+    if (vec) sketch_load<true, NS>(A, lda, m, n, row, col, a0);
+    else sketch_load<false, NS>(A, lda, m, n, row, col, a0);
   }
   __syncthreads();
   for (int64_t c = c0; c < c1; c += 2) {
@@ -213,8 +219,8 @@ __global__ __launch_bounds__(64 * sketch_waves<LT>()) void k_sketch(const float*
     if (c + 1 < c1) {
       sketch_fill<LT, OMEGA>(xs1, pitch, KC, (c + 1) * KC, n, X, ldx, seed, stream, ell);
       const int64_t col = (c + 1) * KC + kofs + 8 * b;
-      if (vec) sketch_load<true>(A, lda, m, n, row, col, a1);
-      else sketch_load<false>(A, lda, m, n, row, col, a1);
+      if (vec) sketch_load<true, NS>(A, lda, m, n, row, col, a1);
+      else sketch_load<false, NS>(A, lda, m, n, row, col, a1);
     }
     sketch_compute<LT, AR, OMEGA>(a0, xs0, pitch, kofs, lane, S, sa, sx);
     __syncthreads();
@@ -223,8 +229,8 @@ __global__ __launch_bounds__(64 * sketch_waves<LT>()) void k_sketch(const float*
     if (c + 2 < c1) {
       sketch_fill<LT, OMEGA>(xs0, pitch, KC, (c + 2) * KC, n, X, ldx, seed, stream, ell);
       const int64_t col = (c + 2) * KC + kofs + 8 * b;
-      if (vec) sketch_load<true>(A, lda, m, n, row, col, a0);
-      else sketch_load<false>(A, lda, m, n, row, col, a0);
+      if (vec) sketch_load<true, NS>(A, lda, m, n, row, col, a0);
+      else sketch_load<false, NS>(A, lda, m, n, row, col, a0);
     }
     sketch_compute<LT, AR, OMEGA>(a1, xs1, pitch, kofs, lane, S, sa, sx);
     __syncthreads();
@@ -237,12 +243,12 @@ __global__ __launch_bounds__(64 * sketch_waves<LT>()) void k_sketch(const float*
 #pragma unroll
   for (int lt = 0; lt < LT; ++lt) unscale[lt] = 1.0f / (sa * sx[lt]);
 #pragma unroll
-  for (int s = 0; s < 4; ++s)
+  for (int s = 0; s < NS; ++s)
 #pragma unroll
     for (int lt = 0; lt < LT; ++lt)
 #pragma unroll
       for (int v = 0; v < 16; ++v) {
-        const int prow = blockIdx.y * 128 * RB + 128 * rb + 4 * mfma32_row(v, lane) + s;
+        const int prow = blockIdx.y * RW * RB + RW * rb + NS * mfma32_row(v, lane) + s;
         if (prow < m) out[(int64_t)prow * KP + 32 * lt + r] = S.acc[s][lt][v] * unscale[lt];
       }
 }
@@ -707,13 +713,14 @@ int num_cus() {
 
 SketchPlan plan_sketch(int m, int64_t n, int kp, int wgs_hint) {
   SketchPlan p;
-  const int NW = (kp == 32) ? 8 : 4;   // waves per WG (sketch_waves)
-  const int rows = m < 128 * NW ? m : 128 * NW;
-  int rb = (rows + 127) / 128;
+  const int NW = 8;                      // waves per WG
+  const int RW = (kp == 32) ? 128 : 64;  // rows per wave (32 x 4 / LT)
+  const int rows = m < RW * NW ? m : RW * NW;
+  int rb = (rows + RW - 1) / RW;
   int RB = 1;
   while (RB < rb) RB *= 2;
   p.RB = RB;
-  p.ysplit = (m + 128 * RB - 1) / (128 * RB);
+  p.ysplit = (m + RW * RB - 1) / (RW * RB);
   int CG = NW / RB;
   const int cg_lds = (kp == 32) ? 8 : 4;   // keep the two X chunks within 64 KB of LDS
   if (CG > cg_lds) CG = cg_lds;
@@ -721,8 +728,8 @@ SketchPlan plan_sketch(int m, int64_t n, int kp, int wgs_hint) {
   const int KC = 16 * CG;
   const int64_t nchunks = (n + KC - 1) / KC;
   const int waves = RB * CG;
-  // one WG per CU at full width; narrower WGs share a CU
-  int64_t target = wgs_hint > 0 ? wgs_hint : (int64_t)num_cus() * (NW / waves) / p.ysplit;
+  // two full-width WGs per CU; narrower WGs share a CU
+  int64_t target = wgs_hint > 0 ? wgs_hint : (int64_t)num_cus() * 2 * (NW / waves) / p.ysplit;
   if (target < 1) target = 1;
   int64_t cpw = (nchunks + target - 1) / target;
   if (cpw < 1) cpw = 1;
