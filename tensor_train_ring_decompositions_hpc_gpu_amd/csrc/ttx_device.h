@@ -48,31 +48,55 @@ __device__ __forceinline__ f32x16 zero16() {
   return z;
 }
 
+// Pairs of floats are converted together (v_cvt_pk_bf16_f32, round to nearest even); the float value
+// of a packed bf16 is its bit pattern shifted into the upper half, so residuals need no conversion back.
+typedef uint32_t u32x4v __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ uint32_t pk_bf16(float a, float b) {
+  typedef float f32x2v __attribute__((ext_vector_type(2)));
+  typedef __bf16 bf16x2v __attribute__((ext_vector_type(2)));
+  const f32x2v v = {a, b};
+  return __builtin_bit_cast(uint32_t, __builtin_convertvector(v, bf16x2v));
+}
+__device__ __forceinline__ float bf_lo(uint32_t w) { return __uint_as_float(w << 16); }
+__device__ __forceinline__ float bf_hi(uint32_t w) { return __uint_as_float(w & 0xFFFF0000u); }
+
 // x = hi + lo with hi = bf16(x) (round to nearest even) and lo = bf16(x - hi)
 __device__ __forceinline__ void split8(const float* x, bf16x8& hi, bf16x8& lo) {
+  u32x4v h, l;
 #pragma unroll
-  for (int i = 0; i < 8; ++i) {
-    const __bf16 h = (__bf16)x[i];
-    hi[i] = h;
-    lo[i] = (__bf16)(x[i] - (float)h);
+  for (int p = 0; p < 4; ++p) {
+    const float a = x[2 * p], b = x[2 * p + 1];
+    const uint32_t w = pk_bf16(a, b);
+    h[p] = w;
+    l[p] = pk_bf16(a - bf_lo(w), b - bf_hi(w));
   }
+  hi = __builtin_bit_cast(bf16x8, h);
+  lo = __builtin_bit_cast(bf16x8, l);
 }
 // x = hi + mid + lo, three bf16 terms (about 24 significant bits): exact products with a bf16-exact
 // operand
 __device__ __forceinline__ void split8x3(const float* x, bf16x8& hi, bf16x8& mid, bf16x8& lo) {
+  u32x4v h, m, l;
 #pragma unroll
-  for (int i = 0; i < 8; ++i) {
-    const __bf16 h = (__bf16)x[i];
-    const float r1 = x[i] - (float)h;
-    const __bf16 mm = (__bf16)r1;
-    hi[i] = h;
-    mid[i] = mm;
-    lo[i] = (__bf16)(r1 - (float)mm);
+  for (int p = 0; p < 4; ++p) {
+    const float a = x[2 * p], b = x[2 * p + 1];
+    const uint32_t wh = pk_bf16(a, b);
+    const float ra = a - bf_lo(wh), rb = b - bf_hi(wh);
+    const uint32_t wm = pk_bf16(ra, rb);
+    h[p] = wh;
+    m[p] = wm;
+    l[p] = pk_bf16(ra - bf_lo(wm), rb - bf_hi(wm));
   }
+  hi = __builtin_bit_cast(bf16x8, h);
+  mid = __builtin_bit_cast(bf16x8, m);
+  lo = __builtin_bit_cast(bf16x8, l);
 }
 __device__ __forceinline__ void round8(const float* x, bf16x8& hi) {
+  u32x4v h;
 #pragma unroll
-  for (int i = 0; i < 8; ++i) hi[i] = (__bf16)x[i];
+  for (int p = 0; p < 4; ++p) h[p] = pk_bf16(x[2 * p], x[2 * p + 1]);
+  hi = __builtin_bit_cast(bf16x8, h);
 }
 __device__ __forceinline__ void round8h(const float* x, f16x8& h) {
 #pragma unroll
