@@ -130,16 +130,73 @@ __device__ __forceinline__ void sketch_compute(const float (&a)[8][4 / LT], cons
   }
 }
 
-// Fill the LDS chunk xs[c][k] (c < kp = 32 LT, k < KC) for columns jc .. jc+KC-1 of X.
+// The LDS chunk xs[c][k] (c < kp = 32 LT, k < KC) holds columns jc .. jc+KC-1 of X.  Its global
+// loads (X mode) go to registers one chunk ahead (fill_load) and are written to LDS after the MFMAs
+// of the current chunk (fill_store), so neither the A loads nor the MFMAs wait on them; with the
+// in-kernel sketch, fill_store generates the Gaussian entries instead.  A thread handles FQ float4
+// pieces per chunk (KC kp / 4 / threads = kp / (16 RB), at least 1).
+template <bool OMEGA, int FQ>
+struct FillRegs {
+  f32x4 v[OMEGA ? 1 : FQ];
+};
+
+template <int LT, bool OMEGA, int FQ>
+__device__ __forceinline__ void fill_load(FillRegs<OMEGA, FQ>& R, int KC, int64_t jc, int64_t n,
+                                          const float* __restrict__ X, int64_t ldx) {
+  if constexpr (!OMEGA) {
+    constexpr int NG = 32 * LT / 4;
+    const int total = KC * NG;
+#pragma unroll
+    for (int q = 0; q < FQ; ++q) {
+      const int idx = threadIdx.x + q * (int)blockDim.x;
+      if (idx < total) {
+        const int jj = idx % KC;
+        const int g = idx / KC;
+        const int64_t j = jc + jj;
+        if (j < n) R.v[q] = *reinterpret_cast<const f32x4*>(X + j * ldx + 4 * g);
+        else R.v[q] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+      }
+    }
+  }
+}
+
+template <int LT, bool OMEGA, int FQ>
+__device__ __forceinline__ void fill_store(const FillRegs<OMEGA, FQ>& R, float* __restrict__ xs, int pitch, int KC,
+                                           int64_t jc, int64_t n, uint64_t seed, uint32_t stream, int ell) {
+  constexpr int NG = 32 * LT / 4;
+  const int total = KC * NG;
+#pragma unroll
+  for (int q = 0; q < (OMEGA ? 4 : FQ); ++q) {
+    const int idx = threadIdx.x + q * (int)blockDim.x;
+    if (idx < total) {
+      const int jj = idx % KC;
+      const int g = idx / KC;
+      const int64_t j = jc + jj;
+      float z[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+      if constexpr (OMEGA) {
+        if (j < n && 4 * g < ell) {
+          normal4(seed, stream, (uint64_t)j, (uint32_t)g, z);
+#pragma unroll
+          for (int e = 0; e < 4; ++e) z[e] = (4 * g + e < ell) ? omega_round(z[e]) : 0.0f;
+        }
+      } else {
+        z[0] = R.v[q][0]; z[1] = R.v[q][1]; z[2] = R.v[q][2]; z[3] = R.v[q][3];
+      }
+#pragma unroll
+      for (int e = 0; e < 4; ++e) xs[(4 * g + e) * pitch + jj] = z[e];
+    }
+  }
+}
+
+// One-shot fill of the LDS chunk (loads, or Gaussian entries, written straight to LDS): the order of
+// the variants whose registers do not hold a chunk of X across the MFMAs.
 template <int LT, bool OMEGA>
 __device__ __forceinline__ void sketch_fill(float* __restrict__ xs, int pitch, int KC, int64_t jc, int64_t n,
                                             const float* __restrict__ X, int64_t ldx,
                                             uint64_t seed, uint32_t stream, int ell) {
-  constexpr int KP = 32 * LT;
-  constexpr int NG = KP / 4;
-  const int tid = threadIdx.x;
+  constexpr int NG = 32 * LT / 4;
   const int total = KC * NG;
-  for (int idx = tid; idx < total; idx += blockDim.x) {
+  for (int idx = threadIdx.x; idx < total; idx += blockDim.x) {
     const int jj = idx % KC;
     const int g = idx / KC;
     const int64_t j = jc + jj;
@@ -161,7 +218,7 @@ __device__ __forceinline__ void sketch_fill(float* __restrict__ xs, int pitch, i
   }
 }
 
-template <int LT, int AR, bool OMEGA>
+template <int LT, int AR, bool OMEGA, int FQ>
 __global__ __launch_bounds__(512) void k_sketch(const float* __restrict__ A, int64_t lda, int m, int64_t n,
                                                 const float* __restrict__ X, int64_t ldx, uint64_t seed,
                                                 uint32_t stream, int ell, int RB, int CG, int64_t chunks_per_wg,
@@ -206,34 +263,65 @@ __global__ __launch_bounds__(512) void k_sketch(const float* __restrict__ A, int
   }
 
   float a0[8][NS], a1[8][NS];
+  [[maybe_unused]] FillRegs<OMEGA, FQ> R;
   const int kofs = 16 * cg;
-  if (c0 < c1) {
-    sketch_fill<LT, OMEGA>(xs0, pitch, KC, c0 * KC, n, X, ldx, seed, stream, ell);
-    const int64_t col = c0 * KC + kofs + 8 * b;
-    if (vec) sketch_load<true, NS>(A, lda, m, n, row, col, a0);
-    else sketch_load<false, NS>(A, lda, m, n, row, col, a0);
-  }
-  __syncthreads();
-  for (int64_t c = c0; c < c1; c += 2) {
-    // even step: compute chunk c from a0 / xs0, prefetch chunk c+1 into a1 / xs1
-    if (c + 1 < c1) {
-      sketch_fill<LT, OMEGA>(xs1, pitch, KC, (c + 1) * KC, n, X, ldx, seed, stream, ell);
-      const int64_t col = (c + 1) * KC + kofs + 8 * b;
-      if (vec) sketch_load<true, NS>(A, lda, m, n, row, col, a1);
-      else sketch_load<false, NS>(A, lda, m, n, row, col, a1);
+  auto loadA = [&](float (&a)[8][NS], int64_t c) {
+    const int64_t col = c * KC + kofs + 8 * b;
+    if (vec) sketch_load<true, NS>(A, lda, m, n, row, col, a);
+    else sketch_load<false, NS>(A, lda, m, n, row, col, a);
+  };
+  // the pipelined order (X chunk loads held in registers across the MFMAs) fits the registers of the
+  // 64-column sketch with at most two pieces per thread; the others fill before computing
+  constexpr bool PIPE = (LT == 2) && (FQ <= 2);
+  if constexpr (PIPE) {
+    if (c0 < c1) {
+      fill_load<LT, OMEGA, FQ>(R, KC, c0 * KC, n, X, ldx);
+      loadA(a0, c0);
+      fill_store<LT, OMEGA, FQ>(R, xs0, pitch, KC, c0 * KC, n, seed, stream, ell);
     }
-    sketch_compute<LT, AR, OMEGA>(a0, xs0, pitch, kofs, lane, S, sa, sx);
     __syncthreads();
-    if (c + 1 >= c1) break;
-    // odd step: compute chunk c+1 from a1 / xs1, prefetch chunk c+2 into a0 / xs0
-    if (c + 2 < c1) {
-      sketch_fill<LT, OMEGA>(xs0, pitch, KC, (c + 2) * KC, n, X, ldx, seed, stream, ell);
-      const int64_t col = (c + 2) * KC + kofs + 8 * b;
-      if (vec) sketch_load<true, NS>(A, lda, m, n, row, col, a0);
-      else sketch_load<false, NS>(A, lda, m, n, row, col, a0);
+    for (int64_t c = c0; c < c1; c += 2) {
+      // even step: chunk c from a0 / xs0; chunk c+1's loads go out first, its LDS fill after the MFMAs
+      if (c + 1 < c1) {
+        loadA(a1, c + 1);
+        fill_load<LT, OMEGA, FQ>(R, KC, (c + 1) * KC, n, X, ldx);
+      }
+      sketch_compute<LT, AR, OMEGA>(a0, xs0, pitch, kofs, lane, S, sa, sx);
+      __builtin_amdgcn_sched_barrier(0);
+      if (c + 1 < c1) fill_store<LT, OMEGA, FQ>(R, xs1, pitch, KC, (c + 1) * KC, n, seed, stream, ell);
+      __syncthreads();
+      if (c + 1 >= c1) break;
+      // odd step: chunk c+1 from a1 / xs1
+      if (c + 2 < c1) {
+        loadA(a0, c + 2);
+        fill_load<LT, OMEGA, FQ>(R, KC, (c + 2) * KC, n, X, ldx);
+      }
+      sketch_compute<LT, AR, OMEGA>(a1, xs1, pitch, kofs, lane, S, sa, sx);
+      __builtin_amdgcn_sched_barrier(0);
+      if (c + 2 < c1) fill_store<LT, OMEGA, FQ>(R, xs0, pitch, KC, (c + 2) * KC, n, seed, stream, ell);
+      __syncthreads();
     }
-    sketch_compute<LT, AR, OMEGA>(a1, xs1, pitch, kofs, lane, S, sa, sx);
+  } else {
+    if (c0 < c1) {
+      sketch_fill<LT, OMEGA>(xs0, pitch, KC, c0 * KC, n, X, ldx, seed, stream, ell);
+      loadA(a0, c0);
+    }
     __syncthreads();
+    for (int64_t c = c0; c < c1; c += 2) {
+      if (c + 1 < c1) {
+        sketch_fill<LT, OMEGA>(xs1, pitch, KC, (c + 1) * KC, n, X, ldx, seed, stream, ell);
+        loadA(a1, c + 1);
+      }
+      sketch_compute<LT, AR, OMEGA>(a0, xs0, pitch, kofs, lane, S, sa, sx);
+      __syncthreads();
+      if (c + 1 >= c1) break;
+      if (c + 2 < c1) {
+        sketch_fill<LT, OMEGA>(xs0, pitch, KC, (c + 2) * KC, n, X, ldx, seed, stream, ell);
+        loadA(a0, c + 2);
+      }
+      sketch_compute<LT, AR, OMEGA>(a1, xs1, pitch, kofs, lane, S, sa, sx);
+      __syncthreads();
+    }
   }
 
   // partial output (row-major, kp columns), the FP16 operand scales undone (exact powers of two)
@@ -747,8 +835,19 @@ static void launch_sketch_t(const SketchPlan& p, const float* A, int64_t lda, in
                             int64_t ldx, uint64_t seed, uint32_t stream, int ell, float* part, hipStream_t st,
                             const float* scales) {
   dim3 grid(p.grid_x, p.ysplit);
-  hipLaunchKernelGGL((k_sketch<LT, AR, OM>), grid, dim3(64 * p.RB * p.CG), p.lds_bytes, st, A, lda, m, n, X, ldx,
-                     seed, stream, ell, p.RB, p.CG, p.chunks_per_wg, part, scales);
+  const int fq = (32 * LT) / (16 * p.RB);   // float4 pieces of the X chunk per thread
+  if constexpr (OM)
+    hipLaunchKernelGGL((k_sketch<LT, AR, OM, 1>), grid, dim3(64 * p.RB * p.CG), p.lds_bytes, st, A, lda, m, n, X, ldx,
+                       seed, stream, ell, p.RB, p.CG, p.chunks_per_wg, part, scales);
+  else if (fq <= 1)
+    hipLaunchKernelGGL((k_sketch<LT, AR, OM, 1>), grid, dim3(64 * p.RB * p.CG), p.lds_bytes, st, A, lda, m, n, X, ldx,
+                       seed, stream, ell, p.RB, p.CG, p.chunks_per_wg, part, scales);
+  else if (fq == 2)
+    hipLaunchKernelGGL((k_sketch<LT, AR, OM, 2>), grid, dim3(64 * p.RB * p.CG), p.lds_bytes, st, A, lda, m, n, X, ldx,
+                       seed, stream, ell, p.RB, p.CG, p.chunks_per_wg, part, scales);
+  else
+    hipLaunchKernelGGL((k_sketch<LT, AR, OM, 4>), grid, dim3(64 * p.RB * p.CG), p.lds_bytes, st, A, lda, m, n, X, ldx,
+                       seed, stream, ell, p.RB, p.CG, p.chunks_per_wg, part, scales);
 }
 
 template <int LT, bool OM>
