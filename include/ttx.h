@@ -224,6 +224,10 @@ ttx_status ttx_mlp_train_step(ttx_handle h, const float* d_X, const float* d_T, 
 /* the model's input features from probe pivots d_pivots[0..kp-1] (device, FP64) */
 ttx_status ttx_mlp_features(ttx_handle h, const double* d_pivots, int32_t kp, double rel_tol, int64_t m, int64_t n,
                             float* d_x);
+/* features and forward pass of one sample in one launch, as a decomposition step predicts its rank
+ * and sketch width: d_y[0..TTX_MLP_OUT-1] (device) equals ttx_mlp_features then ttx_mlp_forward */
+ttx_status ttx_mlp_predict(ttx_handle h, const double* d_pivots, int32_t kp, double rel_tol, int64_t m, int64_t n,
+                           float* d_y);
 /* the probe of one unfolding (device A, m x n column-major): pivots R_ii^2 of the shifted Cholesky of
  * (A[:, :n_p] Omega)^T (A[:, :n_p] Omega), Omega n_p x 64 from (seed, stream), FP32-accurate sketch;
  * h_pivots (host) receives 64 values */

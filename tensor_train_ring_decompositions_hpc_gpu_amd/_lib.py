@@ -127,6 +127,7 @@ SIGNATURES = {
     "ttx_mlp_gradient": (_I32, [_P, _P, _P, _I32, _P, ctypes.POINTER(_D)]),
     "ttx_mlp_train_step": (_I32, [_P, _P, _P, _I32, _F, ctypes.POINTER(_D)]),
     "ttx_mlp_features": (_I32, [_P, _P, _I32, _D, _I64, _I64, _P]),
+    "ttx_mlp_predict": (_I32, [_P, _P, _I32, _D, _I64, _I64, _P]),
     "ttx_probe": (_I32, [_P, _P, _I64, _I64, _U64, _U32, ctypes.POINTER(_D)]),
 }
 

@@ -113,6 +113,17 @@ class Engine:
                                                            ctypes.c_void_p(x.data_ptr())))
         return x
 
+    def mlp_predict(self, pivots_dev, rel_tol, m, n):
+        """features + forward of one sample in one launch (the prediction a decomposition step makes)"""
+        torch = _torch()
+        y = torch.zeros(L.MLP_OUT, device=pivots_dev.device, dtype=torch.float32)
+        if pivots_dev.dtype != torch.float64 or not pivots_dev.is_contiguous():
+            raise TypeError("ttx: pivots must be a contiguous float64 CUDA tensor")
+        _check(self.lib, self.h, self.lib.ttx_mlp_predict(self.h, ctypes.c_void_p(pivots_dev.data_ptr()),
+                                                          pivots_dev.numel(), float(rel_tol), int(m), int(n),
+                                                          ctypes.c_void_p(y.data_ptr())))
+        return y
+
     def probe(self, A, m, n, seed=1234, stream=0):
         piv = np.zeros(64, dtype=np.float64)
         _check(self.lib, self.h, self.lib.ttx_probe(self.h, ctypes.c_void_p(_dev_ptr(A)), int(m), int(n), seed, stream,

@@ -156,104 +156,113 @@ __device__ __forceinline__ double rsqrt_f64(double d) {
 // ------------------------------------------------------------------------------------------------
 // sum S Gram partials, (shifted) Cholesky G + s I = R^T R, explicit Rinv = R^-1 (upper, row-major)
 //
-// Right-looking elimination on the upper triangle: step i scales row i by 1/sqrt(pivot) (it becomes
-// row i of R) and subtracts R[i][l] R[i][j] from the trailing entries.  The same row operations
-// applied to B = I give B = L^-1 with L = R^T, so Rinv = B^T comes out of the same sweep: every
-// step is two parallel phases separated by barriers, with O(1) dependent latency per step.
+// One wavefront factors the matrix in registers: lane x holds column x in W[0..63] (the Gram is
+// symmetric, so column x is also row x).  Step i (unrolled, so every register index is a
+// constant):
+//   * pivot d = W[i] of lane i (read across lanes), inv = 1 / sqrt(d + s);
+//   * R[i][x] = W[i] inv goes to LDS, and the trailing update W[l] -= R[i][l] R[i][x] (l > i) reads
+//     R[i][l] back as an LDS broadcast;
+//   * slot i is then free and takes row i of B = L^-1 (L = R^T):
+//     B[i][x] = (delta_ix - sum_{j<i} R[j][i] B[j][x]) inv, from the rows of B already in slots j < i.
+// The trailing update keeps the full symmetric column, so no lane needs a mask; lanes x <= i carry
+// stale values in slots l > i that are overwritten by B rows before they are read.  No barrier
+// inside the factorisation; the other waves only sum the partials and write the result.
 // ------------------------------------------------------------------------------------------------
+__device__ __forceinline__ double lane_value(double v, int lane) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)(b & 0xFFFFFFFFll), lane);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// KP: k rounded up to a multiple of 8; the matrix is padded with the identity to KP, which leaves
+// the leading k x k factor unchanged and makes every loop a compile-time one.  INV = false (Rinv
+// null) factors only, for callers that want the pivots.
+template <int KP, bool INV>
 __global__ __launch_bounds__(256) void k_chol_rinv(const double* __restrict__ part, int64_t S, int k, int64_t m,
                                                    int shift_mode, double* __restrict__ Rinv,
                                                    int* __restrict__ status, double* __restrict__ pivots) {
-  __shared__ double a[KMAX][KMAX + 1];
-  __shared__ double B[KMAX][KMAX + 1];
-  __shared__ double shift_s, tiny_s;
+  __shared__ double a[KMAX][KMAX + 1];     // summed Gram, then B (row c, column x) for the output
+  __shared__ double Rs[KP][KMAX + 2];      // rows of R as they are produced
   const int tid = threadIdx.x;
   sum_slabs(part, S, k, a);
-  for (int e = tid; e < KMAX * KMAX; e += 256) {
-    const int i = e >> 6, j = e & 63;
-    B[i][j] = (i == j) ? 1.0 : 0.0;
-  }
   __syncthreads();
   if (tid < 64) {
-    double tr = tid < k ? a[tid][tid] : 0.0, md = tr;
+    const int x = tid;
+    double tr = x < k ? a[x][x] : 0.0, md = tr;
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
       tr += __shfl_xor(tr, o);
       md = fmax(md, __shfl_xor(md, o));
     }
-    if (tid == 0) {
-      // shifted CholeskyQR (Fukaya, Kannan, Nakatsukasa, Yamamoto, Yanagisawa 2020):
-      // s = 11 (m k + k (k+1)) u ||X||_2^2 with ||X||_2^2 <= trace(X^T X)
-      shift_s = shift_mode ? 11.0 * ((double)m * k + (double)k * (k + 1)) * 1.1102230246251565e-16 * tr : 0.0;
-      tiny_s = (md + shift_s) * 1e-28;
+    // shifted CholeskyQR (Fukaya, Kannan, Nakatsukasa, Yamamoto, Yanagisawa 2020):
+    // s = 11 (m k + k (k+1)) u ||X||_2^2 with ||X||_2^2 <= trace(X^T X)
+    const double sh = shift_mode ? 11.0 * ((double)m * k + (double)k * (k + 1)) * 1.1102230246251565e-16 * tr : 0.0;
+    const double tiny = (md + sh) * 1e-28;
+    double W[KP];
+#pragma unroll
+    for (int l = 0; l < KP; ++l) {
+      const double g = a[l][x];   // in bounds for every lane: a is 64 x 65
+      W[l] = (x < k && l < k) ? g : (l == x ? 1.0 : 0.0);
     }
+    int nzero = 0;
+#pragma unroll
+    for (int i = 0; i < KP; ++i) {
+      const bool real = i < k;
+      const double d = lane_value(W[i], i) + (real ? sh : 0.0);
+      const bool ok = d > tiny;
+      const double inv = ok ? rsqrt_f64(d) : 0.0;
+      nzero += (ok || !real) ? 0 : 1;
+      if (pivots && x == 0 && real) pivots[i] = ok ? d : 0.0;
+      const double r = W[i] * inv;
+      Rs[i][x] = r;
+      // row i of B from the rows j < i (independent of this step's pivot until the last multiply)
+      double acc0 = (x == i) ? 1.0 : 0.0, acc1 = 0.0;
+      if constexpr (INV) {
+#pragma unroll
+        for (int j = 0; j < i; j += 2) {
+          acc0 -= Rs[j][i] * W[j];
+          if (j + 1 < i) acc1 -= Rs[j + 1][i] * W[j + 1];
+        }
+      }
+      // trailing update of the columns still to be factored
+#pragma unroll
+      for (int l = i + 1; l < KP; ++l) W[l] -= Rs[i][l] * r;
+      W[i] = (acc0 + acc1) * inv;
+      // materialise this step's results here: left alone, the compiler defers each trailing
+      // update to the step that reads the slot, holding O(k^2) loaded factors live
+#pragma unroll
+      for (int l = i; l < KP; ++l) asm volatile("" : "+v"(W[l]));
+    }
+    if (INV && x < k) {
+#pragma unroll
+      for (int c = 0; c < KP; ++c)
+        if (c < k) a[c][x] = W[c];
+    }
+    if (x == 0 && status) status[0] += nzero;
   }
+  if (!INV) return;
   __syncthreads();
-  const double sh = shift_s, tiny = tiny_s;
-  const int tx = tid & 15, ty = tid >> 4;
-  int nzero = 0;
-  for (int i = 0; i < k; ++i) {
-    const double d = a[i][i] + sh;
-    const bool ok = d > tiny;
-    const double inv = ok ? rsqrt_f64(d) : 0.0;
-    if (!ok) ++nzero;
-    if (pivots && tid == 0) pivots[i] = ok ? d : 0.0;
-    // phase 1: row i of R (strict upper part; the diagonal is kept apart) and row i of B
-    if (tid < k) {
-      if (tid > i) a[i][tid] *= inv;
-      else B[i][tid] *= inv;
-    }
-    __syncthreads();
-    // phase 2: trailing rows l > i; every operand is loaded before the first store so the loads
-    // issue back to back
-    {
-      double f[4], rv[4], cur[4][4];
-      bool inB[4];
-#pragma unroll
-      for (int li = 0; li < 4; ++li) {
-        const int l = i + 1 + ty + 16 * li;
-        f[li] = (l < k) ? a[i][l] : 0.0;
-      }
-#pragma unroll
-      for (int xi = 0; xi < 4; ++xi) {
-        const int x = tx + 16 * xi;
-        inB[xi] = x <= i;
-        rv[xi] = (x < k) ? (inB[xi] ? B[i][x] : a[i][x]) : 0.0;
-      }
-#pragma unroll
-      for (int li = 0; li < 4; ++li)
-#pragma unroll
-        for (int xi = 0; xi < 4; ++xi) {
-          const int l = i + 1 + ty + 16 * li, x = tx + 16 * xi;
-          const bool live = l < k && x < k && (inB[xi] || x >= l);
-          cur[li][xi] = live ? (inB[xi] ? B[l][x] : a[l][x]) : 0.0;
-        }
-#pragma unroll
-      for (int li = 0; li < 4; ++li)
-#pragma unroll
-        for (int xi = 0; xi < 4; ++xi) {
-          const int l = i + 1 + ty + 16 * li, x = tx + 16 * xi;
-          const bool live = l < k && x < k && (inB[xi] || x >= l);
-          if (live) {
-            const double nv = cur[li][xi] - f[li] * rv[xi];
-            if (inB[xi]) B[l][x] = nv;
-            else a[l][x] = nv;
-          }
-        }
-    }
-    __syncthreads();
-  }
   // Rinv[i][c] = B[c][i] (upper triangle)
   for (int e = tid; e < k * k; e += 256) {
     const int i = e / k, c = e - (e / k) * k;
-    Rinv[e] = (c >= i) ? B[c][i] : 0.0;
+    Rinv[e] = (c >= i) ? a[c][i] : 0.0;
   }
-  if (tid == 0 && status) status[0] += nzero;
 }
 
 void launch_chol_inv(const double* part, int64_t S, int k, int64_t m, int shift_mode, double* Rinv, int* status,
                      hipStream_t st, double* pivots) {
-  hipLaunchKernelGGL(k_chol_rinv, dim3(1), dim3(256), 0, st, part, S, k, m, shift_mode, Rinv, status, pivots);
+  switch ((k + 7) / 8) {
+#define TTX_CHOL(q) case q:                                                                                  \
+    if (Rinv) hipLaunchKernelGGL((k_chol_rinv<8 * q, true>), dim3(1), dim3(256), 0, st, part, S, k, m, shift_mode,   \
+                                 Rinv, status, pivots);                                                           \
+    else hipLaunchKernelGGL((k_chol_rinv<8 * q, false>), dim3(1), dim3(256), 0, st, part, S, k, m, shift_mode,       \
+                            Rinv, status, pivots);                                                                \
+    break;
+    TTX_CHOL(1) TTX_CHOL(2) TTX_CHOL(3) TTX_CHOL(4) TTX_CHOL(5) TTX_CHOL(6) TTX_CHOL(7) TTX_CHOL(8)
+#undef TTX_CHOL
+    default: break;
+  }
 }
 
 // ------------------------------------------------------------------------------------------------

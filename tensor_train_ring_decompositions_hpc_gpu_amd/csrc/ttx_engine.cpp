@@ -226,7 +226,10 @@ struct StepOut {
 };
 
 // host copies of a step's sigma (kdim values) and status into its StepOut
-static void finish_step_out(const float* sig, const int* stat, StepOut& o) {
+// singular values and solver status of a finished step; with norms = true also the kept and total
+// squared norms from the FP32 singular values (a DELTA / LEARNED step sets those from its FP64
+// eigenvalues when it chooses the rank)
+static void finish_step_out(const float* sig, const int* stat, StepOut& o, bool norms = true) {
   double total = 0.0, kept = 0.0;
   for (int i = 0; i < std::min(o.kdim, TTX_MAX_ELL); ++i) {
     o.sigma[i] = sig[i];
@@ -234,8 +237,10 @@ static void finish_step_out(const float* sig, const int* stat, StepOut& o) {
     total += s2;
     if (i < o.r) kept += s2;
   }
-  o.kept2 = kept;
-  if (o.path == TTX_PATH_EXACT_M) o.total2 = total;
+  if (norms) {
+    o.kept2 = kept;
+    if (o.path == TTX_PATH_EXACT_M) o.total2 = total;
+  }
   o.zero_pivots = stat[0];
   o.sweeps = stat[1];
 }
@@ -312,6 +317,7 @@ static ttx_status step_tail(Handle* h, const StepArgs& a, const Arith3& ar, int 
     TTX_HIP(h, hipStreamSynchronize(st));
     double kept;
     r = choose_rank(a, k, h->h_lam, a.norm2, rel2, &kept);
+    o.kept2 = kept;
   }
   if (r > k) return fail(h, TTX_ERR_BAD_ARG, "rank exceeds the sketch width");
   TTX_TRY(ensure(h, h->Ur, (size_t)kp * r * sizeof(float)));
@@ -548,7 +554,6 @@ static ttx_status probe(Handle* h, const StepArgs& a, const Arith3& ar, double* 
   }
   const int64_t nb = panel_blocks(m);
   TTX_TRY(ensure(h, h->gpart, (size_t)nb * kp * kp * sizeof(double)));
-  TTX_TRY(ensure(h, h->Rinv, (size_t)kp * kp * sizeof(double)));
   {
     KScope ks(h, TTX_K_SMALL, 0);
     launch_reduce_f32(P<float>(h->part), sp.slabs, (int64_t)m * kp, P<float>(h->Y), st);
@@ -557,7 +562,7 @@ static ttx_status probe(Handle* h, const StepArgs& a, const Arith3& ar, double* 
   {
     KScope ks(h, TTX_K_SMALL, 0);
     launch_gram_tall_f32(Ya, kp, m, kp, P<double>(h->gpart), nb, st);
-    launch_chol_inv(P<double>(h->gpart), nb, kp, m, 1, P<double>(h->Rinv), nullptr, st, d_piv);
+    launch_chol_inv(P<double>(h->gpart), nb, kp, m, 1, nullptr, nullptr, st, d_piv);
   }
   o.bytes += a.m * np * 4;
   return check_launch(h, "probe");
@@ -573,11 +578,10 @@ static ttx_status predict(Handle* h, const StepArgs& a, const Arith3& ar, int el
   float* x = reinterpret_cast<float*>(piv + PROBE_K);
   float* y = x + MLP_IN;
   TTX_TRY(probe(h, a, ar, piv, o));
-  const MlpAct act = mlp_act(h, 1);
   {
     KScope ks(h, TTX_K_SMALL, 0);
-    mlp_features(piv, PROBE_K, a.rel_budget > 0 ? a.rel_budget : 1e-12, (double)a.m, (double)a.n, x, MLP_IN, st);
-    mlp_forward(P<float>(h->mP), MLP_IN, MLP_H, MLP_OUT, x, 1, act.Z1, act.H1, act.Z2, act.H2, y, st);
+    mlp_predict(P<float>(h->mP), piv, PROBE_K, a.rel_budget > 0 ? a.rel_budget : 1e-12, (double)a.m, (double)a.n, y,
+                st);
   }
   TTX_TRY(check_launch(h, "rank model"));
   float hy[MLP_OUT];
@@ -655,6 +659,8 @@ static ttx_status run_step(Handle* h, const StepArgs& a_in, StepOut& o) {
       if (aa.delta2 < 0) aa.delta2 = a.delta_rel * total;
       double kept = 0.0;
       r = choose_rank(aa, m, h->h_lam, total, 0.0, &kept);
+      o.total2 = total;
+      o.kept2 = kept;
     }
     if (r > TTX_MAX_ELL) return fail(h, TTX_ERR_UNSUPPORTED, "rank above TTX_MAX_ELL");
     const int kpn = kp_for(r);
@@ -889,7 +895,9 @@ static ttx_status sweep(Handle* h, const float* d_A, const ttx_params* p, float*
     a.sig_prev = (k > 0) ? sig_all + (size_t)(k - 1) * TTX_MAX_ELL : nullptr;
     a.r_prev = prev_kept;
     a.stat_dev = stat_all + 4 * k;
-    a.sync = host_ranks;
+    // the step's own readback is not needed: a DELTA / LEARNED step reads its eigenvalues when it
+    // chooses the rank, and the singular values of every step come back once after the sweep
+    a.sync = false;
     // range-check tolerance: a quarter of the step's share eps^2 / (d-1) of the squared error budget
     // (Oseledets 2011: the squared step errors add up), relative to ||A_k||_F^2
     if (p->rank_mode == TTX_RANK_FIXED || norm2 <= 0.0)
@@ -975,7 +983,7 @@ static ttx_status sweep(Handle* h, const float* d_A, const ttx_params* p, float*
     if (rep) *rep = R;
     return check_launch(h, "batch sweep");
   }
-  if (!host_ranks) {
+  {
     std::vector<float> hs((size_t)TTX_MAX_ORDER * TTX_MAX_ELL);
     std::vector<int> hst((size_t)TTX_MAX_ORDER * 4);
     TTX_HIP(h, hipMemcpyAsync(hs.data(), sig_all, hs.size() * sizeof(float), hipMemcpyDeviceToHost, st));
@@ -983,9 +991,11 @@ static ttx_status sweep(Handle* h, const float* d_A, const ttx_params* p, float*
     TTX_HIP(h, hipStreamSynchronize(st));
     double n2 = -1.0;
     for (int k = 0; k < d - 1; ++k) {
-      finish_step_out(hs.data() + (size_t)k * TTX_MAX_ELL, hst.data() + 4 * k, outs[k]);
-      R.norm2_in[k] = (outs[k].total2 >= 0) ? outs[k].total2 : n2;
-      n2 = outs[k].kept2;
+      finish_step_out(hs.data() + (size_t)k * TTX_MAX_ELL, hst.data() + 4 * k, outs[k], !host_ranks);
+      if (!host_ranks) {
+        R.norm2_in[k] = (outs[k].total2 >= 0) ? outs[k].total2 : n2;
+        n2 = outs[k].kept2;
+      }
     }
   }
   for (int k = 0; k < d - 1; ++k) {
@@ -1525,6 +1535,17 @@ ttx_status ttx_mlp_features(ttx_handle hh, const double* piv, int32_t kp, double
   if (!h || !piv || !x || kp <= 0 || kp > 64 || !(rel_tol > 0.0) || m <= 0 || n <= 0) return TTX_ERR_BAD_ARG;
   ttx::mlp_features(piv, kp, rel_tol, (double)m, (double)n, x, TTX_MLP_IN, h->stream);
   TTX_TRY(ttx::check_launch(h, "features"));
+  TTX_HIP(h, hipStreamSynchronize(h->stream));
+  return TTX_OK;
+}
+
+ttx_status ttx_mlp_predict(ttx_handle hh, const double* piv, int32_t kp, double rel_tol, int64_t m, int64_t n,
+                           float* y) {
+  Handle* h = H(hh);
+  if (!h || !piv || !y || kp <= 0 || kp > 64 || !(rel_tol > 0.0) || m <= 0 || n <= 0) return TTX_ERR_BAD_ARG;
+  if (!h->mlp_ready) return ttx::fail(h, TTX_ERR_BAD_ARG, "mlp_predict: no parameters (ttx_mlp_init / load)");
+  ttx::mlp_predict(ttx::P<float>(h->mP), piv, kp, rel_tol, (double)m, (double)n, y, h->stream);
+  TTX_TRY(ttx::check_launch(h, "predict"));
   TTX_HIP(h, hipStreamSynchronize(h->stream));
   return TTX_OK;
 }

@@ -52,6 +52,7 @@ int64_t panel_blocks(int64_t rows);
 // G + s I = R^T R, Rinv = R^-1.  shift_mode 1 adds s = 11 (m k + k (k+1)) u trace(G); pivots that are
 // not positive zero their row and column of Rinv and count into status[0].  pivots (optional, k doubles)
 // receives the shifted pivots R_ii^2.
+// Rinv null: the factorisation only (pivots and status)
 void launch_chol_inv(const double* part, int64_t S, int k, int64_t m, int shift_mode, double* Rinv, int* status,
                      hipStream_t st, double* pivots = nullptr);
 // W_out = W_in Rinv (rows x k) per 64-row block with the Gram partial of W_out (part[block][k][k]).
@@ -121,6 +122,9 @@ void launch_permute_231(const float* in, float* out, int64_t r0, int64_t r1, int
 void mlp_init(float* params, int in, int hid, int out, uint64_t seed, hipStream_t st);
 // features from the probe's shifted Cholesky pivots d[0..kp-1] (= R_ii^2)
 void mlp_features(const double* d, int kp, double rel_tol, double m, double n, float* x, int in, hipStream_t st);
+// features + forward of one sample in one workgroup (same arithmetic as mlp_features + mlp_forward)
+void mlp_predict(const float* P, const double* d, int kp, double rel_tol, double m, double n, float* y,
+                 hipStream_t st);
 void mlp_forward(const float* P, int in, int hid, int out, const float* X, int B, float* Z1, float* H1, float* Z2,
                  float* H2, float* Y, hipStream_t st);
 void mlp_backward(const float* P, int in, int hid, int out, const float* X, const float* T, int B, const float* Z1,

@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "ttx.h"
 #include "ttx_device.h"
 #include "ttx_kernels.h"
 
@@ -173,6 +174,67 @@ __global__ void k_mlp_features(const double* __restrict__ d, int kp, double rel_
     v = 1.0;
   }
   x[t] = (float)v;
+}
+
+// One sample end to end in one workgroup (the prediction inside a decomposition step): the
+// features of k_mlp_features, then the three layers with the arithmetic of k_mlp_gemm (binary16
+// operands, FP32 products accumulated in k order from zero, bias added last) and k_gelu, so the
+// output equals that of mlp_features + mlp_forward on the same pivots.  HID threads.
+template <int IN, int HID, int OUT>
+__global__ __launch_bounds__(HID) void k_mlp_predict(const float* __restrict__ P, const double* __restrict__ d, int kp,
+                                                     double rel_tol, double m, double n, float* __restrict__ y) {
+  __shared__ float xs[IN];
+  __shared__ float h1[HID];
+  __shared__ float h2[HID];
+  __shared__ double D;
+  const float* W1 = P;
+  const float* b1 = W1 + IN * HID;
+  const float* W2 = b1 + HID;
+  const float* b2 = W2 + HID * HID;
+  const float* W3 = b2 + HID;
+  const float* b3 = W3 + HID * OUT;
+  const int t = threadIdx.x;
+  if (t == 0) {
+    double s = 0.0;
+    for (int i = 0; i < kp; ++i) s += d[i];
+    D = s > 0.0 ? s : 1.0;
+  }
+  __syncthreads();
+  if (t < IN) {
+    double v = 0.0;
+    if (t < 32) {
+      if (2 * t + 1 < kp) v = log10(fmax(d[2 * t] + d[2 * t + 1], 1e-30 * D) / D) / 10.0;
+    } else if (t == 32) {
+      v = log10(rel_tol) / 10.0;
+    } else if (t == 33) {
+      v = log2(m) / 16.0;
+    } else if (t == 34) {
+      v = log2(n) / 32.0;
+    } else if (t == 35) {
+      v = 1.0;
+    }
+    xs[t] = r16((float)v);
+  }
+  __syncthreads();
+  float acc = 0.0f;
+  for (int k = 0; k < IN; ++k) acc += xs[k] * r16(W1[k * HID + t]);
+  h1[t] = r16(gelu(acc + b1[t]));
+  __syncthreads();
+  acc = 0.0f;
+  for (int k = 0; k < HID; ++k) acc += h1[k] * r16(W2[k * HID + t]);
+  h2[t] = r16(gelu(acc + b2[t]));
+  __syncthreads();
+  if (t < OUT) {
+    acc = 0.0f;
+    for (int k = 0; k < HID; ++k) acc += h2[k] * r16(W3[k * OUT + t]);
+    y[t] = acc + b3[t];
+  }
+}
+
+void mlp_predict(const float* P, const double* d, int kp, double rel_tol, double m, double n, float* y,
+                 hipStream_t st) {
+  hipLaunchKernelGGL((k_mlp_predict<TTX_MLP_IN, TTX_MLP_HIDDEN, TTX_MLP_OUT>), dim3(1), dim3(TTX_MLP_HIDDEN), 0, st, P,
+                     d, kp, rel_tol, m, n, y);
 }
 
 static unsigned ew_grid(int64_t count) {

@@ -80,6 +80,24 @@ def test_features_kernel_matches_oracle(engine):
     assert np.max(np.abs(x - ref)) < 1e-6
 
 
+def test_fused_predict_matches_features_and_forward(engine):
+    """The one-launch prediction a step makes equals features + forward bit for bit, and the oracle's
+    FP64 network with binary16 operands within FP32 accumulation error."""
+    import torch
+    rng = np.random.default_rng(9)
+    engine.mlp_init(3)
+    for trial in range(4):
+        piv = 10.0 ** rng.uniform(-14, 3, 64)
+        piv[40 + 5 * trial:] = 0.0
+        dp = torch.from_numpy(piv).cuda()
+        x = engine.mlp_features(dp, 1e-6 * (trial + 1), 1024, 1 << 20)
+        y_ref = engine.mlp_forward(x[None, :]).cpu().numpy()[0]
+        y = engine.mlp_predict(dp, 1e-6 * (trial + 1), 1024, 1 << 20).cpu().numpy()
+        assert np.array_equal(y, y_ref), (y, y_ref)
+        yo, _ = om.forward(engine.mlp_get_params(), om.features(piv, 1e-6 * (trial + 1), 1024, 1 << 20)[None, :])
+        assert np.max(np.abs(y - yo[0])) < 1e-4 * max(1.0, np.max(np.abs(yo)))
+
+
 @pytest.mark.parametrize("m,n,rank", [(96, 5000, 20), (300, 20000, 64), (128, 200000, 8)])
 def test_probe_matches_oracle(engine, m, n, rank):
     rng = np.random.default_rng(m)
