@@ -11,6 +11,8 @@
 // Rows of A are spread over lanes so that every global load is a 16-byte vector along a column
 // (rows 4r..4r+3 of one column per lane); the MFMA row index r of tile s then maps to physical
 // row 4r+s.  The 32x32 MFMA output layout is documented in ttx_device.h.
+#include <cstdlib>
+
 #include "ttx_device.h"
 #include "ttx_kernels.h"
 
@@ -338,6 +340,151 @@ __global__ __launch_bounds__(512) void k_sketch(const float* __restrict__ A, int
       for (int v = 0; v < 16; ++v) {
         const int prow = blockIdx.y * RW * RB + RW * rb + NS * mfma32_row(v, lane) + s;
         if (prow < m) out[(int64_t)prow * KP + 32 * lt + r] = S.acc[s][lt][v] * unscale[lt];
+      }
+}
+
+// ------------------------------------------------------------------------------------------------
+// sketch with the in-kernel Omega, A streamed straight into LDS
+// ------------------------------------------------------------------------------------------------
+// Same tiling, arithmetic and output as k_sketch<LT, AR, true, *>, but each wave's A tile (16
+// columns x RW rows per chunk) travels from HBM to LDS by global_load_lds_dwordx4, ST chunks deep:
+// the bytes in flight no longer cost registers, so a CU keeps (ST - 1) x 32 KB of A requested while
+// it computes (a register double buffer allowed one chunk).  Every wave loads and reads only its
+// own tile, so the A stages need no barrier: s_waitcnt vmcnt((ST - 1) NI) before reading chunk c
+// leaves the ST - 1 later chunks in flight (no other vector-memory operation runs in the loop).
+// Load layout: instruction q of a chunk writes 1 KB = CPI = 256 / RW columns of RW rows; lane l
+// reads rows 4 (l % LPC) .. +3 of column CPI q + l / LPC (LPC = RW / 4 lanes per column), so the
+// stage is the tile in column-major order [16][RW] and lane r of the MFMA reads its NS rows of
+// column 8 b + t as one ds_read.  Requires lda % 4 == 0 and m % 4 == 0 (16-byte rows); rows past m
+// load row 0 of the column instead (their outputs are not written) and columns past n load column
+// n - 1 (zeroed before use).
+template <int LT, int AR, int ST>
+__global__ __launch_bounds__(512) void k_sketch_dl(const float* __restrict__ A, int64_t lda, int m, int64_t n,
+                                                   uint64_t seed, uint32_t stream, int ell, int RB, int CG,
+                                                   int64_t chunks_per_wg, float* __restrict__ part,
+                                                   const float* __restrict__ scales) {
+  constexpr int KP = 32 * LT;
+  constexpr int NS = 4 / LT;
+  constexpr int RW = 32 * NS;
+  constexpr int LPC = RW / 4;            // lanes per column in a load
+  constexpr int CPI = 64 / LPC;          // columns per load instruction
+  constexpr int NI = 16 / CPI;           // load instructions per chunk and wave
+  constexpr int TILE = 16 * RW;          // floats of a wave's tile
+  const int KC = 16 * CG;
+  const int pitch = KC + 4;
+  extern __shared__ __align__(16) float smem[];
+  float* xs0 = smem;
+  float* xs1 = smem + KP * pitch;
+  const int lane = threadIdx.x & 63;
+  const int w = threadIdx.x >> 6;
+  const int nw = blockDim.x >> 6;
+  float* stg = smem + 2 * KP * pitch + w * TILE;   // stage s of this wave at stg + s * nw * TILE
+
+  const int rb = w % RB;
+  const int cg = w / RB;
+  const int r = lane & 31;
+  const int b = lane >> 5;
+  const int row0 = blockIdx.y * RW * RB + RW * rb;
+
+  const int64_t nchunks = (n + KC - 1) / KC;
+  const int64_t c0 = (int64_t)blockIdx.x * chunks_per_wg;
+  const int64_t c1 = (c0 + chunks_per_wg < nchunks) ? c0 + chunks_per_wg : nchunks;
+
+  SketchAcc<LT> S;
+#pragma unroll
+  for (int s = 0; s < NS; ++s)
+#pragma unroll
+    for (int lt = 0; lt < LT; ++lt) S.acc[s][lt] = zero16();
+  float sa = 1.0f, sx[LT];
+#pragma unroll
+  for (int lt = 0; lt < LT; ++lt) sx[lt] = 1.0f;
+  if (AR == AR_FP16 && scales) sa = scales[0];
+
+  // this lane's load: rows lrow .. lrow + 3 (row 0 when past m) of column lcol + CPI q of the chunk
+  const int lrow_raw = row0 + 4 * (lane % LPC);
+  const int lrow = (lrow_raw < m) ? lrow_raw : 0;
+  const int lcol = 16 * cg + lane / LPC;
+  auto issue = [&](int64_t c) {
+    float* dst = stg + (int)(c % ST) * nw * TILE;
+#pragma unroll
+    for (int q = 0; q < NI; ++q) {
+      int64_t j = c * KC + lcol + CPI * q;
+      if (j >= n) j = n - 1;
+      __builtin_amdgcn_global_load_lds((const void*)(A + j * lda + lrow),
+                                       (__attribute__((address_space(3))) void*)(dst + q * 256), 16, 0, 0);
+    }
+  };
+  auto compute = [&](int64_t c, const float* xs, int kofs) {
+    const float* src = stg + (int)(c % ST) * nw * TILE;
+    float a[8][NS];
+#pragma unroll
+    for (int t = 0; t < 8; ++t) {
+      const float* pp = src + (8 * b + t) * RW + NS * r;
+      if constexpr (NS == 4) {
+        const f32x4 v = *reinterpret_cast<const f32x4*>(pp);
+        a[t][0] = v[0]; a[t][1] = v[1]; a[t][2] = v[2]; a[t][3] = v[3];
+      } else {
+        typedef float f32x2 __attribute__((ext_vector_type(2)));
+        const f32x2 v = *reinterpret_cast<const f32x2*>(pp);
+        a[t][0] = v[0]; a[t][1] = v[1];
+      }
+    }
+    if (c * KC + KC > n) {
+      // tail chunk: columns past n hold a copy of column n - 1
+#pragma unroll
+      for (int t = 0; t < 8; ++t)
+        if (c * KC + 16 * cg + 8 * b + t >= n)
+#pragma unroll
+          for (int s2 = 0; s2 < NS; ++s2) a[t][s2] = 0.0f;
+    }
+    sketch_compute<LT, AR, true>(a, xs, pitch, kofs, lane, S, sa, sx);
+  };
+  constexpr int WAIT = (ST - 1) * NI;
+  constexpr int WAIT_IMM = (WAIT & 15) | ((WAIT >> 4) << 14) | 0x0F70;   // vmcnt(WAIT), others unchanged
+  static_assert(WAIT < 64, "vmcnt range");
+  // workgroup barrier for the Omega chunks only: their LDS writes complete (lgkmcnt(0)), then
+  // s_barrier.  __syncthreads would also wait for the A loads still in flight (vmcnt(0)), which
+  // would drain the pipeline every chunk.
+  auto xs_barrier = [] {
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_s_waitcnt(0xC07F);   // lgkmcnt(0); vmcnt, expcnt unchanged
+    __builtin_amdgcn_s_barrier();
+    asm volatile("" ::: "memory");
+  };
+  FillRegs<true, 1> R;
+  for (int d = 0; d < ST - 1; ++d)
+    if (c0 + d < c1) issue(c0 + d);
+  if (c0 < c1) fill_store<LT, true, 1>(R, xs0, pitch, KC, c0 * KC, n, seed, stream, ell);
+  xs_barrier();
+  for (int64_t c = c0; c < c1; ++c) {
+    const float* xcur = ((c - c0) & 1) ? xs1 : xs0;
+    float* xnext = ((c - c0) & 1) ? xs0 : xs1;
+    // chunk c + ST - 1 goes out before chunk c is read: ST chunks requested; the wait leaves ST - 1
+    if (c + ST - 1 < c1) {
+      issue(c + ST - 1);
+      __builtin_amdgcn_s_waitcnt(WAIT_IMM);
+    } else {
+      __builtin_amdgcn_s_waitcnt(0x0F70);   // the last chunks: wait for all
+    }
+    compute(c, xcur, 16 * cg);
+    __builtin_amdgcn_sched_barrier(0);
+    if (c + 1 < c1) fill_store<LT, true, 1>(R, xnext, pitch, KC, (c + 1) * KC, n, seed, stream, ell);
+    xs_barrier();
+  }
+
+  const int64_t slab = (int64_t)blockIdx.x * CG + cg;
+  float* out = part + slab * (int64_t)m * KP;
+  float unscale[LT];
+#pragma unroll
+  for (int lt = 0; lt < LT; ++lt) unscale[lt] = 1.0f / (sa * sx[lt]);
+#pragma unroll
+  for (int s2 = 0; s2 < NS; ++s2)
+#pragma unroll
+    for (int lt = 0; lt < LT; ++lt)
+#pragma unroll
+      for (int v = 0; v < 16; ++v) {
+        const int prow = row0 + NS * mfma32_row(v, lane) + s2;
+        if (prow < m) out[(int64_t)prow * KP + 32 * lt + r] = S.acc[s2][lt][v] * unscale[lt];
       }
 }
 
@@ -788,6 +935,11 @@ __global__ __launch_bounds__(256) void k_reduce(const TI* __restrict__ part, int
 // launchers
 // ------------------------------------------------------------------------------------------------
 static int g_num_cus = 0;
+// direct-to-LDS sketch (k_sketch_dl); TTX_SKETCH_DL=0 selects the register-buffered kernel
+static const bool g_sketch_dl = [] {
+  const char* e = getenv("TTX_SKETCH_DL");
+  return !(e && e[0] == '0');
+}();
 int num_cus() {
   if (g_num_cus == 0) {
     int dev = 0;
@@ -836,9 +988,32 @@ static void launch_sketch_t(const SketchPlan& p, const float* A, int64_t lda, in
                             const float* scales) {
   dim3 grid(p.grid_x, p.ysplit);
   const int fq = (32 * LT) / (16 * p.RB);   // float4 pieces of the X chunk per thread
-  if constexpr (OM)
+  if constexpr (OM) {
+    // A through LDS, as many chunks deep (4 at most) as fit, when the rows are 16-byte aligned
+    constexpr int RW = 128 / LT;
+    const int nw = p.RB * p.CG;
+    const int tile_bytes = 16 * RW * (int)sizeof(float) * nw;
+    constexpr int LDS_MAX = 160 * 1024;
+    if ((lda & 3) == 0 && (m & 3) == 0 && g_sketch_dl) {
+      if (p.lds_bytes + 4 * tile_bytes <= LDS_MAX) {
+        hipLaunchKernelGGL((k_sketch_dl<LT, AR, 4>), grid, dim3(64 * nw), p.lds_bytes + 4 * tile_bytes, st, A, lda, m, n,
+                           seed, stream, ell, p.RB, p.CG, p.chunks_per_wg, part, scales);
+        return;
+      }
+      if (p.lds_bytes + 3 * tile_bytes <= LDS_MAX) {
+        hipLaunchKernelGGL((k_sketch_dl<LT, AR, 3>), grid, dim3(64 * nw), p.lds_bytes + 3 * tile_bytes, st, A, lda, m, n,
+                           seed, stream, ell, p.RB, p.CG, p.chunks_per_wg, part, scales);
+        return;
+      }
+      if (p.lds_bytes + 2 * tile_bytes <= LDS_MAX) {
+        hipLaunchKernelGGL((k_sketch_dl<LT, AR, 2>), grid, dim3(64 * nw), p.lds_bytes + 2 * tile_bytes, st, A, lda, m, n,
+                           seed, stream, ell, p.RB, p.CG, p.chunks_per_wg, part, scales);
+        return;
+      }
+    }
     hipLaunchKernelGGL((k_sketch<LT, AR, OM, 1>), grid, dim3(64 * p.RB * p.CG), p.lds_bytes, st, A, lda, m, n, X, ldx,
                        seed, stream, ell, p.RB, p.CG, p.chunks_per_wg, part, scales);
+  }
   else if (fq <= 1)
     hipLaunchKernelGGL((k_sketch<LT, AR, OM, 1>), grid, dim3(64 * p.RB * p.CG), p.lds_bytes, st, A, lda, m, n, X, ldx,
                        seed, stream, ell, p.RB, p.CG, p.chunks_per_wg, part, scales);
