@@ -28,6 +28,14 @@ struct Handle {
   std::string err;
   DevBuf part, Y, Y2, Qc, Qp, Qn, Z, W1, W2, G, G2, Rinv, lam, V, Ur, gpart, status, next[2], sig, keys, scratch[4];
   DevBuf Yt, tpart, tout;         // held-out test columns of the range check and its partial sums
+  // the sketch of the second step of a TT sweep, taken in the first step's Gram pass (the same
+  // memory read once): valid for the unfolding (pre_A, pre_m x pre_n) with Omega (seed, stream)
+  DevBuf Ypre;
+  bool pre_valid = false;
+  const float* pre_A = nullptr;
+  int64_t pre_m = 0, pre_n = 0;
+  uint64_t pre_seed = 0;
+  uint32_t pre_stream = 0;
   // rank model: parameters, Adam moments, gradient, activations (batch-sized), probe pivots / features
   DevBuf mP, mM1, mM2, mG, mAct, mPart, mIO;
   DevBuf f16s;                    // FP16 operand scales (65 floats) and column-max words (64)
@@ -213,6 +221,8 @@ struct StepArgs {
   // a learned model predicted (the check judges it, the adaptive growth is its fallback)
   double range_tol2 = 0.0;
   int pred_ell = 0;
+  // exact first step whose Gram partials (gram_pre of them, in gpart) the fused first pass made
+  int64_t gram_pre = 0;
 };
 struct StepOut {
   int r = 0, ell = 0, path = 0, power_iters = 0, zero_pivots = 0, sweeps = 0, kdim = 0;
@@ -431,19 +441,26 @@ static ttx_status range_sketch(Handle* h, const StepArgs& a, const Arith3& ar, i
   hipStream_t st = h->stream;
   const int m = (int)a.m;
   float* Y = P<float>(h->Y);
-  float* sc = fp16_scales(h, a, ar, true);
-  {
-    KScope ks(h, TTX_K_SKETCH, a.m * a.n * 4);
-    launch_sketch(ar.range, sp, a.A, a.m, m, a.n, nullptr, 0, a.seed, a.stream_id, w + pt, P<float>(h->part), st,
-                  sc);
+  if (h->pre_valid && a.A == h->pre_A && a.m == h->pre_m && a.n == h->pre_n && kp == 64 && a.seed == h->pre_seed &&
+      a.stream_id == h->pre_stream && ar.range == AR_BF16X3) {
+    // taken in the first step's pass with every Omega column: its first w + pt columns are this
+    // sketch (column c of Omega does not depend on the width)
+    Y = P<float>(h->Ypre);
+  } else {
+    float* sc = fp16_scales(h, a, ar, true);
+    {
+      KScope ks(h, TTX_K_SKETCH, a.m * a.n * 4);
+      launch_sketch(ar.range, sp, a.A, a.m, m, a.n, nullptr, 0, a.seed, a.stream_id, w + pt, P<float>(h->part), st,
+                    sc);
+    }
+    {
+      KScope ks(h, TTX_K_SMALL, 0);
+      launch_reduce_f32(P<float>(h->part), sp.slabs, (int64_t)m * kp, Y, st);
+    }
+    TTX_TRY(check_launch(h, "sketch"));
+    o.bytes += a.m * a.n * 4;
+    o.passes += 1;
   }
-  {
-    KScope ks(h, TTX_K_SMALL, 0);
-    launch_reduce_f32(P<float>(h->part), sp.slabs, (int64_t)m * kp, Y, st);
-  }
-  TTX_TRY(check_launch(h, "sketch"));
-  o.bytes += a.m * a.n * 4;
-  o.passes += 1;
   float* Ya = left_mix(h, a, Y, kp);
   if (pt) {
     KScope ks(h, TTX_K_SMALL, 0);
@@ -628,9 +645,11 @@ static ttx_status run_step(Handle* h, const StepArgs& a_in, StepOut& o) {
     o.path = TTX_PATH_EXACT_M;
     o.ell = 0;
     k = m;
-    const int64_t grid = gram_small_grid(n);
-    TTX_TRY(ensure(h, h->gpart, (size_t)grid * m * m * sizeof(double)));
-    {
+    int64_t grid = gram_small_grid(n);
+    if (a.gram_pre > 0) {
+      grid = a.gram_pre;   // the fused first pass left them in gpart
+    } else {
+      TTX_TRY(ensure(h, h->gpart, (size_t)grid * m * m * sizeof(double)));
       KScope ks(h, TTX_K_GRAM, a.m * n * 4);
       launch_gram_small(ar.accurate, a.A, a.m, m, n, P<double>(h->gpart), grid, nullptr, st);
     }
@@ -866,8 +885,61 @@ static ttx_status sweep(Handle* h, const float* d_A, const ttx_params* p, float*
   R.ranks[0] = p->ranks[0];
   R.ranks[d] = p->ranks[d];
   if (timing) TTX_HIP(h, hipEventRecord(ev[0], st));
+  // One read of the tensor for the first two steps: when step 0 is an exact 32-row step whose rank
+  // ceiling is full (so it defers its projection) and step 1 is a randomized step with a 64-column
+  // sketch, the pass that takes step 0's Gram also takes step 1's sketch (of the same memory viewed
+  // as 32 N_1 rows; the deferred core is applied to the sketch afterwards, as for any fused step).
+  h->pre_valid = false;
+  struct PreGuard {   // the precomputed sketch never outlives this sweep, whatever path leaves it
+    Handle* h;
+    ~PreGuard() { h->pre_valid = false; }
+  } pre_guard{h};
+  int64_t gram_pre = 0;
+  if (!tr && !h->capture && d >= 3 && p->precision == TTX_PREC_BF16 && !(p->flags & TTX_FLAG_FAST_RANGE) &&
+      p->ranks[0] == 1 && p->dims[0] == 32) {
+    const int64_t m0 = 32, n0 = total / 32;
+    const int64_t r0 = std::min<int64_t>(p->ranks[1], std::min(m0, n0));
+    const bool exact0 = (p->method != TTX_METHOD_RSVD) &&
+                        (r0 + p->oversampling >= std::min(m0, n0) || p->method == TTX_METHOD_EXACT);
+    const int64_t m1 = m0 * p->dims[1], n1 = n0 / p->dims[1];
+    const int64_t mn1 = std::min(m1, n1);
+    const int64_t r1 = std::min<int64_t>(p->ranks[2], mn1);
+    int64_t ell1 = r1 + p->oversampling;
+    const bool exact1 = (p->method == TTX_METHOD_EXACT) || (p->method == TTX_METHOD_AUTO && ell1 >= mn1);
+    ell1 = std::min<int64_t>(std::min<int64_t>(ell1, TTX_MAX_ELL), mn1);
+    const bool adaptive1 = (p->flags & TTX_FLAG_ADAPTIVE) || p->rank_mode != TTX_RANK_FIXED;
+    int pt1 = adaptive1 ? (int)std::min<int64_t>(8, TTX_MAX_ELL - ell1) : 0;
+    if (pt1 < 2) pt1 = 0;
+    if (exact0 && r0 == m0 && !exact1 && kp_for((int)ell1 + pt1) == 64 && m1 % 64 == 0 && m1 <= (1 << 30)) {
+      const SketchPlan sp = plan_sketch((int)m1, n1, 64, 0);
+      if (sketch_gram0_ok(sp, (int)m1, m1, 64)) {
+        const int64_t parts = sketch_gram0_partials(sp);
+        TTX_TRY(ensure(h, h->part, (size_t)sp.slabs * m1 * 64 * sizeof(float)));
+        TTX_TRY(ensure(h, h->gpart, (size_t)parts * 32 * 32 * sizeof(double)));
+        TTX_TRY(ensure(h, h->Ypre, (size_t)m1 * 64 * sizeof(float)));
+        {
+          KScope ks(h, TTX_K_SKETCH, m1 * n1 * 4);
+          launch_sketch_gram0(sp, d_A, m1, (int)m1, n1, p->omega_seed, 1u, 64, P<float>(h->part), P<double>(h->gpart),
+                              st);
+        }
+        {
+          KScope ks(h, TTX_K_SMALL, 0);
+          launch_reduce_f32(P<float>(h->part), sp.slabs, m1 * 64, P<float>(h->Ypre), st);
+        }
+        TTX_TRY(check_launch(h, "first pass"));
+        gram_pre = parts;
+        h->pre_valid = true;
+        h->pre_A = d_A;
+        h->pre_m = m1;
+        h->pre_n = n1;
+        h->pre_seed = p->omega_seed;
+        h->pre_stream = 1u;
+      }
+    }
+  }
   for (int k = 0; k < d - 1; ++k) {
     StepArgs a;
+    if (k == 0) a.gram_pre = gram_pre;
     const int64_t m = (int64_t)r_prev * p->dims[k];
     n = n / p->dims[k];
     a.A = cur;
@@ -979,6 +1051,7 @@ static ttx_status sweep(Handle* h, const float* d_A, const ttx_params* p, float*
     }
     if (timing) TTX_HIP(h, hipEventRecord(ev[k + 1], st));
   }
+  h->pre_valid = false;
   if (h->capture) {
     if (rep) *rep = R;
     return check_launch(h, "batch sweep");

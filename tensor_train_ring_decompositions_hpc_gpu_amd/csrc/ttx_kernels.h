@@ -19,6 +19,13 @@ SketchPlan plan_sketch(int m, int64_t n, int kp, int wgs_hint);
 void launch_sketch(int ar, const SketchPlan& p, const float* A, int64_t lda, int m, int64_t n, const float* X,
                    int64_t ldx, uint64_t seed, uint32_t stream, int ell, float* part, hipStream_t st,
                    const float* scales = nullptr);
+// The fused first pass of a TT sweep: the BF16X3 Omega sketch of A (m x n, kp = 64 columns) as
+// launch_sketch, plus the Gram of the 32-row view of the same memory (32 x (m / 32) n) into
+// sketch_gram0_partials(p) FP64 32 x 32 partials.  sketch_gram0_ok: the shapes it takes.
+bool sketch_gram0_ok(const SketchPlan& p, int m, int64_t lda, int kp);
+int64_t sketch_gram0_partials(const SketchPlan& p);
+void launch_sketch_gram0(const SketchPlan& p, const float* A, int64_t lda, int m, int64_t n, uint64_t seed,
+                         uint32_t stream, int ell, float* part, double* gpart, hipStream_t st);
 // scales[0] from sum_i<r sig_prev[i]^2 / numel (RMS entry to [8, 16)); sig_prev == nullptr gives 1
 void launch_fp16_ascale(const float* sig_prev, int r_prev, double numel, float* scales, hipStream_t st);
 // scales[1 + c] from max_j |X[j][c]| (to [128, 256)); mx: kp zeroed words, zeroed again afterwards

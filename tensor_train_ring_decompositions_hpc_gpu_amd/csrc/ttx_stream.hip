@@ -352,24 +352,34 @@ __global__ __launch_bounds__(512) void k_sketch(const float* __restrict__ A, int
 // it computes (a register double buffer allowed one chunk).  Every wave loads and reads only its
 // own tile, so the A stages need no barrier: s_waitcnt vmcnt((ST - 1) NI) before reading chunk c
 // leaves the ST - 1 later chunks in flight (no other vector-memory operation runs in the loop).
-// Load layout: instruction q of a chunk writes 1 KB = CPI = 256 / RW columns of RW rows; lane l
-// reads rows 4 (l % LPC) .. +3 of column CPI q + l / LPC (LPC = RW / 4 lanes per column), so the
-// stage is the tile in column-major order [16][RW] and lane r of the MFMA reads its NS rows of
-// column 8 b + t as one ds_read.  Requires lda % 4 == 0 and m % 4 == 0 (16-byte rows); rows past m
-// load row 0 of the column instead (their outputs are not written) and columns past n load column
-// n - 1 (zeroed before use).
-template <int LT, int AR, int ST>
+// Load layout: instruction q of a chunk writes 1 KB = CPI = 256 / RW columns of RW rows (then PAD
+// floats of padding); lane l reads rows 4 (l % LPC) .. +3 of column CPI q + l / LPC (LPC = RW / 4
+// lanes per column), so the stage is the tile in column-major order and lane r of the MFMA reads
+// its NS rows of column 8 b + t as one ds_read.  Requires lda % 4 == 0 and m % 4 == 0 (16-byte
+// rows); rows past m load row 0 of the column instead (their outputs are not written) and columns
+// past n load column n - 1 (zeroed before use).
+//
+// GRAM0 (LT = 2, m % 64 == 0): the same pass also accumulates the Gram C = A0 A0^T of the 32-row
+// view A0 (32 x 32 n) of the same memory: a 64-row tile holds two 32-row columns of A0 per column
+// of A.  It is the exact first step of a TT sweep whose second step this sketch serves, so the
+// tensor is read once for both.  Products as k_gram_small<1, AR_BF16X4> (hi/lo split, four bf16
+// MFMAs), FP32 accumulators flushed to FP64 every 32 chunks; one FP64 32 x 32 partial per WG in
+// gpart[blockIdx.y * gridDim.x + blockIdx.x].  PAD = 16 makes the Gram's column reads of the two
+// lane halves fall on disjoint banks.
+template <int LT, int AR, int ST, bool GRAM0>
 __global__ __launch_bounds__(512) void k_sketch_dl(const float* __restrict__ A, int64_t lda, int m, int64_t n,
                                                    uint64_t seed, uint32_t stream, int ell, int RB, int CG,
                                                    int64_t chunks_per_wg, float* __restrict__ part,
-                                                   const float* __restrict__ scales) {
+                                                   const float* __restrict__ scales, double* __restrict__ gpart) {
   constexpr int KP = 32 * LT;
   constexpr int NS = 4 / LT;
   constexpr int RW = 32 * NS;
   constexpr int LPC = RW / 4;            // lanes per column in a load
   constexpr int CPI = 64 / LPC;          // columns per load instruction
   constexpr int NI = 16 / CPI;           // load instructions per chunk and wave
-  constexpr int TILE = 16 * RW;          // floats of a wave's tile
+  constexpr int PAD = GRAM0 ? 16 : 0;
+  constexpr int TILE = NI * (256 + PAD); // floats of a wave's tile
+  static_assert(!GRAM0 || LT == 2, "the fused Gram takes 64-row tiles");
   const int KC = 16 * CG;
   const int pitch = KC + 4;
   extern __shared__ __align__(16) float smem[];
@@ -399,7 +409,16 @@ __global__ __launch_bounds__(512) void k_sketch_dl(const float* __restrict__ A, 
 #pragma unroll
   for (int lt = 0; lt < LT; ++lt) sx[lt] = 1.0f;
   if (AR == AR_FP16 && scales) sa = scales[0];
+  [[maybe_unused]] f32x16 gacc;
+  [[maybe_unused]] double gd[16];
+  if constexpr (GRAM0) {
+    gacc = zero16();
+#pragma unroll
+    for (int v = 0; v < 16; ++v) gd[v] = 0.0;
+  }
 
+  // column col of the tile starts at tile_col(col) floats into the stage
+  auto tile_col = [](int col) { return (col / CPI) * (256 + PAD) + (col % CPI) * RW; };
   // this lane's load: rows lrow .. lrow + 3 (row 0 when past m) of column lcol + CPI q of the chunk
   const int lrow_raw = row0 + 4 * (lane % LPC);
   const int lrow = (lrow_raw < m) ? lrow_raw : 0;
@@ -411,15 +430,16 @@ __global__ __launch_bounds__(512) void k_sketch_dl(const float* __restrict__ A, 
       int64_t j = c * KC + lcol + CPI * q;
       if (j >= n) j = n - 1;
       __builtin_amdgcn_global_load_lds((const void*)(A + j * lda + lrow),
-                                       (__attribute__((address_space(3))) void*)(dst + q * 256), 16, 0, 0);
+                                       (__attribute__((address_space(3))) void*)(dst + q * (256 + PAD)), 16, 0, 0);
     }
   };
   auto compute = [&](int64_t c, const float* xs, int kofs) {
     const float* src = stg + (int)(c % ST) * nw * TILE;
+    const bool tail = c * KC + KC > n;
     float a[8][NS];
 #pragma unroll
     for (int t = 0; t < 8; ++t) {
-      const float* pp = src + (8 * b + t) * RW + NS * r;
+      const float* pp = src + tile_col(8 * b + t) + NS * r;
       if constexpr (NS == 4) {
         const f32x4 v = *reinterpret_cast<const f32x4*>(pp);
         a[t][0] = v[0]; a[t][1] = v[1]; a[t][2] = v[2]; a[t][3] = v[3];
@@ -429,7 +449,7 @@ __global__ __launch_bounds__(512) void k_sketch_dl(const float* __restrict__ A, 
         a[t][0] = v[0]; a[t][1] = v[1];
       }
     }
-    if (c * KC + KC > n) {
+    if (tail) {
       // tail chunk: columns past n hold a copy of column n - 1
 #pragma unroll
       for (int t = 0; t < 8; ++t)
@@ -438,6 +458,26 @@ __global__ __launch_bounds__(512) void k_sketch_dl(const float* __restrict__ A, 
           for (int s2 = 0; s2 < NS; ++s2) a[t][s2] = 0.0f;
     }
     sketch_compute<LT, AR, true>(a, xs, pitch, kofs, lane, S, sa, sx);
+    if constexpr (GRAM0) {
+      // K-step h: the 32-row half h of the 16 tile columns; lane (r, b) row r, columns 8 b + t.
+      // A tile past m (whole, as m % 64 == 0) holds row-0 copies: it adds nothing.
+      if (row0 < m)
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        float v[8];
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+          v[t] = src[tile_col(8 * b + t) + 32 * h + r];
+          if (tail && c * KC + 16 * cg + 8 * b + t >= n) v[t] = 0.0f;
+        }
+        bf16x8 hi, lo;
+        split8(v, hi, lo);
+        gacc = mfma_bf16(hi, hi, gacc);
+        gacc = mfma_bf16(lo, hi, gacc);
+        gacc = mfma_bf16(hi, lo, gacc);
+        gacc = mfma_bf16(lo, lo, gacc);
+      }
+    }
   };
   constexpr int WAIT = (ST - 1) * NI;
   constexpr int WAIT_IMM = (WAIT & 15) | ((WAIT >> 4) << 14) | 0x0F70;   // vmcnt(WAIT), others unchanged
@@ -470,6 +510,13 @@ __global__ __launch_bounds__(512) void k_sketch_dl(const float* __restrict__ A, 
     __builtin_amdgcn_sched_barrier(0);
     if (c + 1 < c1) fill_store<LT, true, 1>(R, xnext, pitch, KC, (c + 1) * KC, n, seed, stream, ell);
     xs_barrier();
+    if constexpr (GRAM0) {
+      if (((c - c0) & 31) == 31) {
+#pragma unroll
+        for (int v = 0; v < 16; ++v) gd[v] += (double)gacc[v];
+        gacc = zero16();
+      }
+    }
   }
 
   const int64_t slab = (int64_t)blockIdx.x * CG + cg;
@@ -486,6 +533,22 @@ __global__ __launch_bounds__(512) void k_sketch_dl(const float* __restrict__ A, 
         const int prow = row0 + NS * mfma32_row(v, lane) + s2;
         if (prow < m) out[(int64_t)prow * KP + 32 * lt + r] = S.acc[s2][lt][v] * unscale[lt];
       }
+  if constexpr (GRAM0) {
+#pragma unroll
+    for (int v = 0; v < 16; ++v) gd[v] += (double)gacc[v];
+    // the waves' partials through the (now idle) stage area, summed in wave order
+    __syncthreads();
+    double* red = reinterpret_cast<double*>(smem + 2 * KP * pitch);
+#pragma unroll
+    for (int v = 0; v < 16; ++v) red[w * 1024 + mfma32_row(v, lane) * 32 + r] = gd[v];
+    __syncthreads();
+    double* o = gpart + ((int64_t)blockIdx.y * gridDim.x + blockIdx.x) * 1024;
+    for (int e = threadIdx.x; e < 1024; e += blockDim.x) {
+      double sum = 0.0;
+      for (int ww = 0; ww < nw; ++ww) sum += red[ww * 1024 + e];
+      o[e] = sum;
+    }
+  }
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -996,18 +1059,18 @@ static void launch_sketch_t(const SketchPlan& p, const float* A, int64_t lda, in
     constexpr int LDS_MAX = 160 * 1024;
     if ((lda & 3) == 0 && (m & 3) == 0 && g_sketch_dl) {
       if (p.lds_bytes + 4 * tile_bytes <= LDS_MAX) {
-        hipLaunchKernelGGL((k_sketch_dl<LT, AR, 4>), grid, dim3(64 * nw), p.lds_bytes + 4 * tile_bytes, st, A, lda, m, n,
-                           seed, stream, ell, p.RB, p.CG, p.chunks_per_wg, part, scales);
+        hipLaunchKernelGGL((k_sketch_dl<LT, AR, 4, false>), grid, dim3(64 * nw), p.lds_bytes + 4 * tile_bytes, st, A, lda, m,
+                           n, seed, stream, ell, p.RB, p.CG, p.chunks_per_wg, part, scales, nullptr);
         return;
       }
       if (p.lds_bytes + 3 * tile_bytes <= LDS_MAX) {
-        hipLaunchKernelGGL((k_sketch_dl<LT, AR, 3>), grid, dim3(64 * nw), p.lds_bytes + 3 * tile_bytes, st, A, lda, m, n,
-                           seed, stream, ell, p.RB, p.CG, p.chunks_per_wg, part, scales);
+        hipLaunchKernelGGL((k_sketch_dl<LT, AR, 3, false>), grid, dim3(64 * nw), p.lds_bytes + 3 * tile_bytes, st, A, lda, m,
+                           n, seed, stream, ell, p.RB, p.CG, p.chunks_per_wg, part, scales, nullptr);
         return;
       }
       if (p.lds_bytes + 2 * tile_bytes <= LDS_MAX) {
-        hipLaunchKernelGGL((k_sketch_dl<LT, AR, 2>), grid, dim3(64 * nw), p.lds_bytes + 2 * tile_bytes, st, A, lda, m, n,
-                           seed, stream, ell, p.RB, p.CG, p.chunks_per_wg, part, scales);
+        hipLaunchKernelGGL((k_sketch_dl<LT, AR, 2, false>), grid, dim3(64 * nw), p.lds_bytes + 2 * tile_bytes, st, A, lda, m,
+                           n, seed, stream, ell, p.RB, p.CG, p.chunks_per_wg, part, scales, nullptr);
         return;
       }
     }
@@ -1035,6 +1098,29 @@ static void launch_sketch_ar(int ar, const SketchPlan& p, const float* A, int64_
     case AR_FP16: launch_sketch_t<LT, AR_FP16, OM>(p, A, lda, m, n, X, ldx, seed, stream, ell, part, st, sc); break;
     default: launch_sketch_t<LT, AR_BF16X3, OM>(p, A, lda, m, n, X, ldx, seed, stream, ell, part, st, sc); break;
   }
+}
+
+bool sketch_gram0_ok(const SketchPlan& p, int m, int64_t lda, int kp) {
+  if (kp != 64 || (m % 64) != 0 || (lda & 3) != 0 || p.CG != 1 || !g_sketch_dl) return false;
+  const int nw = p.RB * p.CG;
+  const int tile_bytes = 4 * (256 + 16) * (int)sizeof(float) * nw;
+  return p.lds_bytes + 3 * tile_bytes <= 160 * 1024 && p.RB * 64 * p.ysplit >= m;
+}
+
+int64_t sketch_gram0_partials(const SketchPlan& p) { return (int64_t)p.grid_x * p.ysplit; }
+
+void launch_sketch_gram0(const SketchPlan& p, const float* A, int64_t lda, int m, int64_t n, uint64_t seed,
+                         uint32_t stream, int ell, float* part, double* gpart, hipStream_t st) {
+  dim3 grid(p.grid_x, p.ysplit);
+  const int nw = p.RB * p.CG;
+  const int tile_bytes = 4 * (256 + 16) * (int)sizeof(float) * nw;
+  const int lds = p.lds_bytes + (p.lds_bytes + 4 * tile_bytes <= 160 * 1024 ? 4 : 3) * tile_bytes;
+  if (p.lds_bytes + 4 * tile_bytes <= 160 * 1024)
+    hipLaunchKernelGGL((k_sketch_dl<2, AR_BF16X3, 4, true>), grid, dim3(64 * nw), lds, st, A, lda, m, n, seed, stream,
+                       ell, p.RB, p.CG, p.chunks_per_wg, part, (const float*)nullptr, gpart);
+  else
+    hipLaunchKernelGGL((k_sketch_dl<2, AR_BF16X3, 3, true>), grid, dim3(64 * nw), lds, st, A, lda, m, n, seed, stream,
+                       ell, p.RB, p.CG, p.chunks_per_wg, part, (const float*)nullptr, gpart);
 }
 
 void launch_sketch(int ar, const SketchPlan& p, const float* A, int64_t lda, int m, int64_t n, const float* X,

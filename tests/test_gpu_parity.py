@@ -209,6 +209,34 @@ def test_tt_svd_full_rank_first_step(engine, prec, dims, ranks, s, q):
         assert rel(fmat(g, *shape), c_ref[k]) < 10 * tol["vec"], k
 
 
+@pytest.mark.parametrize("dims,ranks,s,q,mode", [
+    ([32, 32, 24, 20], [1, 32, 40, 12, 1], 10, 0, "fixed"),
+    ([32, 32, 24, 20], [1, 32, 40, 12, 1], 10, 1, "fixed"),
+    ([32, 16, 32, 8, 6], [1, 32, 48, 30, 6, 1], 8, 0, "fixed"),
+])
+def test_tt_svd_fused_first_pass(engine, dims, ranks, s, q, mode):
+    """BF16 sweeps whose first read of the tensor serves both the exact 32-row first step (its Gram)
+    and the second step's 64-column sketch: the same results as the oracle, which sketches the
+    projected remainder, and one pass fewer (step 0 counts the shared pass, step 1 only its
+    projections)."""
+    A = engine.generate(dims, ranks, std=1.0, noise_rel=1e-6, seed=2024)
+    cores, sig, rep = engine.tt_svd(A, dims, ranks, s=s, q=q, precision="bf16", rank_mode=mode)
+    An = host(A).astype(np.float64)
+    c_ref, s_ref, p_ref, used = ttsvd.tt_svd(An, dims, ranks, s=s, q=q)
+    tol = TOL["bf16"]
+    assert rep["ranks"] == used
+    assert rep["passes"][0] == 1 and rep["passes"][1] == 1 + 2 * q, rep["passes"]
+    for k in range(len(dims) - 1):
+        r = ranks[k + 1]
+        assert np.allclose(sig[k][:r], s_ref[k][:r], rtol=tol["sig"], atol=tol["sig"] * s_ref[k][0]), k
+    err = engine.rel_error(cores, dims, used, A)
+    err_ref = ttsvd.rel_error(An, ttsvd.tt_full(c_ref, dims, used))
+    assert abs(err - err_ref) <= tol["err"] + 2e-2 * err_ref, (err, err_ref)
+    for k in range(len(dims) - 1):
+        g, shape = cores[k]
+        assert rel(fmat(g, *shape), c_ref[k]) < 10 * tol["vec"], k
+
+
 # ------------------------------------------------------------------------------------------------
 # adaptive range finder: the engine takes the oracle's decisions (widths, power iterations, checks)
 # ------------------------------------------------------------------------------------------------
