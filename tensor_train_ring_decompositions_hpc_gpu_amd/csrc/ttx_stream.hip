@@ -363,9 +363,9 @@ __global__ __launch_bounds__(512) void k_sketch(const float* __restrict__ A, int
 // view A0 (32 x 32 n) of the same memory: a 64-row tile holds two 32-row columns of A0 per column
 // of A.  It is the exact first step of a TT sweep whose second step this sketch serves, so the
 // tensor is read once for both.  Products as k_gram_small<1, AR_BF16X4> (hi/lo split, four bf16
-// MFMAs), FP32 accumulators flushed to FP64 every 32 chunks; one FP64 32 x 32 partial per WG in
-// gpart[blockIdx.y * gridDim.x + blockIdx.x].  PAD = 16 makes the Gram's column reads of the two
-// lane halves fall on disjoint banks.
+// MFMAs, one FP32 accumulator per 32-row half so the two chains interleave), flushed to FP64
+// every 32 chunks; one FP64 32 x 32 partial per WG in gpart[blockIdx.y * gridDim.x + blockIdx.x].
+// PAD = 16 makes the Gram's column reads of the two lane halves fall on disjoint banks.
 template <int LT, int AR, int ST, bool GRAM0>
 __global__ __launch_bounds__(512) void k_sketch_dl(const float* __restrict__ A, int64_t lda, int m, int64_t n,
                                                    uint64_t seed, uint32_t stream, int ell, int RB, int CG,
@@ -409,10 +409,11 @@ __global__ __launch_bounds__(512) void k_sketch_dl(const float* __restrict__ A, 
 #pragma unroll
   for (int lt = 0; lt < LT; ++lt) sx[lt] = 1.0f;
   if (AR == AR_FP16 && scales) sa = scales[0];
-  [[maybe_unused]] f32x16 gacc;
+  [[maybe_unused]] f32x16 gacc[2];   // one per half: two independent MFMA chains
   [[maybe_unused]] double gd[16];
   if constexpr (GRAM0) {
-    gacc = zero16();
+    gacc[0] = zero16();
+    gacc[1] = zero16();
 #pragma unroll
     for (int v = 0; v < 16; ++v) gd[v] = 0.0;
   }
@@ -472,10 +473,10 @@ __global__ __launch_bounds__(512) void k_sketch_dl(const float* __restrict__ A, 
         }
         bf16x8 hi, lo;
         split8(v, hi, lo);
-        gacc = mfma_bf16(hi, hi, gacc);
-        gacc = mfma_bf16(lo, hi, gacc);
-        gacc = mfma_bf16(hi, lo, gacc);
-        gacc = mfma_bf16(lo, lo, gacc);
+        gacc[h] = mfma_bf16(hi, hi, gacc[h]);
+        gacc[h] = mfma_bf16(lo, hi, gacc[h]);
+        gacc[h] = mfma_bf16(hi, lo, gacc[h]);
+        gacc[h] = mfma_bf16(lo, lo, gacc[h]);
       }
     }
   };
@@ -513,8 +514,9 @@ __global__ __launch_bounds__(512) void k_sketch_dl(const float* __restrict__ A, 
     if constexpr (GRAM0) {
       if (((c - c0) & 31) == 31) {
 #pragma unroll
-        for (int v = 0; v < 16; ++v) gd[v] += (double)gacc[v];
-        gacc = zero16();
+        for (int v = 0; v < 16; ++v) gd[v] += (double)gacc[0][v] + (double)gacc[1][v];
+        gacc[0] = zero16();
+        gacc[1] = zero16();
       }
     }
   }
@@ -535,7 +537,7 @@ __global__ __launch_bounds__(512) void k_sketch_dl(const float* __restrict__ A, 
       }
   if constexpr (GRAM0) {
 #pragma unroll
-    for (int v = 0; v < 16; ++v) gd[v] += (double)gacc[v];
+    for (int v = 0; v < 16; ++v) gd[v] += (double)gacc[0][v] + (double)gacc[1][v];
     // the waves' partials through the (now idle) stage area, summed in wave order
     __syncthreads();
     double* red = reinterpret_cast<double*>(smem + 2 * KP * pitch);
