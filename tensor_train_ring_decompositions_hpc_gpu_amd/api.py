@@ -157,14 +157,20 @@ class Engine:
         p.flags = int(flags)
         return p
 
-    def _sweep(self, fn, A, dims, ranks, tr, **kw):
+    def _sweep(self, fn, A, dims, ranks, tr, cores=None, **kw):
         torch = _torch()
         p = self.params(dims, ranks, **kw)
         if p.rank_mode == L.RANK_MODES["learned"] and not self.model_loaded:
             self.load_rank_model()
         d = len(dims)
-        cores = [torch.empty(int(self.lib.ttx_core_size(ctypes.byref(p), int(tr), k)), device=A.device,
-                             dtype=torch.float32) for k in range(d)]
+        sizes = [int(self.lib.ttx_core_size(ctypes.byref(p), int(tr), k)) for k in range(d)]
+        if cores is not None:
+            # caller-owned output buffers (core_buffers), written in place
+            if len(cores) != d or any(not (c.is_cuda and c.dtype == torch.float32 and c.is_contiguous()) or
+                                      c.numel() < sz for c, sz in zip(cores, sizes)):
+                raise ValueError("ttx: cores must be d contiguous float32 CUDA tensors of at least ttx_core_size floats")
+        else:
+            cores = [torch.empty(sz, device=A.device, dtype=torch.float32) for sz in sizes]
         sigma = np.zeros((d - 1) * L.MAX_ELL, dtype=np.float32)
         rep = L.Report()
         st = fn(self.h, ctypes.c_void_p(_dev_ptr(A)), ctypes.byref(p), L.ptr_array([c.data_ptr() for c in cores]),
@@ -183,8 +189,17 @@ class Engine:
         info = report_dict(rep, d)
         return out, sigma.reshape(d - 1, L.MAX_ELL), info
 
+    def core_buffers(self, dims, ranks, tr=False, **kw):
+        """Output buffers for tt_svd / tr_svd(..., cores=...): one flat float32 CUDA tensor per core
+        of ttx_core_size floats (the largest the ranks allow)."""
+        torch = _torch()
+        p = self.params(dims, ranks, **kw)
+        return [torch.empty(int(self.lib.ttx_core_size(ctypes.byref(p), int(tr), k)), device="cuda",
+                            dtype=torch.float32) for k in range(len(dims))]
+
     def tt_svd(self, A, dims, ranks, **kw):
-        """TT-SVD of the column-major tensor A; returns ([(core, (rows, cols))], sigma, report)."""
+        """TT-SVD of the column-major tensor A; returns ([(core, (rows, cols))], sigma, report).
+        cores=core_buffers(...) (optional): write the cores there instead of new tensors."""
         return self._sweep(self.lib.ttx_tt_svd, A, dims, ranks, False, **kw)
 
     def tr_svd(self, A, dims, ranks, **kw):

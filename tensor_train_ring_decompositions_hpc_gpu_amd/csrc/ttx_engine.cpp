@@ -31,6 +31,9 @@ struct Handle {
   // the sketch of the second step of a TT sweep, taken in the first step's Gram pass (the same
   // memory read once): valid for the unfolding (pre_A, pre_m x pre_n) with Omega (seed, stream)
   DevBuf Ypre;
+  // the current randomized step's sketch with every Omega column (kp of them), reduced and before
+  // the fused left factor: Ypre or Y; any basis width takes its first columns (step-local)
+  float* Yfull = nullptr;
   bool pre_valid = false;
   const float* pre_A = nullptr;
   int64_t pre_m = 0, pre_n = 0;
@@ -441,11 +444,10 @@ static ttx_status range_sketch(Handle* h, const StepArgs& a, const Arith3& ar, i
   hipStream_t st = h->stream;
   const int m = (int)a.m;
   float* Y = P<float>(h->Y);
-  if (h->pre_valid && a.A == h->pre_A && a.m == h->pre_m && a.n == h->pre_n && kp == 64 && a.seed == h->pre_seed &&
-      a.stream_id == h->pre_stream && ar.range == AR_BF16X3) {
-    // taken in the first step's pass with every Omega column: its first w + pt columns are this
-    // sketch (column c of Omega does not depend on the width)
-    Y = P<float>(h->Ypre);
+  if (h->Yfull) {
+    // the step's sketch with every Omega column: its first w + pt columns are this sketch (column c
+    // of Omega does not depend on the width)
+    Y = h->Yfull;
   } else {
     float* sc = fp16_scales(h, a, ar, true);
     {
@@ -553,9 +555,22 @@ static MlpAct mlp_act(Handle* h, int B) {
 }
 
 // the probe: pivots of the shifted Cholesky of the Gram of Y_p = A[:, :n_p] Omega (64 columns)
-static ttx_status probe(Handle* h, const StepArgs& a, const Arith3& ar, double* d_piv, StepOut& o) {
+// Ysrc != nullptr: the pivots of that sketch (the step's own 64-column sketch of every column of A,
+// reduced, before the fused left factor) instead of a probe pass
+static ttx_status probe(Handle* h, const StepArgs& a, const Arith3& ar, double* d_piv, StepOut& o,
+                        const float* Ysrc = nullptr) {
   hipStream_t st = h->stream;
   const int m = (int)a.m;
+  if (Ysrc) {
+    const int kp = PROBE_K;
+    const int64_t nb = panel_blocks(m);
+    TTX_TRY(ensure(h, h->gpart, (size_t)nb * kp * kp * sizeof(double)));
+    float* Ya = left_mix(h, a, const_cast<float*>(Ysrc), kp);
+    KScope ks(h, TTX_K_SMALL, 0);
+    launch_gram_tall_f32(Ya, kp, m, kp, P<double>(h->gpart), nb, st);
+    launch_chol_inv(P<double>(h->gpart), nb, kp, m, 1, nullptr, nullptr, st, d_piv);
+    return check_launch(h, "sketch pivots");
+  }
   int64_t np = std::max<int64_t>(std::min<int64_t>(a.n, 512), a.n / 16);
   if (np > a.n) np = a.n;
   const int kp = PROBE_K;
@@ -587,14 +602,14 @@ static ttx_status probe(Handle* h, const StepArgs& a, const Arith3& ar, double* 
 
 // predicted (rank, width) of one randomized step; host sync
 static ttx_status predict(Handle* h, const StepArgs& a, const Arith3& ar, int ell, int* pred_r, int* pred_w,
-                          StepOut& o) {
+                          StepOut& o, const float* Ysrc = nullptr) {
   if (!h->mlp_ready) return fail(h, TTX_ERR_BAD_ARG, "rank mode LEARNED needs a rank model (ttx_mlp_load)");
   hipStream_t st = h->stream;
   TTX_TRY(mlp_buffers(h, 1));
   double* piv = P<double>(h->mIO);
   float* x = reinterpret_cast<float*>(piv + PROBE_K);
   float* y = x + MLP_IN;
-  TTX_TRY(probe(h, a, ar, piv, o));
+  TTX_TRY(probe(h, a, ar, piv, o, Ysrc));
   {
     KScope ks(h, TTX_K_SMALL, 0);
     mlp_predict(P<float>(h->mP), piv, PROBE_K, a.rel_budget > 0 ? a.rel_budget : 1e-12, (double)a.m, (double)a.n, y,
@@ -737,9 +752,39 @@ static ttx_status run_step(Handle* h, const StepArgs& a_in, StepOut& o) {
     TTX_TRY(ensure(h, h->Qp, (size_t)kp * m * sizeof(float)));
     TTX_TRY(ensure(h, h->Z, (size_t)n * kp * sizeof(float)));
     if (pt) TTX_TRY(ensure(h, h->Yt, (size_t)m * pt * sizeof(float)));
+    struct YfullGuard {   // the full sketch belongs to this step only
+      Handle* h;
+      ~YfullGuard() { h->Yfull = nullptr; }
+    } yfull_guard{h};
+    h->Yfull = nullptr;
+    if (h->pre_valid && a.A == h->pre_A && a.m == h->pre_m && a.n == h->pre_n && kp == 64 && a.seed == h->pre_seed &&
+        a.stream_id == h->pre_stream && ar.range == AR_BF16X3)
+      h->Yfull = P<float>(h->Ypre);   // taken in the first step's pass of the sweep
     if (a.rank_mode == TTX_RANK_LEARNED && pt) {
       int pr = 0, pw = 0;
-      TTX_TRY(predict(h, a, ar, ell, &pr, &pw, o));
+      if (kp == PROBE_K) {
+        // the model reads the pivots of this step's own sketch with every Omega column (the range
+        // finder's first pass, which the chosen width then reuses), so no separate probe pass
+        if (!h->Yfull) {
+          float* sc = fp16_scales(h, a, ar, true);
+          {
+            KScope ks(h, TTX_K_SKETCH, a.m * n * 4);
+            launch_sketch(ar.range, sp, a.A, a.m, m, n, nullptr, 0, a.seed, a.stream_id, kp, P<float>(h->part), st,
+                          sc);
+          }
+          {
+            KScope ks(h, TTX_K_SMALL, 0);
+            launch_reduce_f32(P<float>(h->part), sp.slabs, (int64_t)m * kp, P<float>(h->Y), st);
+          }
+          TTX_TRY(check_launch(h, "full sketch"));
+          o.bytes += a.m * n * 4;
+          o.passes += 1;
+          h->Yfull = P<float>(h->Y);
+        }
+        TTX_TRY(predict(h, a, ar, ell, &pr, &pw, o, h->Yfull));
+      } else {
+        TTX_TRY(predict(h, a, ar, ell, &pr, &pw, o));
+      }
       a.pred_ell = pw;
       o.pred_rank = pr;
     }
@@ -771,7 +816,8 @@ static ttx_status run_step(Handle* h, const StepArgs& a_in, StepOut& o) {
       }
     }
     // power iterations (basicRandSVD_LU :628-683): Z = A^T Q, Y = A Z, Q = orth(Y); adaptive: only
-    // while the check fails, q at most
+    // while the check fails, q at most (they overwrite Y)
+    h->Yfull = nullptr;
     int it = 0;
     while (it < a.q && (!pt || rel2 > a.range_tol2)) {
       TTX_TRY(power_iter(h, a, ar, k, kp, sp, o));
