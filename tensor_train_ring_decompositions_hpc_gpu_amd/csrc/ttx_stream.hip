@@ -359,13 +359,16 @@ __global__ __launch_bounds__(512) void k_sketch(const float* __restrict__ A, int
 // rows); rows past m load row 0 of the column instead (their outputs are not written) and columns
 // past n load column n - 1 (zeroed before use).
 //
-// GRAM0 (LT = 2, m % 64 == 0): the same pass also accumulates the Gram C = A0 A0^T of the 32-row
-// view A0 (32 x 32 n) of the same memory: a 64-row tile holds two 32-row columns of A0 per column
-// of A.  It is the exact first step of a TT sweep whose second step this sketch serves, so the
-// tensor is read once for both.  Products as k_gram_small<1, AR_BF16X4> (hi/lo split, four bf16
-// MFMAs, one FP32 accumulator per 32-row half so the two chains interleave), flushed to FP64
-// every 32 chunks; one FP64 32 x 32 partial per WG in gpart[blockIdx.y * gridDim.x + blockIdx.x].
-// PAD = 16 makes the Gram's column reads of the two lane halves fall on disjoint banks.
+// GRAM0 (LT = 2, BF16X3, m % 64 == 0): the same pass also accumulates the Gram C = A0 A0^T of the
+// 32-row view A0 (32 x 32 n) of the same memory: a 64-row tile holds two 32-row columns of A0 per
+// column of A.  It is the exact first step of a TT sweep whose second step this sketch serves, so
+// the tensor is read once for both.  MFMA row r of sketch tile s is physical row 32 s + r, so the
+// tile's A operand is also the Gram operand of half s: the hi and mid terms of the sketch's
+// three-way split are the hi / lo of k_gram_small<1, AR_BF16X4>'s two-way split, and the four
+// products hi hi, lo hi, hi lo, lo lo go to one FP32 accumulator per half (two interleaved chains),
+// flushed to FP64 every 32 chunks; one FP64 32 x 32 partial per WG in
+// gpart[blockIdx.y * gridDim.x + blockIdx.x].  PAD = 16 puts the two lane halves' column reads on
+// disjoint banks.
 template <int LT, int AR, int ST, bool GRAM0>
 __global__ __launch_bounds__(512) void k_sketch_dl(const float* __restrict__ A, int64_t lda, int m, int64_t n,
                                                    uint64_t seed, uint32_t stream, int ell, int RB, int CG,
@@ -379,7 +382,7 @@ __global__ __launch_bounds__(512) void k_sketch_dl(const float* __restrict__ A, 
   constexpr int NI = 16 / CPI;           // load instructions per chunk and wave
   constexpr int PAD = GRAM0 ? 16 : 0;
   constexpr int TILE = NI * (256 + PAD); // floats of a wave's tile
-  static_assert(!GRAM0 || LT == 2, "the fused Gram takes 64-row tiles");
+  static_assert(!GRAM0 || (LT == 2 && AR == AR_BF16X3), "the fused Gram takes 64-row tiles, BF16X3");
   const int KC = 16 * CG;
   const int pitch = KC + 4;
   extern __shared__ __align__(16) float smem[];
@@ -440,13 +443,18 @@ __global__ __launch_bounds__(512) void k_sketch_dl(const float* __restrict__ A, 
     float a[8][NS];
 #pragma unroll
     for (int t = 0; t < 8; ++t) {
-      const float* pp = src + tile_col(8 * b + t) + NS * r;
-      if constexpr (NS == 4) {
-        const f32x4 v = *reinterpret_cast<const f32x4*>(pp);
+      const float* pp = src + tile_col(8 * b + t);
+      if constexpr (GRAM0) {
+        // MFMA row r of tile s is physical row 32 s + r: tile s is then also the Gram operand of
+        // the 32-row half s, with the same lanes and the same split
+        a[t][0] = pp[r];
+        a[t][1] = pp[32 + r];
+      } else if constexpr (NS == 4) {
+        const f32x4 v = *reinterpret_cast<const f32x4*>(pp + NS * r);
         a[t][0] = v[0]; a[t][1] = v[1]; a[t][2] = v[2]; a[t][3] = v[3];
       } else {
         typedef float f32x2 __attribute__((ext_vector_type(2)));
-        const f32x2 v = *reinterpret_cast<const f32x2*>(pp);
+        const f32x2 v = *reinterpret_cast<const f32x2*>(pp + NS * r);
         a[t][0] = v[0]; a[t][1] = v[1];
       }
     }
@@ -458,26 +466,42 @@ __global__ __launch_bounds__(512) void k_sketch_dl(const float* __restrict__ A, 
 #pragma unroll
           for (int s2 = 0; s2 < NS; ++s2) a[t][s2] = 0.0f;
     }
-    sketch_compute<LT, AR, true>(a, xs, pitch, kofs, lane, S, sa, sx);
     if constexpr (GRAM0) {
-      // K-step h: the 32-row half h of the 16 tile columns; lane (r, b) row r, columns 8 b + t.
-      // A tile past m (whole, as m % 64 == 0) holds row-0 copies: it adds nothing.
-      if (row0 < m)
+      // sketch_compute<2, AR_BF16X3, true> with the Gram products on the same split operands
+      const int cc = lane & 31;
+      bf16x8 bh[LT];
 #pragma unroll
-      for (int h = 0; h < 2; ++h) {
+      for (int lt = 0; lt < LT; ++lt) {
+        const float* p = xs + (32 * lt + cc) * pitch + kofs + 8 * b;
+        const f32x4 u0 = *reinterpret_cast<const f32x4*>(p);
+        const f32x4 u1 = *reinterpret_cast<const f32x4*>(p + 4);
+        const float xv[8] = {u0[0], u0[1], u0[2], u0[3], u1[0], u1[1], u1[2], u1[3]};
+        round8(xv, bh[lt]);   // Omega entries are exact in bf16
+      }
+      const bool gram_rows_valid = row0 < m;   // a tile past m (m % 64 == 0) holds row-0 copies
+#pragma unroll
+      for (int s2 = 0; s2 < NS; ++s2) {
         float v[8];
 #pragma unroll
-        for (int t = 0; t < 8; ++t) {
-          v[t] = src[tile_col(8 * b + t) + 32 * h + r];
-          if (tail && c * KC + 16 * cg + 8 * b + t >= n) v[t] = 0.0f;
+        for (int t = 0; t < 8; ++t) v[t] = a[t][s2];
+        bf16x8 ah, am, al;
+        split8x3(v, ah, am, al);
+#pragma unroll
+        for (int lt = 0; lt < LT; ++lt) {
+          S.acc[s2][lt] = mfma_bf16(ah, bh[lt], S.acc[s2][lt]);
+          S.acc[s2][lt] = mfma_bf16(al, bh[lt], S.acc[s2][lt]);
+          S.acc[s2][lt] = mfma_bf16(am, bh[lt], S.acc[s2][lt]);
         }
-        bf16x8 hi, lo;
-        split8(v, hi, lo);
-        gacc[h] = mfma_bf16(hi, hi, gacc[h]);
-        gacc[h] = mfma_bf16(lo, hi, gacc[h]);
-        gacc[h] = mfma_bf16(hi, lo, gacc[h]);
-        gacc[h] = mfma_bf16(lo, lo, gacc[h]);
+        // Gram of the half s2: hi / lo of the two-term split are hi / mid of the three-term one
+        if (gram_rows_valid) {
+          gacc[s2] = mfma_bf16(ah, ah, gacc[s2]);
+          gacc[s2] = mfma_bf16(am, ah, gacc[s2]);
+          gacc[s2] = mfma_bf16(ah, am, gacc[s2]);
+          gacc[s2] = mfma_bf16(am, am, gacc[s2]);
+        }
       }
+    } else {
+      sketch_compute<LT, AR, true>(a, xs, pitch, kofs, lane, S, sa, sx);
     }
   };
   constexpr int WAIT = (ST - 1) * NI;
@@ -532,7 +556,7 @@ __global__ __launch_bounds__(512) void k_sketch_dl(const float* __restrict__ A, 
     for (int lt = 0; lt < LT; ++lt)
 #pragma unroll
       for (int v = 0; v < 16; ++v) {
-        const int prow = row0 + NS * mfma32_row(v, lane) + s2;
+        const int prow = GRAM0 ? row0 + 32 * s2 + mfma32_row(v, lane) : row0 + NS * mfma32_row(v, lane) + s2;
         if (prow < m) out[(int64_t)prow * KP + 32 * lt + r] = S.acc[s2][lt][v] * unscale[lt];
       }
   if constexpr (GRAM0) {
