@@ -828,18 +828,21 @@ __global__ __launch_bounds__(512) void k_proj(const float* __restrict__ A, int64
     if constexpr (GRAM) {
       // tile D (kp x 32 columns, zero for columns j >= n) -> LDS dt[j][c], then
       // G += D D^T with MFMA 32x32x2: A = D[c][j] (lane: c = l%32, j = 2 kk + l/32), B = the same
+      // row pitch KP + 1: the 32 lanes of a column write to distinct banks
+      constexpr int DP = KP + 1;
+      static_assert(32 * DP <= 32 * SP, "D tile fits the staging area");
       float* d = stg[w];
 #pragma unroll
       for (int lt = 0; lt < LT; ++lt)
 #pragma unroll
-        for (int v = 0; v < 16; ++v) d[jl * KP + 32 * lt + mfma32_row(v, lane)] = acc[lt][v];
+        for (int v = 0; v < 16; ++v) d[jl * DP + 32 * lt + mfma32_row(v, lane)] = acc[lt][v];
       __syncthreads();
 #pragma unroll
       for (int kk = 0; kk < 16; ++kk) {
         const int jr = 2 * kk + b;
         float x[LT];
 #pragma unroll
-        for (int lt = 0; lt < LT; ++lt) x[lt] = d[jr * KP + 32 * lt + jl];
+        for (int lt = 0; lt < LT; ++lt) x[lt] = d[jr * DP + 32 * lt + jl];
         int pidx = 0;
 #pragma unroll
         for (int ta = 0; ta < LT; ++ta)
