@@ -344,118 +344,146 @@ void launch_apply_ns_out(const double* Win, int64_t rows, int k, int kp, const d
 int64_t panel_blocks(int64_t rows) { return (rows + RB64 - 1) / RB64; }
 
 // ------------------------------------------------------------------------------------------------
-// cyclic Jacobi eigensolver (round-robin ordering), FP64, k <= 64
-// Each round rotates k2/2 disjoint pairs; A <- J^T A J and V <- V J are applied together on 2 x 2
-// blocks.  The angle is computed in FP32 and (c, s) normalised in FP64, so every applied rotation is
-// orthogonal to FP64 accuracy.  status[1] = sweeps.  keys[0..63] (optional) are zeroed for the
-// core assembly that follows.
+// cyclic Jacobi eigensolver (round-robin ordering), FP64, k <= 64, one workgroup of 1024 threads
+// Each round rotates k2/2 disjoint pairs: the first np threads compute the angles (FP32 angle,
+// (c, s) normalised in FP64 so every applied rotation is orthogonal to FP64 accuracy), then
+// A <- J^T A J is applied on 2 x 2 blocks, one block per thread.  V <- V J of a round runs while the
+// next round's angles are computed (the pair tables are double-buffered), off the critical path.
+// The sweep loop ends when the off-diagonal mass after a sweep is at most 0.5e-24 ||A||_F^2 (the
+// last round's blocks tile the matrix, so it comes from registers).  status[1] = sweeps.
+// keys[0..63] (optional) are zeroed for the core assembly that follows.
 // ------------------------------------------------------------------------------------------------
+__device__ __forceinline__ void jacobi_angle(double app, double aqq, double apq, double& c, double& s) {
+  c = 1.0;
+  s = 0.0;
+  if (apq == 0.0) return;
+  // the angle only needs FP32 accuracy (every applied rotation is re-normalised in FP64)
+  const float tau = (float)((aqq - app) / (2.0 * apq));
+  float t;
+  if (!(fabsf(tau) <= 1e18f)) t = 0.0f;
+  else t = (tau >= 0.0f ? 1.0f : -1.0f) * __builtin_amdgcn_rcpf(fabsf(tau) + __builtin_amdgcn_sqrtf(1.0f + tau * tau));
+  const float c0 = __builtin_amdgcn_rsqf(1.0f + t * t);
+  const double cd = (double)c0, sd = (double)(t * c0);
+  const double nrm = cd * cd + sd * sd;
+  double rs = (double)__builtin_amdgcn_rsqf((float)nrm);
+  rs = rs * (1.5 - 0.5 * nrm * rs * rs);
+  rs = rs * (1.5 - 0.5 * nrm * rs * rs);
+  c = cd * rs;
+  s = sd * rs;
+}
+
 __global__ __launch_bounds__(1024) void k_syevj(const double* __restrict__ G, int k, double* __restrict__ lam,
                                                double* __restrict__ Vout, int* __restrict__ status,
                                                unsigned long long* __restrict__ keys) {
+  constexpr int NT = 1024, NW = NT / 64;
   __shared__ double a[KMAX][KMAX + 1];
   __shared__ double v[KMAX][KMAX + 1];
-  __shared__ double cs[KMAX / 2], sn[KMAX / 2];
-  __shared__ int pp[KMAX / 2], qq[KMAX / 2];
-  __shared__ double red[2][1024];
+  __shared__ double cs[2][KMAX / 2], sn[2][KMAX / 2];
+  __shared__ int pp[2][KMAX / 2], qq[2][KMAX / 2];
+  __shared__ double red[NW];
   __shared__ int done;
-  const int tid = threadIdx.x;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   if (keys && tid < 64) keys[tid] = 0ull;
   const int k2 = k + (k & 1);   // an odd size gets one zero row / column (its rotations are identities)
-  for (int e = tid; e < k2 * k2; e += 1024) {
+  const int np = k2 / 2, M = k2 - 1;
+  double tot = 0.0;
+  for (int e = tid; e < k2 * k2; e += NT) {
     const int i = e / k2, j = e % k2;
     // the Gram passes accumulate (i, j) and (j, i) in different orders: use the symmetric part
-    a[i][j] = (i < k && j < k) ? 0.5 * (G[i * k + j] + G[j * k + i]) : 0.0;
+    const double x = (i < k && j < k) ? 0.5 * (G[i * k + j] + G[j * k + i]) : 0.0;
+    a[i][j] = x;
     v[i][j] = (i == j) ? 1.0 : 0.0;
+    tot += x * x;
   }
-  __syncthreads();
-  const int np = k2 / 2;
-  // fixed work assignment of this thread: 2 x 2 blocks (bu, bw) and V entries (vi, vw)
-  int bu[1], bw[1], vi[2], vw[2];
-  int nblk = 0, nvi = 0;
-  for (int e = tid; e < np * np && nblk < 1; e += 1024) { bu[nblk] = e / np; bw[nblk] = e % np; ++nblk; }
-  for (int e = tid; e < k2 * np && nvi < 2; e += 1024) { vi[nvi] = e / np; vw[nvi] = e % np; ++nvi; }
   // ||A||_F^2 is invariant under the rotations: one reduction up front
-  {
-    double tot = 0.0;
-    for (int e = tid; e < k2 * k2; e += 1024) tot += a[e / k2][e % k2] * a[e / k2][e % k2];
-    red[0][tid] = tot;
-    __syncthreads();
-    for (int s = 512; s > 0; s >>= 1) {
-      if (tid < s) red[0][tid] += red[0][tid + s];
-      __syncthreads();
-    }
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) tot += __shfl_xor(tot, o);
+  if (lane == 0) red[wv] = tot;
+  __syncthreads();
+  tot = 0.0;
+#pragma unroll
+  for (int w = 0; w < NW; ++w) tot += red[w];
+  // fixed work: the 2 x 2 block (bu, bw) and the V row pairs (vi[t], vw[t]); np^2 <= NT, k2 np <= 2 NT
+  const bool hasblk = tid < np * np;
+  const int bu = hasblk ? tid / np : 0, bw = hasblk ? tid % np : 0;
+  int vi[2], vw[2];
+  bool hasv[2];
+#pragma unroll
+  for (int t = 0; t < 2; ++t) {
+    const int e = tid + t * NT;
+    hasv[t] = e < k2 * np;
+    vi[t] = hasv[t] ? e / np : 0;
+    vw[t] = hasv[t] ? e % np : 0;
   }
-  const double tot = red[0][0];
-  int sweeps = 0;
+  auto v_apply = [&](int b) {
+#pragma unroll
+    for (int t = 0; t < 2; ++t)
+      if (hasv[t]) {
+        const int p2 = pp[b][vw[t]], q2 = qq[b][vw[t]];
+        const double c2 = cs[b][vw[t]], s2 = sn[b][vw[t]];
+        const double vp = v[vi[t]][p2], vq = v[vi[t]][q2];
+        v[vi[t]][p2] = c2 * vp - s2 * vq;
+        v[vi[t]][q2] = s2 * vp + c2 * vq;
+      }
+  };
+  int sweeps = 0, gr = 0;   // gr: rounds so far (table parity)
   while (sweeps < 30 && tot > 0.0) {
     ++sweeps;
-    // sum of the squared pivots a_pq^2 met during the sweep (each off-diagonal pair once): the
-    // sweep's off-diagonal mass, accumulated by the angle threads (all in the first wave)
-    double offacc = 0.0;
-    for (int rnd = 0; rnd < k2 - 1; ++rnd) {
+    double off = 0.0;
+    for (int rnd = 0; rnd < M; ++rnd, ++gr) {
+      const int b = gr & 1;
       if (tid < np) {
         int x, y;
-        if (tid == 0) { x = rnd; y = k2 - 1; }
+        if (tid == 0) { x = rnd; y = M; }
         else {
-          x = rnd + tid; if (x >= k2 - 1) x -= k2 - 1;
-          y = rnd - tid; if (y < 0) y += k2 - 1;
+          x = rnd + tid; if (x >= M) x -= M;
+          y = rnd - tid; if (y < 0) y += M;
         }
         const int p = x < y ? x : y, q = x < y ? y : x;
-        const double apq = a[p][q];
-        offacc += apq * apq;
-        double c = 1.0, s = 0.0;
-        if (apq != 0.0) {
-          // the angle only needs FP32 accuracy (every applied rotation is re-normalised in FP64)
-          const float tau = (float)((a[q][q] - a[p][p]) / (2.0 * apq));
-          float t;
-          if (!(fabsf(tau) <= 1e18f)) t = 0.0f;
-          else t = (tau >= 0.0f ? 1.0f : -1.0f) * __builtin_amdgcn_rcpf(fabsf(tau) + __builtin_amdgcn_sqrtf(1.0f + tau * tau));
-          const float c0 = __builtin_amdgcn_rsqf(1.0f + t * t);
-          const double cd = (double)c0, sd = (double)(t * c0);
-          const double nrm = cd * cd + sd * sd;
-          double rs = (double)__builtin_amdgcn_rsqf((float)nrm);
-          rs = rs * (1.5 - 0.5 * nrm * rs * rs);
-          rs = rs * (1.5 - 0.5 * nrm * rs * rs);
-          c = cd * rs;
-          s = sd * rs;
-        }
-        pp[tid] = p; qq[tid] = q; cs[tid] = c; sn[tid] = s;
+        double c, s;
+        jacobi_angle(a[p][p], a[q][q], a[p][q], c, s);
+        pp[b][tid] = p; qq[b][tid] = q; cs[b][tid] = c; sn[b][tid] = s;
       }
+      if (gr > 0) v_apply(b ^ 1);   // the previous round's V update
       __syncthreads();
-      // A <- J^T A J on 2 x 2 blocks (pair u rows, pair w columns), V <- V J
-      for (int t = 0; t < nblk; ++t) {
-        const int u = bu[t], w = bw[t];
-        const int p1 = pp[u], q1 = qq[u], p2 = pp[w], q2 = qq[w];
-        const double c1 = cs[u], s1 = sn[u], c2 = cs[w], s2 = sn[w];
+      if (hasblk) {
+        const int p1 = pp[b][bu], q1 = qq[b][bu], p2 = pp[b][bw], q2 = qq[b][bw];
+        const double c1 = cs[b][bu], s1 = sn[b][bu], c2 = cs[b][bw], s2 = sn[b][bw];
         const double app = a[p1][p2], apq = a[p1][q2], aqp = a[q1][p2], aqq = a[q1][q2];
         const double rpp = c1 * app - s1 * aqp, rpq = c1 * apq - s1 * aqq;
         const double rqp = s1 * app + c1 * aqp, rqq = s1 * apq + c1 * aqq;
-        a[p1][p2] = c2 * rpp - s2 * rpq;
-        a[p1][q2] = s2 * rpp + c2 * rpq;
-        a[q1][p2] = c2 * rqp - s2 * rqq;
-        a[q1][q2] = s2 * rqp + c2 * rqq;
-      }
-      for (int t = 0; t < nvi; ++t) {
-        const int i = vi[t], w = vw[t];
-        const int p2 = pp[w], q2 = qq[w];
-        const double c2 = cs[w], s2 = sn[w];
-        const double vp = v[i][p2], vq = v[i][q2];
-        v[i][p2] = c2 * vp - s2 * vq;
-        v[i][q2] = s2 * vp + c2 * vq;
+        const double npp = c2 * rpp - s2 * rpq, npq = s2 * rpp + c2 * rpq;
+        const double nqp = c2 * rqp - s2 * rqq, nqq = s2 * rqp + c2 * rqq;
+        a[p1][p2] = npp;
+        a[p1][q2] = npq;
+        a[q1][p2] = nqp;
+        a[q1][q2] = nqq;
+        if (rnd == M - 1) {
+          off = npq * npq + nqp * nqp;
+          if (bu != bw) off += npp * npp + nqq * nqq;
+        }
       }
       __syncthreads();
     }
-    if (tid < 64) {
 #pragma unroll
-      for (int o = 32; o > 0; o >>= 1) offacc += __shfl_xor(offacc, o);
-      if (tid == 0) done = offacc <= 0.5e-24 * tot;
+    for (int o = 32; o > 0; o >>= 1) off += __shfl_xor(off, o);
+    if (lane == 0) red[wv] = off;
+    __syncthreads();
+    if (tid == 0) {
+      double sum = 0.0;
+#pragma unroll
+      for (int w = 0; w < NW; ++w) sum += red[w];
+      done = sum <= 0.5e-24 * tot;
     }
     __syncthreads();
     if (done) break;
   }
+  if (gr > 0) {
+    v_apply((gr - 1) & 1);
+    __syncthreads();
+  }
   // sort descending: rank of each eigenvalue (ties broken by index)
-  for (int i = tid; i < k; i += 1024) {
+  for (int i = tid; i < k; i += NT) {
     const double li = a[i][i];
     int rank = 0;
     for (int j = 0; j < k; ++j) {
