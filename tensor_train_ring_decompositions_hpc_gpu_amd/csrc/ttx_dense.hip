@@ -143,14 +143,10 @@ __device__ __forceinline__ void sum_slabs(const double* __restrict__ part, int64
 // 1 / sqrt(d) for a normal d > 0 to FP64 accuracy: exact power-of-two scaling to [1, 4), the FP32
 // reciprocal square root, then two Newton steps in FP64 (each squares the relative error)
 __device__ __forceinline__ double rsqrt_f64(double d) {
-  const long long b = __double_as_longlong(d);
-  const int e = (int)((b >> 52) & 0x7FF) - 1023;
-  const int h = e >> 1;   // floor(e / 2)
-  const double ds = d * __longlong_as_double((long long)(1023 - 2 * h) << 52);
-  double r = (double)__builtin_amdgcn_rsqf((float)ds);
-  r = r * (1.5 - 0.5 * ds * r * r);
-  r = r * (1.5 - 0.5 * ds * r * r);
-  return r * __longlong_as_double((long long)(1023 - h) << 52);
+  // hardware estimate r0, then r0 (1 - e)^(-1/2) with e = 1 - d r0^2 to third order in e
+  const double r0 = __builtin_amdgcn_rsq(d);
+  const double e = fma(-(d * r0), r0, 1.0);
+  return fma(r0 * e, fma(fma(e, 0.3125, 0.375), e, 0.5), r0);
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -159,7 +155,8 @@ __device__ __forceinline__ double rsqrt_f64(double d) {
 // One wavefront factors the matrix in registers: lane x holds column x in W[0..63] (the Gram is
 // symmetric, so column x is also row x).  Step i (unrolled, so every register index is a
 // constant):
-//   * pivot d = W[i] of lane i (read across lanes), inv = 1 / sqrt(d + s);
+//   * pivot d = W[i] of lane i (read across lanes), inv = 1 / sqrt(d + s), computed one step ahead
+//     from lane i's own update;
 //   * R[i][x] = W[i] inv goes to LDS, and the trailing update W[l] -= R[i][l] R[i][x] (l > i) reads
 //     R[i][l] back as an LDS broadcast;
 //   * slot i is then free and takes row i of B = L^-1 (L = R^T):
@@ -206,16 +203,25 @@ __global__ __launch_bounds__(256) void k_chol_rinv(const double* __restrict__ pa
       W[l] = (x < k && l < k) ? g : (l == x ? 1.0 : 0.0);
     }
     int nzero = 0;
-#pragma unroll
-    for (int i = 0; i < KP; ++i) {
+    // pivot of step i + 1, computed during step i: lane i + 1 updates its own diagonal with its own
+    // R[i][i + 1] (the same fma as its trailing update), so the next inverse square root does not
+    // wait for the LDS round trip of row i
+    auto pivot = [&](int i, double dv, double& inv) {
       const bool real = i < k;
-      const double d = lane_value(W[i], i) + (real ? sh : 0.0);
+      const double d = lane_value(dv, i) + (real ? sh : 0.0);
       const bool ok = d > tiny;
-      const double inv = ok ? rsqrt_f64(d) : 0.0;
+      inv = ok ? rsqrt_f64(d) : 0.0;
       nzero += (ok || !real) ? 0 : 1;
       if (pivots && x == 0 && real) pivots[i] = ok ? d : 0.0;
+    };
+    double inv;
+    pivot(0, W[0], inv);
+#pragma unroll
+    for (int i = 0; i < KP; ++i) {
       const double r = W[i] * inv;
       Rs[i][x] = r;
+      double inv_next = 0.0;
+      if (i + 1 < KP) pivot(i + 1, fma(-r, r, W[i + 1]), inv_next);
       // row i of B from the rows j < i (independent of this step's pivot until the last multiply)
       double acc0 = (x == i) ? 1.0 : 0.0, acc1 = 0.0;
       if constexpr (INV) {
@@ -227,12 +233,13 @@ __global__ __launch_bounds__(256) void k_chol_rinv(const double* __restrict__ pa
       }
       // trailing update of the columns still to be factored
 #pragma unroll
-      for (int l = i + 1; l < KP; ++l) W[l] -= Rs[i][l] * r;
+      for (int l = i + 1; l < KP; ++l) W[l] = fma(-Rs[i][l], r, W[l]);
       W[i] = (acc0 + acc1) * inv;
       // materialise this step's results here: left alone, the compiler defers each trailing
       // update to the step that reads the slot, holding O(k^2) loaded factors live
 #pragma unroll
       for (int l = i; l < KP; ++l) asm volatile("" : "+v"(W[l]));
+      inv = inv_next;
     }
     if (INV && x < k) {
 #pragma unroll
