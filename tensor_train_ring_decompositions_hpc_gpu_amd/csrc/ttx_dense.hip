@@ -31,6 +31,8 @@ __device__ __forceinline__ void block_gram(const double (*xs)[KMAX + 1], int nro
     double acc[16];
 #pragma unroll
     for (int e = 0; e < 16; ++e) acc[e] = 0.0;
+    // unrolled so the LDS loads of the next rows are in flight during this row's FMAs
+#pragma unroll 4
     for (int rr = 0; rr < nrows; ++rr) {
       double xa[4], xb[4];
 #pragma unroll
@@ -304,13 +306,35 @@ __global__ __launch_bounds__(256) void k_apply_gram(const TI* __restrict__ Win, 
     ws[rr][c] = (rr < nr) ? (double)Win[(i0 + rr) * ldi + c] : 0.0;
   }
   __syncthreads();
-  const int alim = gpart ? k - 1 : -1;   // full product for M, triangular for Rinv
-  for (int e = tid; e < RB64 * k; e += 256) {
-    const int rr = e % RB64, c = e / RB64;
-    const int top = alim >= 0 ? alim : c;
-    double s = 0.0;
-    for (int a2 = 0; a2 <= top; ++a2) s += ws[rr][a2] * ri[a2][c];
-    os[rr][c] = (rr < nr) ? s : 0.0;
+  // os = ws ri: 4 x 4 output tiles, one per thread (16 row groups x ceil(k / 4) column groups <= 256),
+  // sixteen independent FMA chains; a triangular Rinv is zero below the diagonal, so the terms
+  // a2 > c of a tile add zeros (the sum over a2 <= c is unchanged)
+  {
+    const int ncb = (k + 3) / 4;
+    if (tid < 16 * ncb) {
+      const int r0 = 4 * (tid / ncb), c0 = 4 * (tid % ncb);
+      const int top = gpart ? k - 1 : ((c0 + 3 < k - 1) ? c0 + 3 : k - 1);
+      double acc[4][4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) acc[u][v] = 0.0;
+#pragma unroll 2
+      for (int a2 = 0; a2 <= top; ++a2) {
+        double x[4], y[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) { x[u] = ws[r0 + u][a2]; y[u] = ri[a2][c0 + u]; }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+          for (int v = 0; v < 4; ++v) acc[u][v] += x[u] * y[v];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+          if (c0 + v < k) os[r0 + u][c0 + v] = (r0 + u < nr) ? acc[u][v] : 0.0;
+    }
   }
   __syncthreads();
   if (OUT == 0) {
@@ -550,6 +574,10 @@ __global__ __launch_bounds__(256) void k_assemble_g(const float* __restrict__ Qc
   if (tid < rk) atomicMax(&keys[tid], kmax[tid]);
 }
 
+// row stride of Ur: r rounded up to 8, 16, 32 or 64 (the remainder kernel's register width)
+__host__ __device__ constexpr int ur_ld(int r) { return r <= 8 ? 8 : r <= 16 ? 16 : r <= 32 ? 32 : 64; }
+int next_from_z_ld(int r) { return ur_ld(r); }
+
 __global__ __launch_bounds__(256) void k_assemble_fix(const unsigned long long* __restrict__ keys, int m, int k,
                                                       const double* __restrict__ V, const double* __restrict__ lam,
                                                       int r, float* __restrict__ Gc, float* __restrict__ Ur,
@@ -575,11 +603,14 @@ __global__ __launch_bounds__(256) void k_assemble_fix(const unsigned long long* 
       if (i < m) Qn[(int64_t)c * m + i] = 0.0f;
     }
   if (blockIdx.x == 0) {
-    if (Ur)
-      for (int e = tid; e < k * r; e += 256) {
-        const int a2 = e / r, c = e % r;
-        Ur[e] = (c < rk) ? (float)(V[a2 * k + c] * (double)sg[c]) : 0.0f;
+    if (Ur) {
+      // round4(k) x ur_ld(r), zero-padded, as k_next_from_z reads it
+      const int ld = ur_ld(r), k4 = (k + 3) & ~3;
+      for (int e = tid; e < k4 * ld; e += 256) {
+        const int a2 = e / ld, c = e % ld;
+        Ur[e] = (a2 < k && c < rk) ? (float)(V[a2 * k + c] * (double)sg[c]) : 0.0f;
       }
+    }
     if (sigma)
       for (int i = tid; i < k; i += 256) sigma[i] = (float)sqrt(fmax(lam[i], 0.0));
   }
@@ -599,14 +630,55 @@ void launch_assemble(const float* Qc, int64_t ldq, int m, int k, const double* l
 template <int RM>
 __global__ __launch_bounds__(256) void k_next_from_z(const float* __restrict__ Z, int64_t ldz, int64_t n, int k,
                                                      const float* __restrict__ Ur, int r, float* __restrict__ out) {
-  // thread -> (row jj of this WG's 256 / RM rows, output column c): a wave writes whole consecutive
-  // output rows (coalesced); the RM lanes of one row read the same Z row with 16-byte loads
+  // one output row per lane: the lane reads its own Z row with 16-byte loads (a wave reads 64
+  // consecutive rows, every byte of them) and keeps RM sums in registers; Ur (round4(k) x RM,
+  // zero-padded) is read at wave-uniform addresses, so its values come through scalar loads as
+  // SGPR operands; the lane writes its r consecutive floats (a wave writes 64 r contiguous floats)
+  const bool vec = (r % 4) == 0 && (reinterpret_cast<uintptr_t>(out) & 15) == 0;
+  const int kq = (k + 3) / 4;
+  for (int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x; j < n; j += (int64_t)gridDim.x * 256) {
+    // the whole Z row goes out first (up to 16 loads in flight per lane)
+    const f32x4* z = reinterpret_cast<const f32x4*>(Z + j * ldz);
+    f32x4 zr[KMAX / 4];
+#pragma unroll
+    for (int q = 0; q < KMAX / 4; ++q)
+      if (q < kq) zr[q] = z[q];
+    float acc[RM];
+#pragma unroll
+    for (int c = 0; c < RM; ++c) acc[c] = 0.0f;
+#pragma unroll
+    for (int q = 0; q < KMAX / 4; ++q) {
+      if (q < kq) {
+        const float* u = Ur + 4 * q * RM;
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+#pragma unroll
+          for (int c = 0; c < RM; ++c) acc[c] += zr[q][t] * u[t * RM + c];
+      }
+    }
+    float* o = out + j * r;
+    if (vec) {
+#pragma unroll
+      for (int c4 = 0; c4 < RM / 4; ++c4)
+        if (4 * c4 < r)
+          *reinterpret_cast<f32x4*>(o + 4 * c4) = f32x4{acc[4 * c4], acc[4 * c4 + 1], acc[4 * c4 + 2], acc[4 * c4 + 3]};
+    } else {
+#pragma unroll
+      for (int c = 0; c < RM; ++c)
+        if (c < r) o[c] = acc[c];
+    }
+  }
+}
+
+// small n: RM lanes per output row (lane c computes column c), so short sweeps still fill the
+// machine; the RM lanes of one row read the same Z row with 16-byte loads
+template <int RM>
+__global__ __launch_bounds__(256) void k_next_from_z_cols(const float* __restrict__ Z, int64_t ldz, int64_t n,
+                                                          int k, const float* __restrict__ Ur, int r,
+                                                          float* __restrict__ out) {
   __shared__ float us[(KMAX + 4) * RM];
   const int k4 = (k + 3) & ~3;
-  for (int e = threadIdx.x; e < k4 * RM; e += 256) {
-    const int a2 = e / RM, c = e % RM;
-    us[e] = (a2 < k && c < r) ? Ur[a2 * r + c] : 0.0f;
-  }
+  for (int e = threadIdx.x; e < k4 * RM; e += 256) us[e] = Ur[e];   // Ur is k4 x RM, zero-padded
   __syncthreads();
   constexpr int ROWS = 256 / RM;
   const int jj = threadIdx.x / RM, c = threadIdx.x % RM;
@@ -626,16 +698,23 @@ __global__ __launch_bounds__(256) void k_next_from_z(const float* __restrict__ Z
 
 void launch_next_from_z(const float* Z, int64_t ldz, int64_t n, int k, const float* Ur, int r, float* out,
                         hipStream_t st) {
-  const int RM = r <= 8 ? 8 : r <= 16 ? 16 : r <= 32 ? 32 : 64;
-  const int rows = 256 / RM;
-  int64_t grid = (n + rows - 1) / rows;
+  const int RM = ur_ld(r);
   const int64_t cap = (int64_t)num_cus() * 16;
+  if (n < (int64_t)1 << 17) {
+    // a row per lane would leave most of the machine idle: RM lanes per row
+    int64_t grid = (n * RM + 255) / 256;
+    if (grid > cap) grid = cap;
+    if (grid < 1) grid = 1;
+#define TTX_NFZ(R) hipLaunchKernelGGL((k_next_from_z_cols<R>), dim3((unsigned)grid), dim3(256), 0, st, Z, ldz, n, k, Ur, r, out)
+    if (RM == 8) TTX_NFZ(8); else if (RM == 16) TTX_NFZ(16); else if (RM == 32) TTX_NFZ(32); else TTX_NFZ(64);
+#undef TTX_NFZ
+    return;
+  }
+  int64_t grid = (n + 255) / 256;
   if (grid > cap) grid = cap;
-  if (grid < 1) grid = 1;
-  if (RM == 8) hipLaunchKernelGGL((k_next_from_z<8>), dim3((unsigned)grid), dim3(256), 0, st, Z, ldz, n, k, Ur, r, out);
-  else if (RM == 16) hipLaunchKernelGGL((k_next_from_z<16>), dim3((unsigned)grid), dim3(256), 0, st, Z, ldz, n, k, Ur, r, out);
-  else if (RM == 32) hipLaunchKernelGGL((k_next_from_z<32>), dim3((unsigned)grid), dim3(256), 0, st, Z, ldz, n, k, Ur, r, out);
-  else hipLaunchKernelGGL((k_next_from_z<64>), dim3((unsigned)grid), dim3(256), 0, st, Z, ldz, n, k, Ur, r, out);
+#define TTX_NFZ(R) hipLaunchKernelGGL((k_next_from_z<R>), dim3((unsigned)grid), dim3(256), 0, st, Z, ldz, n, k, Ur, r, out)
+  if (RM == 8) TTX_NFZ(8); else if (RM == 16) TTX_NFZ(16); else if (RM == 32) TTX_NFZ(32); else TTX_NFZ(64);
+#undef TTX_NFZ
 }
 
 // ------------------------------------------------------------------------------------------------

@@ -333,7 +333,7 @@ static ttx_status step_tail(Handle* h, const StepArgs& a, const Arith3& ar, int 
     o.kept2 = kept;
   }
   if (r > k) return fail(h, TTX_ERR_BAD_ARG, "rank exceeds the sketch width");
-  TTX_TRY(ensure(h, h->Ur, (size_t)kp * r * sizeof(float)));
+  TTX_TRY(ensure(h, h->Ur, (size_t)((k + 3) & ~3) * next_from_z_ld(r) * sizeof(float)));
   // G = Q U_B[:, :r] with the sign gauge; sigma = sqrt(eig(B B^T)) (basicRandSVD_LU :717-763)
   {
     KScope ks(h, TTX_K_SMALL, 0);

@@ -79,7 +79,8 @@ void launch_syevj(const double* G, int k, double* lam, double* V, int* status, u
                   hipStream_t st);
 // core assembly after the eigen-decomposition of the projected Gram:
 //   Gc = Q V[:, :r] (m x r column-major, FP32) with the sign of each column fixed so that its largest
-//   |entry| is positive (first index on ties); Ur = V[:, :r] diag(sign) (k x r, row-major, FP32);
+//   |entry| is positive (first index on ties); Ur = V[:, :r] diag(sign) (row-major FP32, round4(k) rows
+//   of next_from_z_ld(r) floats, zero beyond k rows and r columns);
 //   Qn = Gc padded with zero columns to kpn (column-major kpn x m) for a following projection;
 //   sigma = sqrt(max(lam, 0)) for the first k values.  Qc == nullptr means Q = I (exact path).
 //   keys: 64 zeroed 64-bit words (launch_syevj zeroes them).
@@ -87,7 +88,8 @@ void launch_assemble(const float* Qc, int64_t ldq, int m, int k, const double* l
                      float* Gc, float* Ur, float* Qn, int kpn, float* sigma, unsigned long long* keys,
                      hipStream_t st);
 // A_next[c + r j] = sum_a Ur[a][c] Z[j][a]  (c < r, j < n); Z row-major with stride ldz (a multiple of 4,
-// 16-byte aligned rows, columns k .. round_up(k, 4) - 1 readable)
+// 16-byte aligned rows, columns k .. round_up(k, 4) - 1 readable); Ur as launch_assemble writes it
+int next_from_z_ld(int r);
 void launch_next_from_z(const float* Z, int64_t ldz, int64_t n, int k, const float* Ur, int r, float* out,
                         hipStream_t st);
 // pending full-rank left factor P = I_N (x) G^T of a fused step (G: ms x ms column-major core):
