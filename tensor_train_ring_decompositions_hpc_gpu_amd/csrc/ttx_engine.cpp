@@ -47,7 +47,8 @@ struct Handle {
   int* h_status = nullptr;       // pinned
   float* h_sigma = nullptr;      // pinned, TTX_MAX_ELL
   double* h_lam = nullptr;       // pinned
-  double* h_aux = nullptr;       // pinned, 16 doubles (range-check sums)
+  double* h_aux = nullptr;       // pinned, 16 doubles (range-check sums; words 8.. the rank model output)
+  float* h_sweep = nullptr;      // pinned, the sweep's sigma and status words (one copy at the end)
   int dev = 0;
   std::vector<float*> last_cores;  // device buffers kept for the reference-name shims
   // per-kernel timing (TTX_FLAG_TIMING): event pairs recorded on the stream around launches
@@ -616,8 +617,10 @@ static ttx_status predict(Handle* h, const StepArgs& a, const Arith3& ar, int el
                 st);
   }
   TTX_TRY(check_launch(h, "rank model"));
-  float hy[MLP_OUT];
-  TTX_HIP(h, hipMemcpyAsync(hy, y, sizeof(hy), hipMemcpyDeviceToHost, st));
+  // through the pinned scratch words (a pageable destination would add a staging copy)
+  float* hy = reinterpret_cast<float*>(h->h_aux + 8);
+  static_assert(MLP_OUT * sizeof(float) <= 8 * sizeof(double), "rank model output fits the pinned words");
+  TTX_HIP(h, hipMemcpyAsync(hy, y, MLP_OUT * sizeof(float), hipMemcpyDeviceToHost, st));
   TTX_HIP(h, hipStreamSynchronize(st));
   const double r = std::isfinite(hy[0]) ? 64.0 * hy[0] : 1.0;
   const double w = std::isfinite(hy[1]) ? 64.0 * hy[1] : (double)ell;
@@ -1103,14 +1106,15 @@ static ttx_status sweep(Handle* h, const float* d_A, const ttx_params* p, float*
     return check_launch(h, "batch sweep");
   }
   {
-    std::vector<float> hs((size_t)TTX_MAX_ORDER * TTX_MAX_ELL);
-    std::vector<int> hst((size_t)TTX_MAX_ORDER * 4);
-    TTX_HIP(h, hipMemcpyAsync(hs.data(), sig_all, hs.size() * sizeof(float), hipMemcpyDeviceToHost, st));
-    TTX_HIP(h, hipMemcpyAsync(hst.data(), stat_all, hst.size() * sizeof(int), hipMemcpyDeviceToHost, st));
+    // sigma and status words are contiguous on the device: one copy into pinned memory
+    const float* hs = h->h_sweep;
+    const int* hst = reinterpret_cast<const int*>(h->h_sweep + (size_t)TTX_MAX_ORDER * TTX_MAX_ELL);
+    TTX_HIP(h, hipMemcpyAsync(h->h_sweep, sig_all, (size_t)TTX_MAX_ORDER * (TTX_MAX_ELL + 4) * sizeof(float),
+                              hipMemcpyDeviceToHost, st));
     TTX_HIP(h, hipStreamSynchronize(st));
     double n2 = -1.0;
     for (int k = 0; k < d - 1; ++k) {
-      finish_step_out(hs.data() + (size_t)k * TTX_MAX_ELL, hst.data() + 4 * k, outs[k], !host_ranks);
+      finish_step_out(hs + (size_t)k * TTX_MAX_ELL, hst + 4 * k, outs[k], !host_ranks);
       if (!host_ranks) {
         R.norm2_in[k] = (outs[k].total2 >= 0) ? outs[k].total2 : n2;
         n2 = outs[k].kept2;
@@ -1335,7 +1339,8 @@ ttx_status ttx_create(ttx_handle* out, void* stream) {
       hipHostMalloc((void**)&h->h_status, 16 * sizeof(int), 0) != hipSuccess ||
       hipHostMalloc((void**)&h->h_sigma, TTX_MAX_ELL * sizeof(float), 0) != hipSuccess ||
       hipHostMalloc((void**)&h->h_lam, TTX_MAX_ELL * sizeof(double), 0) != hipSuccess ||
-      hipHostMalloc((void**)&h->h_aux, 16 * sizeof(double), 0) != hipSuccess) {
+      hipHostMalloc((void**)&h->h_aux, 16 * sizeof(double), 0) != hipSuccess ||
+      hipHostMalloc((void**)&h->h_sweep, (size_t)TTX_MAX_ORDER * (TTX_MAX_ELL + 4) * sizeof(float), 0) != hipSuccess) {
     (void)hipGetLastError();
     delete h;
     *out = nullptr;
@@ -1367,6 +1372,7 @@ static void free_handle(Handle* h, bool own_stream) {
   if (h->h_sigma) (void)hipHostFree(h->h_sigma);
   if (h->h_lam) (void)hipHostFree(h->h_lam);
   if (h->h_aux) (void)hipHostFree(h->h_aux);
+  if (h->h_sweep) (void)hipHostFree(h->h_sweep);
   if (own_stream && h->stream) (void)hipStreamDestroy(h->stream);
   delete h;
 }
