@@ -525,15 +525,18 @@ __global__ __launch_bounds__(512) void k_sketch_dl(const float* __restrict__ A, 
     const float* xcur = ((c - c0) & 1) ? xs1 : xs0;
     float* xnext = ((c - c0) & 1) ? xs0 : xs1;
     // chunk c + ST - 1 goes out before chunk c is read: ST chunks requested; the wait leaves ST - 1
+    if (c + ST - 1 < c1) issue(c + ST - 1);
+    // the next chunk's Omega entries are generated while this chunk's loads are still in flight
+    // (its buffer was last read before the previous barrier); measured 8% faster than after the MFMAs
+    if (c + 1 < c1) fill_store<LT, true, 1>(R, xnext, pitch, KC, (c + 1) * KC, n, seed, stream, ell);
+    __builtin_amdgcn_sched_barrier(0);
     if (c + ST - 1 < c1) {
-      issue(c + ST - 1);
       __builtin_amdgcn_s_waitcnt(WAIT_IMM);
     } else {
       __builtin_amdgcn_s_waitcnt(0x0F70);   // the last chunks: wait for all
     }
     compute(c, xcur, 16 * cg);
     __builtin_amdgcn_sched_barrier(0);
-    if (c + 1 < c1) fill_store<LT, true, 1>(R, xnext, pitch, KC, (c + 1) * KC, n, seed, stream, ell);
     xs_barrier();
     if constexpr (GRAM0) {
       if (((c - c0) & 31) == 31) {
