@@ -1020,8 +1020,11 @@ __global__ __launch_bounds__(256) void k_reduce(const TI* __restrict__ part, int
   const int el = threadIdx.x & 31, g = threadIdx.x >> 5;
   const int64_t e = (int64_t)blockIdx.x * 32 + el;
   double acc = 0.0;
-  if (e < count)
+  // unrolled: sixteen independent loads in flight per thread, added in slab order
+  if (e < count) {
+#pragma unroll 16
     for (int64_t k = g; k < S; k += 8) acc += (double)part[k * count + e];
+  }
   red[g][el] = acc;
   __syncthreads();
   if (g == 0 && e < count) {
