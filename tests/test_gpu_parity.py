@@ -123,6 +123,14 @@ def test_step_randomized(engine, prec, m, n, r, s, q):
 
 
 @pytest.mark.parametrize("prec", ["fp32", "bf16"])
+@pytest.mark.parametrize("m,n,r,s,q", [(64, 140000, 16, 8, 0), (48, 131100, 10, 6, 1)])
+def test_step_randomized_long(engine, prec, m, n, r, s, q):
+    # n >= 2^17 takes the one-row-per-lane remainder kernel (vector stores for r % 4 == 0, scalar
+    # stores otherwise)
+    _check_step(engine, lowrank(m, n, r, 29, decay=1.5), r, s, q, prec, 2)
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16"])
 def test_rsvd_matches_oracle(engine, prec):
     A = lowrank(700, 3000, 24, 3, decay=2.0)
     U, S, SVT = engine.rsvd(dev(A), 700, 3000, 24, s=8, q=1, seed=1234, precision=prec)
