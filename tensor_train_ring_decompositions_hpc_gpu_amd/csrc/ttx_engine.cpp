@@ -31,6 +31,7 @@ struct Handle {
   // the sketch of the second step of a TT sweep, taken in the first step's Gram pass (the same
   // memory read once): valid for the unfolding (pre_A, pre_m x pre_n) with Omega (seed, stream)
   DevBuf Ypre;
+  DevBuf omg;                     // Omega of that pass, precomputed (2 KB chunks of 16 rows, bf16)
   // the current randomized step's sketch with every Omega column (kp of them), reduced and before
   // the fused left factor: Ypre or Y; any basis width takes its first columns (step-local)
   float* Yfull = nullptr;
@@ -966,10 +967,18 @@ static ttx_status sweep(Handle* h, const float* d_A, const ttx_params* p, float*
         TTX_TRY(ensure(h, h->part, (size_t)sp.slabs * m1 * 64 * sizeof(float)));
         TTX_TRY(ensure(h, h->gpart, (size_t)parts * 32 * 32 * sizeof(double)));
         TTX_TRY(ensure(h, h->Ypre, (size_t)m1 * 64 * sizeof(float)));
+        // Omega once for the whole pass (a full-machine kernel): the pass itself then only streams A,
+        // where generating it per workgroup put the Philox / Box-Muller work (twice, once per row
+        // half) on the critical path of every chunk
+        TTX_TRY(ensure(h, h->omg, (size_t)omega_chunks_bytes(n1)));
+        {
+          KScope ks(h, TTX_K_SMALL, 0);
+          launch_omega_chunks(p->omega_seed, 1u, n1, 64, h->omg.p, st);
+        }
         {
           KScope ks(h, TTX_K_SKETCH, m1 * n1 * 4);
           launch_sketch_gram0(sp, d_A, m1, (int)m1, n1, p->omega_seed, 1u, 64, P<float>(h->part), P<double>(h->gpart),
-                              st);
+                              st, h->omg.p);
         }
         {
           KScope ks(h, TTX_K_SMALL, 0);
@@ -1363,7 +1372,7 @@ static void free_handle(Handle* h, bool own_stream) {
                          &h->V, &h->Ur, &h->gpart, &h->status, &h->next[0], &h->next[1], &h->sig, &h->keys,
                          &h->scratch[0], &h->scratch[1], &h->scratch[2], &h->scratch[3], &h->Yt, &h->tpart,
                          &h->tout, &h->mP, &h->mM1, &h->mM2, &h->mG, &h->mAct, &h->mPart, &h->mIO,
-                         &h->f16s};
+                         &h->f16s, &h->Ypre, &h->omg};
   for (auto* b : bufs)
     if (b->p) (void)hipFree(b->p);
   for (float* c : h->last_cores) (void)hipFree(c);

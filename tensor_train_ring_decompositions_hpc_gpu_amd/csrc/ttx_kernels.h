@@ -24,8 +24,15 @@ void launch_sketch(int ar, const SketchPlan& p, const float* A, int64_t lda, int
 // sketch_gram0_partials(p) FP64 32 x 32 partials.  sketch_gram0_ok: the shapes it takes.
 bool sketch_gram0_ok(const SketchPlan& p, int m, int64_t lda, int kp);
 int64_t sketch_gram0_partials(const SketchPlan& p);
+// omega: Omega precomputed by launch_omega_chunks for (seed, stream, n, ell) (the pass then spends no
+// arithmetic on it), or nullptr (generated in the pass).
 void launch_sketch_gram0(const SketchPlan& p, const float* A, int64_t lda, int m, int64_t n, uint64_t seed,
-                         uint32_t stream, int ell, float* part, double* gpart, hipStream_t st);
+                         uint32_t stream, int ell, float* part, double* gpart, hipStream_t st,
+                         const void* omega = nullptr);
+// Omega (kp = 64 columns, the first ell nonzero) of an n-column sketch in 2 KB chunks of 16 rows (bf16,
+// exact): omega_chunks_bytes(n) bytes at out
+int64_t omega_chunks_bytes(int64_t n);
+void launch_omega_chunks(uint64_t seed, uint32_t stream, int64_t n, int ell, void* out, hipStream_t st);
 // scales[0] from sum_i<r sig_prev[i]^2 / numel (RMS entry to [8, 16)); sig_prev == nullptr gives 1
 void launch_fp16_ascale(const float* sig_prev, int r_prev, double numel, float* scales, hipStream_t st);
 // scales[1 + c] from max_j |X[j][c]| (to [128, 256)); mx: kp zeroed words, zeroed again afterwards

@@ -369,11 +369,44 @@ __global__ __launch_bounds__(512) void k_sketch(const float* __restrict__ A, int
 // flushed to FP64 every 32 chunks; one FP64 32 x 32 partial per WG in
 // gpart[blockIdx.y * gridDim.x + blockIdx.x].  PAD = 16 puts the two lane halves' column reads on
 // disjoint banks.
-template <int LT, int AR, int ST, bool GRAM0>
+// Omega for the precomputed-Omega first pass: chunk c holds rows 16 c .. 16 c + 15 of Omega (the
+// Philox counters j), column-major inside the chunk: entry (col, jj) at out[c * 1024 + col * 16 + jj],
+// so a chunk is 2 KB and a column's 16 entries are 32 contiguous bytes.  The entries are exactly the
+// ones fill_store writes for kp = 64 (omega_round of the normals, exact in bf16; zero for col >= ell
+// and for j >= n).  Thread: one chunk and one group of four columns.
+__global__ __launch_bounds__(256) void k_omega_chunks(uint64_t seed, uint32_t stream, int64_t n, int ell,
+                                                      int64_t nchunks, __bf16* __restrict__ out) {
+  for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < nchunks * 16; t += (int64_t)gridDim.x * 256) {
+    const int64_t c = t >> 4;
+    const int g = (int)(t & 15);
+    bf16x8 v[4][2];
+#pragma unroll
+    for (int jj = 0; jj < 16; ++jj) {
+      const int64_t j = c * 16 + jj;
+      float z[4] = {0.0f, 0.0f, 0.0f, 0.0f};
+      if (j < n && 4 * g < ell) {
+        normal4(seed, stream, (uint64_t)j, (uint32_t)g, z);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) z[e] = (4 * g + e < ell) ? omega_round(z[e]) : 0.0f;
+      }
+#pragma unroll
+      for (int e = 0; e < 4; ++e) v[e][jj >> 3][jj & 7] = (__bf16)z[e];
+    }
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      bf16x8* o = reinterpret_cast<bf16x8*>(out + c * 1024 + (4 * g + e) * 16);
+      o[0] = v[e][0];
+      o[1] = v[e][1];
+    }
+  }
+}
+
+template <int LT, int AR, int ST, bool GRAM0, bool OP = false>
 __global__ __launch_bounds__(512) void k_sketch_dl(const float* __restrict__ A, int64_t lda, int m, int64_t n,
                                                    uint64_t seed, uint32_t stream, int ell, int RB, int CG,
                                                    int64_t chunks_per_wg, float* __restrict__ part,
-                                                   const float* __restrict__ scales, double* __restrict__ gpart) {
+                                                   const float* __restrict__ scales, double* __restrict__ gpart,
+                                                   const __bf16* __restrict__ omega = nullptr) {
   constexpr int KP = 32 * LT;
   constexpr int NS = 4 / LT;
   constexpr int RW = 32 * NS;
@@ -383,6 +416,10 @@ __global__ __launch_bounds__(512) void k_sketch_dl(const float* __restrict__ A, 
   constexpr int PAD = GRAM0 ? 16 : 0;
   constexpr int TILE = NI * (256 + PAD); // floats of a wave's tile
   static_assert(!GRAM0 || (LT == 2 && AR == AR_BF16X3), "the fused Gram takes 64-row tiles, BF16X3");
+  // OP: Omega comes precomputed (k_omega_chunks layout, bf16) through a ring of ST + 1 chunk slots
+  // of 2 KB in the space of the two generated chunks; waves 0 and 1 load a chunk's halves
+  // direct-to-LDS with their A tiles
+  static_assert(!OP || (GRAM0 && (ST + 1) * 2048 <= 2 * KP * 20 * 4), "the Omega ring fits the chunk space");
   const int KC = 16 * CG;
   const int pitch = KC + 4;
   extern __shared__ __align__(16) float smem[];
@@ -427,7 +464,15 @@ __global__ __launch_bounds__(512) void k_sketch_dl(const float* __restrict__ A, 
   const int lrow_raw = row0 + 4 * (lane % LPC);
   const int lrow = (lrow_raw < m) ? lrow_raw : 0;
   const int lcol = 16 * cg + lane / LPC;
+  __attribute__((address_space(3))) char* om_lds = (__attribute__((address_space(3))) char*)smem;
   auto issue = [&](int64_t c) {
+    if constexpr (OP) {
+      if (w < 2)
+        __builtin_amdgcn_global_load_lds((const void*)(omega + c * 1024 + w * 512 + lane * 8),
+                                         (__attribute__((address_space(3))) void*)(om_lds + (int)(c % (ST + 1)) * 2048 +
+                                                                                   w * 1024),
+                                         16, 0, 0);
+    }
     float* dst = stg + (int)(c % ST) * nw * TILE;
 #pragma unroll
     for (int q = 0; q < NI; ++q) {
@@ -472,11 +517,16 @@ __global__ __launch_bounds__(512) void k_sketch_dl(const float* __restrict__ A, 
       bf16x8 bh[LT];
 #pragma unroll
       for (int lt = 0; lt < LT; ++lt) {
-        const float* p = xs + (32 * lt + cc) * pitch + kofs + 8 * b;
-        const f32x4 u0 = *reinterpret_cast<const f32x4*>(p);
-        const f32x4 u1 = *reinterpret_cast<const f32x4*>(p + 4);
-        const float xv[8] = {u0[0], u0[1], u0[2], u0[3], u1[0], u1[1], u1[2], u1[3]};
-        round8(xv, bh[lt]);   // Omega entries are exact in bf16
+        if constexpr (OP) {
+          const __bf16* om = reinterpret_cast<const __bf16*>(smem) + (int)(c % (ST + 1)) * 1024;
+          bh[lt] = *reinterpret_cast<const bf16x8*>(om + (32 * lt + cc) * 16 + 8 * b);
+        } else {
+          const float* p = xs + (32 * lt + cc) * pitch + kofs + 8 * b;
+          const f32x4 u0 = *reinterpret_cast<const f32x4*>(p);
+          const f32x4 u1 = *reinterpret_cast<const f32x4*>(p + 4);
+          const float xv[8] = {u0[0], u0[1], u0[2], u0[3], u1[0], u1[1], u1[2], u1[3]};
+          round8(xv, bh[lt]);   // Omega entries are exact in bf16
+        }
       }
       const bool gram_rows_valid = row0 < m;   // a tile past m (m % 64 == 0) holds row-0 copies
 #pragma unroll
@@ -516,6 +566,36 @@ __global__ __launch_bounds__(512) void k_sketch_dl(const float* __restrict__ A, 
     __builtin_amdgcn_s_barrier();
     asm volatile("" ::: "memory");
   };
+  if constexpr (OP) {
+    // waves 0 and 1 issue one more load per chunk (their Omega halves, ahead of the A tiles)
+    constexpr int WAIT_O = (ST - 1) * (NI + 1);
+    constexpr int WAIT_O_IMM = (WAIT_O & 15) | ((WAIT_O >> 4) << 14) | 0x0F70;
+    static_assert(WAIT_O < 64, "vmcnt range");
+    for (int d = 0; d < ST - 1; ++d)
+      if (c0 + d < c1) issue(c0 + d);
+    for (int64_t c = c0; c < c1; ++c) {
+      if (c + ST - 1 < c1) {
+        issue(c + ST - 1);
+        if (w < 2) __builtin_amdgcn_s_waitcnt(WAIT_O_IMM);
+        else __builtin_amdgcn_s_waitcnt(WAIT_IMM);
+      } else {
+        __builtin_amdgcn_s_waitcnt(0x0F70);   // the last chunks: wait for all
+      }
+      // chunk c's Omega (loaded by waves 0 and 1) is in LDS for every wave; every wave is also past
+      // compute(c - 1), so the slot that issue(c + ST) overwrites next is free
+      xs_barrier();
+      compute(c, nullptr, 16 * cg);
+      __builtin_amdgcn_sched_barrier(0);
+      if constexpr (GRAM0) {
+        if (((c - c0) & 31) == 31) {
+#pragma unroll
+          for (int v = 0; v < 16; ++v) gd[v] += (double)gacc[0][v] + (double)gacc[1][v];
+          gacc[0] = zero16();
+          gacc[1] = zero16();
+        }
+      }
+    }
+  } else {
   FillRegs<true, 1> R;
   for (int d = 0; d < ST - 1; ++d)
     if (c0 + d < c1) issue(c0 + d);
@@ -548,6 +628,7 @@ __global__ __launch_bounds__(512) void k_sketch_dl(const float* __restrict__ A, 
     }
   }
 
+  }
   const int64_t slab = (int64_t)blockIdx.x * CG + cg;
   float* out = part + slab * (int64_t)m * KP;
   float unscale[LT];
@@ -1150,13 +1231,32 @@ bool sketch_gram0_ok(const SketchPlan& p, int m, int64_t lda, int kp) {
 
 int64_t sketch_gram0_partials(const SketchPlan& p) { return (int64_t)p.grid_x * p.ysplit; }
 
+int64_t omega_chunks_bytes(int64_t n) { return (n + 15) / 16 * 2048; }
+
+void launch_omega_chunks(uint64_t seed, uint32_t stream, int64_t n, int ell, void* out, hipStream_t st) {
+  const int64_t nchunks = (n + 15) / 16;
+  int64_t grid = (nchunks * 16 + 255) / 256;
+  const int64_t cap = (int64_t)num_cus() * 32;
+  if (grid > cap) grid = cap;
+  if (grid < 1) grid = 1;
+  hipLaunchKernelGGL(k_omega_chunks, dim3((unsigned)grid), dim3(256), 0, st, seed, stream, n, ell, nchunks,
+                     reinterpret_cast<__bf16*>(out));
+}
+
 void launch_sketch_gram0(const SketchPlan& p, const float* A, int64_t lda, int m, int64_t n, uint64_t seed,
-                         uint32_t stream, int ell, float* part, double* gpart, hipStream_t st) {
+                         uint32_t stream, int ell, float* part, double* gpart, hipStream_t st, const void* omega) {
   dim3 grid(p.grid_x, p.ysplit);
   const int nw = p.RB * p.CG;
   const int tile_bytes = 4 * (256 + 16) * (int)sizeof(float) * nw;
   const int lds = p.lds_bytes + (p.lds_bytes + 4 * tile_bytes <= 160 * 1024 ? 4 : 3) * tile_bytes;
-  if (p.lds_bytes + 4 * tile_bytes <= 160 * 1024)
+  const __bf16* om = reinterpret_cast<const __bf16*>(omega);
+  if (om && p.lds_bytes + 4 * tile_bytes <= 160 * 1024)
+    hipLaunchKernelGGL((k_sketch_dl<2, AR_BF16X3, 4, true, true>), grid, dim3(64 * nw), lds, st, A, lda, m, n, seed,
+                       stream, ell, p.RB, p.CG, p.chunks_per_wg, part, (const float*)nullptr, gpart, om);
+  else if (om)
+    hipLaunchKernelGGL((k_sketch_dl<2, AR_BF16X3, 3, true, true>), grid, dim3(64 * nw), lds, st, A, lda, m, n, seed,
+                       stream, ell, p.RB, p.CG, p.chunks_per_wg, part, (const float*)nullptr, gpart, om);
+  else if (p.lds_bytes + 4 * tile_bytes <= 160 * 1024)
     hipLaunchKernelGGL((k_sketch_dl<2, AR_BF16X3, 4, true>), grid, dim3(64 * nw), lds, st, A, lda, m, n, seed, stream,
                        ell, p.RB, p.CG, p.chunks_per_wg, part, (const float*)nullptr, gpart);
   else
