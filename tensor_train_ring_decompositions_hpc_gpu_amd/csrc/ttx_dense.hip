@@ -789,15 +789,29 @@ void launch_gram_mix(const float* G, int ms, int m, const double* Cs, double* Ca
 // ------------------------------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_test_proj(const float* __restrict__ Qc, int64_t ldq, int m, int k,
                                                    const float* __restrict__ Yt, int p, double* __restrict__ part) {
+  // the block's rows of Yt (RB64 x p, p <= 16) go to LDS once; thread (c, t) then sums its 64 rows in
+  // four independent chains (the Q column segment is 64 consecutive floats)
+  __shared__ double ys[RB64][17];
   const int64_t i0 = (int64_t)blockIdx.x * RB64;
+  for (int e = threadIdx.x; e < RB64 * p; e += 256) {
+    const int rr = e / p, t = e % p;
+    ys[rr][t] = (i0 + rr < m) ? (double)Yt[(i0 + rr) * p + t] : 0.0;
+  }
+  __syncthreads();
+  const int nr = (m - i0) < RB64 ? (int)(m - i0) : RB64;
   for (int e = threadIdx.x; e < k * p; e += 256) {
     const int c = e / p, t = e % p;
-    double s = 0.0;
-    for (int rr = 0; rr < RB64; ++rr) {
-      const int64_t i = i0 + rr;
-      if (i < m) s += (double)Qc[(int64_t)c * ldq + i] * (double)Yt[i * p + t];
+    const float* q = Qc + (int64_t)c * ldq + i0;
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+    int rr = 0;
+    for (; rr + 3 < nr; rr += 4) {
+      s0 += (double)q[rr] * ys[rr][t];
+      s1 += (double)q[rr + 1] * ys[rr + 1][t];
+      s2 += (double)q[rr + 2] * ys[rr + 2][t];
+      s3 += (double)q[rr + 3] * ys[rr + 3][t];
     }
-    part[(int64_t)blockIdx.x * k * p + e] = s;
+    for (; rr < nr; ++rr) s0 += (double)q[rr] * ys[rr][t];
+    part[(int64_t)blockIdx.x * k * p + e] = (s0 + s1) + (s2 + s3);
   }
 }
 
@@ -820,9 +834,17 @@ __global__ __launch_bounds__(256) void k_test_resid(const float* __restrict__ Qc
     const int64_t i = i0 + rr;
     if (i >= m) continue;
     const double y = (double)Yt[i * p + t];
-    double proj = 0.0;
-    for (int c = 0; c < k; ++c) proj += (double)Qc[(int64_t)c * ldq + i] * C[c * p + t];
-    const double d = y - proj;
+    // four independent chains over the basis columns
+    double p0 = 0.0, p1 = 0.0, p2 = 0.0, p3 = 0.0;
+    int c = 0;
+    for (; c + 3 < k; c += 4) {
+      p0 += (double)Qc[(int64_t)c * ldq + i] * C[c * p + t];
+      p1 += (double)Qc[(int64_t)(c + 1) * ldq + i] * C[(c + 1) * p + t];
+      p2 += (double)Qc[(int64_t)(c + 2) * ldq + i] * C[(c + 2) * p + t];
+      p3 += (double)Qc[(int64_t)(c + 3) * ldq + i] * C[(c + 3) * p + t];
+    }
+    for (; c < k; ++c) p0 += (double)Qc[(int64_t)c * ldq + i] * C[c * p + t];
+    const double d = y - ((p0 + p1) + (p2 + p3));
     r2 += d * d;
     y2 += y * y;
   }

@@ -187,6 +187,8 @@ __global__ __launch_bounds__(HID) void k_mlp_predict(const float* __restrict__ P
   __shared__ float h1[HID];
   __shared__ float h2[HID];
   __shared__ double D;
+  __shared__ double ds[HID];   // the pivots, loaded once in parallel
+  static_assert(HID >= 64, "kp <= 64 pivots (checked by the callers) fit the staging array");
   const float* W1 = P;
   const float* b1 = W1 + IN * HID;
   const float* W2 = b1 + HID;
@@ -194,16 +196,18 @@ __global__ __launch_bounds__(HID) void k_mlp_predict(const float* __restrict__ P
   const float* W3 = b2 + HID;
   const float* b3 = W3 + HID * OUT;
   const int t = threadIdx.x;
+  if (t < kp) ds[t] = d[t];
+  __syncthreads();
   if (t == 0) {
     double s = 0.0;
-    for (int i = 0; i < kp; ++i) s += d[i];
+    for (int i = 0; i < kp; ++i) s += ds[i];
     D = s > 0.0 ? s : 1.0;
   }
   __syncthreads();
   if (t < IN) {
     double v = 0.0;
     if (t < 32) {
-      if (2 * t + 1 < kp) v = log10(fmax(d[2 * t] + d[2 * t + 1], 1e-30 * D) / D) / 10.0;
+      if (2 * t + 1 < kp) v = log10(fmax(ds[2 * t] + ds[2 * t + 1], 1e-30 * D) / D) / 10.0;
     } else if (t == 32) {
       v = log10(rel_tol) / 10.0;
     } else if (t == 33) {
@@ -216,16 +220,21 @@ __global__ __launch_bounds__(HID) void k_mlp_predict(const float* __restrict__ P
     xs[t] = r16((float)v);
   }
   __syncthreads();
+  // the layers keep the k-order FP32 chains of mlp_forward; unrolled so the (L2) weight loads of
+  // later k are in flight while the chain runs
   float acc = 0.0f;
+#pragma unroll
   for (int k = 0; k < IN; ++k) acc += xs[k] * r16(W1[k * HID + t]);
   h1[t] = r16(gelu(acc + b1[t]));
   __syncthreads();
   acc = 0.0f;
+#pragma unroll 32
   for (int k = 0; k < HID; ++k) acc += h1[k] * r16(W2[k * HID + t]);
   h2[t] = r16(gelu(acc + b2[t]));
   __syncthreads();
   if (t < OUT) {
     acc = 0.0f;
+#pragma unroll 32
     for (int k = 0; k < HID; ++k) acc += h2[k] * r16(W3[k * OUT + t]);
     y[t] = acc + b3[t];
   }
