@@ -283,7 +283,7 @@ __global__ __launch_bounds__(256) void k_apply_gram(const TI* __restrict__ Win, 
                                                     int kp, const double* __restrict__ Rinv,
                                                     double* __restrict__ Wd, float* __restrict__ Qc, int64_t ldq,
                                                     double* __restrict__ part, const double* __restrict__ gpart,
-                                                    int64_t S) {
+                                                    int64_t S, int ldr) {
   __shared__ double ri[KMAX][KMAX + 1];
   __shared__ double ws[RB64][KMAX + 1];
   __shared__ double os[RB64][KMAX + 1];
@@ -297,7 +297,9 @@ __global__ __launch_bounds__(256) void k_apply_gram(const TI* __restrict__ Win, 
       ri[i][j] = ((i == j) ? 1.5 : 0.0) - 0.5 * ri[i][j];
     }
   } else {
-    for (int e = tid; e < k * k; e += 256) ri[e / k][e - (e / k) * k] = Rinv[e];
+    // Rinv row-major with row stride ldr (>= k): the leading k x k block of a wider factor's inverse
+    // is the inverse of its leading block
+    for (int e = tid; e < k * k; e += 256) ri[e / k][e - (e / k) * k] = Rinv[(int64_t)(e / k) * ldr + (e - (e / k) * k)];
   }
   const int64_t i0 = (int64_t)blockIdx.x * RB64;
   const int nr = (int)((rows - i0) < RB64 ? (rows - i0) : RB64);
@@ -349,28 +351,28 @@ __global__ __launch_bounds__(256) void k_apply_gram(const TI* __restrict__ Win, 
 }
 
 void launch_apply_gram_f32(const float* Win, int64_t ldi, int64_t rows, int k, const double* Rinv, double* Wout,
-                           double* part, hipStream_t st) {
+                           double* part, hipStream_t st, int ldr) {
   const int64_t grid = (rows + RB64 - 1) / RB64;
   hipLaunchKernelGGL((k_apply_gram<float, 0>), dim3((unsigned)grid), dim3(256), 0, st, Win, ldi, rows, k, k, Rinv,
-                     Wout, (float*)nullptr, (int64_t)0, part, (const double*)nullptr, (int64_t)0);
+                     Wout, (float*)nullptr, (int64_t)0, part, (const double*)nullptr, (int64_t)0, ldr > 0 ? ldr : k);
 }
 void launch_apply_gram_f64(const double* Win, int64_t rows, int k, const double* Rinv, double* Wout, double* part,
                            hipStream_t st) {
   const int64_t grid = (rows + RB64 - 1) / RB64;
   hipLaunchKernelGGL((k_apply_gram<double, 0>), dim3((unsigned)grid), dim3(256), 0, st, Win, (int64_t)k, rows, k, k,
-                     Rinv, Wout, (float*)nullptr, (int64_t)0, part, (const double*)nullptr, (int64_t)0);
+                     Rinv, Wout, (float*)nullptr, (int64_t)0, part, (const double*)nullptr, (int64_t)0, k);
 }
 void launch_apply_out(const double* Win, int64_t rows, int k, int kp, const double* Rinv, float* Qc, int64_t ldq,
                       hipStream_t st) {
   const int64_t grid = (rows + RB64 - 1) / RB64;
   hipLaunchKernelGGL((k_apply_gram<double, 1>), dim3((unsigned)grid), dim3(256), 0, st, Win, (int64_t)k, rows, k, kp,
-                     Rinv, (double*)nullptr, Qc, ldq, (double*)nullptr, (const double*)nullptr, (int64_t)0);
+                     Rinv, (double*)nullptr, Qc, ldq, (double*)nullptr, (const double*)nullptr, (int64_t)0, k);
 }
 void launch_apply_ns_out(const double* Win, int64_t rows, int k, int kp, const double* gpart, int64_t S, float* Qc,
                          int64_t ldq, hipStream_t st) {
   const int64_t grid = (rows + RB64 - 1) / RB64;
   hipLaunchKernelGGL((k_apply_gram<double, 1>), dim3((unsigned)grid), dim3(256), 0, st, Win, (int64_t)k, rows, k, kp,
-                     (const double*)nullptr, (double*)nullptr, Qc, ldq, (double*)nullptr, gpart, S);
+                     (const double*)nullptr, (double*)nullptr, Qc, ldq, (double*)nullptr, gpart, S, k);
 }
 int64_t panel_blocks(int64_t rows) { return (rows + RB64 - 1) / RB64; }
 
@@ -823,7 +825,8 @@ __global__ __launch_bounds__(256) void k_test_resid(const float* __restrict__ Qc
   __shared__ double red[2][256];
   for (int e = threadIdx.x; e < k * p; e += 256) {
     double s = 0.0;
-    for (int64_t q = 0; q < S; ++q) s += part[q * k * p + e];
+#pragma unroll 16
+    for (int64_t q = 0; q < S; ++q) s += part[q * k * p + e];   // loads in flight together, added in order
     C[e] = s;
   }
   __syncthreads();

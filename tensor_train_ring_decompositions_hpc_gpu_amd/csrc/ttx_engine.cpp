@@ -35,6 +35,11 @@ struct Handle {
   // the current randomized step's sketch with every Omega column (kp of them), reduced and before
   // the fused left factor: Ypre or Y; any basis width takes its first columns (step-local)
   float* Yfull = nullptr;
+  // the inverse shifted Cholesky factor (64 x 64) of the Gram of the mixed full sketch of Rprobe_y,
+  // taken for the rank model's pivots: its leading w x w block starts the basis of any width w
+  DevBuf Rprobe;
+  bool rprobe_valid = false;
+  const float* rprobe_y = nullptr;
   bool pre_valid = false;
   const float* pre_A = nullptr;
   int64_t pre_m = 0, pre_n = 0;
@@ -164,7 +169,8 @@ static Arith3 arith_for(int precision, int flags) {
 // orthonormal basis of the columns of Y (m x k, row-major stride kpy): shifted CholeskyQR then two
 // CholeskyQR rounds, all in FP64; output Qc column-major (kp x m), columns k..kp-1 zero
 // ------------------------------------------------------------------------------------------------
-static ttx_status orth(Handle* h, const float* Y, int64_t kpy, int m, int k, int kp, float* Qc, int* stat) {
+static ttx_status orth(Handle* h, const float* Y, int64_t kpy, int m, int k, int kp, float* Qc, int* stat,
+                       const double* Rlead = nullptr, int ldr = 0) {
   hipStream_t st = h->stream;
   const int64_t nb = panel_blocks(m);
   TTX_TRY(ensure(h, h->gpart, (size_t)nb * k * k * sizeof(double)));
@@ -176,9 +182,14 @@ static ttx_status orth(Handle* h, const float* Y, int64_t kpy, int m, int k, int
   double* W2 = P<double>(h->W2);
   double* gp = P<double>(h->gpart);
   KScope ks(h, TTX_K_SMALL, 0);
-  launch_gram_tall_f32(Y, kpy, m, k, gp, nb, st);
-  launch_chol_inv(gp, nb, k, m, 1, Ri, stat, st);
-  launch_apply_gram_f32(Y, kpy, m, k, Ri, W1, gp, st);
+  if (Rlead) {
+    // the first (shifted) round's factor is the leading k x k block of a factor already taken
+    launch_apply_gram_f32(Y, kpy, m, k, Rlead, W1, gp, st, ldr);
+  } else {
+    launch_gram_tall_f32(Y, kpy, m, k, gp, nb, st);
+    launch_chol_inv(gp, nb, k, m, 1, Ri, stat, st);
+    launch_apply_gram_f32(Y, kpy, m, k, Ri, W1, gp, st);
+  }
   launch_chol_inv(gp, nb, k, m, 0, Ri, stat, st);
   launch_apply_gram_f64(W1, m, k, Ri, W2, gp, st);
   // after two CholeskyQR rounds ||W2^T W2 - I|| is at rounding level: one Newton-Schulz step
@@ -470,7 +481,11 @@ static ttx_status range_sketch(Handle* h, const StepArgs& a, const Arith3& ar, i
     KScope ks(h, TTX_K_SMALL, 0);
     launch_copy_block(Ya, kp, m, w, pt, P<float>(h->Yt), st);
   }
-  return orth(h, Ya, kp, m, w, kp, P<float>(h->Qc), a.stat_dev);
+  // the rank model's factor of this same mixed sketch (all kp = 64 columns, shift for 64 columns)
+  // gives the first round for any width: the Cholesky factor of a leading principal block is the
+  // leading block of the factor, and so is its inverse
+  const bool reuse = h->rprobe_valid && Y == h->rprobe_y && kp == 64 && w <= kp;
+  return orth(h, Ya, kp, m, w, kp, P<float>(h->Qc), a.stat_dev, reuse ? P<double>(h->Rprobe) : nullptr, 64);
 }
 
 // one power iteration on the current basis: Z = A^T Q, Y = A Z, Q = orth(Y)
@@ -568,9 +583,13 @@ static ttx_status probe(Handle* h, const StepArgs& a, const Arith3& ar, double* 
     const int64_t nb = panel_blocks(m);
     TTX_TRY(ensure(h, h->gpart, (size_t)nb * kp * kp * sizeof(double)));
     float* Ya = left_mix(h, a, const_cast<float*>(Ysrc), kp);
+    TTX_TRY(ensure(h, h->Rprobe, (size_t)kp * kp * sizeof(double)));
     KScope ks(h, TTX_K_SMALL, 0);
     launch_gram_tall_f32(Ya, kp, m, kp, P<double>(h->gpart), nb, st);
-    launch_chol_inv(P<double>(h->gpart), nb, kp, m, 1, nullptr, nullptr, st, d_piv);
+    // with the inverse: the basis of the chosen width starts from its leading block
+    launch_chol_inv(P<double>(h->gpart), nb, kp, m, 1, P<double>(h->Rprobe), nullptr, st, d_piv);
+    h->rprobe_valid = true;
+    h->rprobe_y = Ysrc;
     return check_launch(h, "sketch pivots");
   }
   int64_t np = std::max<int64_t>(std::min<int64_t>(a.n, 512), a.n / 16);
@@ -756,11 +775,15 @@ static ttx_status run_step(Handle* h, const StepArgs& a_in, StepOut& o) {
     TTX_TRY(ensure(h, h->Qp, (size_t)kp * m * sizeof(float)));
     TTX_TRY(ensure(h, h->Z, (size_t)n * kp * sizeof(float)));
     if (pt) TTX_TRY(ensure(h, h->Yt, (size_t)m * pt * sizeof(float)));
-    struct YfullGuard {   // the full sketch belongs to this step only
+    struct YfullGuard {   // the full sketch (and the factor taken from it) belongs to this step only
       Handle* h;
-      ~YfullGuard() { h->Yfull = nullptr; }
+      ~YfullGuard() {
+        h->Yfull = nullptr;
+        h->rprobe_valid = false;
+      }
     } yfull_guard{h};
     h->Yfull = nullptr;
+    h->rprobe_valid = false;
     if (h->pre_valid && a.A == h->pre_A && a.m == h->pre_m && a.n == h->pre_n && kp == 64 && a.seed == h->pre_seed &&
         a.stream_id == h->pre_stream && ar.range == AR_BF16X3)
       h->Yfull = P<float>(h->Ypre);   // taken in the first step's pass of the sweep
@@ -822,6 +845,7 @@ static ttx_status run_step(Handle* h, const StepArgs& a_in, StepOut& o) {
     // power iterations (basicRandSVD_LU :628-683): Z = A^T Q, Y = A Z, Q = orth(Y); adaptive: only
     // while the check fails, q at most (they overwrite Y)
     h->Yfull = nullptr;
+    h->rprobe_valid = false;
     int it = 0;
     while (it < a.q && (!pt || rel2 > a.range_tol2)) {
       TTX_TRY(power_iter(h, a, ar, k, kp, sp, o));
@@ -1372,7 +1396,7 @@ static void free_handle(Handle* h, bool own_stream) {
                          &h->V, &h->Ur, &h->gpart, &h->status, &h->next[0], &h->next[1], &h->sig, &h->keys,
                          &h->scratch[0], &h->scratch[1], &h->scratch[2], &h->scratch[3], &h->Yt, &h->tpart,
                          &h->tout, &h->mP, &h->mM1, &h->mM2, &h->mG, &h->mAct, &h->mPart, &h->mIO,
-                         &h->f16s, &h->Ypre, &h->omg};
+                         &h->f16s, &h->Ypre, &h->omg, &h->Rprobe};
   for (auto* b : bufs)
     if (b->p) (void)hipFree(b->p);
   for (float* c : h->last_cores) (void)hipFree(c);

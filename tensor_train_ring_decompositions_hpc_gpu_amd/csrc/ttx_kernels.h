@@ -70,8 +70,9 @@ int64_t panel_blocks(int64_t rows);
 void launch_chol_inv(const double* part, int64_t S, int k, int64_t m, int shift_mode, double* Rinv, int* status,
                      hipStream_t st, double* pivots = nullptr);
 // W_out = W_in Rinv (rows x k) per 64-row block with the Gram partial of W_out (part[block][k][k]).
+// ldr: row stride of Rinv (0: k), so the leading block of a wider factor's inverse can be used.
 void launch_apply_gram_f32(const float* Win, int64_t ldi, int64_t rows, int k, const double* Rinv, double* Wout,
-                           double* part, hipStream_t st);
+                           double* part, hipStream_t st, int ldr = 0);
 void launch_apply_gram_f64(const double* Win, int64_t rows, int k, const double* Rinv, double* Wout, double* part,
                            hipStream_t st);
 // final round: Qc (column-major kp x m, FP32) = W_in Rinv, columns k..kp-1 zeroed
