@@ -180,7 +180,7 @@ __device__ __forceinline__ double lane_value(double v, int lane) {
 template <int KP, bool INV>
 __global__ __launch_bounds__(256) void k_chol_rinv(const double* __restrict__ part, int64_t S, int k, int64_t m,
                                                    int shift_mode, double* __restrict__ Rinv,
-                                                   int* __restrict__ status, double* __restrict__ pivots) {
+                                                   int* __restrict__ status, double* __restrict__ pivots, int kinv) {
   __shared__ double a[KMAX][KMAX + 1];     // summed Gram, then B (row c, column x) for the output
   __shared__ double Rs[KP][KMAX + 2];      // rows of R as they are produced
   const int tid = threadIdx.x;
@@ -226,7 +226,8 @@ __global__ __launch_bounds__(256) void k_chol_rinv(const double* __restrict__ pa
       if (i + 1 < KP) pivot(i + 1, fma(-r, r, W[i + 1]), inv_next);
       // row i of B from the rows j < i (independent of this step's pivot until the last multiply)
       double acc0 = (x == i) ? 1.0 : 0.0, acc1 = 0.0;
-      if constexpr (INV) {
+      // rows i >= kinv of B are not wanted (only the leading kinv x kinv block of Rinv is)
+      if (INV && i < kinv) {
 #pragma unroll
         for (int j = 0; j < i; j += 2) {
           acc0 -= Rs[j][i] * W[j];
@@ -260,13 +261,14 @@ __global__ __launch_bounds__(256) void k_chol_rinv(const double* __restrict__ pa
 }
 
 void launch_chol_inv(const double* part, int64_t S, int k, int64_t m, int shift_mode, double* Rinv, int* status,
-                     hipStream_t st, double* pivots) {
+                     hipStream_t st, double* pivots, int kinv) {
+  if (kinv <= 0 || kinv > k) kinv = k;
   switch ((k + 7) / 8) {
 #define TTX_CHOL(q) case q:                                                                                  \
     if (Rinv) hipLaunchKernelGGL((k_chol_rinv<8 * q, true>), dim3(1), dim3(256), 0, st, part, S, k, m, shift_mode,   \
-                                 Rinv, status, pivots);                                                           \
+                                 Rinv, status, pivots, kinv);                                                     \
     else hipLaunchKernelGGL((k_chol_rinv<8 * q, false>), dim3(1), dim3(256), 0, st, part, S, k, m, shift_mode,       \
-                            Rinv, status, pivots);                                                                \
+                            Rinv, status, pivots, kinv);                                                          \
     break;
     TTX_CHOL(1) TTX_CHOL(2) TTX_CHOL(3) TTX_CHOL(4) TTX_CHOL(5) TTX_CHOL(6) TTX_CHOL(7) TTX_CHOL(8)
 #undef TTX_CHOL

@@ -40,6 +40,7 @@ struct Handle {
   DevBuf Rprobe;
   bool rprobe_valid = false;
   const float* rprobe_y = nullptr;
+  int rprobe_k = 0;                // the leading rprobe_k x rprobe_k block of Rprobe is computed
   bool pre_valid = false;
   const float* pre_A = nullptr;
   int64_t pre_m = 0, pre_n = 0;
@@ -484,7 +485,7 @@ static ttx_status range_sketch(Handle* h, const StepArgs& a, const Arith3& ar, i
   // the rank model's factor of this same mixed sketch (all kp = 64 columns, shift for 64 columns)
   // gives the first round for any width: the Cholesky factor of a leading principal block is the
   // leading block of the factor, and so is its inverse
-  const bool reuse = h->rprobe_valid && Y == h->rprobe_y && kp == 64 && w <= kp;
+  const bool reuse = h->rprobe_valid && Y == h->rprobe_y && kp == 64 && w <= h->rprobe_k;
   return orth(h, Ya, kp, m, w, kp, P<float>(h->Qc), a.stat_dev, reuse ? P<double>(h->Rprobe) : nullptr, 64);
 }
 
@@ -575,7 +576,7 @@ static MlpAct mlp_act(Handle* h, int B) {
 // Ysrc != nullptr: the pivots of that sketch (the step's own 64-column sketch of every column of A,
 // reduced, before the fused left factor) instead of a probe pass
 static ttx_status probe(Handle* h, const StepArgs& a, const Arith3& ar, double* d_piv, StepOut& o,
-                        const float* Ysrc = nullptr) {
+                        const float* Ysrc = nullptr, int kinv = 0) {
   hipStream_t st = h->stream;
   const int m = (int)a.m;
   if (Ysrc) {
@@ -587,9 +588,10 @@ static ttx_status probe(Handle* h, const StepArgs& a, const Arith3& ar, double* 
     KScope ks(h, TTX_K_SMALL, 0);
     launch_gram_tall_f32(Ya, kp, m, kp, P<double>(h->gpart), nb, st);
     // with the inverse: the basis of the chosen width starts from its leading block
-    launch_chol_inv(P<double>(h->gpart), nb, kp, m, 1, P<double>(h->Rprobe), nullptr, st, d_piv);
+    launch_chol_inv(P<double>(h->gpart), nb, kp, m, 1, P<double>(h->Rprobe), nullptr, st, d_piv, kinv);
     h->rprobe_valid = true;
     h->rprobe_y = Ysrc;
+    h->rprobe_k = kinv > 0 ? kinv : kp;
     return check_launch(h, "sketch pivots");
   }
   int64_t np = std::max<int64_t>(std::min<int64_t>(a.n, 512), a.n / 16);
@@ -630,7 +632,9 @@ static ttx_status predict(Handle* h, const StepArgs& a, const Arith3& ar, int el
   double* piv = P<double>(h->mIO);
   float* x = reinterpret_cast<float*>(piv + PROBE_K);
   float* y = x + MLP_IN;
-  TTX_TRY(probe(h, a, ar, piv, o, Ysrc));
+  // the basis widths tried afterwards are at most ell: only that leading block of the factor's
+  // inverse is needed
+  TTX_TRY(probe(h, a, ar, piv, o, Ysrc, ell));
   {
     KScope ks(h, TTX_K_SMALL, 0);
     mlp_predict(P<float>(h->mP), piv, PROBE_K, a.rel_budget > 0 ? a.rel_budget : 1e-12, (double)a.m, (double)a.n, y,

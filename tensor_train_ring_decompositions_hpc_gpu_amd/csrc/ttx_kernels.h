@@ -67,8 +67,9 @@ int64_t panel_blocks(int64_t rows);
 // not positive zero their row and column of Rinv and count into status[0].  pivots (optional, k doubles)
 // receives the shifted pivots R_ii^2.
 // Rinv null: the factorisation only (pivots and status)
+// kinv (0: k): only the leading kinv x kinv block of Rinv is computed (the rest of Rinv is unspecified).
 void launch_chol_inv(const double* part, int64_t S, int k, int64_t m, int shift_mode, double* Rinv, int* status,
-                     hipStream_t st, double* pivots = nullptr);
+                     hipStream_t st, double* pivots = nullptr, int kinv = 0);
 // W_out = W_in Rinv (rows x k) per 64-row block with the Gram partial of W_out (part[block][k][k]).
 // ldr: row stride of Rinv (0: k), so the leading block of a wider factor's inverse can be used.
 void launch_apply_gram_f32(const float* Win, int64_t ldi, int64_t rows, int k, const double* Rinv, double* Wout,
