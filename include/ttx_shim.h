@@ -20,6 +20,22 @@
  * ignored.  Differences: h_A is never overwritten; errors print a message and return -1 instead of
  * aborting through assert; the cores are not freed but kept for ttx_last_cores.
  *
+ * The third-order drivers, one decomposition per call, returning its seconds (host copy included):
+ *
+ *   libttx_ttdec6.so  <- ttdecomposition/third_order_GPUs/opt/ttdec_gpu_float_rsvd_lu_half.cu:778 (func.h:16)
+ *   libttx_trdec6.so  <- trdecomposition/third_order_GPU/opt_GPU/trdec_gpu_float_rsvd_lu_half.cu:734 (func.h:15)
+ *
+ *   float ttdec_gpu_float(float* h_tensor, int* h_n_array, int* h_r_array, double* eps,
+ *                         float* singleError, float* singleComRatio);
+ *   float trdec_gpu_float(float* h_tensor, int* h_n_array, int* h_r_array, double* eps,
+ *                         float* singleError, float* singleComRatio);
+ *
+ * h_n_array[0..2] are N1..N3; h_r_array[0..3] the TT ranks (1, r1, r2, 1) or the TR ranks (r0, r1,
+ * r2, r0).  The reference declares singleError and singleComRatio and never writes them; here, when
+ * non-null, singleError[0] receives ||A - A_hat||_F / ||A||_F (computed after the timer stops) and
+ * singleComRatio[0] the tensor's entries divided by the cores' entries.  eps is not read, as in
+ * the reference.
+ *
  * Engine settings (environment, read at each call): TTX_PRECISION = fp32 | bf16 (default) | fp16,
  * TTX_OVERSAMPLING (default 10), TTX_POWER_ITERS (default 2, the ceiling of the adaptive range
  * check), TTX_ADAPTIVE = 0 | 1 (default 1).
@@ -31,6 +47,11 @@ float ttdec_half_lu_device_6(float* h_A, const int* ttRanks, const int* ttDemns,
                              const int& p, bool calError);
 float ttdec_half_lu_host_6(float* h_A, int* ttRanks, const int* ttDemns, const int& calculateTimes, const int& p,
                            const bool calError);
+
+float ttdec_gpu_float(float* h_tensor, int* h_n_array, int* h_r_array, double* eps, float* singleError,
+                      float* singleComRatio);
+float trdec_gpu_float(float* h_tensor, int* h_n_array, int* h_r_array, double* eps, float* singleError,
+                      float* singleComRatio);
 
 extern "C" {
 /* Side channel for the cores the reference discards (ttdec6_singleGPU.cu:947-952): device pointers

@@ -35,9 +35,12 @@ static void release_cores() {
   g.order = 0;
 }
 
-static float run(const float* h_A, const int* ttRanks, const int* ttDemns, int times, bool calError, bool tr) {
-  const int d = 6;
-  if (!h_A || !ttRanks || !ttDemns || times <= 0) {
+// d-mode sweep over the host tensor h_A, timed from the host copy to the cores; err_mode 0: no
+// error, 1: the relative error of every run, printed inside the timed loop (the six-mode drivers),
+// 2: the error of the last run, computed after the timer stops (the third-order drivers)
+static float run(const float* h_A, const int* ttRanks, const int* ttDemns, int times, int err_mode, bool tr,
+                 int d = 6) {
+  if (!h_A || !ttRanks || !ttDemns || times <= 0 || d < 3 || d > TTX_MAX_ORDER) {
     std::fprintf(stderr, "ttx shim: bad arguments\n");
     return -1.0f;
   }
@@ -46,8 +49,8 @@ static float run(const float* h_A, const int* ttRanks, const int* ttDemns, int t
     return -1.0f;
   }
   ttx_params p;
-  int64_t dims[6];
-  int32_t ranks[7];
+  int64_t dims[TTX_MAX_ORDER];
+  int32_t ranks[TTX_MAX_ORDER + 1];
   for (int k = 0; k < d; ++k) dims[k] = ttDemns[k];
   for (int k = 0; k <= d; ++k) ranks[k] = ttRanks[k];
   ttx_params_init(&p, d, dims, ranks);
@@ -90,7 +93,7 @@ static float run(const float* h_A, const int* ttRanks, const int* ttDemns, int t
     st = tr ? ttx_tr_svd(g.h, d_A, &p, g.cores.data(), nullptr, &rep)
             : ttx_tt_svd(g.h, d_A, &p, g.cores.data(), nullptr, &rep);
     if (st == TTX_RESIDUAL_FALLBACK_USED) st = TTX_OK;
-    if (st == TTX_OK && calError) {
+    if (st == TTX_OK && err_mode == 1) {
       double e = -1.0;
       st = tr ? ttx_tr_rel_error(g.h, d, dims, rep.ranks, g.cores.data(), d_A, &e)
               : ttx_tt_rel_error(g.h, d, dims, rep.ranks, g.cores.data(), d_A, &e);
@@ -100,6 +103,12 @@ static float run(const float* h_A, const int* ttRanks, const int* ttDemns, int t
   }
   (void)hipDeviceSynchronize();
   const double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  if (st == TTX_OK && err_mode == 2) {
+    double e = -1.0;
+    st = tr ? ttx_tr_rel_error(g.h, d, dims, rep.ranks, g.cores.data(), d_A, &e)
+            : ttx_tt_rel_error(g.h, d, dims, rep.ranks, g.cores.data(), d_A, &e);
+    g.err = e;
+  }
   (void)hipFree(d_A);
   if (st != TTX_OK) {
     std::fprintf(stderr, "ttx shim: %s: %s\n", ttx_status_string(st), ttx_last_error(g.h));
@@ -109,6 +118,27 @@ static float run(const float* h_A, const int* ttRanks, const int* ttDemns, int t
   g.order = d;
   for (int k = 0; k <= d; ++k) g.ranks[k] = rep.ranks[k];
   return (float)(sec / times);
+}
+
+// the third-order drivers: one decomposition of the N1 x N2 x N3 tensor with ranks h_r_array
+// (TT: 1, r1, r2, 1; TR: r0, r1, r2, r0, first step of rank r0 r1); the relative error and the
+// compression ratio (tensor entries / core entries, the ratio the reference's tntorch baseline
+// prints, ttdecomposition/third_order_GPUs/python/tntorch.py:11) go to the output arguments the
+// reference declares but leaves unset; eps is not read (neither does the reference read it)
+static float run3(const float* h_tensor, const int* h_n_array, const int* h_r_array, float* singleError,
+                  float* singleComRatio, bool tr) {
+  const float sec = run(h_tensor, h_r_array, h_n_array, 1, singleError ? 2 : 0, tr, 3);
+  if (sec < 0.0f) return sec;
+  if (singleError) *singleError = (float)g.err;
+  if (singleComRatio) {
+    double entries = 1.0, core_entries = 0.0;
+    for (int k = 0; k < 3; ++k) {
+      entries *= (double)h_n_array[k];
+      core_entries += (double)g.ranks[k] * h_n_array[k] * g.ranks[k + 1];
+    }
+    *singleComRatio = (float)(entries / core_entries);
+  }
+  return sec;
 }
 
 }  // namespace ttx_shim

@@ -5,7 +5,7 @@
 float ttdec_half_lu_device_6(float* h_A, const int* ttRanks, const int* ttDemns, const int& calculateTimes,
                              const int& p, bool calError) {
   (void)p;
-  return ttx_shim::run(h_A, ttRanks, ttDemns, calculateTimes, calError, true);
+  return ttx_shim::run(h_A, ttRanks, ttDemns, calculateTimes, calError ? 1 : 0, true);
 }
 
 // device-resident sweep (the reference's host-streamed variant, trdec6_singleGPU.cu:668-826, exists
@@ -13,5 +13,13 @@ float ttdec_half_lu_device_6(float* h_A, const int* ttRanks, const int* ttDemns,
 float ttdec_half_lu_host_6(float* h_A, int* ttRanks, const int* ttDemns, const int& calculateTimes, const int& p,
                            const bool calError) {
   (void)p;
-  return ttx_shim::run(h_A, ttRanks, ttDemns, calculateTimes, calError, true);
+  return ttx_shim::run(h_A, ttRanks, ttDemns, calculateTimes, calError ? 1 : 0, true);
+}
+
+// third-order TR driver with the reference's name and C++ signature
+// (trdecomposition/third_order_GPU/opt_GPU/trdec_gpu_float_rsvd_lu_half.cu:734, func.h:15)
+float trdec_gpu_float(float* h_tensor, int* h_n_array, int* h_r_array, double* eps, float* singleError,
+                      float* singleComRatio) {
+  (void)eps;
+  return ttx_shim::run3(h_tensor, h_n_array, h_r_array, singleError, singleComRatio, true);
 }

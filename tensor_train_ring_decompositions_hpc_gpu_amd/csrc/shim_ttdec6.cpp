@@ -5,7 +5,7 @@
 float ttdec_half_lu_device_6(float* h_A, const int* ttRanks, const int* ttDemns, const int& calculateTimes,
                              const int& p, bool calError) {
   (void)p;
-  return ttx_shim::run(h_A, ttRanks, ttDemns, calculateTimes, calError, false);
+  return ttx_shim::run(h_A, ttRanks, ttDemns, calculateTimes, calError ? 1 : 0, false);
 }
 
 // The reference streams h_A in p slices because its tensors outgrow 32 GB devices
@@ -14,5 +14,13 @@ float ttdec_half_lu_device_6(float* h_A, const int* ttRanks, const int* ttDemns,
 float ttdec_half_lu_host_6(float* h_A, int* ttRanks, const int* ttDemns, const int& calculateTimes, const int& p,
                            const bool calError) {
   (void)p;
-  return ttx_shim::run(h_A, ttRanks, ttDemns, calculateTimes, calError, false);
+  return ttx_shim::run(h_A, ttRanks, ttDemns, calculateTimes, calError ? 1 : 0, false);
+}
+
+// third-order TT driver with the reference's name and C++ signature
+// (ttdecomposition/third_order_GPUs/opt/ttdec_gpu_float_rsvd_lu_half.cu:778, func.h:16)
+float ttdec_gpu_float(float* h_tensor, int* h_n_array, int* h_r_array, double* eps, float* singleError,
+                      float* singleComRatio) {
+  (void)eps;
+  return ttx_shim::run3(h_tensor, h_n_array, h_r_array, singleError, singleComRatio, false);
 }

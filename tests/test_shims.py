@@ -96,3 +96,85 @@ def test_shim_runs_like_reference_main(engine, kind, name):
     An = before.astype(np.float64)
     full = ttsvd.tr_full(hosts, dims, ranks) if kind == "tr" else ttsvd.tt_full(hosts, dims, ranks)
     assert ttsvd.rel_error(An, full) < 1e-4
+
+
+THIRD = {"tt": "_Z15ttdec_gpu_floatPfPiS0_PdS_S_", "tr": "_Z15trdec_gpu_floatPfPiS0_PdS_S_"}
+
+
+def _load3(kind):
+    lib = _load(kind)
+    fn = getattr(lib, THIRD[kind])
+    fn.restype = ctypes.c_float
+    PI, PF = ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_float)
+    fn.argtypes = [PF, PI, PI, ctypes.POINTER(ctypes.c_double), PF, PF]
+    return lib, fn
+
+
+@pytest.mark.parametrize("kind", ["tt", "tr"])
+def test_third_order_symbols_demangle_to_reference_signatures(kind):
+    """ttdec_gpu_float / trdec_gpu_float (third_order_GPUs/opt/func.h:16, opt_GPU/func.h:15)."""
+    lib, _ = _load3(kind)
+    cxxfilt = shutil.which("c++filt")
+    if cxxfilt is None:
+        pytest.skip("c++filt not available")
+    out = subprocess.run([cxxfilt, THIRD[kind]], stdout=subprocess.PIPE, text=True).stdout.strip()
+    name = "ttdec_gpu_float" if kind == "tt" else "trdec_gpu_float"
+    assert out == name + "(float*, int*, int*, double*, float*, float*)"
+
+
+def _cores_to_host(lib, dims, kind):
+    ptrs = ctypes.POINTER(ctypes.c_void_p)()
+    used = (ctypes.c_int32 * 17)()
+    d = lib.ttx_last_cores(ctypes.byref(ptrs), used)
+    assert d == len(dims)
+    ranks = list(used[:d + 1])
+    hip = ctypes.CDLL("libamdhip64.so")
+    hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+    hosts = []
+    for k in range(d):
+        n = ranks[k] * dims[k] * ranks[k + 1]
+        h = np.empty(n, dtype=np.float32)
+        assert hip.hipMemcpy(h.ctypes.data, ptrs[k], 4 * n, 2) == 0     # hipMemcpyDeviceToHost
+        hosts.append(h)
+    return hosts, ranks
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kind,gen_ranks,ranks", [
+    ("tt", [1, 5, 4, 1], [1, 5, 4, 1]),      # exact ranks: error at the noise level
+    ("tt", [1, 6, 5, 1], [1, 3, 3, 1]),      # truncated: an error of order one
+    ("tr", [2, 2, 2, 2], [2, 2, 8, 2]),      # ring of ranks 2, inner rank r0^2 r2 reproduces it
+    ("tr", [2, 3, 3, 2], [2, 2, 4, 2]),      # truncated ring
+])
+def test_third_order_driver_fills_error_and_ratio(engine, kind, gen_ranks, ranks):
+    """The third-order drivers run like the reference's main (one decomposition, seconds back) and
+    fill singleError / singleComRatio, which the reference declares and leaves unset: the error
+    agrees with the oracle's reconstruction from the returned cores, and the ratio is the tensor's
+    entries over the cores' entries."""
+    lib, fn = _load3(kind)
+    dims = [24, 20, 16]
+    A = engine.generate(dims, gen_ranks, std=1.0, noise_rel=1e-6, seed=1134, tr=(kind == "tr"))
+    hA = np.ascontiguousarray(A.cpu().numpy(), dtype=np.float32)
+    before = hA.copy()
+    n3 = (ctypes.c_int * 3)(*dims)
+    r4 = (ctypes.c_int * 4)(*ranks)
+    eps = (ctypes.c_double * 2)(1e-6, 1e-8)
+    err = ctypes.c_float(-7.0)
+    ratio = ctypes.c_float(-7.0)
+    t = fn(hA.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), n3, r4, eps, ctypes.byref(err), ctypes.byref(ratio))
+    assert t > 0
+    assert np.array_equal(hA, before)
+    hosts, used = _cores_to_host(lib, dims, kind)
+    assert used == ranks
+    full = ttsvd.tr_full(hosts, dims, used) if kind == "tr" else ttsvd.tt_full(hosts, dims, used)
+    e_or = ttsvd.rel_error(before.astype(np.float64), full)
+    assert abs(err.value - e_or) <= 2e-7 + 1e-4 * e_or
+    if ranks == gen_ranks or (kind == "tr" and ranks[2] >= ranks[0] ** 2 * gen_ranks[2]):
+        assert err.value < 1e-4
+    else:
+        assert err.value > 1e-3
+    core_entries = sum(used[k] * dims[k] * used[k + 1] for k in range(3))
+    assert ratio.value == pytest.approx(np.prod(dims) / core_entries, rel=1e-6)
+    # a null singleError / singleComRatio is allowed (the reference's callers may pass either)
+    t2 = fn(hA.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), n3, r4, eps, None, None)
+    assert t2 > 0
