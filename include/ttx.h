@@ -75,6 +75,9 @@ typedef struct {
  * above the step's tolerance, and power_iters becomes their ceiling.  In DELTA rank mode the basis
  * also grows (ell = 16, 32, ..., r + s) while the check fails. */
 #define TTX_FLAG_ADAPTIVE 4
+/* ttx_tt_svd_host: read the host tensor twice in slices (Gram, then projection) even when a device
+ * copy of it would fit; by default one read fills a device copy and the projection reads that */
+#define TTX_FLAG_STREAM_TWICE 8
 
 typedef struct {
   int32_t ranks[TTX_MAX_ORDER + 1];   /* ranks used */
@@ -126,6 +129,20 @@ int64_t ttx_core_size(const ttx_params* p, int32_t tr, int32_t k);
  * (d-1) x TTX_MAX_ELL singular values per step, descending, zero-filled. */
 ttx_status ttx_tt_svd(ttx_handle h, const float* d_A, const ttx_params* p, float* const* d_cores, float* h_sigma,
                       ttx_report* rep);
+/* Replaces ttdec_half_lu_host_6 (ttdecomposition/high_order_GPUs/ttdec6_singleGPU.cu:656-796, its slice
+ * loops :540-652): the tensor stays in host memory h_A (column-major; pinned, e.g. hipHostMalloc, for
+ * copies that overlap the kernels).  The first step (exact Gram on the N_1 side, N_1 <= TTX_MAX_ELL)
+ * reads it in column slices of slice_cols columns (0: about 64 MB per slice), the copy of the next
+ * slice on the handle's copy stream overlapping the kernels of the current one: once, into a device
+ * copy that the projection then reads, when that copy fits beside the remainder with 2 GB to spare
+ * (and TTX_FLAG_STREAM_TWICE is not set), else twice through two slice buffers.  Its remainder
+ * (r_1 x N_2..N_d, N_1 / r_1 times smaller than the tensor) stays on the device for the rest of the
+ * sweep, as in ttx_tt_svd.  Order >= 3.  DELTA / LEARNED: the first rank by the delta rule on the
+ * exact spectrum, the later steps with the same per-step budget.  Device memory: two slices (or the
+ * device copy), the remainder and the sweep's own working set; these buffers stay with the handle
+ * for the next call (ttx_destroy frees them). */
+ttx_status ttx_tt_svd_host(ttx_handle h, const float* h_A, const ttx_params* p, int64_t slice_cols,
+                           float* const* d_cores, float* h_sigma, ttx_report* rep);
 /* Replaces ttdec_half_lu_device_6 of trdecomposition/high_order_GPUs/trdec6_singleGPU.cu:827-1004:
  * first step of rank r_0 r_1, the 231 relayout, then the TT sweep; TR core layouts above. */
 ttx_status ttx_tr_svd(ttx_handle h, const float* d_A, const ttx_params* p, float* const* d_cores, float* h_sigma,

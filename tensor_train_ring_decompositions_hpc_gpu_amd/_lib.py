@@ -17,7 +17,7 @@ PREC_FP32, PREC_BF16, PREC_FP16 = 0, 1, 2
 RANK_FIXED, RANK_DELTA, RANK_LEARNED = 0, 1, 2
 METHOD_AUTO, METHOD_EXACT, METHOD_RSVD = 0, 1, 2
 PATH_EXACT_M, PATH_EXACT_N, PATH_RSVD = 0, 1, 2
-FLAG_FAST_RANGE, FLAG_TIMING, FLAG_ADAPTIVE = 1, 2, 4
+FLAG_FAST_RANGE, FLAG_TIMING, FLAG_ADAPTIVE, FLAG_STREAM_TWICE = 1, 2, 4, 8
 NUM_KCLASS = 8
 KCLASS = ["sketch", "proj", "gram", "permute", "tall", "small", "unused6", "unused7"]
 
@@ -103,6 +103,7 @@ SIGNATURES = {
     "ttx_core_size": (_I64, [ctypes.POINTER(Params), _I32, _I32]),
     "ttx_tt_svd": (_I32, [_P, _P, ctypes.POINTER(Params), _FPP, ctypes.POINTER(_F), ctypes.POINTER(Report)]),
     "ttx_tr_svd": (_I32, [_P, _P, ctypes.POINTER(Params), _FPP, ctypes.POINTER(_F), ctypes.POINTER(Report)]),
+    "ttx_tt_svd_host": (_I32, [_P, _P, ctypes.POINTER(Params), _I64, _FPP, ctypes.POINTER(_F), ctypes.POINTER(Report)]),
     "ttx_rsvd": (_I32, [_P, _P, _I64, _I64, _I32, _I32, _I32, _U64, _I32, _P, _P, _P]),
     "ttx_step": (_I32, [_P, _P, _I64, _I64, _I32, _I32, _I32, _U64, _U32, _I32, _I32, _P, _P,
                         ctypes.POINTER(_F), ctypes.POINTER(_I32)]),

@@ -41,6 +41,19 @@ def _dev_ptr(t):
     return t.data_ptr()
 
 
+def _host_ptr(t):
+    torch = _torch()
+    if isinstance(t, np.ndarray):
+        if t.dtype != np.float32 or not t.flags["F_CONTIGUOUS"] and not t.flags["C_CONTIGUOUS"]:
+            raise TypeError("ttx: expected a contiguous float32 array")
+        return t.ctypes.data
+    if not isinstance(t, torch.Tensor) or t.is_cuda:
+        raise TypeError("ttx: expected a host (CPU) tensor")
+    if t.dtype != torch.float32 or not t.is_contiguous():
+        raise TypeError("ttx: expected a contiguous float32 tensor")
+    return t.data_ptr()
+
+
 class Engine:
     """One ttx handle bound to the current torch stream of the current device."""
 
@@ -157,7 +170,7 @@ class Engine:
         p.flags = int(flags)
         return p
 
-    def _sweep(self, fn, A, dims, ranks, tr, cores=None, **kw):
+    def _sweep(self, fn, A, dims, ranks, tr, cores=None, host_slice=None, **kw):
         torch = _torch()
         p = self.params(dims, ranks, **kw)
         if p.rank_mode == L.RANK_MODES["learned"] and not self.model_loaded:
@@ -170,11 +183,17 @@ class Engine:
                                       c.numel() < sz for c, sz in zip(cores, sizes)):
                 raise ValueError("ttx: cores must be d contiguous float32 CUDA tensors of at least ttx_core_size floats")
         else:
-            cores = [torch.empty(sz, device=A.device, dtype=torch.float32) for sz in sizes]
+            dev = "cuda" if host_slice is not None else A.device
+            cores = [torch.empty(sz, device=dev, dtype=torch.float32) for sz in sizes]
         sigma = np.zeros((d - 1) * L.MAX_ELL, dtype=np.float32)
         rep = L.Report()
-        st = fn(self.h, ctypes.c_void_p(_dev_ptr(A)), ctypes.byref(p), L.ptr_array([c.data_ptr() for c in cores]),
-                sigma.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), ctypes.byref(rep))
+        sig_p = sigma.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+        core_p = L.ptr_array([c.data_ptr() for c in cores])
+        if host_slice is None:
+            st = fn(self.h, ctypes.c_void_p(_dev_ptr(A)), ctypes.byref(p), core_p, sig_p, ctypes.byref(rep))
+        else:
+            st = fn(self.h, ctypes.c_void_p(_host_ptr(A)), ctypes.byref(p), int(host_slice), core_p, sig_p,
+                    ctypes.byref(rep))
         _check(self.lib, self.h, st)
         used = [int(rep.ranks[k]) for k in range(d + 1)]
         out = []
@@ -201,6 +220,12 @@ class Engine:
         """TT-SVD of the column-major tensor A; returns ([(core, (rows, cols))], sigma, report).
         cores=core_buffers(...) (optional): write the cores there instead of new tensors."""
         return self._sweep(self.lib.ttx_tt_svd, A, dims, ranks, False, **kw)
+
+    def tt_svd_host(self, A, dims, ranks, slice_cols=0, **kw):
+        """TT-SVD of a column-major tensor held in host memory (a contiguous float32 CPU torch tensor,
+        pinned for overlapped copies, or a numpy array): the first step streams it in column slices
+        of slice_cols columns (0: about 64 MB each); the cores land on the GPU.  Returns as tt_svd."""
+        return self._sweep(self.lib.ttx_tt_svd_host, A, dims, ranks, False, host_slice=slice_cols, **kw)
 
     def tr_svd(self, A, dims, ranks, **kw):
         """TR-SVD (first step rank ranks[0] * ranks[1]); returns ([(core, shape)], sigma, report)."""

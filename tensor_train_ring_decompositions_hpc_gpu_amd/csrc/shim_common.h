@@ -38,8 +38,10 @@ static void release_cores() {
 // d-mode sweep over the host tensor h_A, timed from the host copy to the cores; err_mode 0: no
 // error, 1: the relative error of every run, printed inside the timed loop (the six-mode drivers),
 // 2: the error of the last run, computed after the timer stops (the third-order drivers)
+// host_slices > 0 (TT, N_1 <= TTX_MAX_ELL): the host-streamed sweep, the tensor read from h_A in
+// that many column slices; the device copy of the tensor is then made only for the error
 static float run(const float* h_A, const int* ttRanks, const int* ttDemns, int times, int err_mode, bool tr,
-                 int d = 6) {
+                 int d = 6, int host_slices = 0) {
   if (!h_A || !ttRanks || !ttDemns || times <= 0 || d < 3 || d > TTX_MAX_ORDER) {
     std::fprintf(stderr, "ttx shim: bad arguments\n");
     return -1.0f;
@@ -64,8 +66,10 @@ static float run(const float* h_A, const int* ttRanks, const int* ttDemns, int t
   for (int k = 0; k < d; ++k) total *= dims[k];
 
   release_cores();
+  const bool host = host_slices > 0 && !tr && dims[0] <= TTX_MAX_ELL;
+  const int64_t slice_cols = host ? (total / dims[0] + host_slices - 1) / host_slices : 0;
   float* d_A = nullptr;
-  if (hipMalloc(&d_A, (size_t)total * sizeof(float)) != hipSuccess) {
+  if ((!host || err_mode != 0) && hipMalloc(&d_A, (size_t)total * sizeof(float)) != hipSuccess) {
     std::fprintf(stderr, "ttx shim: out of device memory for the tensor\n");
     return -1.0f;
   }
@@ -85,14 +89,21 @@ static float run(const float* h_A, const int* ttRanks, const int* ttDemns, int t
   (void)hipDeviceSynchronize();
   const auto t0 = std::chrono::steady_clock::now();
   for (int i = 0; i < times && st == TTX_OK; ++i) {
-    // the reference copies the host tensor inside its timed loop (ttdec6_singleGPU.cu:824)
-    if (hipMemcpy(d_A, h_A, (size_t)total * sizeof(float), hipMemcpyHostToDevice) != hipSuccess) {
-      st = TTX_ERR_HIP;
-      break;
+    if (host) {
+      st = ttx_tt_svd_host(g.h, h_A, &p, slice_cols, g.cores.data(), nullptr, &rep);
+    } else {
+      // the reference copies the host tensor inside its timed loop (ttdec6_singleGPU.cu:824)
+      if (hipMemcpy(d_A, h_A, (size_t)total * sizeof(float), hipMemcpyHostToDevice) != hipSuccess) {
+        st = TTX_ERR_HIP;
+        break;
+      }
+      st = tr ? ttx_tr_svd(g.h, d_A, &p, g.cores.data(), nullptr, &rep)
+              : ttx_tt_svd(g.h, d_A, &p, g.cores.data(), nullptr, &rep);
     }
-    st = tr ? ttx_tr_svd(g.h, d_A, &p, g.cores.data(), nullptr, &rep)
-            : ttx_tt_svd(g.h, d_A, &p, g.cores.data(), nullptr, &rep);
     if (st == TTX_RESIDUAL_FALLBACK_USED) st = TTX_OK;
+    if (st == TTX_OK && err_mode == 1 && host &&
+        hipMemcpy(d_A, h_A, (size_t)total * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)
+      st = TTX_ERR_HIP;
     if (st == TTX_OK && err_mode == 1) {
       double e = -1.0;
       st = tr ? ttx_tr_rel_error(g.h, d, dims, rep.ranks, g.cores.data(), d_A, &e)
@@ -103,13 +114,16 @@ static float run(const float* h_A, const int* ttRanks, const int* ttDemns, int t
   }
   (void)hipDeviceSynchronize();
   const double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  if (st == TTX_OK && err_mode == 2 && host &&
+      hipMemcpy(d_A, h_A, (size_t)total * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)
+    st = TTX_ERR_HIP;
   if (st == TTX_OK && err_mode == 2) {
     double e = -1.0;
     st = tr ? ttx_tr_rel_error(g.h, d, dims, rep.ranks, g.cores.data(), d_A, &e)
             : ttx_tt_rel_error(g.h, d, dims, rep.ranks, g.cores.data(), d_A, &e);
     g.err = e;
   }
-  (void)hipFree(d_A);
+  if (d_A) (void)hipFree(d_A);
   if (st != TTX_OK) {
     std::fprintf(stderr, "ttx shim: %s: %s\n", ttx_status_string(st), ttx_last_error(g.h));
     release_cores();

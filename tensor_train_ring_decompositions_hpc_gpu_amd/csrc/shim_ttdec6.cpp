@@ -8,13 +8,11 @@ float ttdec_half_lu_device_6(float* h_A, const int* ttRanks, const int* ttDemns,
   return ttx_shim::run(h_A, ttRanks, ttDemns, calculateTimes, calError ? 1 : 0, false);
 }
 
-// The reference streams h_A in p slices because its tensors outgrow 32 GB devices
-// (ttdec6_singleGPU.cu:540-796); the whole tensor fits in 288 GB of HBM, so this runs the
-// device-resident sweep.
+// The reference streams h_A in p column slices (ttdec6_singleGPU.cu:540-796); so does this, through
+// ttx_tt_svd_host (N_1 <= 64; a wider first mode runs the device-resident sweep)
 float ttdec_half_lu_host_6(float* h_A, int* ttRanks, const int* ttDemns, const int& calculateTimes, const int& p,
                            const bool calError) {
-  (void)p;
-  return ttx_shim::run(h_A, ttRanks, ttDemns, calculateTimes, calError ? 1 : 0, false);
+  return ttx_shim::run(h_A, ttRanks, ttDemns, calculateTimes, calError ? 1 : 0, false, 6, p > 0 ? p : 1);
 }
 
 // third-order TT driver with the reference's name and C++ signature

@@ -38,6 +38,10 @@ struct Handle {
   // the inverse shifted Cholesky factor (64 x 64) of the Gram of the mixed full sketch of Rprobe_y,
   // taken for the rank model's pivots: its leading w x w block starts the basis of any width w
   DevBuf Rprobe;
+  // host-streamed first step: two column-slice buffers, per-slice Gram sums, the remainder
+  DevBuf hs[2], hGs, hB, hA;
+  hipStream_t cstream = nullptr;  // host-to-device copies of the slices (default priority)
+  hipEvent_t hev[4] = {nullptr, nullptr, nullptr, nullptr};   // copied[2], free[2]
   bool rprobe_valid = false;
   const float* rprobe_y = nullptr;
   int rprobe_k = 0;                // the leading rprobe_k x rprobe_k block of Rprobe is computed
@@ -1192,6 +1196,173 @@ static ttx_status sweep(Handle* h, const float* d_A, const ttx_params* p, float*
 }
 
 // ------------------------------------------------------------------------------------------------
+// host-streamed TT sweep (ttdec6_singleGPU.cu:540-652, 656-796): the tensor stays in host memory;
+// the first step (exact Gram on the N_1 side) streams it twice in column slices, the H2D copy of
+// slice s + 1 on its own stream overlapping the kernels of slice s; its remainder, N_1 / r_1 times
+// smaller than the tensor, is the device-resident input of the rest of the sweep.
+// ------------------------------------------------------------------------------------------------
+static ttx_status tt_svd_host(Handle* h, const float* h_A, const ttx_params* p, int64_t slice_cols,
+                              float* const* d_cores, float* h_sigma, ttx_report* rep) {
+  TTX_TRY(validate(h, p, false));
+  if (!h_A || !d_cores) return fail(h, TTX_ERR_BAD_ARG, "null tensor or core pointers");
+  if (slice_cols < 0) return fail(h, TTX_ERR_BAD_ARG, "negative slice width");
+  if (h->capture) return fail(h, TTX_ERR_UNSUPPORTED, "host-streamed sweep inside a batch");
+  const int d = p->order;
+  if (d < 3) return fail(h, TTX_ERR_UNSUPPORTED, "host-streamed sweep needs order >= 3");
+  const int m = (int)p->dims[0];
+  if (p->dims[0] > TTX_MAX_ELL) return fail(h, TTX_ERR_UNSUPPORTED, "host-streamed sweep needs N_1 <= TTX_MAX_ELL");
+  int64_t n = 1;
+  for (int k = 1; k < d; ++k) n *= p->dims[k];
+  const int64_t w = slice_cols > 0 ? std::min(slice_cols, n)
+                                   : std::min<int64_t>(n, std::max<int64_t>(1024, ((int64_t)64 << 20) / (4 * (int64_t)m)));
+  const int64_t S = (n + w - 1) / w;
+  hipStream_t st = h->stream;
+  const Arith3 ar = arith_for(p->precision, p->flags);
+  h->ktime = false;
+  h->pre_valid = false;
+  if (!h->cstream) TTX_HIP(h, hipStreamCreateWithFlags(&h->cstream, hipStreamNonBlocking));
+  for (hipEvent_t& e : h->hev)
+    if (!e) TTX_HIP(h, hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  // one read over PCIe when a device copy of the tensor fits beside the remainder with 2 GB to
+  // spare: the slices land in that copy and the projection reads it from HBM
+  const int64_t rmax = std::min<int64_t>({(int64_t)p->ranks[1], (int64_t)m, n});
+  size_t free_b = 0, total_b = 0;
+  TTX_HIP(h, hipMemGetInfo(&free_b, &total_b));
+  const size_t need = (size_t)(m + rmax) * n * sizeof(float) + ((size_t)2 << 30);
+  const bool resident = !(p->flags & TTX_FLAG_STREAM_TWICE) && free_b + h->hA.bytes + h->hB.bytes >= need;
+  if (resident) {
+    TTX_TRY(ensure(h, h->hA, (size_t)m * n * sizeof(float)));
+  } else {
+    for (int b = 0; b < 2; ++b) TTX_TRY(ensure(h, h->hs[b], (size_t)m * w * sizeof(float)));
+  }
+  float* Ad = resident ? P<float>(h->hA) : nullptr;
+  const int64_t grid = gram_small_grid(w);
+  TTX_TRY(ensure(h, h->gpart, (size_t)grid * m * m * sizeof(double)));
+  TTX_TRY(ensure(h, h->hGs, (size_t)S * m * m * sizeof(double)));
+  TTX_TRY(ensure(h, h->status, 4 * sizeof(int)));
+  TTX_TRY(ensure(h, h->sig, TTX_MAX_ELL * sizeof(float)));
+  TTX_TRY(ensure(h, h->lam, TTX_MAX_ELL * sizeof(double)));
+  TTX_TRY(ensure(h, h->keys, 64 * sizeof(unsigned long long)));
+  TTX_TRY(ensure(h, h->V, TTX_MAX_ELL * TTX_MAX_ELL * sizeof(double)));
+  TTX_TRY(ensure(h, h->G, TTX_MAX_ELL * TTX_MAX_ELL * sizeof(double)));
+  // the caller's stream may hold earlier work on these buffers: the copies start after it
+  TTX_HIP(h, hipEventRecord(h->hev[2], st));
+  TTX_HIP(h, hipEventRecord(h->hev[3], st));
+  TTX_HIP(h, hipMemsetAsync(h->status.p, 0, 4 * sizeof(int), st));
+  // one pass over the slices: copy s into buffer s & 1 once the kernels of slice s - 2 are done, or
+  // (resident) into its place in the device copy, after the work queued before this call
+  auto stream_pass = [&](auto&& kernels) -> ttx_status {
+    for (int64_t s = 0; s < S; ++s) {
+      const int b = (int)(s & 1);
+      const int64_t c0 = s * w, ws = std::min(w, n - c0);
+      float* dst = resident ? Ad + c0 * m : P<float>(h->hs[b]);
+      if (!resident || s == 0) TTX_HIP(h, hipStreamWaitEvent(h->cstream, h->hev[2 + b], 0));
+      TTX_HIP(h, hipMemcpyAsync(dst, h_A + c0 * m, (size_t)ws * m * sizeof(float), hipMemcpyHostToDevice,
+                                h->cstream));
+      TTX_HIP(h, hipEventRecord(h->hev[b], h->cstream));
+      TTX_HIP(h, hipStreamWaitEvent(st, h->hev[b], 0));
+      kernels(s, dst, c0, ws);
+      if (!resident) TTX_HIP(h, hipEventRecord(h->hev[2 + b], st));
+      TTX_TRY(check_launch(h, "host-streamed slice"));
+    }
+    return TTX_OK;
+  };
+  // pass 1: C = A_1 A_1^T as the sum of the slices' Grams (each reduced in FP64, then summed in
+  // slice order)
+  double* Gs = P<double>(h->hGs);
+  TTX_TRY(stream_pass([&](int64_t s, const float* X, int64_t, int64_t ws) {
+    launch_gram_small(ar.accurate, X, m, m, ws, P<double>(h->gpart), grid, nullptr, st);
+    launch_reduce_f64(P<double>(h->gpart), grid, (int64_t)m * m, Gs + s * m * m, st);
+  }));
+  launch_reduce_f64(Gs, S, (int64_t)m * m, P<double>(h->G), st);
+  int* stat = P<int>(h->status);
+  launch_syevj(P<double>(h->G), m, P<double>(h->lam), P<double>(h->V), stat, P<unsigned long long>(h->keys), st);
+  TTX_TRY(check_launch(h, "host-streamed eig"));
+  TTX_HIP(h, hipMemcpyAsync(h->h_lam, h->lam.p, m * sizeof(double), hipMemcpyDeviceToHost, st));
+  TTX_HIP(h, hipStreamSynchronize(st));
+  double total = 0.0;
+  for (int i = 0; i < m; ++i) total += std::max(h->h_lam[i], 0.0);
+  // rank: fixed (capped at min(N_1, n)), or the delta rule on the exact spectrum with
+  // delta^2 = eps^2 ||A||^2 / (d - 1), r_1 as the ceiling
+  int r = (int)std::min<int64_t>({(int64_t)p->ranks[1], (int64_t)m, n});
+  const double delta2 = p->eps * p->eps * total / (double)(d - 1);
+  if (p->rank_mode != TTX_RANK_FIXED) {
+    double tail = total;
+    int rr = 0;
+    while (rr < r && tail > delta2) tail -= std::max(h->h_lam[rr++], 0.0);
+    r = std::max(rr, 1);
+  }
+  double kept = 0.0;
+  for (int i = 0; i < r; ++i) kept += std::max(h->h_lam[i], 0.0);
+  const int kpn = kp_for(r);
+  TTX_TRY(ensure(h, h->Qn, (size_t)kpn * m * sizeof(float)));
+  launch_assemble(nullptr, 0, m, m, P<double>(h->lam), P<double>(h->V), r, d_cores[0], nullptr, P<float>(h->Qn), kpn,
+                  P<float>(h->sig), P<unsigned long long>(h->keys), st);
+  TTX_TRY(check_launch(h, "host-streamed core"));
+  // pass 2: the remainder B = G_1^T A_1 (r x n column-major), slice by slice
+  TTX_TRY(ensure(h, h->hB, (size_t)r * n * sizeof(float)));
+  float* B = P<float>(h->hB);
+  if (resident) {
+    launch_proj(ar.accurate, kpn, Ad, m, m, n, P<float>(h->Qn), m, 1, B, 0, r, st);
+    TTX_TRY(check_launch(h, "host-streamed projection"));
+  } else {
+    TTX_TRY(stream_pass([&](int64_t, const float* X, int64_t c0, int64_t ws) {
+      launch_proj(ar.accurate, kpn, X, m, m, ws, P<float>(h->Qn), m, 1, B + c0 * r, 0, r, st);
+    }));
+  }
+  float sig0[TTX_MAX_ELL];
+  int st0[4];
+  TTX_HIP(h, hipMemcpyAsync(sig0, h->sig.p, sizeof(sig0), hipMemcpyDeviceToHost, st));
+  TTX_HIP(h, hipMemcpyAsync(st0, h->status.p, sizeof(st0), hipMemcpyDeviceToHost, st));
+  TTX_HIP(h, hipStreamSynchronize(st));
+  // the rest of the sweep on B seen as an order d - 1 tensor (r N_2) x N_3 x ... x N_d: its first
+  // core (r N_2) x r_2 is the TT core G_2 in place, its later cores G_3 .. G_d
+  ttx_params q = *p;
+  q.order = d - 1;
+  q.dims[0] = (int64_t)r * p->dims[1];
+  for (int k = 1; k < d - 1; ++k) q.dims[k] = p->dims[k + 1];
+  q.ranks[0] = 1;
+  for (int k = 1; k <= d - 1; ++k) q.ranks[k] = p->ranks[k + 1];
+  if (p->rank_mode != TTX_RANK_FIXED) {
+    // the same per-step budget delta^2 for the remaining d - 2 steps: eps'^2 ||B||^2 / (d - 2) = delta^2
+    q.eps = (kept > 0.0 && d > 2) ? std::sqrt(delta2 * (double)(d - 2) / kept) : p->eps;
+  }
+  ttx_report sub;
+  float* sub_sigma = h_sigma ? h_sigma + TTX_MAX_ELL : nullptr;
+  TTX_TRY(sweep(h, B, &q, d_cores + 1, sub_sigma, &sub, false));
+  if (h_sigma) memcpy(h_sigma, sig0, sizeof(sig0));
+  if (rep) {
+    ttx_report R;
+    memset(&R, 0, sizeof(R));
+    for (int k = 0; k < TTX_MAX_ORDER; ++k) R.norm2_in[k] = -1.0;
+    R.ranks[0] = 1;
+    R.ranks[1] = r;
+    for (int k = 2; k <= d; ++k) R.ranks[k] = sub.ranks[k - 1];
+    R.path[0] = TTX_PATH_EXACT_M;
+    R.passes[0] = resident ? 1 : 2;   // reads of the host tensor
+    R.jacobi_sweeps[0] = st0[1];
+    R.norm2_in[0] = total;
+    for (int k = 1; k < d - 1; ++k) {
+      R.ell[k] = sub.ell[k - 1];
+      R.path[k] = sub.path[k - 1];
+      R.power_iters[k] = sub.power_iters[k - 1];
+      R.zero_pivots[k] = sub.zero_pivots[k - 1];
+      R.jacobi_sweeps[k] = sub.jacobi_sweeps[k - 1];
+      R.norm2_in[k] = sub.norm2_in[k - 1];
+      R.range_resid[k] = sub.range_resid[k - 1];
+      R.range_checks[k] = sub.range_checks[k - 1];
+      R.passes[k] = sub.passes[k - 1];
+      R.fallback[k] = sub.fallback[k - 1];
+      R.predicted_rank[k] = sub.predicted_rank[k - 1];
+    }
+    R.range_resid[0] = -1.0;
+    R.bytes_alg = (int64_t)2 * m * n * 4 + (int64_t)r * n * 4 + sub.bytes_alg;
+    *rep = R;
+  }
+  return TTX_OK;
+}
+
+// ------------------------------------------------------------------------------------------------
 // contractions: L (left modes) x R (right modes)
 // ------------------------------------------------------------------------------------------------
 struct Halves {
@@ -1395,12 +1566,18 @@ static void free_handle(Handle* h, bool own_stream) {
   if (h->gexec) (void)hipGraphExecDestroy(h->gexec);
   if (h->graph) (void)hipGraphDestroy(h->graph);
   if (h->bstream) (void)hipStreamDestroy(h->bstream);
+  if (h->cstream) {
+    (void)hipStreamSynchronize(h->cstream);
+    (void)hipStreamDestroy(h->cstream);
+  }
+  for (hipEvent_t e : h->hev)
+    if (e) (void)hipEventDestroy(e);
   ttx::DevBuf* bufs[] = {&h->part, &h->Y, &h->Y2, &h->Qc, &h->Qp, &h->Qn, &h->Z, &h->W1, &h->W2, &h->G, &h->G2,
                          &h->Rinv, &h->lam,
                          &h->V, &h->Ur, &h->gpart, &h->status, &h->next[0], &h->next[1], &h->sig, &h->keys,
                          &h->scratch[0], &h->scratch[1], &h->scratch[2], &h->scratch[3], &h->Yt, &h->tpart,
                          &h->tout, &h->mP, &h->mM1, &h->mM2, &h->mG, &h->mAct, &h->mPart, &h->mIO,
-                         &h->f16s, &h->Ypre, &h->omg, &h->Rprobe};
+                         &h->f16s, &h->Ypre, &h->omg, &h->Rprobe, &h->hs[0], &h->hs[1], &h->hGs, &h->hB, &h->hA};
   for (auto* b : bufs)
     if (b->p) (void)hipFree(b->p);
   for (float* c : h->last_cores) (void)hipFree(c);
@@ -1460,6 +1637,12 @@ ttx_status ttx_tr_svd(ttx_handle h, const float* d_A, const ttx_params* p, float
                       ttx_report* rep) {
   if (!h) return TTX_ERR_BAD_ARG;
   return ttx::sweep(H(h), d_A, p, d_cores, h_sigma, rep, true);
+}
+
+ttx_status ttx_tt_svd_host(ttx_handle h, const float* h_A, const ttx_params* p, int64_t slice_cols,
+                           float* const* d_cores, float* h_sigma, ttx_report* rep) {
+  if (!h) return TTX_ERR_BAD_ARG;
+  return ttx::tt_svd_host(H(h), h_A, p, slice_cols, d_cores, h_sigma, rep);
 }
 
 ttx_status ttx_step(ttx_handle hh, const float* d_A, int64_t m, int64_t n, int32_t r, int32_t s, int32_t q,
