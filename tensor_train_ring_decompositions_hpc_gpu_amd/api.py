@@ -54,6 +54,41 @@ def _host_ptr(t):
     return t.data_ptr()
 
 
+def save_cores(path, cores, dims, ranks, tr=False):
+    """Write a decomposition to one .npz file (numpy's container; plain arrays only, nothing pickled):
+    core_k is core k flattened in the engine's layout (TT: (r_{k-1} N_k) x r_k column-major; TR: the
+    first core N_1 x r_0 r_1, the last r_{d-1} x N_d r_0), with dims, ranks and kind ("tt" / "tr").
+    cores: as tt_svd / tr_svd return them ((tensor, shape) pairs), or tensors / arrays."""
+    d = len(dims)
+    if len(cores) != d or len(ranks) != d + 1:
+        raise ValueError("ttx: need d cores and d + 1 ranks")
+    arrays = {}
+    for k, c in enumerate(cores):
+        t = c[0] if isinstance(c, tuple) else c
+        a = t.detach().cpu().numpy() if hasattr(t, "detach") else np.asarray(t)
+        size = int(ranks[k]) * int(dims[k]) * int(ranks[k + 1])
+        a = a.reshape(-1)
+        if a.size < size:
+            raise ValueError("ttx: core %d holds %d values, the ranks need %d" % (k, a.size, size))
+        arrays["core_%d" % k] = np.ascontiguousarray(a[:size], dtype=np.float32)
+    np.savez(path, dims=np.asarray(dims, dtype=np.int64), ranks=np.asarray(ranks, dtype=np.int32),
+             kind=np.array("tr" if tr else "tt"), **arrays)
+
+
+def load_cores(path, device="cuda"):
+    """Read a file written by save_cores: (cores, dims, ranks, tr).  device=None keeps numpy arrays;
+    otherwise the cores are float32 torch tensors on that device, ready for Engine.full / rel_error."""
+    with np.load(path, allow_pickle=False) as z:
+        dims = [int(x) for x in z["dims"]]
+        ranks = [int(x) for x in z["ranks"]]
+        tr = str(z["kind"]) == "tr"
+        cores = [np.array(z["core_%d" % k], dtype=np.float32) for k in range(len(dims))]
+    if device is not None:
+        torch = _torch()
+        cores = [torch.from_numpy(c).to(device) for c in cores]
+    return cores, dims, ranks, tr
+
+
 class Engine:
     """One ttx handle bound to the current torch stream of the current device."""
 
