@@ -143,6 +143,12 @@ ttx_status ttx_tt_svd(ttx_handle h, const float* d_A, const ttx_params* p, float
  * for the next call (ttx_destroy frees them). */
 ttx_status ttx_tt_svd_host(ttx_handle h, const float* h_A, const ttx_params* p, int64_t slice_cols,
                            float* const* d_cores, float* h_sigma, ttx_report* rep);
+/* Replaces ttdec_half_lu_host_6 of trdecomposition/high_order_GPUs/trdec6_singleGPU.cu:668-826: the TR
+ * form of ttx_tt_svd_host (first step of rank r_0 r_1 <= TTX_MAX_ELL from the streamed exact Gram, the
+ * 231 relayout of its remainder on the device, then the sweep over r_1 N_2, N_3, .., N_d r_0); fixed
+ * ranks; TR core layouts as ttx_tr_svd. */
+ttx_status ttx_tr_svd_host(ttx_handle h, const float* h_A, const ttx_params* p, int64_t slice_cols,
+                           float* const* d_cores, float* h_sigma, ttx_report* rep);
 /* Replaces ttdec_half_lu_device_6 of trdecomposition/high_order_GPUs/trdec6_singleGPU.cu:827-1004:
  * first step of rank r_0 r_1, the 231 relayout, then the TT sweep; TR core layouts above. */
 ttx_status ttx_tr_svd(ttx_handle h, const float* d_A, const ttx_params* p, float* const* d_cores, float* h_sigma,

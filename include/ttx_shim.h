@@ -16,10 +16,10 @@
  * first step of rank r0 r1); the timed loop runs calculateTimes decompositions, each including the
  * host-to-device copy of h_A, and the function returns the mean seconds per decomposition; with
  * calError the relative reconstruction error ||A - A_hat||_F / ||A||_F is printed per run, as the
- * reference's calMSE line.  p is the number of column slices in which the TT
- * ttdec_half_lu_host_6 streams h_A from host memory (ttx_tt_svd_host; N_1 <= 64, else the device
- * path), as the reference's host variant does; the other three entry points ignore it (the
- * reference's device variant slices its GEMMs with it).  Differences: h_A is never overwritten; errors print a message and return -1 instead of
+ * reference's calMSE line.  p is the number of column slices in which
+ * ttdec_half_lu_host_6 streams h_A from host memory (ttx_tt_svd_host / ttx_tr_svd_host; N_1 <= 64,
+ * TR also r_0 r_1 <= 64, else the device path), as the reference's host variants do; the device
+ * variants ignore it (the reference slices its GEMMs with it).  Differences: h_A is never overwritten; errors print a message and return -1 instead of
  * aborting through assert; the cores are not freed but kept for ttx_last_cores.
  *
  * The third-order drivers, one decomposition per call, returning its seconds (host copy included):

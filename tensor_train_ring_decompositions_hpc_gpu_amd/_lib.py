@@ -104,6 +104,7 @@ SIGNATURES = {
     "ttx_tt_svd": (_I32, [_P, _P, ctypes.POINTER(Params), _FPP, ctypes.POINTER(_F), ctypes.POINTER(Report)]),
     "ttx_tr_svd": (_I32, [_P, _P, ctypes.POINTER(Params), _FPP, ctypes.POINTER(_F), ctypes.POINTER(Report)]),
     "ttx_tt_svd_host": (_I32, [_P, _P, ctypes.POINTER(Params), _I64, _FPP, ctypes.POINTER(_F), ctypes.POINTER(Report)]),
+    "ttx_tr_svd_host": (_I32, [_P, _P, ctypes.POINTER(Params), _I64, _FPP, ctypes.POINTER(_F), ctypes.POINTER(Report)]),
     "ttx_rsvd": (_I32, [_P, _P, _I64, _I64, _I32, _I32, _I32, _U64, _I32, _P, _P, _P]),
     "ttx_step": (_I32, [_P, _P, _I64, _I64, _I32, _I32, _I32, _U64, _U32, _I32, _I32, _P, _P,
                         ctypes.POINTER(_F), ctypes.POINTER(_I32)]),

@@ -262,6 +262,10 @@ class Engine:
         of slice_cols columns (0: about 64 MB each); the cores land on the GPU.  Returns as tt_svd."""
         return self._sweep(self.lib.ttx_tt_svd_host, A, dims, ranks, False, host_slice=slice_cols, **kw)
 
+    def tr_svd_host(self, A, dims, ranks, slice_cols=0, **kw):
+        """TR-SVD of a tensor in host memory (as tt_svd_host; first step of rank ranks[0] * ranks[1])."""
+        return self._sweep(self.lib.ttx_tr_svd_host, A, dims, ranks, True, host_slice=slice_cols, **kw)
+
     def tr_svd(self, A, dims, ranks, **kw):
         """TR-SVD (first step rank ranks[0] * ranks[1]); returns ([(core, shape)], sigma, report)."""
         return self._sweep(self.lib.ttx_tr_svd, A, dims, ranks, True, **kw)

@@ -38,8 +38,9 @@ static void release_cores() {
 // d-mode sweep over the host tensor h_A, timed from the host copy to the cores; err_mode 0: no
 // error, 1: the relative error of every run, printed inside the timed loop (the six-mode drivers),
 // 2: the error of the last run, computed after the timer stops (the third-order drivers)
-// host_slices > 0 (TT, N_1 <= TTX_MAX_ELL): the host-streamed sweep, the tensor read from h_A in
-// that many column slices; the device copy of the tensor is then made only for the error
+// host_slices > 0 (N_1 <= TTX_MAX_ELL; TR also r_0 r_1 <= TTX_MAX_ELL): the host-streamed sweep, the
+// tensor read from h_A in that many column slices; the device copy of the tensor is then made only
+// for the error
 static float run(const float* h_A, const int* ttRanks, const int* ttDemns, int times, int err_mode, bool tr,
                  int d = 6, int host_slices = 0) {
   if (!h_A || !ttRanks || !ttDemns || times <= 0 || d < 3 || d > TTX_MAX_ORDER) {
@@ -66,7 +67,7 @@ static float run(const float* h_A, const int* ttRanks, const int* ttDemns, int t
   for (int k = 0; k < d; ++k) total *= dims[k];
 
   release_cores();
-  const bool host = host_slices > 0 && !tr && dims[0] <= TTX_MAX_ELL;
+  const bool host = host_slices > 0 && dims[0] <= TTX_MAX_ELL && (!tr || (int64_t)ranks[0] * ranks[1] <= TTX_MAX_ELL);
   const int64_t slice_cols = host ? (total / dims[0] + host_slices - 1) / host_slices : 0;
   float* d_A = nullptr;
   if ((!host || err_mode != 0) && hipMalloc(&d_A, (size_t)total * sizeof(float)) != hipSuccess) {
@@ -90,7 +91,8 @@ static float run(const float* h_A, const int* ttRanks, const int* ttDemns, int t
   const auto t0 = std::chrono::steady_clock::now();
   for (int i = 0; i < times && st == TTX_OK; ++i) {
     if (host) {
-      st = ttx_tt_svd_host(g.h, h_A, &p, slice_cols, g.cores.data(), nullptr, &rep);
+      st = tr ? ttx_tr_svd_host(g.h, h_A, &p, slice_cols, g.cores.data(), nullptr, &rep)
+              : ttx_tt_svd_host(g.h, h_A, &p, slice_cols, g.cores.data(), nullptr, &rep);
     } else {
       // the reference copies the host tensor inside its timed loop (ttdec6_singleGPU.cu:824)
       if (hipMemcpy(d_A, h_A, (size_t)total * sizeof(float), hipMemcpyHostToDevice) != hipSuccess) {

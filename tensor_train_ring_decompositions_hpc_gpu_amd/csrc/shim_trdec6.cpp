@@ -8,12 +8,11 @@ float ttdec_half_lu_device_6(float* h_A, const int* ttRanks, const int* ttDemns,
   return ttx_shim::run(h_A, ttRanks, ttDemns, calculateTimes, calError ? 1 : 0, true);
 }
 
-// device-resident sweep (the reference's host-streamed variant, trdec6_singleGPU.cu:668-826, exists
-// for tensors larger than its device memory)
+// The reference streams h_A in p column slices (trdec6_singleGPU.cu:668-826); so does this, through
+// ttx_tr_svd_host (N_1 <= 64 and r_0 r_1 <= 64; otherwise the device-resident sweep)
 float ttdec_half_lu_host_6(float* h_A, int* ttRanks, const int* ttDemns, const int& calculateTimes, const int& p,
                            const bool calError) {
-  (void)p;
-  return ttx_shim::run(h_A, ttRanks, ttDemns, calculateTimes, calError ? 1 : 0, true);
+  return ttx_shim::run(h_A, ttRanks, ttDemns, calculateTimes, calError ? 1 : 0, true, 6, p > 0 ? p : 1);
 }
 
 // third-order TR driver with the reference's name and C++ signature

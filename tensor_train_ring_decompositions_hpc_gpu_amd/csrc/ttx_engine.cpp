@@ -39,7 +39,7 @@ struct Handle {
   // taken for the rank model's pivots: its leading w x w block starts the basis of any width w
   DevBuf Rprobe;
   // host-streamed first step: two column-slice buffers, per-slice Gram sums, the remainder
-  DevBuf hs[2], hGs, hB, hA;
+  DevBuf hs[2], hGs, hB, hB2, hA;
   hipStream_t cstream = nullptr;  // host-to-device copies of the slices (default priority)
   hipEvent_t hev[4] = {nullptr, nullptr, nullptr, nullptr};   // copied[2], free[2]
   bool rprobe_valid = false;
@@ -1196,14 +1196,16 @@ static ttx_status sweep(Handle* h, const float* d_A, const ttx_params* p, float*
 }
 
 // ------------------------------------------------------------------------------------------------
-// host-streamed TT sweep (ttdec6_singleGPU.cu:540-652, 656-796): the tensor stays in host memory;
-// the first step (exact Gram on the N_1 side) streams it twice in column slices, the H2D copy of
-// slice s + 1 on its own stream overlapping the kernels of slice s; its remainder, N_1 / r_1 times
-// smaller than the tensor, is the device-resident input of the rest of the sweep.
+// host-streamed TT / TR sweep (ttdec6_singleGPU.cu:540-652, 656-796; trdec6_singleGPU.cu:668-826):
+// the tensor stays in host memory; the first step (exact Gram on the N_1 side) reads it in column
+// slices, the H2D copy of slice s + 1 on its own stream overlapping the kernels of slice s; its
+// remainder (TR: after the 231 relayout) is the device-resident input of the rest of the sweep,
+// which is a TT sweep in both cases (TR: over r_1 N_2, N_3, .., N_d r_0, as the device TR sweep).
 // ------------------------------------------------------------------------------------------------
-static ttx_status tt_svd_host(Handle* h, const float* h_A, const ttx_params* p, int64_t slice_cols,
-                              float* const* d_cores, float* h_sigma, ttx_report* rep) {
-  TTX_TRY(validate(h, p, false));
+static ttx_status sweep_host(Handle* h, const float* h_A, const ttx_params* p, int64_t slice_cols,
+                             float* const* d_cores, float* h_sigma, ttx_report* rep, bool tr) {
+  TTX_TRY(validate(h, p, tr));
+  if (tr && p->rank_mode != TTX_RANK_FIXED) return fail(h, TTX_ERR_UNSUPPORTED, "TR-SVD supports fixed ranks only");
   if (!h_A || !d_cores) return fail(h, TTX_ERR_BAD_ARG, "null tensor or core pointers");
   if (slice_cols < 0) return fail(h, TTX_ERR_BAD_ARG, "negative slice width");
   if (h->capture) return fail(h, TTX_ERR_UNSUPPORTED, "host-streamed sweep inside a batch");
@@ -1225,11 +1227,15 @@ static ttx_status tt_svd_host(Handle* h, const float* h_A, const ttx_params* p, 
     if (!e) TTX_HIP(h, hipEventCreateWithFlags(&e, hipEventDisableTiming));
   // one read over PCIe when a device copy of the tensor fits beside the remainder with 2 GB to
   // spare: the slices land in that copy and the projection reads it from HBM
-  const int64_t rmax = std::min<int64_t>({(int64_t)p->ranks[1], (int64_t)m, n});
+  // TR: the first step keeps its r_0 r_1 columns (zero-padded beyond N_1), as the device sweep
+  const int64_t rmax = tr ? (int64_t)p->ranks[0] * p->ranks[1]
+                          : std::min<int64_t>({(int64_t)p->ranks[1], (int64_t)m, n});
+  if (tr && rmax > TTX_MAX_ELL) return fail(h, TTX_ERR_UNSUPPORTED, "TR first-step rank r_0 r_1 above TTX_MAX_ELL");
   size_t free_b = 0, total_b = 0;
   TTX_HIP(h, hipMemGetInfo(&free_b, &total_b));
-  const size_t need = (size_t)(m + rmax) * n * sizeof(float) + ((size_t)2 << 30);
-  const bool resident = !(p->flags & TTX_FLAG_STREAM_TWICE) && free_b + h->hA.bytes + h->hB.bytes >= need;
+  const size_t need = (size_t)(m + (tr ? 2 : 1) * rmax) * n * sizeof(float) + ((size_t)2 << 30);
+  const bool resident = !(p->flags & TTX_FLAG_STREAM_TWICE) &&
+                        free_b + h->hA.bytes + h->hB.bytes + h->hB2.bytes >= need;
   if (resident) {
     TTX_TRY(ensure(h, h->hA, (size_t)m * n * sizeof(float)));
   } else {
@@ -1284,16 +1290,16 @@ static ttx_status tt_svd_host(Handle* h, const float* h_A, const ttx_params* p, 
   for (int i = 0; i < m; ++i) total += std::max(h->h_lam[i], 0.0);
   // rank: fixed (capped at min(N_1, n)), or the delta rule on the exact spectrum with
   // delta^2 = eps^2 ||A||^2 / (d - 1), r_1 as the ceiling
-  int r = (int)std::min<int64_t>({(int64_t)p->ranks[1], (int64_t)m, n});
+  int r = (int)rmax;
   const double delta2 = p->eps * p->eps * total / (double)(d - 1);
-  if (p->rank_mode != TTX_RANK_FIXED) {
+  if (!tr && p->rank_mode != TTX_RANK_FIXED) {
     double tail = total;
     int rr = 0;
     while (rr < r && tail > delta2) tail -= std::max(h->h_lam[rr++], 0.0);
     r = std::max(rr, 1);
   }
   double kept = 0.0;
-  for (int i = 0; i < r; ++i) kept += std::max(h->h_lam[i], 0.0);
+  for (int i = 0; i < std::min(r, m); ++i) kept += std::max(h->h_lam[i], 0.0);
   const int kpn = kp_for(r);
   TTX_TRY(ensure(h, h->Qn, (size_t)kpn * m * sizeof(float)));
   launch_assemble(nullptr, 0, m, m, P<double>(h->lam), P<double>(h->V), r, d_cores[0], nullptr, P<float>(h->Qn), kpn,
@@ -1316,28 +1322,41 @@ static ttx_status tt_svd_host(Handle* h, const float* h_A, const ttx_params* p, 
   TTX_HIP(h, hipMemcpyAsync(st0, h->status.p, sizeof(st0), hipMemcpyDeviceToHost, st));
   TTX_HIP(h, hipStreamSynchronize(st));
   // the rest of the sweep on B seen as an order d - 1 tensor (r N_2) x N_3 x ... x N_d: its first
-  // core (r N_2) x r_2 is the TT core G_2 in place, its later cores G_3 .. G_d
+  // core (r N_2) x r_2 is the TT core G_2 in place, its later cores G_3 .. G_d.  TR: B relaid out
+  // (r_0, r_1, n) -> (r_1, n, r_0) (trdec6_singleGPU.cu:865-877), then the order d - 1 tensor
+  // (r_1 N_2) x N_3 x ... x (N_d r_0) whose last core r_{d-1} x N_d r_0 is the TR layout
+  const float* Bsub = B;
+  const int r1 = tr ? p->ranks[1] : r;
+  if (tr) {
+    TTX_TRY(ensure(h, h->hB2, (size_t)r * n * sizeof(float)));
+    launch_permute_231(B, P<float>(h->hB2), p->ranks[0], p->ranks[1], n, st);
+    TTX_TRY(check_launch(h, "host-streamed permute_231"));
+    Bsub = P<float>(h->hB2);
+  }
   ttx_params q = *p;
   q.order = d - 1;
-  q.dims[0] = (int64_t)r * p->dims[1];
+  q.dims[0] = (int64_t)r1 * p->dims[1];
   for (int k = 1; k < d - 1; ++k) q.dims[k] = p->dims[k + 1];
+  if (tr) q.dims[d - 2] *= p->ranks[0];
   q.ranks[0] = 1;
-  for (int k = 1; k <= d - 1; ++k) q.ranks[k] = p->ranks[k + 1];
+  for (int k = 1; k < d - 1; ++k) q.ranks[k] = p->ranks[k + 1];
+  q.ranks[d - 1] = 1;
   if (p->rank_mode != TTX_RANK_FIXED) {
     // the same per-step budget delta^2 for the remaining d - 2 steps: eps'^2 ||B||^2 / (d - 2) = delta^2
     q.eps = (kept > 0.0 && d > 2) ? std::sqrt(delta2 * (double)(d - 2) / kept) : p->eps;
   }
   ttx_report sub;
   float* sub_sigma = h_sigma ? h_sigma + TTX_MAX_ELL : nullptr;
-  TTX_TRY(sweep(h, B, &q, d_cores + 1, sub_sigma, &sub, false));
+  TTX_TRY(sweep(h, Bsub, &q, d_cores + 1, sub_sigma, &sub, false));
   if (h_sigma) memcpy(h_sigma, sig0, sizeof(sig0));
   if (rep) {
     ttx_report R;
     memset(&R, 0, sizeof(R));
     for (int k = 0; k < TTX_MAX_ORDER; ++k) R.norm2_in[k] = -1.0;
-    R.ranks[0] = 1;
-    R.ranks[1] = r;
-    for (int k = 2; k <= d; ++k) R.ranks[k] = sub.ranks[k - 1];
+    R.ranks[0] = tr ? p->ranks[0] : 1;
+    R.ranks[1] = r1;
+    for (int k = 2; k < d; ++k) R.ranks[k] = sub.ranks[k - 1];
+    R.ranks[d] = tr ? p->ranks[0] : 1;
     R.path[0] = TTX_PATH_EXACT_M;
     R.passes[0] = resident ? 1 : 2;   // reads of the host tensor
     R.jacobi_sweeps[0] = st0[1];
@@ -1356,7 +1375,7 @@ static ttx_status tt_svd_host(Handle* h, const float* h_A, const ttx_params* p, 
       R.predicted_rank[k] = sub.predicted_rank[k - 1];
     }
     R.range_resid[0] = -1.0;
-    R.bytes_alg = (int64_t)2 * m * n * 4 + (int64_t)r * n * 4 + sub.bytes_alg;
+    R.bytes_alg = (int64_t)2 * m * n * 4 + (int64_t)r * n * 4 * (tr ? 3 : 1) + sub.bytes_alg;
     *rep = R;
   }
   return TTX_OK;
@@ -1577,7 +1596,7 @@ static void free_handle(Handle* h, bool own_stream) {
                          &h->V, &h->Ur, &h->gpart, &h->status, &h->next[0], &h->next[1], &h->sig, &h->keys,
                          &h->scratch[0], &h->scratch[1], &h->scratch[2], &h->scratch[3], &h->Yt, &h->tpart,
                          &h->tout, &h->mP, &h->mM1, &h->mM2, &h->mG, &h->mAct, &h->mPart, &h->mIO,
-                         &h->f16s, &h->Ypre, &h->omg, &h->Rprobe, &h->hs[0], &h->hs[1], &h->hGs, &h->hB, &h->hA};
+                         &h->f16s, &h->Ypre, &h->omg, &h->Rprobe, &h->hs[0], &h->hs[1], &h->hGs, &h->hB, &h->hB2, &h->hA};
   for (auto* b : bufs)
     if (b->p) (void)hipFree(b->p);
   for (float* c : h->last_cores) (void)hipFree(c);
@@ -1642,7 +1661,13 @@ ttx_status ttx_tr_svd(ttx_handle h, const float* d_A, const ttx_params* p, float
 ttx_status ttx_tt_svd_host(ttx_handle h, const float* h_A, const ttx_params* p, int64_t slice_cols,
                            float* const* d_cores, float* h_sigma, ttx_report* rep) {
   if (!h) return TTX_ERR_BAD_ARG;
-  return ttx::tt_svd_host(H(h), h_A, p, slice_cols, d_cores, h_sigma, rep);
+  return ttx::sweep_host(H(h), h_A, p, slice_cols, d_cores, h_sigma, rep, false);
+}
+
+ttx_status ttx_tr_svd_host(ttx_handle h, const float* h_A, const ttx_params* p, int64_t slice_cols,
+                           float* const* d_cores, float* h_sigma, ttx_report* rep) {
+  if (!h) return TTX_ERR_BAD_ARG;
+  return ttx::sweep_host(H(h), h_A, p, slice_cols, d_cores, h_sigma, rep, true);
 }
 
 ttx_status ttx_step(ttx_handle hh, const float* d_A, int64_t m, int64_t n, int32_t r, int32_t s, int32_t q,

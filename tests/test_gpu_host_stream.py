@@ -82,6 +82,31 @@ def test_host_stream_delta_ranks(engine):
     assert e < 1e-3
 
 
+@pytest.mark.parametrize("twice", [False, True])
+@pytest.mark.parametrize("dims,ranks", [
+    ([8, 8, 8, 8, 8, 8], [2, 2, 8, 8, 8, 8, 2]),   # ring of ranks 2, inner ranks r0^2 r_k: exact
+    ([4, 6, 6, 6], [2, 4, 8, 8, 2]),               # r0 r1 = 8 > N_1 = 4: the first core zero-padded
+])
+def test_tr_host_stream_matches_device_sweep(engine, dims, ranks, twice):
+    """TR form (trdec6_singleGPU.cu:668-826): the streamed first step of rank r0 r1, the 231 relayout,
+    then the sweep over r1 N2, .., Nd r0; compared with the device TR sweep and the oracle."""
+    A = engine.generate(dims, [2] * (len(dims) + 1), std=1.0, noise_rel=1e-6, seed=1134, tr=True)
+    Ah = A.cpu().pin_memory()
+    print(f"tr host stream: dims {dims} ranks {ranks} twice {twice}", flush=True)
+    flags = L.FLAG_STREAM_TWICE if twice else 0
+    ch, sh, rh = engine.tr_svd_host(Ah, dims, ranks, slice_cols=37, precision="fp32", flags=flags)
+    cd, sd, rd = engine.tr_svd(A, dims, ranks, precision="fp32")
+    torch.cuda.synchronize()
+    assert rh["ranks"] == rd["ranks"] == ranks
+    k0 = min(dims[0], ranks[0] * ranks[1])
+    np.testing.assert_allclose(sh[0][:k0], sd[0][:k0], rtol=0, atol=2e-6 * sd[0][0])
+    np.testing.assert_allclose(_cores_host(ch)[0], _cores_host(cd)[0], rtol=0, atol=2e-5)
+    a = A.cpu().numpy().astype(np.float64)
+    e_h = ttsvd.rel_error(a, ttsvd.tr_full(_cores_host(ch), dims, ranks))
+    e_d = ttsvd.rel_error(a, ttsvd.tr_full(_cores_host(cd), dims, ranks))
+    assert e_h < 1e-4 and abs(e_h - e_d) < 2e-6
+
+
 def test_host_stream_rejects_what_it_does_not_cover(engine):
     A = engine.generate([80, 8, 8], [1, 4, 4, 1], seed=3).cpu()
     with pytest.raises(L.TtxError, match="N_1"):
@@ -94,3 +119,7 @@ def test_host_stream_rejects_what_it_does_not_cover(engine):
         engine.tt_svd_host(C, [16, 8, 8], [1, 4, 4, 1], slice_cols=-1)
     with pytest.raises(TypeError):
         engine.tt_svd_host(C.cuda(), [16, 8, 8], [1, 4, 4, 1])
+    with pytest.raises(L.TtxError, match="fixed ranks"):
+        engine.tr_svd_host(C, [16, 8, 8], [2, 2, 4, 2], rank_mode="delta")
+    with pytest.raises(L.TtxError, match="r_0 r_1"):
+        engine.tr_svd_host(C, [16, 8, 8], [8, 9, 4, 8])

@@ -56,7 +56,7 @@ def _call(lib, name, A, ranks, dims, times=1, cal_error=True):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("kind,name", [("tt", DEVICE6), ("tt", HOST6), ("tr", DEVICE6)])
+@pytest.mark.parametrize("kind,name", [("tt", DEVICE6), ("tt", HOST6), ("tr", DEVICE6), ("tr", HOST6)])
 def test_shim_runs_like_reference_main(engine, kind, name):
     lib = _load(kind)
     dims = [8, 8, 8, 8, 8, 8]
