@@ -707,9 +707,13 @@ __global__ __launch_bounds__(512) void k_proj(const float* __restrict__ A, int64
   // for the 1024 x 1M step)
   const int kofs = (int)(blockIdx.x % nk);
   const bool vec = ((lda & 3) == 0) && ((m & 3) == 0);
+  // whole 64-row chunks and 16-byte rows of A and Q: no per-load row guards
+  const bool rows_whole = vec && ((m & 63) == 0) && ((ldq & 3) == 0);
 
   for (int64_t blk = blockIdx.x; blk < blocks_total; blk += gridDim.x) {
     const int64_t j = blk * 256 + 32 * w + jl;
+    // an interior block (all 256 columns below n) loads without guards (a wave-uniform branch)
+    const bool fast = rows_whole && blk * 256 + 256 <= n;
     f32x16 acc[LT];
 #pragma unroll
     for (int lt = 0; lt < LT; ++lt) acc[lt] = zero16();
@@ -725,7 +729,7 @@ __global__ __launch_bounds__(512) void k_proj(const float* __restrict__ A, int64
         const int c = idx / 16;
         const int i = ((kc + kofs) % nk) * 64 + (idx % 16) * 4;
         f32x4 v;
-        if (((ldq & 3) == 0) && i + 3 < m) {
+        if (rows_whole || (((ldq & 3) == 0) && i + 3 < m)) {
           v = *reinterpret_cast<const f32x4*>(Qc + (int64_t)c * ldq + i);
         } else {
 #pragma unroll
@@ -763,6 +767,15 @@ __global__ __launch_bounds__(512) void k_proj(const float* __restrict__ A, int64
     const int lc = lane >> 4, lr = (lane & 15) * 4;
     auto load = [&](float (&x)[32], int kc) {
       const int i = ((kc + kofs) % nk) * 64 + lr;
+      if (fast) {
+        const float* p0 = A + (blk * 256 + 32 * w + lc) * lda + i;
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+          const f32x4 v = *reinterpret_cast<const f32x4*>(p0 + (int64_t)(4 * u) * lda);
+          x[4 * u] = v[0]; x[4 * u + 1] = v[1]; x[4 * u + 2] = v[2]; x[4 * u + 3] = v[3];
+        }
+        return;
+      }
 #pragma unroll
       for (int u = 0; u < 8; ++u) {
         const int64_t jj = blk * 256 + 32 * w + 4 * u + lc;
