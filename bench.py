@@ -38,6 +38,8 @@ def parse():
     ap.add_argument("--q", type=int, default=2, help="power iterations of the headline run")
     ap.add_argument("--s", type=int, default=10, help="oversampling")
     ap.add_argument("--fast-range", action="store_true", help="single bf16 MFMA in the range finder")
+    ap.add_argument("--tail-x4", action="store_true",
+                    help="four bf16 products in the randomized steps' projection (default: three)")
     ap.add_argument("--no-adaptive", action="store_true",
                     help="headline leg with a fixed q instead of the a-posteriori range check")
     ap.add_argument("--rank-mode", default="learned", choices=["learned", "fixed"],
@@ -273,6 +275,8 @@ def main():
     torch.cuda.synchronize()
     in_bytes = A.numel() * 4
     flags = L.FLAG_FAST_RANGE if args.fast_range else 0
+    if not args.tail_x4:
+        flags |= L.FLAG_TAIL_X3
     if not args.no_adaptive:
         flags |= L.FLAG_ADAPTIVE
 
@@ -322,6 +326,7 @@ def main():
                    "rank_mode": args.rank_mode if "bf16" in legs else "fixed",
                    "eps": 1e-4, "ranks_used": rep["ranks"], "predicted_rank": rep["predicted_rank"],
                    "range_arith": "bf16" if args.fast_range else "bf16x3",
+                   "projection_arith": "bf16x4" if args.tail_x4 else "bf16x3 (randomized steps' Z = A^T Q)",
                    "range_check": None if args.no_adaptive else "a-posteriori, 8 test columns, eps=1e-4",
                    "paths": rep["path"], "ell": rep["ell"], "power_iters_run": rep["power_iters"],
                    "passes": rep["passes"], "range_resid": rep["range_resid"]},

@@ -17,7 +17,7 @@ PREC_FP32, PREC_BF16, PREC_FP16 = 0, 1, 2
 RANK_FIXED, RANK_DELTA, RANK_LEARNED = 0, 1, 2
 METHOD_AUTO, METHOD_EXACT, METHOD_RSVD = 0, 1, 2
 PATH_EXACT_M, PATH_EXACT_N, PATH_RSVD = 0, 1, 2
-FLAG_FAST_RANGE, FLAG_TIMING, FLAG_ADAPTIVE, FLAG_STREAM_TWICE = 1, 2, 4, 8
+FLAG_FAST_RANGE, FLAG_TIMING, FLAG_ADAPTIVE, FLAG_STREAM_TWICE, FLAG_TAIL_X3 = 1, 2, 4, 8, 16
 NUM_KCLASS = 8
 KCLASS = ["sketch", "proj", "gram", "permute", "tall", "small", "unused6", "unused7"]
 

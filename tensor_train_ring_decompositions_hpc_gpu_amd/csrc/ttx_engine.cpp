@@ -161,12 +161,14 @@ static int kp_for(int k) { return k <= 32 ? 32 : 64; }
 struct Arith3 {
   int range;    // sketch and power passes
   int accurate; // remainder / exact Gram / exact projection
+  int tail;     // the randomized step's final projection Z = A^T Q
 };
 static Arith3 arith_for(int precision, int flags) {
   Arith3 a;
   if (precision == TTX_PREC_FP32) { a.range = AR_FP32; a.accurate = AR_FP32; }
   else if (precision == TTX_PREC_FP16) { a.range = AR_FP16; a.accurate = AR_BF16X4; }
   else { a.range = (flags & TTX_FLAG_FAST_RANGE) ? AR_BF16 : AR_BF16X3; a.accurate = AR_BF16X4; }
+  a.tail = (precision == TTX_PREC_BF16 && (flags & TTX_FLAG_TAIL_X3)) ? AR_BF16X3 : a.accurate;
   return a;
 }
 
@@ -330,7 +332,7 @@ static ttx_status step_tail(Handle* h, const StepArgs& a, const Arith3& ar, int 
   const float* Qop = proj_operand(h, a, P<float>(h->Qc), kp);
   {
     KScope ks(h, TTX_K_PROJ, a.m * n * 4 + n * kz * 4);
-    launch_proj(ar.accurate, kp, a.A, a.m, m, n, Qop, m, 0, Z, kz, kz, st, P<double>(h->gpart), k);
+    launch_proj(ar.tail, kp, a.A, a.m, m, n, Qop, m, 0, Z, kz, kz, st, P<double>(h->gpart), k);
   }
   o.bytes += (int64_t)(a.m * n * 4);
   o.passes += 1;

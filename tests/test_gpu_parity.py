@@ -245,6 +245,35 @@ def test_tt_svd_fused_first_pass(engine, dims, ranks, s, q, mode):
         assert rel(fmat(g, *shape), c_ref[k]) < 10 * tol["vec"], k
 
 
+@pytest.mark.parametrize("dims,ranks,s,q", [
+    ([8, 9, 10, 11, 6], [1, 4, 6, 6, 5, 1], 4, 1),
+    ([32, 32, 24, 20], [1, 32, 40, 12, 1], 10, 0),
+])
+def test_tt_svd_tail_x3_matches_oracle(engine, dims, ranks, s, q):
+    """TTX_FLAG_TAIL_X3 (three bf16 products in the randomized steps' projection): the oracle's ranks,
+    singular values, cores and error to the BF16 tolerances, and the error within 2e-6 of the
+    four-product sweep's."""
+    from tensor_train_ring_decompositions_hpc_gpu_amd import _lib as L
+    A = engine.generate(dims, ranks, std=1.0, noise_rel=1e-6, seed=1134)
+    cores, sig, rep = engine.tt_svd(A, dims, ranks, s=s, q=q, precision="bf16", flags=L.FLAG_TAIL_X3)
+    c4, _, rep4 = engine.tt_svd(A, dims, ranks, s=s, q=q, precision="bf16")
+    An = host(A).astype(np.float64)
+    c_ref, s_ref, p_ref, used = ttsvd.tt_svd(An, dims, ranks, s=s, q=q)
+    tol = TOL["bf16"]
+    assert rep["ranks"] == used == rep4["ranks"]
+    for k in range(len(dims) - 1):
+        r = ranks[k + 1]
+        assert np.allclose(sig[k][:r], s_ref[k][:r], rtol=tol["sig"], atol=tol["sig"] * s_ref[k][0]), k
+    err = engine.rel_error(cores, dims, used, A)
+    err4 = engine.rel_error(c4, dims, used, A)
+    err_ref = ttsvd.rel_error(An, ttsvd.tt_full(c_ref, dims, used))
+    assert abs(err - err_ref) <= tol["err"] + 2e-2 * err_ref, (err, err_ref)
+    assert abs(err - err4) <= 2e-6, (err, err4)
+    for k in range(len(dims) - 1):
+        g, shape = cores[k]
+        assert rel(fmat(g, *shape), c_ref[k]) < 10 * tol["vec"], k
+
+
 # ------------------------------------------------------------------------------------------------
 # adaptive range finder: the engine takes the oracle's decisions (widths, power iterations, checks)
 # ------------------------------------------------------------------------------------------------
