@@ -78,10 +78,11 @@ typedef struct {
 /* ttx_tt_svd_host: read the host tensor twice in slices (Gram, then projection) even when a device
  * copy of it would fit; by default one read fills a device copy and the projection reads that */
 #define TTX_FLAG_STREAM_TWICE 8
-/* BF16 mode: the randomized steps' projection Z = A^T Q (and its fused Gram) with three bf16 products
- * per term (hi hi, lo hi, hi lo) instead of four: the dropped lo lo term is below 2^-16 of the
- * product, so a term carries about 2^-16 relative error instead of 2^-17; a quarter fewer MFMAs in
- * the largest pass of the sweep.  Exact steps and remainders keep four products. */
+/* BF16 mode: the randomized steps' projection Z = A^T Q (and its fused Gram), and the Gram that the
+ * first pass of a sweep takes for a full-rank 32-row first step, with three bf16 products per term
+ * (hi hi, lo hi, hi lo) instead of four: the dropped lo lo term is below 2^-16 of the product, so a
+ * term carries about 2^-16 relative error instead of 2^-17; a quarter fewer MFMAs in the projection,
+ * a tenth fewer in the first pass.  Other exact steps and the remainders keep four products. */
 #define TTX_FLAG_TAIL_X3 16
 
 typedef struct {

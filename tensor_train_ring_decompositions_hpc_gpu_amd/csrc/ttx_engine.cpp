@@ -1012,7 +1012,7 @@ static ttx_status sweep(Handle* h, const float* d_A, const ttx_params* p, float*
         {
           KScope ks(h, TTX_K_SKETCH, m1 * n1 * 4);
           launch_sketch_gram0(sp, d_A, m1, (int)m1, n1, p->omega_seed, 1u, 64, P<float>(h->part), P<double>(h->gpart),
-                              st, h->omg.p);
+                              st, h->omg.p, (p->flags & TTX_FLAG_TAIL_X3) != 0);
         }
         {
           KScope ks(h, TTX_K_SMALL, 0);

@@ -25,10 +25,11 @@ void launch_sketch(int ar, const SketchPlan& p, const float* A, int64_t lda, int
 bool sketch_gram0_ok(const SketchPlan& p, int m, int64_t lda, int kp);
 int64_t sketch_gram0_partials(const SketchPlan& p);
 // omega: Omega precomputed by launch_omega_chunks for (seed, stream, n, ell) (the pass then spends no
-// arithmetic on it), or nullptr (generated in the pass).
+// arithmetic on it), or nullptr (generated in the pass).  gram3 (with omega): the Gram with three bf16
+// products (no mid mid term) instead of four.
 void launch_sketch_gram0(const SketchPlan& p, const float* A, int64_t lda, int m, int64_t n, uint64_t seed,
                          uint32_t stream, int ell, float* part, double* gpart, hipStream_t st,
-                         const void* omega = nullptr);
+                         const void* omega = nullptr, bool gram3 = false);
 // Omega (kp = 64 columns, the first ell nonzero) of an n-column sketch in 2 KB chunks of 16 rows (bf16,
 // exact): omega_chunks_bytes(n) bytes at out
 int64_t omega_chunks_bytes(int64_t n);

@@ -367,7 +367,8 @@ __global__ __launch_bounds__(512) void k_sketch(const float* __restrict__ A, int
 // three-way split are the hi / lo of k_gram_small<1, AR_BF16X4>'s two-way split, and the four
 // products hi hi, lo hi, hi lo, lo lo go to one FP32 accumulator per half (two interleaved chains),
 // flushed to FP64 every 32 chunks; one FP64 32 x 32 partial per WG in
-// gpart[blockIdx.y * gridDim.x + blockIdx.x].  PAD = 16 puts the two lane halves' column reads on
+// gpart[blockIdx.y * gridDim.x + blockIdx.x].  G3: the Gram without the mid mid product (three bf16 products, about
+// 2^-16 relative per term instead of 2^-17).  PAD = 16 puts the two lane halves' column reads on
 // disjoint banks.
 // Omega for the precomputed-Omega first pass: chunk c holds rows 16 c .. 16 c + 15 of Omega (the
 // Philox counters j), column-major inside the chunk: entry (col, jj) at out[c * 1024 + col * 16 + jj],
@@ -401,7 +402,7 @@ __global__ __launch_bounds__(256) void k_omega_chunks(uint64_t seed, uint32_t st
   }
 }
 
-template <int LT, int AR, int ST, bool GRAM0, bool OP = false>
+template <int LT, int AR, int ST, bool GRAM0, bool OP = false, bool G3 = false>
 __global__ __launch_bounds__(512) void k_sketch_dl(const float* __restrict__ A, int64_t lda, int m, int64_t n,
                                                    uint64_t seed, uint32_t stream, int ell, int RB, int CG,
                                                    int64_t chunks_per_wg, float* __restrict__ part,
@@ -547,7 +548,7 @@ __global__ __launch_bounds__(512) void k_sketch_dl(const float* __restrict__ A, 
           gacc[s2] = mfma_bf16(ah, ah, gacc[s2]);
           gacc[s2] = mfma_bf16(am, ah, gacc[s2]);
           gacc[s2] = mfma_bf16(ah, am, gacc[s2]);
-          gacc[s2] = mfma_bf16(am, am, gacc[s2]);
+          if constexpr (!G3) gacc[s2] = mfma_bf16(am, am, gacc[s2]);
         }
       }
     } else {
@@ -1257,13 +1258,20 @@ void launch_omega_chunks(uint64_t seed, uint32_t stream, int64_t n, int ell, voi
 }
 
 void launch_sketch_gram0(const SketchPlan& p, const float* A, int64_t lda, int m, int64_t n, uint64_t seed,
-                         uint32_t stream, int ell, float* part, double* gpart, hipStream_t st, const void* omega) {
+                         uint32_t stream, int ell, float* part, double* gpart, hipStream_t st, const void* omega,
+                         bool gram3) {
   dim3 grid(p.grid_x, p.ysplit);
   const int nw = p.RB * p.CG;
   const int tile_bytes = 4 * (256 + 16) * (int)sizeof(float) * nw;
   const int lds = p.lds_bytes + (p.lds_bytes + 4 * tile_bytes <= 160 * 1024 ? 4 : 3) * tile_bytes;
   const __bf16* om = reinterpret_cast<const __bf16*>(omega);
-  if (om && p.lds_bytes + 4 * tile_bytes <= 160 * 1024)
+  if (om && gram3 && p.lds_bytes + 4 * tile_bytes <= 160 * 1024)
+    hipLaunchKernelGGL((k_sketch_dl<2, AR_BF16X3, 4, true, true, true>), grid, dim3(64 * nw), lds, st, A, lda, m, n,
+                       seed, stream, ell, p.RB, p.CG, p.chunks_per_wg, part, (const float*)nullptr, gpart, om);
+  else if (om && gram3)
+    hipLaunchKernelGGL((k_sketch_dl<2, AR_BF16X3, 3, true, true, true>), grid, dim3(64 * nw), lds, st, A, lda, m, n,
+                       seed, stream, ell, p.RB, p.CG, p.chunks_per_wg, part, (const float*)nullptr, gpart, om);
+  else if (om && p.lds_bytes + 4 * tile_bytes <= 160 * 1024)
     hipLaunchKernelGGL((k_sketch_dl<2, AR_BF16X3, 4, true, true>), grid, dim3(64 * nw), lds, st, A, lda, m, n, seed,
                        stream, ell, p.RB, p.CG, p.chunks_per_wg, part, (const float*)nullptr, gpart, om);
   else if (om)

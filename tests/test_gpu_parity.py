@@ -247,7 +247,8 @@ def test_tt_svd_fused_first_pass(engine, dims, ranks, s, q, mode):
 
 @pytest.mark.parametrize("dims,ranks,s,q", [
     ([8, 9, 10, 11, 6], [1, 4, 6, 6, 5, 1], 4, 1),
-    ([32, 32, 24, 20], [1, 32, 40, 12, 1], 10, 0),
+    ([32, 32, 24, 20], [1, 32, 40, 12, 1], 10, 0),       # fused first pass (three-product Gram)
+    ([32, 16, 32, 8, 6], [1, 32, 48, 30, 6, 1], 8, 0),    # fused first pass, five modes
 ])
 def test_tt_svd_tail_x3_matches_oracle(engine, dims, ranks, s, q):
     """TTX_FLAG_TAIL_X3 (three bf16 products in the randomized steps' projection): the oracle's ranks,
