@@ -44,8 +44,10 @@ def _dev_ptr(t):
 def _host_ptr(t):
     torch = _torch()
     if isinstance(t, np.ndarray):
-        if t.dtype != np.float32 or not t.flags["F_CONTIGUOUS"] and not t.flags["C_CONTIGUOUS"]:
-            raise TypeError("ttx: expected a contiguous float32 array")
+        # the engine reads the buffer as the column-major tensor: a flat array, or an N-d array in
+        # Fortran order (a C-ordered N-d array would be read transposed)
+        if t.dtype != np.float32 or not (t.ndim == 1 and t.flags["C_CONTIGUOUS"] or t.flags["F_CONTIGUOUS"]):
+            raise TypeError("ttx: expected a float32 array, flat or in Fortran (column-major) order")
         return t.ctypes.data
     if not isinstance(t, torch.Tensor) or t.is_cuda:
         raise TypeError("ttx: expected a host (CPU) tensor")
