@@ -870,30 +870,39 @@ __global__ __launch_bounds__(512) void k_proj(const float* __restrict__ A, int64
       }
     };
 
+    // workgroup barrier for the Q chunks in LDS only: their writes complete (lgkmcnt(0)), then
+    // s_barrier.  __syncthreads would also wait for the next A chunk's loads (vmcnt(0)) and drain
+    // the wave's only chunk in flight at every Q chunk.
+    auto q_barrier = [] {
+      asm volatile("" ::: "memory");
+      __builtin_amdgcn_s_waitcnt(0xC07F);   // lgkmcnt(0); vmcnt, expcnt unchanged
+      __builtin_amdgcn_s_barrier();
+      asm volatile("" ::: "memory");
+    };
     float x0[32], x1[32];
     f32x4 qr[QF];
     qload(qr, 0);
     load(x0, 0);
     qstore(qr, qs[0]);
-    __syncthreads();
+    q_barrier();
     for (int kc = 0; kc < nk; kc += 2) {
       if (kc + 1 < nk) {
-        load(x1, kc + 1);
         qload(qr, kc + 1);
+        load(x1, kc + 1);
       }
       compute(x0, qs[0]);
       __builtin_amdgcn_sched_barrier(0);
       if (kc + 1 < nk) qstore(qr, qs[1]);
-      __syncthreads();
+      q_barrier();
       if (kc + 1 >= nk) break;
       if (kc + 2 < nk) {
-        load(x0, kc + 2);
         qload(qr, kc + 2);
+        load(x0, kc + 2);
       }
       compute(x1, qs[1]);
       __builtin_amdgcn_sched_barrier(0);
       if (kc + 2 < nk) qstore(qr, qs[0]);
-      __syncthreads();
+      q_barrier();
     }
 
     if (AR == AR_FP16 && scales) {
