@@ -205,6 +205,10 @@ ttx_status ttx_tr_full(ttx_handle h, int32_t order, const int64_t* dims, const i
  * dims: count x order, ranks: count x (order + 1) (r_0 = r_d = 1), d_cores: count x order device
  * pointers (ttx_core_size of each tensor's parameters). */
 #define TTX_BATCH_GRAPH 1
+/* largest batch that TTX_BATCH_GRAPH captures (the size its tests cover; the launch of one
+ * 512-tensor graph faulted the GPU and no cause has been found, so larger batches are refused with
+ * TTX_ERR_UNSUPPORTED: split them, or run them without TTX_BATCH_GRAPH) */
+#define TTX_BATCH_GRAPH_MAX 16
 #define TTX_BATCH_PER_TENSOR 2   /* also time each tensor alone: p50 / p99 */
 #define TTX_BATCH_AUTOTUNE 4
 typedef struct {

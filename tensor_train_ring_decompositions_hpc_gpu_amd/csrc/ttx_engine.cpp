@@ -2011,6 +2011,9 @@ static ttx_status batch_run(Handle* h, int count, int order, const int64_t* dims
   if (S < 1 || S > 16) return fail(h, TTX_ERR_BAD_ARG, "batch: 1..16 streams");
   if (common->rank_mode != TTX_RANK_FIXED) return fail(h, TTX_ERR_UNSUPPORTED, "batch: fixed ranks only");
   if (common->flags & TTX_FLAG_TIMING) return fail(h, TTX_ERR_UNSUPPORTED, "batch: no per-kernel timing");
+  if ((bflags & TTX_BATCH_GRAPH) && count > TTX_BATCH_GRAPH_MAX)
+    return fail(h, TTX_ERR_UNSUPPORTED, "batch: graph capture takes at most " + std::to_string(TTX_BATCH_GRAPH_MAX) +
+                                            " tensors; split the batch or run it without TTX_BATCH_GRAPH");
   std::vector<ttx_params> P(count);
   for (int i = 0; i < count; ++i) {
     ttx_params_init(&P[i], order, dims + (size_t)i * order, ranks + (size_t)i * (order + 1));

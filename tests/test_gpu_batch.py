@@ -74,3 +74,14 @@ def test_batch_autotuned_reconstructs(engine):
         c_ref, _, _, used = ttsvd.tt_svd(An, dims, ranks, s=10, q=2)
         err_ref = ttsvd.rel_error(An, ttsvd.tt_full(c_ref, dims, used))
         assert abs(err - err_ref) <= 1e-4 + 2e-2 * err_ref, (err, err_ref)
+
+
+def test_graph_capture_refuses_batches_above_its_tested_size(engine):
+    """TTX_BATCH_GRAPH_MAX: a graph batch above the size the graph tests cover is refused before any
+    work is queued; the same batch runs without the graph."""
+    from tensor_train_ring_decompositions_hpc_gpu_amd import _lib as L
+    tensors, dims_l, ranks_l = _batch(engine, 17, 21, dims_range=(8, 16), rank_range=(2, 4))
+    with pytest.raises(L.TtxError, match="at most 16"):
+        engine.batch_tt_svd(tensors, dims_l, ranks_l, s=4, q=1, streams=2, graph=True, autotune=False)
+    out, info = engine.batch_tt_svd(tensors, dims_l, ranks_l, s=4, q=1, streams=2, graph=False, autotune=False)
+    assert info["graph"] == 0 and len(out) == 17
