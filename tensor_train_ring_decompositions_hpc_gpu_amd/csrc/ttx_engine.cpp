@@ -1257,6 +1257,11 @@ static ttx_status sweep_host(Handle* h, const float* h_A, const ttx_params* p, i
   TTX_HIP(h, hipEventRecord(h->hev[2], st));
   TTX_HIP(h, hipEventRecord(h->hev[3], st));
   TTX_HIP(h, hipMemsetAsync(h->status.p, 0, 4 * sizeof(int), st));
+  // whatever path leaves this function, no copy from the caller's h_A is still in flight
+  struct CopyGuard {
+    hipStream_t cs;
+    ~CopyGuard() { (void)hipStreamSynchronize(cs); }
+  } copy_guard{h->cstream};
   // one pass over the slices: copy s into buffer s & 1 once the kernels of slice s - 2 are done, or
   // (resident) into its place in the device copy, after the work queued before this call
   auto stream_pass = [&](auto&& kernels) -> ttx_status {
