@@ -30,7 +30,9 @@ typedef enum {
   TTX_OK = 0,
   TTX_ERR_BAD_ARG = 1,
   TTX_ERR_OOM = 2,
-  TTX_ERR_NO_CONVERGE = 3,
+  TTX_ERR_NO_CONVERGE = 3,        /* the FP64 Jacobi eigensolver of a step reached its sweep cap (30, or
+                                     TTX_JACOBI_MAX_SWEEPS in the environment at ttx_create); the outputs
+                                     are written but not trusted */
   TTX_RESIDUAL_FALLBACK_USED = 4, /* success; the adaptive range finder had to add passes */
   TTX_ERR_HIP = 5,
   TTX_ERR_UNSUPPORTED = 6
@@ -182,6 +184,17 @@ ttx_status ttx_tt_generate(ttx_handle h, int32_t order, const int64_t* dims, con
                            float noise_rel, uint64_t seed, float* d_A, float* const* d_cores);
 ttx_status ttx_tr_generate(ttx_handle h, int32_t order, const int64_t* dims, const int32_t* ranks, float std_,
                            float noise_rel, uint64_t seed, float* d_A, float* const* d_cores);
+/* the reference's own inputs (generalTTTensor, ttdecomposition/high_order_GPUs/ttdec6_singleGPU.cu:185-271;
+ * generalTRTensor, trdecomposition/high_order_GPUs/trdec6_singleGPU.cu:231-329): cores with entries
+ * (float)(rand() / RAND_MAX) * 2 - 1 from glibc's rand() after srand(seed) (the reference seeds 1134),
+ * drawn on the host in the reference's order G_{d-1}, G_d, G_{d-2}, ..., G_1 (order 6: G5, G6, G4, G3,
+ * G2, G1) and each filled in its storage order (the core layouts above), then contracted on the device
+ * in FP32 as ttx_tt_full / ttx_tr_full.  The entries are bit-identical to the reference's; the tensor
+ * differs from the reference's only by the FP32 summation order of the contraction. */
+ttx_status ttx_tt_generate_ref(ttx_handle h, int32_t order, const int64_t* dims, const int32_t* ranks, uint32_t seed,
+                               float* d_A, float* const* d_cores);
+ttx_status ttx_tr_generate_ref(ttx_handle h, int32_t order, const int64_t* dims, const int32_t* ranks, uint32_t seed,
+                               float* d_A, float* const* d_cores);
 /* ||A - reconstruction(cores)||_F / ||A||_F in FP64 without writing the reconstruction (calMSE,
  * ttdecomposition/high_order_GPUs/ttdec6_singleGPU.cu:134-182, non-destructive) */
 ttx_status ttx_tt_rel_error(ttx_handle h, int32_t order, const int64_t* dims, const int32_t* ranks,
@@ -201,7 +214,10 @@ ttx_status ttx_tr_full(ttx_handle h, int32_t order, const int64_t* dims, const i
  * (fork / join over the worker streams) and every call with the same tensors, shapes and parameters
  * relaunches that graph as one launch.  TTX_BATCH_AUTOTUNE times candidate sketch grids per shape
  * class (ceil log2 m, ceil log2 n, sketch width, arithmetic) on the first unfolding of that class and
- * keeps the fastest for the handle's lifetime.  Fixed ranks only (no host round trips inside).
+ * keeps the fastest for the handle's lifetime.  Fixed ranks, widths and power iterations only (no host
+ * round trips inside): DELTA / LEARNED rank modes, TTX_FLAG_ADAPTIVE and TTX_FLAG_TIMING are refused
+ * with TTX_ERR_UNSUPPORTED.  A captured graph is relaunched only while no worker buffer has been
+ * reallocated since its capture (a larger batch on the same handle triggers a new capture).
  * dims: count x order, ranks: count x (order + 1) (r_0 = r_d = 1), d_cores: count x order device
  * pointers (ttx_core_size of each tensor's parameters). */
 #define TTX_BATCH_GRAPH 1

@@ -55,6 +55,14 @@ float ttdec_gpu_float(float* h_tensor, int* h_n_array, int* h_r_array, double* e
 float trdec_gpu_float(float* h_tensor, int* h_n_array, int* h_r_array, double* eps, float* singleError,
                       float* singleComRatio);
 
+/* The harness's input generators (ttdecomposition/high_order_GPUs/ttdec6_singleGPU.cu:185-271 in
+ * libttx_ttdec6.so, trdecomposition/high_order_GPUs/trdec6_singleGPU.cu:231-329 in libttx_trdec6.so):
+ * fill the caller's host h_A (six modes ttDemns, ranks ttRanks) with the reference's tensor, cores from
+ * glibc's rand() after srand(1134) in the order G5, G6, G4, G3, G2, G1 (ttx_tt_generate_ref /
+ * ttx_tr_generate_ref); p is the reference's GEMM slice count and is not needed. */
+void generalTTTensor(float* h_A, const int* ttRanks, const int* ttDemns, const int& p);
+void generalTRTensor(float* h_A, const int* ttRanks, const int* ttDemns, const int& p);
+
 extern "C" {
 /* Side channel for the cores the reference discards (ttdec6_singleGPU.cu:947-952): device pointers
  * of the last call's d cores (TT or TR layouts, ttx.h) and its ranks (d + 1 entries); valid until

@@ -113,6 +113,8 @@ SIGNATURES = {
     "ttx_omega": (_I32, [_P, _P, _I64, _I32, _U64, _U32]),
     "ttx_tt_generate": (_I32, [_P, _I32, ctypes.POINTER(_I64), ctypes.POINTER(_I32), _F, _F, _U64, _P, _FPP]),
     "ttx_tr_generate": (_I32, [_P, _I32, ctypes.POINTER(_I64), ctypes.POINTER(_I32), _F, _F, _U64, _P, _FPP]),
+    "ttx_tt_generate_ref": (_I32, [_P, _I32, ctypes.POINTER(_I64), ctypes.POINTER(_I32), ctypes.c_uint32, _P, _FPP]),
+    "ttx_tr_generate_ref": (_I32, [_P, _I32, ctypes.POINTER(_I64), ctypes.POINTER(_I32), ctypes.c_uint32, _P, _FPP]),
     "ttx_tt_rel_error": (_I32, [_P, _I32, ctypes.POINTER(_I64), ctypes.POINTER(_I32), _FPP, _P,
                                 ctypes.POINTER(_D)]),
     "ttx_tr_rel_error": (_I32, [_P, _I32, ctypes.POINTER(_I64), ctypes.POINTER(_I32), _FPP, _P,

@@ -32,12 +32,17 @@ def _check(lib, h, status):
     return status
 
 
-def _dev_ptr(t):
+def _dev_ptr(t, flat=True):
+    """Device pointer of a contiguous float32 CUDA tensor.  flat: the buffer is a column-major tensor
+    or matrix, passed as a 1-d tensor (a C-contiguous N-d tensor would be read transposed); the rank
+    model's row-major batches (flat=False) may be 2-d."""
     torch = _torch()
     if not isinstance(t, torch.Tensor) or not t.is_cuda:
         raise TypeError("ttx: expected a CUDA tensor")
     if t.dtype != torch.float32 or not t.is_contiguous():
         raise TypeError("ttx: expected a contiguous float32 tensor")
+    if flat and t.dim() != 1:
+        raise TypeError("ttx: expected a flat (1-d) tensor holding column-major data")
     return t.data_ptr()
 
 
@@ -53,6 +58,8 @@ def _host_ptr(t):
         raise TypeError("ttx: expected a host (CPU) tensor")
     if t.dtype != torch.float32 or not t.is_contiguous():
         raise TypeError("ttx: expected a contiguous float32 tensor")
+    if t.dim() != 1:
+        raise TypeError("ttx: expected a flat (1-d) tensor holding column-major data")
     return t.data_ptr()
 
 
@@ -133,7 +140,7 @@ class Engine:
     def mlp_forward(self, X):
         torch = _torch()
         Y = torch.empty(X.shape[0] * L.MLP_OUT, device=X.device, dtype=torch.float32)
-        _check(self.lib, self.h, self.lib.ttx_mlp_forward(self.h, ctypes.c_void_p(_dev_ptr(X)), X.shape[0],
+        _check(self.lib, self.h, self.lib.ttx_mlp_forward(self.h, ctypes.c_void_p(_dev_ptr(X, flat=False)), X.shape[0],
                                                           ctypes.c_void_p(Y.data_ptr())))
         return Y.view(X.shape[0], L.MLP_OUT)
 
@@ -141,15 +148,15 @@ class Engine:
         torch = _torch()
         g = torch.empty(L.MLP_PARAMS, device=X.device, dtype=torch.float32)
         loss = ctypes.c_double(0.0)
-        _check(self.lib, self.h, self.lib.ttx_mlp_gradient(self.h, ctypes.c_void_p(_dev_ptr(X)),
-                                                           ctypes.c_void_p(_dev_ptr(T)), X.shape[0],
+        _check(self.lib, self.h, self.lib.ttx_mlp_gradient(self.h, ctypes.c_void_p(_dev_ptr(X, flat=False)),
+                                                           ctypes.c_void_p(_dev_ptr(T, flat=False)), X.shape[0],
                                                            ctypes.c_void_p(g.data_ptr()), ctypes.byref(loss)))
         return g, float(loss.value)
 
     def mlp_train_step(self, X, T, lr, want_loss=False):
         loss = ctypes.c_double(0.0)
-        _check(self.lib, self.h, self.lib.ttx_mlp_train_step(self.h, ctypes.c_void_p(_dev_ptr(X)),
-                                                             ctypes.c_void_p(_dev_ptr(T)), X.shape[0], float(lr),
+        _check(self.lib, self.h, self.lib.ttx_mlp_train_step(self.h, ctypes.c_void_p(_dev_ptr(X, flat=False)),
+                                                             ctypes.c_void_p(_dev_ptr(T, flat=False)), X.shape[0], float(lr),
                                                              ctypes.byref(loss) if want_loss else None))
         return float(loss.value) if want_loss else None
 
@@ -373,6 +380,24 @@ class Engine:
         st = fn(self.h, d, L.i64_array(dims), L.i32_array(ranks), std, noise_rel, seed,
                 ctypes.c_void_p(A.data_ptr()), cptr)
         _check(self.lib, self.h, st)
+        return (A, cores) if return_cores else A
+
+    def generate_ref(self, dims, ranks, seed=1134, tr=False, return_cores=False, device="cuda"):
+        """The reference harness's input (generalTTTensor / generalTRTensor, ttdec6_singleGPU.cu:185-271,
+        trdec6_singleGPU.cu:231-329): U[-1, 1] cores from glibc's rand() after srand(seed), drawn in the
+        order G_{d-1}, G_d, G_{d-2}, ..., G_1, contracted on the device."""
+        torch = _torch()
+        d = len(dims)
+        A = torch.empty(int(np.prod(dims)), device=device, dtype=torch.float32)
+        cores, cptr = None, None
+        if return_cores:
+            p = self.params(dims, ranks)
+            cores = [torch.empty(int(self.lib.ttx_core_size(ctypes.byref(p), int(tr), k)), device=device,
+                                 dtype=torch.float32) for k in range(d)]
+            cptr = L.ptr_array([c.data_ptr() for c in cores])
+        fn = self.lib.ttx_tr_generate_ref if tr else self.lib.ttx_tt_generate_ref
+        _check(self.lib, self.h, fn(self.h, d, L.i64_array(dims), L.i32_array(ranks), int(seed),
+                                    ctypes.c_void_p(A.data_ptr()), cptr))
         return (A, cores) if return_cores else A
 
     def rel_error(self, cores, dims, ranks, A, tr=False):

@@ -157,6 +157,38 @@ static float run3(const float* h_tensor, const int* h_n_array, const int* h_r_ar
   return sec;
 }
 
+// the harness's input generators (generalTTTensor / generalTRTensor): the six-mode tensor of cores
+// drawn from glibc's rand() after srand(1134) (the reference seeds inside the generator), contracted
+// on the device, copied into the caller's host buffer h_A; p (the reference's GEMM slice count) is
+// not needed
+static void gen_ref(float* h_A, const int* ttRanks, const int* ttDemns, bool tr) {
+  if (!h_A || !ttRanks || !ttDemns) {
+    std::fprintf(stderr, "ttx shim: bad arguments\n");
+    return;
+  }
+  if (!g.h && ttx_create(&g.h, nullptr) != TTX_OK) {
+    std::fprintf(stderr, "ttx shim: ttx_create failed\n");
+    return;
+  }
+  int64_t dims[6];
+  int32_t ranks[7];
+  int64_t total = 1;
+  for (int k = 0; k < 6; ++k) total *= (dims[k] = ttDemns[k]);
+  for (int k = 0; k <= 6; ++k) ranks[k] = ttRanks[k];
+  float* d_A = nullptr;
+  if (hipMalloc(&d_A, (size_t)total * sizeof(float)) != hipSuccess) {
+    std::fprintf(stderr, "ttx shim: out of device memory for the tensor\n");
+    return;
+  }
+  const ttx_status st = tr ? ttx_tr_generate_ref(g.h, 6, dims, ranks, 1134u, d_A, nullptr)
+                           : ttx_tt_generate_ref(g.h, 6, dims, ranks, 1134u, d_A, nullptr);
+  if (st != TTX_OK)
+    std::fprintf(stderr, "ttx shim: %s: %s\n", ttx_status_string(st), ttx_last_error(g.h));
+  else if (hipMemcpy(h_A, d_A, (size_t)total * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess)
+    std::fprintf(stderr, "ttx shim: copy of the generated tensor failed\n");
+  (void)hipFree(d_A);
+}
+
 }  // namespace ttx_shim
 
 extern "C" int32_t ttx_last_cores(float* const** d_cores, int32_t* ranks) {

@@ -22,3 +22,9 @@ float ttdec_gpu_float(float* h_tensor, int* h_n_array, int* h_r_array, double* e
   (void)eps;
   return ttx_shim::run3(h_tensor, h_n_array, h_r_array, singleError, singleComRatio, false);
 }
+
+// the harness's input generator with the reference's name and C++ signature (ttdecomposition/high_order_GPUs/ttdec6_singleGPU.cu:185)
+void generalTTTensor(float* h_A, const int* ttRanks, const int* ttDemns, const int& p) {
+  (void)p;
+  ttx_shim::gen_ref(h_A, ttRanks, ttDemns, false);
+}

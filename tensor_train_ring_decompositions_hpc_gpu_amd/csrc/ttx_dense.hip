@@ -385,7 +385,8 @@ int64_t panel_blocks(int64_t rows) { return (rows + RB64 - 1) / RB64; }
 // A <- J^T A J is applied on 2 x 2 blocks, one block per thread.  V <- V J of a round runs while the
 // next round's angles are computed (the pair tables are double-buffered), off the critical path.
 // The sweep loop ends when the off-diagonal mass after a sweep is at most 0.5e-24 ||A||_F^2 (the
-// last round's blocks tile the matrix, so it comes from registers).  status[1] = sweeps.
+// last round's blocks tile the matrix, so it comes from registers).  status[1] = sweeps, status[2] = 1
+// when the sweep cap (max_sweeps) was reached before that test passed, else 0.
 // keys[0..63] (optional) are zeroed for the core assembly that follows.
 // ------------------------------------------------------------------------------------------------
 __device__ __forceinline__ void jacobi_angle(double app, double aqq, double apq, double& c, double& s) {
@@ -409,7 +410,7 @@ __device__ __forceinline__ void jacobi_angle(double app, double aqq, double apq,
 
 __global__ __launch_bounds__(1024) void k_syevj(const double* __restrict__ G, int k, double* __restrict__ lam,
                                                double* __restrict__ Vout, int* __restrict__ status,
-                                               unsigned long long* __restrict__ keys) {
+                                               unsigned long long* __restrict__ keys, int max_sweeps) {
   constexpr int NT = 1024, NW = NT / 64;
   __shared__ double a[KMAX][KMAX + 1];
   __shared__ double v[KMAX][KMAX + 1];
@@ -462,7 +463,8 @@ __global__ __launch_bounds__(1024) void k_syevj(const double* __restrict__ G, in
       }
   };
   int sweeps = 0, gr = 0;   // gr: rounds so far (table parity)
-  while (sweeps < 30 && tot > 0.0) {
+  bool converged = !(tot > 0.0);   // a zero matrix is diagonal
+  while (sweeps < max_sweeps && tot > 0.0) {
     ++sweeps;
     double off = 0.0;
     for (int rnd = 0; rnd < M; ++rnd, ++gr) {
@@ -511,7 +513,7 @@ __global__ __launch_bounds__(1024) void k_syevj(const double* __restrict__ G, in
       done = sum <= 0.5e-24 * tot;
     }
     __syncthreads();
-    if (done) break;
+    if (done) { converged = true; break; }
   }
   if (gr > 0) {
     v_apply((gr - 1) & 1);
@@ -528,12 +530,15 @@ __global__ __launch_bounds__(1024) void k_syevj(const double* __restrict__ G, in
     lam[rank] = li;
     for (int r = 0; r < k; ++r) Vout[r * k + rank] = v[r][i];
   }
-  if (tid == 0 && status) status[1] = sweeps;
+  if (tid == 0 && status) {
+    status[1] = sweeps;
+    status[2] = converged ? 0 : 1;
+  }
 }
 
 void launch_syevj(const double* G, int k, double* lam, double* V, int* status, unsigned long long* keys,
-                  hipStream_t st) {
-  hipLaunchKernelGGL(k_syevj, dim3(1), dim3(1024), 0, st, G, k, lam, V, status, keys);
+                  hipStream_t st, int max_sweeps) {
+  hipLaunchKernelGGL(k_syevj, dim3(1), dim3(1024), 0, st, G, k, lam, V, status, keys, max_sweeps);
 }
 
 // ------------------------------------------------------------------------------------------------

@@ -7,6 +7,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <string>
@@ -55,6 +56,10 @@ struct Handle {
   DevBuf f16s;                    // FP16 operand scales (65 floats) and column-max words (64)
   bool mlp_ready = false;
   int mlp_t = 0;
+  int jacobi_max_sweeps = kJacobiMaxSweeps;   // sweep cap of the eigensolver (TTX_JACOBI_MAX_SWEEPS)
+  // Omega stream of a sweep's step k is stream_base + k: the host-streamed sweep runs its steps 2..d-1
+  // as a sweep of its own with base 1, so they draw the same sketches as the device sweep's
+  uint32_t stream_base = 0;
   int* h_status = nullptr;       // pinned
   float* h_sigma = nullptr;      // pinned, TTX_MAX_ELL
   double* h_lam = nullptr;       // pinned
@@ -80,6 +85,10 @@ struct Handle {
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
   uint64_t gsig = 0;
+  // workspace generation: bumped whenever ensure() (re)allocates a buffer of this handle, so a graph
+  // captured over the workers' buffers is relaunched only while none of them has moved
+  uint64_t wsgen = 0;
+  uint64_t gws = 0;   // the workers' combined generation when gexec was captured
 };
 
 static hipEvent_t next_event(Handle* h) {
@@ -130,6 +139,7 @@ static ttx_status ensure(Handle* h, DevBuf& b, size_t bytes) {
     b.bytes = 0;
   }
   size_t want = bytes < 256 ? 256 : bytes;
+  ++h->wsgen;
   if (hipMalloc(&b.p, want) != hipSuccess) {
     (void)hipGetLastError();
     return fail(h, TTX_ERR_OOM, "hipMalloc of " + std::to_string(want) + " bytes failed");
@@ -249,6 +259,7 @@ struct StepArgs {
 };
 struct StepOut {
   int r = 0, ell = 0, path = 0, power_iters = 0, zero_pivots = 0, sweeps = 0, kdim = 0;
+  int nonconv = 0;         // the eigensolver reached its sweep cap before converging
   double kept2 = 0.0;      // sum of kept sigma^2 = ||A_next||_F^2
   double total2 = -1.0;    // ||A||_F^2 when the step knows it
   int64_t bytes = 0;
@@ -276,6 +287,7 @@ static void finish_step_out(const float* sig, const int* stat, StepOut& o, bool 
   }
   o.zero_pivots = stat[0];
   o.sweeps = stat[1];
+  o.nonconv = stat[2];
 }
 
 // rank of the delta rule: the smallest r with tail(r) <= delta^2 (Oseledets 2011, TT-SVD; the
@@ -340,7 +352,7 @@ static ttx_status step_tail(Handle* h, const StepArgs& a, const Arith3& ar, int 
     KScope ks(h, TTX_K_SMALL, 0);
     launch_reduce_f64(P<double>(h->gpart), gg, (int64_t)k * k, P<double>(h->G), st);
     launch_syevj(P<double>(h->G), k, P<double>(h->lam), P<double>(h->V), a.stat_dev,
-                 P<unsigned long long>(h->keys), st);
+                 P<unsigned long long>(h->keys), st, h->jacobi_max_sweeps);
   }
   TTX_TRY(check_launch(h, "step tail"));
   // rank decision on the host when needed
@@ -710,7 +722,7 @@ static ttx_status run_step(Handle* h, const StepArgs& a_in, StepOut& o) {
         launch_reduce_f64(P<double>(h->gpart), grid, (int64_t)m * m, P<double>(h->G), st);
       }
       launch_syevj(P<double>(h->G), m, P<double>(h->lam), P<double>(h->V), a.stat_dev,
-                   P<unsigned long long>(h->keys), st);
+                   P<unsigned long long>(h->keys), st, h->jacobi_max_sweeps);
     }
     TTX_TRY(check_launch(h, "exact gram"));
     o.bytes += a.m * n * 4;
@@ -1007,11 +1019,12 @@ static ttx_status sweep(Handle* h, const float* d_A, const ttx_params* p, float*
         TTX_TRY(ensure(h, h->omg, (size_t)omega_chunks_bytes(n1)));
         {
           KScope ks(h, TTX_K_SMALL, 0);
-          launch_omega_chunks(p->omega_seed, 1u, n1, 64, h->omg.p, st);
+          launch_omega_chunks(p->omega_seed, h->stream_base + 1u, n1, 64, h->omg.p, st);
         }
         {
           KScope ks(h, TTX_K_SKETCH, m1 * n1 * 4);
-          launch_sketch_gram0(sp, d_A, m1, (int)m1, n1, p->omega_seed, 1u, 64, P<float>(h->part), P<double>(h->gpart),
+          launch_sketch_gram0(sp, d_A, m1, (int)m1, n1, p->omega_seed, h->stream_base + 1u, 64, P<float>(h->part),
+                              P<double>(h->gpart),
                               st, h->omg.p, (p->flags & TTX_FLAG_TAIL_X3) != 0);
         }
         {
@@ -1025,7 +1038,7 @@ static ttx_status sweep(Handle* h, const float* d_A, const ttx_params* p, float*
         h->pre_m = m1;
         h->pre_n = n1;
         h->pre_seed = p->omega_seed;
-        h->pre_stream = 1u;
+        h->pre_stream = h->stream_base + 1u;
       }
     }
   }
@@ -1044,7 +1057,7 @@ static ttx_status sweep(Handle* h, const float* d_A, const ttx_params* p, float*
     a.s = p->oversampling;
     a.q = p->power_iters;
     a.seed = p->omega_seed;
-    a.stream_id = (uint32_t)k;
+    a.stream_id = h->stream_base + (uint32_t)k;
     a.precision = p->precision;
     a.method = p->method;
     a.flags = p->flags;
@@ -1194,6 +1207,10 @@ static ttx_status sweep(Handle* h, const float* d_A, const ttx_params* p, float*
   h->ktime = false;
   h->krecs.clear();
   if (rep) *rep = R;
+  for (int k = 0; k < d - 1; ++k)
+    if (outs[k].nonconv)
+      return fail(h, TTX_ERR_NO_CONVERGE, "step " + std::to_string(k) + ": the Jacobi eigensolver reached its cap of " +
+                                              std::to_string(h->jacobi_max_sweeps) + " sweeps before converging");
   return TTX_OK;
 }
 
@@ -1289,7 +1306,8 @@ static ttx_status sweep_host(Handle* h, const float* h_A, const ttx_params* p, i
   }));
   launch_reduce_f64(Gs, S, (int64_t)m * m, P<double>(h->G), st);
   int* stat = P<int>(h->status);
-  launch_syevj(P<double>(h->G), m, P<double>(h->lam), P<double>(h->V), stat, P<unsigned long long>(h->keys), st);
+  launch_syevj(P<double>(h->G), m, P<double>(h->lam), P<double>(h->V), stat, P<unsigned long long>(h->keys), st,
+               h->jacobi_max_sweeps);
   TTX_TRY(check_launch(h, "host-streamed eig"));
   TTX_HIP(h, hipMemcpyAsync(h->h_lam, h->lam.p, m * sizeof(double), hipMemcpyDeviceToHost, st));
   TTX_HIP(h, hipStreamSynchronize(st));
@@ -1328,6 +1346,9 @@ static ttx_status sweep_host(Handle* h, const float* h_A, const ttx_params* p, i
   TTX_HIP(h, hipMemcpyAsync(sig0, h->sig.p, sizeof(sig0), hipMemcpyDeviceToHost, st));
   TTX_HIP(h, hipMemcpyAsync(st0, h->status.p, sizeof(st0), hipMemcpyDeviceToHost, st));
   TTX_HIP(h, hipStreamSynchronize(st));
+  if (st0[2])
+    return fail(h, TTX_ERR_NO_CONVERGE, "step 0: the Jacobi eigensolver reached its cap of " +
+                                            std::to_string(h->jacobi_max_sweeps) + " sweeps before converging");
   // the rest of the sweep on B seen as an order d - 1 tensor (r N_2) x N_3 x ... x N_d: its first
   // core (r N_2) x r_2 is the TT core G_2 in place, its later cores G_3 .. G_d.  TR: B relaid out
   // (r_0, r_1, n) -> (r_1, n, r_0) (trdec6_singleGPU.cu:865-877), then the order d - 1 tensor
@@ -1354,6 +1375,11 @@ static ttx_status sweep_host(Handle* h, const float* h_A, const ttx_params* p, i
   }
   ttx_report sub;
   float* sub_sigma = h_sigma ? h_sigma + TTX_MAX_ELL : nullptr;
+  struct BaseGuard {
+    Handle* h;
+    ~BaseGuard() { h->stream_base = 0; }
+  } base_guard{h};
+  h->stream_base = 1;   // the device sweep's step k + 1 draws Omega stream k + 1
   TTX_TRY(sweep(h, Bsub, &q, d_cores + 1, sub_sigma, &sub, false));
   if (h_sigma) memcpy(h_sigma, sig0, sizeof(sig0));
   if (rep) {
@@ -1492,8 +1518,66 @@ static int64_t core_elems(int d, const int64_t* dims, const int32_t* ranks, bool
   return (int64_t)ranks[k] * dims[k] * ranks[k + 1];
 }
 
+// glibc's rand() after srand(seed) (the generator of the reference's inputs, generalTTTensor /
+// generalTRTensor, ttdec6_singleGPU.cu:185-271, trdec6_singleGPU.cu:231-329): srandom_r's Park-Miller
+// sequence r_0 = seed, r_i = 16807 r_{i-1} mod (2^31 - 1) for i < 31, then r_i = r_{i-31} for
+// i = 31..33 and r_i = r_{i-3} + r_{i-31} mod 2^32 beyond (random_r, TYPE_3 additive feedback);
+// output k is r_{k+344} >> 1 (the first 310 outputs are discarded by srandom_r).
+class GlibcRand {
+ public:
+  explicit GlibcRand(uint32_t seed) {
+    int64_t w = seed == 0 ? 1 : (int32_t)seed;
+    s_[0] = (uint32_t)w;
+    for (int i = 1; i < 31; ++i) {
+      // Schrage's form of 16807 w mod (2^31 - 1), as srandom_r computes it on signed 32-bit words
+      const int64_t hi = w / 127773, lo = w % 127773;
+      w = 16807 * lo - 2836 * hi;
+      if (w < 0) w += 2147483647;
+      s_[i] = (uint32_t)w;
+    }
+    i_ = 31;
+    for (int k = 0; k < 310; ++k) (void)next();
+  }
+  // RAND_MAX = 2^31 - 1
+  uint32_t next() {
+    const int slot = i_ % 31;
+    if (i_ >= 34) s_[slot] += s_[(i_ - 3) % 31];   // i = 31..33 keep r_{i-31}
+    ++i_;
+    return s_[slot] >> 1;
+  }
+
+ private:
+  uint32_t s_[31];
+  int64_t i_;
+};
+
+// entry e of a reference core: (float)(rand() / RAND_MAX) * 2.0 - 1.0, rounded to float
+// (the reference's expression: the quotient is cast to float before the affine map in double)
+static float glibc_uniform(GlibcRand& g) {
+  const float u = (float)((double)g.next() / 2147483647.0);
+  return (float)((double)u * 2.0 - 1.0);
+}
+
+// cores from glibc's rand() in the reference's order: G_{d-1}, G_d, then G_{d-2} down to G_1
+// (order 6: G5, G6, G4, G3, G2, G1), each filled in storage order
+static void glibc_cores(int d, const std::vector<int64_t>& sz, uint32_t seed, std::vector<std::vector<float>>& out) {
+  GlibcRand g(seed);
+  out.assign(d, {});
+  std::vector<int> order;
+  if (d >= 2) {
+    order.push_back(d - 2);
+    order.push_back(d - 1);
+    for (int k = d - 3; k >= 0; --k) order.push_back(k);
+  }
+  for (int k : order) {
+    out[k].resize((size_t)sz[k]);
+    for (int64_t e = 0; e < sz[k]; ++e) out[k][(size_t)e] = glibc_uniform(g);
+  }
+}
+
+// source: 0 = Philox normal cores (std_, noise_rel), 1 = the reference's glibc uniform cores (no noise)
 static ttx_status generate(Handle* h, int d, const int64_t* dims, const int32_t* ranks, float std_, float noise_rel,
-                           uint64_t seed, float* A, float* const* d_cores, bool tr) {
+                           uint64_t seed, float* A, float* const* d_cores, bool tr, int source = 0) {
   if (d < 2 || d > TTX_MAX_ORDER || !A) return fail(h, TTX_ERR_BAD_ARG, "generate: bad order or null output");
   for (int k = 0; k < d; ++k)
     if (dims[k] <= 0) return fail(h, TTX_ERR_BAD_ARG, "generate: bad mode size");
@@ -1513,8 +1597,28 @@ static ttx_status generate(Handle* h, int d, const int64_t* dims, const int32_t*
   } else {
     for (int k = 0; k < d; ++k) cp[k] = d_cores[k];
   }
-  for (int k = 0; k < d; ++k)
-    launch_fill_normal(cp[k], sz[k], seed, (uint32_t)((tr ? 0x200 : 0x100) + k), std_, h->stream);
+  if (source == 1) {
+    std::vector<std::vector<float>> hc;
+    glibc_cores(d, sz, (uint32_t)seed, hc);
+    for (int k = 0; k < d; ++k) {
+      const hipError_t e = hipMemcpyAsync(cp[k], hc[k].data(), (size_t)sz[k] * sizeof(float), hipMemcpyHostToDevice,
+                                          h->stream);
+      if (e != hipSuccess) {
+        if (cb.p) (void)hipFree(cb.p);
+        return fail(h, TTX_ERR_HIP, std::string("generate: core copy: ") + hipGetErrorString(e));
+      }
+    }
+    // the pageable sources must outlive the copies
+    const hipError_t e = hipStreamSynchronize(h->stream);
+    if (e != hipSuccess) {
+      if (cb.p) (void)hipFree(cb.p);
+      return fail(h, TTX_ERR_HIP, std::string("generate: core copy: ") + hipGetErrorString(e));
+    }
+    noise_rel = 0.0f;
+  } else {
+    for (int k = 0; k < d; ++k)
+      launch_fill_normal(cp[k], sz[k], seed, (uint32_t)((tr ? 0x200 : 0x100) + k), std_, h->stream);
+  }
   ttx_status s = full(h, d, dims, ranks, cp.data(), A, tr);
   if (s == TTX_OK && noise_rel != 0.0f) {
     int64_t total = 1;
@@ -1581,6 +1685,11 @@ ttx_status ttx_create(ttx_handle* out, void* stream) {
     return TTX_ERR_HIP;
   }
   (void)ttx::num_cus();
+  // TTX_JACOBI_MAX_SWEEPS (1..30): a lower sweep cap, so callers and tests can see TTX_ERR_NO_CONVERGE
+  if (const char* js = getenv("TTX_JACOBI_MAX_SWEEPS")) {
+    const int v = atoi(js);
+    if (v >= 1 && v <= ttx::kJacobiMaxSweeps) h->jacobi_max_sweeps = v;
+  }
   *out = reinterpret_cast<ttx_handle>(h);
   return TTX_OK;
 }
@@ -1693,6 +1802,7 @@ ttx_status ttx_step(ttx_handle hh, const float* d_A, int64_t m, int64_t n, int32
   if (st != TTX_OK) return st;
   if (h_sigma) memcpy(h_sigma, o.sigma, sizeof(float) * TTX_MAX_ELL);
   if (path_out) *path_out = o.path;
+  if (o.nonconv) return ttx::fail(h, TTX_ERR_NO_CONVERGE, "ttx_step: the Jacobi eigensolver did not converge");
   return TTX_OK;
 }
 
@@ -1714,6 +1824,7 @@ ttx_status ttx_rsvd(ttx_handle hh, const float* d_A, int64_t m, int64_t n, int32
   ttx::StepOut o;
   ttx_status st = ttx::run_step(h, a, o);
   if (st != TTX_OK) return st;
+  if (o.nonconv) return ttx::fail(h, TTX_ERR_NO_CONVERGE, "ttx_rsvd: the Jacobi eigensolver did not converge");
   if (d_S) {
     hipError_t e = hipMemcpyAsync(d_S, o.sigma, ell * sizeof(float), hipMemcpyHostToDevice, h->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
@@ -1765,6 +1876,20 @@ ttx_status ttx_tr_generate(ttx_handle h, int32_t order, const int64_t* dims, con
   if (!h || !dims || !ranks) return TTX_ERR_BAD_ARG;
   if (ranks[0] != ranks[order]) { H(h)->err = "TR ranks need r_0 = r_d"; return TTX_ERR_BAD_ARG; }
   return ttx::generate(H(h), order, dims, ranks, std_, noise_rel, seed, d_A, d_cores, true);
+}
+
+ttx_status ttx_tt_generate_ref(ttx_handle h, int32_t order, const int64_t* dims, const int32_t* ranks, uint32_t seed,
+                               float* d_A, float* const* d_cores) {
+  if (!h || !dims || !ranks) return TTX_ERR_BAD_ARG;
+  if (ranks[0] != 1 || ranks[order] != 1) { H(h)->err = "TT ranks need r_0 = r_d = 1"; return TTX_ERR_BAD_ARG; }
+  return ttx::generate(H(h), order, dims, ranks, 1.0f, 0.0f, seed, d_A, d_cores, false, 1);
+}
+
+ttx_status ttx_tr_generate_ref(ttx_handle h, int32_t order, const int64_t* dims, const int32_t* ranks, uint32_t seed,
+                               float* d_A, float* const* d_cores) {
+  if (!h || !dims || !ranks) return TTX_ERR_BAD_ARG;
+  if (ranks[0] != ranks[order]) { H(h)->err = "TR ranks need r_0 = r_d"; return TTX_ERR_BAD_ARG; }
+  return ttx::generate(H(h), order, dims, ranks, 1.0f, 0.0f, seed, d_A, d_cores, true, 1);
 }
 
 ttx_status ttx_tt_rel_error(ttx_handle h, int32_t order, const int64_t* dims, const int32_t* ranks,
@@ -1973,6 +2098,7 @@ static ttx_status make_subs(Handle* h, int S) {
       return fail(h, TTX_ERR_HIP, "cannot create a batch worker stream");
     }
     s->tune = &tune_table(h);
+    s->jacobi_max_sweeps = h->jacobi_max_sweeps;
     h->subs.push_back(s);
   }
   return TTX_OK;
@@ -2016,6 +2142,10 @@ static ttx_status batch_run(Handle* h, int count, int order, const int64_t* dims
   if (S < 1 || S > 16) return fail(h, TTX_ERR_BAD_ARG, "batch: 1..16 streams");
   if (common->rank_mode != TTX_RANK_FIXED) return fail(h, TTX_ERR_UNSUPPORTED, "batch: fixed ranks only");
   if (common->flags & TTX_FLAG_TIMING) return fail(h, TTX_ERR_UNSUPPORTED, "batch: no per-kernel timing");
+  // the range check reads its residual back to the host and chooses widths and power iterations from
+  // it: a host round trip per check, which a captured graph could neither contain nor replay
+  if (common->flags & TTX_FLAG_ADAPTIVE)
+    return fail(h, TTX_ERR_UNSUPPORTED, "batch: no adaptive range finder (fixed widths and power iterations only)");
   if ((bflags & TTX_BATCH_GRAPH) && count > TTX_BATCH_GRAPH_MAX)
     return fail(h, TTX_ERR_UNSUPPORTED, "batch: graph capture takes at most " + std::to_string(TTX_BATCH_GRAPH_MAX) +
                                             " tensors; split the batch or run it without TTX_BATCH_GRAPH");
@@ -2049,7 +2179,14 @@ static ttx_status batch_run(Handle* h, int count, int order, const int64_t* dims
   sig = fnv(sig, &P[0].omega_seed, sizeof(uint64_t));
   sig = fnv(sig, &P[0].precision, sizeof(int32_t) * 2);
   sig = fnv(sig, &P[0].flags, sizeof(int32_t));
-  const bool reuse = graph && h->gexec && h->gsig == sig;
+  // the workers' buffers the graph's kernels were captured with: any reallocation since (a larger
+  // batch, the per-tensor timing loop) invalidates it
+  auto workspace_gen = [&]() {
+    uint64_t g = 0xCBF29CE484222325ull;
+    for (int s = 0; s < S; ++s) g = fnv(g, &h->subs[s]->wsgen, sizeof(uint64_t));
+    return g;
+  };
+  const bool reuse = graph && h->gexec && h->gsig == sig && h->gws == workspace_gen();
 
   hipEvent_t fork = nullptr, e0 = nullptr, e1 = nullptr;
   std::vector<hipEvent_t> joins(S, nullptr);
@@ -2105,6 +2242,7 @@ static ttx_status batch_run(Handle* h, int count, int order, const int64_t* dims
           break;
         }
         h->gsig = sig;
+        h->gws = workspace_gen();
       }
     }
     // the measured run

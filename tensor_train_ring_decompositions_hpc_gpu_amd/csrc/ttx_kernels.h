@@ -84,9 +84,11 @@ void launch_apply_out(const double* Win, int64_t rows, int k, int kp, const doub
 void launch_apply_ns_out(const double* Win, int64_t rows, int k, int kp, const double* gpart, int64_t S, float* Qc,
                          int64_t ldq, hipStream_t st);
 // symmetric eigen-decomposition (cyclic Jacobi, FP64): lam descending, V row-major k x k (columns are
-// eigenvectors); status[1] = sweeps; keys[0..63] are zeroed for the following assembly
+// eigenvectors); status[1] = sweeps, status[2] = 1 when max_sweeps ran out before convergence;
+// keys[0..63] are zeroed for the following assembly
+constexpr int kJacobiMaxSweeps = 30;
 void launch_syevj(const double* G, int k, double* lam, double* V, int* status, unsigned long long* keys,
-                  hipStream_t st);
+                  hipStream_t st, int max_sweeps = kJacobiMaxSweeps);
 // core assembly after the eigen-decomposition of the projected Gram:
 //   Gc = Q V[:, :r] (m x r column-major, FP32) with the sign of each column fixed so that its largest
 //   |entry| is positive (first index on ties); Ur = V[:, :r] diag(sign) (row-major FP32, round4(k) rows

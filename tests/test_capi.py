@@ -57,6 +57,7 @@ def test_status_strings():
     lib = L.load()
     assert lib.ttx_status_string(0) == b"ok"
     assert lib.ttx_status_string(1) == b"bad argument"
+    assert lib.ttx_status_string(L.ERR_NO_CONVERGE) == b"no convergence"
     assert lib.ttx_status_string(99) == b"unknown status"
     assert lib.ttx_version() >= 100
 

@@ -85,3 +85,33 @@ def test_graph_capture_refuses_batches_above_its_tested_size(engine):
         engine.batch_tt_svd(tensors, dims_l, ranks_l, s=4, q=1, streams=2, graph=True, autotune=False)
     out, info = engine.batch_tt_svd(tensors, dims_l, ranks_l, s=4, q=1, streams=2, graph=False, autotune=False)
     assert info["graph"] == 0 and len(out) == 17
+
+
+def test_graph_not_relaunched_after_worker_buffers_moved(engine):
+    """A larger eager batch on the same handle reallocates the workers' buffers; the captured graph of
+    the first batch refers to the old ones, so the next identical graph batch must capture again
+    (graph_reused == 0) and still give the eager result bit for bit."""
+    small, dims_s, ranks_s = _batch(engine, 6, 11, dims_range=(8, 12), rank_range=(2, 4))
+    ref_out, _ = engine.batch_tt_svd(small, dims_s, ranks_s, s=4, q=1, streams=2, graph=False, autotune=False)
+    ref = [_host(c) for c in ref_out]
+    out, info = engine.batch_tt_svd(small, dims_s, ranks_s, s=4, q=1, streams=2, graph=True, autotune=False)
+    bufs = info["core_buffers"]
+    big, dims_b, ranks_b = _batch(engine, 4, 12, dims_range=(24, 32), rank_range=(8, 12))
+    engine.batch_tt_svd(big, dims_b, ranks_b, s=10, q=1, streams=2, graph=False, autotune=False)
+    out, info = engine.batch_tt_svd(small, dims_s, ranks_s, s=4, q=1, streams=2, graph=True, autotune=False,
+                                    cores=bufs)
+    assert info["graph"] == 1 and info["graph_reused"] == 0
+    for r, cores in zip(ref, out):
+        for a, b in zip(r, _host(cores)):
+            assert np.array_equal(a, b)
+
+
+def test_batch_refuses_adaptive_range_finder(engine):
+    """The range check reads its residual back to the host between passes: batches (eager or captured)
+    refuse TTX_FLAG_ADAPTIVE before queueing any work."""
+    from tensor_train_ring_decompositions_hpc_gpu_amd import _lib as L
+    tensors, dims_l, ranks_l = _batch(engine, 3, 13)
+    for graph in (False, True):
+        with pytest.raises(L.TtxError, match="adaptive"):
+            engine.batch_tt_svd(tensors, dims_l, ranks_l, s=4, q=1, streams=2, graph=graph, autotune=False,
+                                flags=L.FLAG_ADAPTIVE)

@@ -26,3 +26,12 @@ def test_refuses_what_would_be_misread():
         _host_ptr(torch.zeros(24, dtype=torch.float64))
     with pytest.raises(TypeError):
         _host_ptr(torch.zeros(4, 6, dtype=torch.float32).t())   # not contiguous
+
+
+def test_refuses_nd_torch_tensors():
+    """A C-contiguous N-d torch tensor would be read transposed as the column-major tensor: only flat
+    tensors pass (numpy arrays can say Fortran order; torch tensors cannot)."""
+    with pytest.raises(TypeError, match="flat"):
+        _host_ptr(torch.zeros(2, 3, 4, dtype=torch.float32))
+    with pytest.raises(TypeError, match="flat"):
+        _host_ptr(torch.zeros(4, 6, dtype=torch.float32))
