@@ -1535,13 +1535,14 @@ class GlibcRand {
       if (w < 0) w += 2147483647;
       s_[i] = (uint32_t)w;
     }
-    i_ = 31;
+    // r_31..r_33 repeat r_0..r_2 (they sit in the same slots); the additive outputs start at r_34
+    i_ = 34;
     for (int k = 0; k < 310; ++k) (void)next();
   }
   // RAND_MAX = 2^31 - 1
   uint32_t next() {
-    const int slot = i_ % 31;
-    if (i_ >= 34) s_[slot] += s_[(i_ - 3) % 31];   // i = 31..33 keep r_{i-31}
+    const int slot = (int)(i_ % 31);
+    s_[slot] += s_[(i_ - 3) % 31];   // r_i = r_{i-31} + r_{i-3}: slot i mod 31 holds r_{i-31}
     ++i_;
     return s_[slot] >> 1;
   }

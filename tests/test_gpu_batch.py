@@ -90,7 +90,19 @@ def test_graph_capture_refuses_batches_above_its_tested_size(engine):
 def test_graph_not_relaunched_after_worker_buffers_moved(engine):
     """A larger eager batch on the same handle reallocates the workers' buffers; the captured graph of
     the first batch refers to the old ones, so the next identical graph batch must capture again
-    (graph_reused == 0) and still give the eager result bit for bit."""
+    (graph_reused == 0) and still give the eager result bit for bit.  A fresh handle, so that the
+    workers' buffers start at the small batch's sizes."""
+    import torch
+    from tensor_train_ring_decompositions_hpc_gpu_amd.api import Engine
+    fresh = Engine()
+    try:
+        _graph_after_growth(fresh)
+    finally:
+        torch.cuda.synchronize()
+        fresh.close()
+
+
+def _graph_after_growth(engine):
     small, dims_s, ranks_s = _batch(engine, 6, 11, dims_range=(8, 12), rank_range=(2, 4))
     ref_out, _ = engine.batch_tt_svd(small, dims_s, ranks_s, s=4, q=1, streams=2, graph=False, autotune=False)
     ref = [_host(c) for c in ref_out]
@@ -101,6 +113,13 @@ def test_graph_not_relaunched_after_worker_buffers_moved(engine):
     out, info = engine.batch_tt_svd(small, dims_s, ranks_s, s=4, q=1, streams=2, graph=True, autotune=False,
                                     cores=bufs)
     assert info["graph"] == 1 and info["graph_reused"] == 0
+    for r, cores in zip(ref, out):
+        for a, b in zip(r, _host(cores)):
+            assert np.array_equal(a, b)
+    # with nothing reallocated since, the new capture is relaunched
+    out, info = engine.batch_tt_svd(small, dims_s, ranks_s, s=4, q=1, streams=2, graph=True, autotune=False,
+                                    cores=bufs)
+    assert info["graph_reused"] == 1
     for r, cores in zip(ref, out):
         for a, b in zip(r, _host(cores)):
             assert np.array_equal(a, b)
