@@ -137,6 +137,18 @@ int64_t ttx_core_size(const ttx_params* p, int32_t tr, int32_t k);
  * (d-1) x TTX_MAX_ELL singular values per step, descending, zero-filled. */
 ttx_status ttx_tt_svd(ttx_handle h, const float* d_A, const ttx_params* p, float* const* d_cores, float* h_sigma,
                       ttx_report* rep);
+/* No counterpart in the reference.  ttx_tt_svd for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc): every work buffer is a piece
+ * of one static device array of TTX_NOALLOC_SCRATCH_BYTES (256 MiB) that each GPU holds from library
+ * load until the process ends; no hipMalloc / hipFree is called.  The caller passes the tensor and the
+ * core buffers as for ttx_tt_svd, and gets the same results bit for bit.  Sizes whose workspace bound
+ * (every buffer request of every step counted separately) exceeds the array are refused with
+ * TTX_ERR_UNSUPPORTED before any work is queued; TTX_FLAG_TIMING is refused too.  Every call on a GPU
+ * shares that GPU's array and leaves its contents behind: calls that overlap on one GPU (two handles,
+ * two threads) race and must be serialised by the caller.  LEARNED ranks use the handle's loaded model. */
+#define TTX_NOALLOC_SCRATCH_BYTES (256u << 20)
+ttx_status ttx_tt_svd_noalloc(ttx_handle h, const float* d_A, const ttx_params* p, float* const* d_cores,
+                              float* h_sigma, ttx_report* rep);
 /* Replaces ttdec_half_lu_host_6 (ttdecomposition/high_order_GPUs/ttdec6_singleGPU.cu:656-796, its slice
  * loops :540-652): the tensor stays in host memory h_A (column-major; pinned, e.g. hipHostMalloc, for
  * copies that overlap the kernels).  The first step (exact Gram on the N_1 side, N_1 <= TTX_MAX_ELL)

@@ -11,6 +11,7 @@ LIB_PATH = os.path.join(HERE, "libttx.so")
 
 MAX_ORDER = 16
 MAX_ELL = 64
+NOALLOC_SCRATCH_BYTES = 256 << 20
 
 OK, ERR_BAD_ARG, ERR_OOM, ERR_NO_CONVERGE, RESIDUAL_FALLBACK_USED, ERR_HIP, ERR_UNSUPPORTED = range(7)
 PREC_FP32, PREC_BF16, PREC_FP16 = 0, 1, 2
@@ -102,6 +103,7 @@ SIGNATURES = {
     "ttx_params_init": (None, [ctypes.POINTER(Params), _I32, ctypes.POINTER(_I64), ctypes.POINTER(_I32)]),
     "ttx_core_size": (_I64, [ctypes.POINTER(Params), _I32, _I32]),
     "ttx_tt_svd": (_I32, [_P, _P, ctypes.POINTER(Params), _FPP, ctypes.POINTER(_F), ctypes.POINTER(Report)]),
+    "ttx_tt_svd_noalloc": (_I32, [_P, _P, ctypes.POINTER(Params), _FPP, ctypes.POINTER(_F), ctypes.POINTER(Report)]),
     "ttx_tr_svd": (_I32, [_P, _P, ctypes.POINTER(Params), _FPP, ctypes.POINTER(_F), ctypes.POINTER(Report)]),
     "ttx_tt_svd_host": (_I32, [_P, _P, ctypes.POINTER(Params), _I64, _FPP, ctypes.POINTER(_F), ctypes.POINTER(Report)]),
     "ttx_tr_svd_host": (_I32, [_P, _P, ctypes.POINTER(Params), _I64, _FPP, ctypes.POINTER(_F), ctypes.POINTER(Report)]),

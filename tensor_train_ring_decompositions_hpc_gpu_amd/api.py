@@ -265,6 +265,12 @@ class Engine:
         cores=core_buffers(...) (optional): write the cores there instead of new tensors."""
         return self._sweep(self.lib.ttx_tt_svd, A, dims, ranks, False, **kw)
 
+    def tt_svd_noalloc(self, A, dims, ranks, **kw):
+        """tt_svd without device allocation inside the call (ttx_tt_svd_noalloc): the engine's work
+        buffers come from its static scratch array; pass cores=core_buffers(...) so that nothing is
+        allocated here either.  Same results as tt_svd."""
+        return self._sweep(self.lib.ttx_tt_svd_noalloc, A, dims, ranks, False, **kw)
+
     def tt_svd_host(self, A, dims, ranks, slice_cols=0, **kw):
         """TT-SVD of a column-major tensor held in host memory (a contiguous float32 CPU torch tensor,
         pinned for overlapped copies, or a numpy array): the first step streams it in column slices

@@ -20,7 +20,7 @@ FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17",
          "-I" + os.path.join(ROOT, "include")]
 
 SOURCES = ["ttx_stream.hip", "ttx_dense.hip", "ttx_rng.hip", "ttx_gemm.hip", "ttx_permute.hip",
-           "ttx_mlp.hip", "ttx_engine.cpp"]
+           "ttx_mlp.hip", "ttx_noalloc.hip", "ttx_engine.cpp"]
 LIB = os.path.join(HERE, "libttx.so")
 SHIMS = {"libttx_ttdec6.so": "shim_ttdec6.cpp", "libttx_trdec6.so": "shim_trdec6.cpp"}
 

@@ -89,7 +89,15 @@ struct Handle {
   // captured over the workers' buffers is relaunched only while none of them has moved
   uint64_t wsgen = 0;
   uint64_t gws = 0;   // the workers' combined generation when gexec was captured
+  // allocation-free mode (ttx_tt_svd_noalloc): ensure() carves buffers from [arena, arena + arena_cap)
+  // instead of allocating; noalloc is the parent handle's shadow that runs those calls
+  unsigned char* arena = nullptr;
+  size_t arena_cap = 0, arena_used = 0;
+  Handle* noalloc = nullptr;
 };
+
+hipError_t noalloc_init(int dev);
+void* noalloc_scratch(int dev);
 
 static hipEvent_t next_event(Handle* h) {
   if (h->evnext == h->evpool.size()) {
@@ -132,6 +140,18 @@ static ttx_status fail(Handle* h, ttx_status s, const std::string& msg) {
 static ttx_status ensure(Handle* h, DevBuf& b, size_t bytes) {
   if (b.bytes >= bytes && b.p) return TTX_OK;
   if (h->strict) return fail(h, TTX_ERR_HIP, "workspace growth under graph capture");
+  if (h->arena) {
+    // a fresh 256-byte aligned piece of the arena; an outgrown piece is not reused within the call
+    size_t want = bytes < 256 ? 256 : bytes;
+    want = (want + 255) & ~(size_t)255;
+    if (want > h->arena_cap - h->arena_used)
+      return fail(h, TTX_ERR_UNSUPPORTED, "no-alloc scratch exhausted: " + std::to_string(h->arena_used + want) +
+                                              " bytes needed, " + std::to_string(h->arena_cap) + " available");
+    b.p = h->arena + h->arena_used;
+    b.bytes = want;
+    h->arena_used += want;
+    return TTX_OK;
+  }
   if (b.p) {
     (void)hipStreamSynchronize(h->stream);
     (void)hipFree(b.p);
@@ -462,7 +482,7 @@ static float* fp16_scales(Handle* h, const StepArgs& a, const Arith3& ar, bool s
   if (ar.range != AR_FP16) return nullptr;
   if (!h->f16s.p) {
     if (ensure(h, h->f16s, 129 * sizeof(float)) != TTX_OK) return nullptr;
-    (void)hipMemsetAsync(h->f16s.p, 0, 129 * sizeof(float), h->stream);
+    launch_zero(h->f16s.p, 129 * sizeof(float), h->stream);
   }
   float* sc = P<float>(h->f16s);
   if (set_a) launch_fp16_ascale(a.sig_prev, a.r_prev, (double)a.m * (double)a.n, sc, h->stream);
@@ -1215,6 +1235,64 @@ static ttx_status sweep(Handle* h, const float* d_A, const ttx_params* p, float*
 }
 
 // ------------------------------------------------------------------------------------------------
+// allocation-free TT sweep: an upper bound of the workspace that sweep() lays out for p, summing every
+// buffer request of every step as if each were a fresh piece (the arena never reuses an outgrown
+// piece within a call), plus 256 bytes of alignment per piece
+// ------------------------------------------------------------------------------------------------
+static size_t noalloc_bound(const ttx_params* p) {
+  const int d = p->order;
+  const size_t F = sizeof(float), D = sizeof(double), E2 = (size_t)TTX_MAX_ELL * TTX_MAX_ELL;
+  size_t total_el = 1;
+  for (int k = 0; k < d; ++k) total_el *= (size_t)p->dims[k];
+  size_t b = 0;
+  int pieces = 0;
+  auto add = [&](size_t bytes) { b += bytes; ++pieces; };
+  add((size_t)TTX_MAX_ORDER * (TTX_MAX_ELL * F + 4 * sizeof(int)));           // sigma / status words
+  add((size_t)(sumsq_grid((int64_t)total_el) + 1) * D);                       // input norm
+  // the fused first pass of a BF16 sweep with a 32-row first mode
+  if (d >= 3 && p->dims[0] == 32) {
+    const int64_t m1 = 32 * p->dims[1], n1 = (int64_t)(total_el / (size_t)m1);
+    if (m1 <= (1 << 30)) {
+      const SketchPlan sp = plan_sketch((int)m1, n1, 64, 0);
+      add((size_t)sp.slabs * m1 * 64 * F);
+      add((size_t)sketch_gram0_partials(sp) * 32 * 32 * D);
+      add((size_t)m1 * 64 * F);
+      add((size_t)omega_chunks_bytes(n1));
+    }
+  }
+  int64_t n = (int64_t)total_el;
+  for (int k = 0; k < d - 1; ++k) {
+    const int64_t m = (int64_t)p->ranks[k] * p->dims[k];
+    n /= p->dims[k];
+    if (m > ((int64_t)1 << 30)) return (size_t)-1;
+    const int64_t r = std::min<int64_t>(p->ranks[k + 1], std::min(m, n));
+    const int64_t nb = panel_blocks(m);
+    // fixed small buffers: status, sigma, lam, keys, V, G, G2, Rinv, Rprobe, Ur, FP16 scales
+    add(16); add(TTX_MAX_ELL * F); add(TTX_MAX_ELL * D); add(64 * 8);
+    add(E2 * D); add(E2 * D); add(E2 * D); add(E2 * D); add(E2 * D); add(E2 * F); add(129 * F);
+    // sketch partials (the step's own pass and the rank model's probe pass)
+    add((size_t)plan_sketch((int)m, n, 64, 0).slabs * m * 64 * F);
+    const int64_t np = std::min<int64_t>(n, std::max<int64_t>(std::min<int64_t>(n, 512), n / 16));
+    add((size_t)plan_sketch((int)m, np, 64, 0).slabs * m * 64 * F);
+    // Gram partials: exact Gram, projection Gram, CholeskyQR panels
+    if (m <= TTX_MAX_ELL) add((size_t)gram_small_grid(n) * m * m * D);
+    add((size_t)proj_grid(n) * E2 * D);
+    add((size_t)nb * E2 * D);
+    // tall factors: Y, Y2, Qc, Qp, Qn, Yt; W1, W2; test partials and sums
+    for (int i = 0; i < 6; ++i) add((size_t)m * 64 * F);
+    add((size_t)m * 64 * D); add((size_t)m * 64 * D);
+    add((size_t)nb * 64 * 8 * D); add((size_t)(2 * nb + 2) * D);
+    // Z = A^T Q and the next unfolding
+    add((size_t)n * 64 * F);
+    add((size_t)r * n * F);
+  }
+  // rank model work buffers (its parameters are the parent handle's)
+  add(3 * (size_t)TTX_MLP_PARAMS * F); add((6 * MLP_H + 2 * TTX_MLP_OUT) * F); add(64 * D);
+  add(64 * D + (TTX_MLP_IN + TTX_MLP_OUT) * F);
+  return b + (size_t)pieces * 256;
+}
+
+// ------------------------------------------------------------------------------------------------
 // host-streamed TT / TR sweep (ttdec6_singleGPU.cu:540-652, 656-796; trdec6_singleGPU.cu:668-826):
 // the tensor stays in host memory; the first step (exact Gram on the N_1 side) reads it in column
 // slices, the H2D copy of slice s + 1 on its own stream overlapping the kernels of slice s; its
@@ -1670,6 +1748,8 @@ const char* ttx_status_string(int32_t s) {
   }
 }
 
+static void free_handle(Handle* h, bool own_stream);
+
 ttx_status ttx_create(ttx_handle* out, void* stream) {
   if (!out) return TTX_ERR_BAD_ARG;
   Handle* h = new Handle();
@@ -1686,6 +1766,15 @@ ttx_status ttx_create(ttx_handle* out, void* stream) {
     return TTX_ERR_HIP;
   }
   (void)ttx::num_cus();
+  // the allocation-free entry point's scratch array: its device address on this device, resolved once
+  // per device at set-up; and the shadow handle that runs those calls (host memory only)
+  if (ttx::noalloc_init(h->dev) != hipSuccess) {
+    (void)hipGetLastError();
+    free_handle(h, false);
+    *out = nullptr;
+    return TTX_ERR_HIP;
+  }
+  h->noalloc = new Handle();
   // TTX_JACOBI_MAX_SWEEPS (1..30): a lower sweep cap, so callers and tests can see TTX_ERR_NO_CONVERGE
   if (const char* js = getenv("TTX_JACOBI_MAX_SWEEPS")) {
     const int v = atoi(js);
@@ -1695,8 +1784,22 @@ ttx_status ttx_create(ttx_handle* out, void* stream) {
   return TTX_OK;
 }
 
+// every device workspace buffer of a handle
+static std::vector<ttx::DevBuf*> handle_bufs(Handle* h) {
+  return {&h->part, &h->Y, &h->Y2, &h->Qc, &h->Qp, &h->Qn, &h->Z, &h->W1, &h->W2, &h->G, &h->G2,
+          &h->Rinv, &h->lam,
+          &h->V, &h->Ur, &h->gpart, &h->status, &h->next[0], &h->next[1], &h->sig, &h->keys,
+          &h->scratch[0], &h->scratch[1], &h->scratch[2], &h->scratch[3], &h->Yt, &h->tpart,
+          &h->tout, &h->mP, &h->mM1, &h->mM2, &h->mG, &h->mAct, &h->mPart, &h->mIO,
+          &h->f16s, &h->Ypre, &h->omg, &h->Rprobe, &h->hs[0], &h->hs[1], &h->hGs, &h->hB, &h->hB2, &h->hA};
+}
+
 static void free_handle(Handle* h, bool own_stream) {
   (void)hipStreamSynchronize(h->stream);
+  // the allocation-free shadow owns nothing: its buffers are pieces of the static scratch array or
+  // the parent's, its host words are the parent's
+  if (h->noalloc) delete h->noalloc;
+  h->noalloc = nullptr;
   for (Handle* sub : h->subs) free_handle(sub, true);
   h->subs.clear();
   if (h->gexec) (void)hipGraphExecDestroy(h->gexec);
@@ -1708,13 +1811,7 @@ static void free_handle(Handle* h, bool own_stream) {
   }
   for (hipEvent_t e : h->hev)
     if (e) (void)hipEventDestroy(e);
-  ttx::DevBuf* bufs[] = {&h->part, &h->Y, &h->Y2, &h->Qc, &h->Qp, &h->Qn, &h->Z, &h->W1, &h->W2, &h->G, &h->G2,
-                         &h->Rinv, &h->lam,
-                         &h->V, &h->Ur, &h->gpart, &h->status, &h->next[0], &h->next[1], &h->sig, &h->keys,
-                         &h->scratch[0], &h->scratch[1], &h->scratch[2], &h->scratch[3], &h->Yt, &h->tpart,
-                         &h->tout, &h->mP, &h->mM1, &h->mM2, &h->mG, &h->mAct, &h->mPart, &h->mIO,
-                         &h->f16s, &h->Ypre, &h->omg, &h->Rprobe, &h->hs[0], &h->hs[1], &h->hGs, &h->hB, &h->hB2, &h->hA};
-  for (auto* b : bufs)
+  for (auto* b : handle_bufs(h))
     if (b->p) (void)hipFree(b->p);
   for (float* c : h->last_cores) (void)hipFree(c);
   for (hipEvent_t e : h->evpool) (void)hipEventDestroy(e);
@@ -1767,6 +1864,44 @@ ttx_status ttx_tt_svd(ttx_handle h, const float* d_A, const ttx_params* p, float
                       ttx_report* rep) {
   if (!h) return TTX_ERR_BAD_ARG;
   return ttx::sweep(H(h), d_A, p, d_cores, h_sigma, rep, false);
+}
+
+ttx_status ttx_tt_svd_noalloc(ttx_handle hh, const float* d_A, const ttx_params* p, float* const* d_cores,
+                              float* h_sigma, ttx_report* rep) {
+  Handle* h = H(hh);
+  if (!h) return TTX_ERR_BAD_ARG;
+  Handle* s = h->noalloc;
+  if (!s) return ttx::fail(h, TTX_ERR_HIP, "ttx_tt_svd_noalloc: handle without its scratch set-up");
+  TTX_TRY(ttx::validate(h, p, false));
+  if (p->flags & TTX_FLAG_TIMING)
+    return ttx::fail(h, TTX_ERR_UNSUPPORTED, "ttx_tt_svd_noalloc: no per-kernel timing (its events are created per call)");
+  const size_t need = ttx::noalloc_bound(p);
+  if (need > (size_t)TTX_NOALLOC_SCRATCH_BYTES)
+    return ttx::fail(h, TTX_ERR_UNSUPPORTED,
+                     "ttx_tt_svd_noalloc: the workspace of this decomposition (up to " + std::to_string(need) +
+                         " bytes) exceeds the scratch maximum of " + std::to_string((size_t)TTX_NOALLOC_SCRATCH_BYTES) +
+                         " bytes; use ttx_tt_svd");
+  unsigned char* base = static_cast<unsigned char*>(ttx::noalloc_scratch(h->dev));
+  if (!base) return ttx::fail(h, TTX_ERR_HIP, "ttx_tt_svd_noalloc: scratch address not resolved");
+  // the shadow runs on the parent's stream with the parent's host words and rank model; every buffer
+  // is a fresh piece of this device's scratch array (the previous call's pieces are forgotten, not freed)
+  for (ttx::DevBuf* b : handle_bufs(s)) *b = ttx::DevBuf();
+  s->stream = h->stream;
+  s->dev = h->dev;
+  s->h_status = h->h_status;
+  s->h_sigma = h->h_sigma;
+  s->h_lam = h->h_lam;
+  s->h_aux = h->h_aux;
+  s->h_sweep = h->h_sweep;
+  s->jacobi_max_sweeps = h->jacobi_max_sweeps;
+  s->mP = h->mP;
+  s->mlp_ready = h->mlp_ready;
+  s->arena = base;
+  s->arena_cap = TTX_NOALLOC_SCRATCH_BYTES;
+  s->arena_used = 0;
+  const ttx_status st = ttx::sweep(s, d_A, p, d_cores, h_sigma, rep, false);
+  if (st != TTX_OK) h->err = s->err;
+  return st;
 }
 
 ttx_status ttx_tr_svd(ttx_handle h, const float* d_A, const ttx_params* p, float* const* d_cores, float* h_sigma,
