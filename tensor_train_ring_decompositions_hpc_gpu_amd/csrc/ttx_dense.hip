@@ -891,6 +891,24 @@ void launch_copy_block(const float* Y, int kpy, int m, int c0, int p, float* Yt,
   hipLaunchKernelGGL(k_copy_block, dim3((unsigned)grid), dim3(256), 0, st, Y, kpy, m, c0, p, Yt);
 }
 
+__global__ void k_expand_rows(const float* __restrict__ compact, int r1c, int r1, int64_t N, int r,
+                              float* __restrict__ full) {
+  const int64_t total = (int64_t)r1 * N * r;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += (int64_t)gridDim.x * blockDim.x) {
+    const int b = (int)(e % r1);
+    const int64_t rest = e / r1;   // i + N c
+    full[e] = (b < r1c) ? compact[b + (int64_t)r1c * rest] : 0.0f;
+  }
+}
+
+void launch_expand_rows(const float* compact, int r1c, int r1, int64_t N, int r, float* full, hipStream_t st) {
+  const int64_t total = (int64_t)r1 * N * r;
+  int64_t grid = (total + 255) / 256;
+  if (grid > (int64_t)num_cus() * 8) grid = (int64_t)num_cus() * 8;
+  if (grid < 1) grid = 1;
+  hipLaunchKernelGGL(k_expand_rows, dim3((unsigned)grid), dim3(256), 0, st, compact, r1c, r1, N, r, full);
+}
+
 __global__ void k_copy_cols(const float* __restrict__ A, int64_t lda, int m, int n, float* __restrict__ Y, int kp) {
   for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < (int64_t)m * kp;
        e += (int64_t)gridDim.x * blockDim.x) {

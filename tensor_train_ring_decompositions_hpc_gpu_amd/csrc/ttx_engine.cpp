@@ -276,6 +276,12 @@ struct StepArgs {
   int pred_ell = 0;
   // exact first step whose Gram partials (gram_pre of them, in gpart) the fused first pass made
   int64_t gram_pre = 0;
+  // TR first step (exact path): tr_r1c > 0 writes the remainder straight into the 231 layout, rows
+  // c = a + r0 b with b < tr_r1c only (the rows beyond the first step's rank bound are zero)
+  int tr_r0 = 0, tr_r1c = 0;
+  // rows the path decision is made on (0: m): the compacted second TR step decides as its zero-padded
+  // unfolding would
+  int64_t m_path = 0;
 };
 struct StepOut {
   int r = 0, ell = 0, path = 0, power_iters = 0, zero_pivots = 0, sweeps = 0, kdim = 0;
@@ -703,7 +709,8 @@ static ttx_status run_step(Handle* h, const StepArgs& a_in, StepOut& o) {
   if (a.pend && (a.pend_ms <= 0 || a.m % a.pend_ms != 0 || a.pend_ms > TTX_MAX_ELL))
     return fail(h, TTX_ERR_BAD_ARG, "step: bad fused left factor");
   int ell = a.r + a.s;
-  const int64_t mn = std::min<int64_t>(a.m, a.n);
+  const int64_t mpath = a.m_path > 0 ? a.m_path : a.m;
+  const int64_t mn = std::min<int64_t>(mpath, a.n);
   const bool exact = (a.method == TTX_METHOD_EXACT) || (a.method == TTX_METHOD_AUTO && ell >= mn);
   memset(o.sigma, 0, sizeof(o.sigma));
   o.bytes = 0;
@@ -720,7 +727,7 @@ static ttx_status run_step(Handle* h, const StepArgs& a_in, StepOut& o) {
   TTX_TRY(ensure(h, h->G, TTX_MAX_ELL * TTX_MAX_ELL * sizeof(double)));
   TTX_TRY(ensure(h, h->G2, TTX_MAX_ELL * TTX_MAX_ELL * sizeof(double)));
   int k = 0;
-  if (exact && a.m <= TTX_MAX_ELL) {
+  if (exact && mpath <= TTX_MAX_ELL) {
     // exact Gram path on the short side (ttdec6_singleGPU.cu:510-536): C = A A^T, eig, top r, G^T A
     o.path = TTX_PATH_EXACT_M;
     o.ell = 0;
@@ -773,9 +780,12 @@ static ttx_status run_step(Handle* h, const StepArgs& a_in, StepOut& o) {
     o.deferred = a.defer || (a.allow_defer && r == m);
     if (!o.deferred) {
       const float* Qop = proj_operand(h, a, P<float>(h->Qn), kpn);
-      KScope ks(h, TTX_K_PROJ, a.m * n * 4 + (int64_t)r * n * 4);
-      launch_proj(ar.accurate, kpn, a.A, a.m, m, n, Qop, m, 1, a.next, 0, r, st);
-      o.bytes += a.m * n * 4 + (int64_t)r * n * 4;
+      // rows written: r, or (TR first step, relaid out on the way) r0 times the kept b range
+      const int64_t rw = a.tr_r1c > 0 ? (int64_t)a.tr_r0 * a.tr_r1c : r;
+      KScope ks(h, TTX_K_PROJ, a.m * n * 4 + rw * n * 4);
+      if (a.tr_r1c > 0) launch_proj(ar.accurate, kpn, a.A, a.m, m, n, Qop, m, 2, a.next, a.tr_r0, a.tr_r1c, st);
+      else launch_proj(ar.accurate, kpn, a.A, a.m, m, n, Qop, m, 1, a.next, 0, r, st);
+      o.bytes += a.m * n * 4 + rw * n * 4;
       o.passes += 1;
     }
     o.r = r;
@@ -997,6 +1007,7 @@ static ttx_status sweep(Handle* h, const float* d_A, const ttx_params* p, float*
   int64_t n = total;
   const float* pend = nullptr;   // square core of a deferred full-rank step
   int prev_kept = 0;             // singular values kept by the previous step
+  bool tr_compact = false;       // the second TR step runs on the rows of its unfolding that can be nonzero
   int pend_ms = 0;
   R.ranks[0] = p->ranks[0];
   R.ranks[d] = p->ranks[d];
@@ -1071,9 +1082,12 @@ static ttx_status sweep(Handle* h, const float* d_A, const ttx_params* p, float*
     a.m = m;
     a.n = n;
     a.r = (tr && k == 0) ? p->ranks[0] * p->ranks[1] : p->ranks[k + 1];
+    // the second TR step on the compacted unfolding decides as on its zero-padded rows
+    const int64_t m_pad = (tr_compact && k == 1) ? (int64_t)p->ranks[1] * p->dims[1] : m;
+    if (tr_compact && k == 1) a.m_path = m_pad;
     // a rank above min(m, n) cannot be reached: the step keeps min(m, n) (the TR first step keeps its
     // r0 r1 columns, zero-padded, because the relayout needs them)
-    if (!(tr && k == 0)) a.r = (int)std::min<int64_t>(a.r, std::min<int64_t>(m, n));
+    if (!(tr && k == 0)) a.r = (int)std::min<int64_t>(a.r, std::min<int64_t>(m_pad, n));
     a.s = p->oversampling;
     a.q = p->power_iters;
     a.seed = p->omega_seed;
@@ -1105,24 +1119,57 @@ static ttx_status sweep(Handle* h, const float* d_A, const ttx_params* p, float*
     const bool last = (k == d - 2);
     // A full-rank exact step (r = m, square orthogonal core) that is not the last one is not
     // projected: the next step reads this step's input with the core folded into its small factors.
-    const int64_t mn = std::min<int64_t>(m, n);
+    const int64_t mn = std::min<int64_t>(m_pad, n);
     const bool exact_m = (a.method != TTX_METHOD_RSVD) && ((int64_t)a.r + a.s >= mn || a.method == TTX_METHOD_EXACT) &&
-                         m <= TTX_MAX_ELL;
-    const bool can_defer = !last && !(tr && k == 0) && !pend && exact_m;
+                         m_pad <= TTX_MAX_ELL;
+    const bool can_defer = !last && !(tr && k == 0) && !pend && exact_m && !(tr_compact && k == 1);
     a.defer = can_defer && !host_ranks && a.r == m;
     a.allow_defer = can_defer && host_ranks;
+    // TR first step on the exact path (order >= 3): the projection writes the 231 layout itself (no
+    // permute pass), and when r0 r1 exceeds the step's rank bound min(r0 r1, N_1) the rows b >= r1c =
+    // ceil(bound / r0) are all zero and are not written; the second step then runs on the r1c N_2 rows
+    // that can be nonzero (same results: zero rows add nothing to any sketch, Gram or projection) unless
+    // its zero-padded unfolding would take the exact short-side path
+    if (tr && k == 0 && exact_m && d >= 3) {
+      const int64_t r0 = p->ranks[0], r1 = p->ranks[1];
+      const int64_t bound = std::min<int64_t>(r0 * r1, m);
+      int64_t r1c = (bound + r0 - 1) / r0;
+      const int64_t m1 = r1 * p->dims[1], n1 = n * r0 / p->dims[1];
+      const int64_t r2 = std::min<int64_t>(p->ranks[2], std::min(m1, n1));
+      const bool exact1 = (p->method == TTX_METHOD_EXACT) ||
+                          (p->method == TTX_METHOD_AUTO && r2 + p->oversampling >= std::min(m1, n1));
+      if (r1c >= r1 || (exact1 && m1 <= TTX_MAX_ELL)) r1c = r1;
+      a.tr_r0 = (int)r0;
+      a.tr_r1c = (int)r1c;
+    }
     float* next = nullptr;
     int wbuf = -1;
     if (last) {
       next = d_cores[d - 1];
     } else if (!a.defer) {
       wbuf = (cur_buf == 0) ? 1 : 0;
-      TTX_TRY(ensure(h, h->next[wbuf], (size_t)a.r * n * sizeof(float)));
+      const int64_t rows = a.tr_r1c > 0 ? (int64_t)a.tr_r0 * a.tr_r1c : a.r;
+      TTX_TRY(ensure(h, h->next[wbuf], (size_t)rows * n * sizeof(float)));
       next = P<float>(h->next[wbuf]);
     }
     a.next = next;
+    // the compacted second TR step writes its (r1c N_2) x r_2 core to scratch; the zero rows are
+    // restored into the caller's core after the step
+    float* core_compact = nullptr;
+    if (tr_compact && k == 1) {
+      TTX_TRY(ensure(h, h->scratch[2], (size_t)m * a.r * sizeof(float)));
+      core_compact = P<float>(h->scratch[2]);
+      a.core = core_compact;
+    }
     StepOut& o = outs[k];
     TTX_TRY(run_step(h, a, o));
+    if (core_compact) {
+      {
+        KScope ks(h, TTX_K_SMALL, 0);
+        launch_expand_rows(core_compact, r_prev, p->ranks[1], p->dims[1], o.r, d_cores[1], st);
+      }
+      TTX_TRY(check_launch(h, "TR core rows"));
+    }
     R.ell[k] = o.ell;
     R.path[k] = o.path;
     R.power_iters[k] = o.power_iters;
@@ -1151,7 +1198,17 @@ static ttx_status sweep(Handle* h, const float* d_A, const ttx_params* p, float*
     }
     pend = nullptr;
     pend_ms = 0;
-    if (tr && k == 0) {
+    if (tr && k == 0 && a.tr_r1c > 0) {
+      // the projection wrote (r0, r1c, N') as (r1c, N', r0) already; the ring rank joins the columns
+      if (o.path != TTX_PATH_EXACT_M) return fail(h, TTX_ERR_HIP, "TR first step: relayout expected on the exact path");
+      if (o.r != p->ranks[0] * p->ranks[1]) return fail(h, TTX_ERR_BAD_ARG, "TR first step rank mismatch");
+      cur = next;
+      cur_buf = wbuf;
+      r_prev = a.tr_r1c;
+      tr_compact = a.tr_r1c < p->ranks[1];
+      n = n * p->ranks[0];
+      R.ranks[1] = p->ranks[1];
+    } else if (tr && k == 0) {
       // (r0, r1, N') -> (r1, N', r0) (trdec6_singleGPU.cu:865-877); the ring rank joins the columns
       const int64_t r0 = p->ranks[0], r1 = p->ranks[1];
       if (o.r != r0 * r1) return fail(h, TTX_ERR_BAD_ARG, "TR first step rank mismatch");

@@ -39,7 +39,9 @@ void launch_fp16_ascale(const float* sig_prev, int r_prev, double numel, float* 
 // scales[1 + c] from max_j |X[j][c]| (to [128, 256)); mx: kp zeroed words, zeroed again afterwards
 void launch_fp16_xscale(const float* X, int64_t n, int64_t ldx, int kp, unsigned* mx, float* scales, hipStream_t st);
 // D = Q^T A; mode 0 writes Z (n x ldo row-major: kp columns, or the first r_out when r_out > 0, a multiple
-// of 4), mode 1 writes r_out x n column-major.
+// of 4), mode 1 writes r_out x n column-major, mode 2 writes the TR first-step remainder straight into the
+// 231 layout: row c = a + r0 b (b < r1c) of column j to out[b + r1c (j + n a)], with r0 = ldo and
+// r1c = r_out (rows c >= r0 r1c are not written).
 // gpart != nullptr: also the Gram D D^T, first kg rows / columns, one FP64 partial per WG
 // (proj_grid(n) partials).
 int64_t proj_grid(int64_t n);
@@ -116,6 +118,9 @@ void launch_test_resid(const float* Qc, int64_t ldq, int m, int k, const float* 
                        hipStream_t st);
 // Yt (m x p row-major) <- columns c0 .. c0+p-1 of Y (m x kpy row-major)
 void launch_copy_block(const float* Y, int kpy, int m, int c0, int p, float* Yt, hipStream_t st);
+// TR core with zero rows restored: full[(b + r1 i) + r1 N c] = b < r1c ? compact[(b + r1c i) + r1c N c] : 0
+// for b < r1, i < N, c < r (both column-major (r1 N) x r and (r1c N) x r)
+void launch_expand_rows(const float* compact, int r1c, int r1, int64_t N, int r, float* full, hipStream_t st);
 // Y (m x kp row-major) <- A (m x n column-major, n <= kp), zero columns n..kp-1
 void launch_copy_cols(const float* A, int64_t lda, int m, int n, float* Y, int kp, hipStream_t st);
 

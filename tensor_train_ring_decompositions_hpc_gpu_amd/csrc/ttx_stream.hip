@@ -926,7 +926,7 @@ __global__ __launch_bounds__(512) void k_proj(const float* __restrict__ A, int64
             v[0] = acc[lt][4 * g]; v[1] = acc[lt][4 * g + 1]; v[2] = acc[lt][4 * g + 2]; v[3] = acc[lt][4 * g + 3];
             *reinterpret_cast<f32x4*>(o + 32 * lt + 8 * g + 4 * b) = v;
           }
-      } else {
+      } else if (out_mode == 1) {
         // next unfolding, column-major r_out x n
         float* o = out + j * (int64_t)r_out;
 #pragma unroll
@@ -935,6 +935,20 @@ __global__ __launch_bounds__(512) void k_proj(const float* __restrict__ A, int64
           for (int v = 0; v < 16; ++v) {
             const int c = 32 * lt + mfma32_row(v, lane);
             if (c < r_out) o[c] = acc[lt][v];
+          }
+      } else {
+        // TR first-step remainder in the 231 layout (trdec6_singleGPU.cu:49-58, 865-877): row
+        // c = a + r0 b of column j, b < r1c, goes to out[b + r1c (j + n a)] (r0 = ldo, r1c = r_out)
+        const int r0 = (int)ldo, r1c = r_out;
+#pragma unroll
+        for (int lt = 0; lt < LT; ++lt)
+#pragma unroll
+          for (int v = 0; v < 16; ++v) {
+            const int c = 32 * lt + mfma32_row(v, lane);
+            if (c < r0 * r1c) {
+              const int a2 = c % r0, b2 = c / r0;
+              out[b2 + (int64_t)r1c * (j + n * a2)] = acc[lt][v];
+            }
           }
       }
     }
