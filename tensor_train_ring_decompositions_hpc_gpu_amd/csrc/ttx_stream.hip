@@ -912,7 +912,36 @@ __global__ __launch_bounds__(512) void k_proj(const float* __restrict__ A, int64
 #pragma unroll
         for (int v = 0; v < 16; ++v) acc[lt][v] *= inv;
     }
-    if (j < n) {
+    if (out_mode == 2) {
+      // TR first-step remainder in the 231 layout (trdec6_singleGPU.cu:49-58, 865-877): row c = a + r0 b
+      // (b < r1c) of column j goes to out[b + r1c (j + n a)] (r0 = ldo, r1c = r_out).  For each a the
+      // wave's 32 columns fill r1c * 32 consecutive floats: the tile goes through the wave's staging
+      // area and every a is written as one contiguous run
+      const int r0 = (int)ldo, r1c = r_out;
+      constexpr int DP = 33;
+      static_assert(KP * DP <= 32 * SP, "the D tile fits the staging area");
+      float* d = stg[w];
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+#pragma unroll
+      for (int lt = 0; lt < LT; ++lt)
+#pragma unroll
+        for (int v = 0; v < 16; ++v) d[(32 * lt + mfma32_row(v, lane)) * DP + jl] = acc[lt][v];
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      const int64_t j0 = blk * 256 + 32 * w;
+      const int ncols = (n - j0 >= 32) ? 32 : (n > j0 ? (int)(n - j0) : 0);
+      for (int a2 = 0; a2 < r0; ++a2) {
+        float* o = out + (int64_t)r1c * (j0 + n * a2);
+        for (int e = lane; e < r1c * ncols; e += 64) {
+          const int b2 = e % r1c, jj = e / r1c;
+          o[e] = d[(a2 + r0 * b2) * DP + jj];
+        }
+      }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+    } else if (j < n) {
       if (out_mode == 0) {
         // Z = D^T, row-major n x ldo: kp columns, or the first r_out (a multiple of 4)
         float* o = out + j * ldo;
@@ -926,7 +955,7 @@ __global__ __launch_bounds__(512) void k_proj(const float* __restrict__ A, int64
             v[0] = acc[lt][4 * g]; v[1] = acc[lt][4 * g + 1]; v[2] = acc[lt][4 * g + 2]; v[3] = acc[lt][4 * g + 3];
             *reinterpret_cast<f32x4*>(o + 32 * lt + 8 * g + 4 * b) = v;
           }
-      } else if (out_mode == 1) {
+      } else {
         // next unfolding, column-major r_out x n
         float* o = out + j * (int64_t)r_out;
 #pragma unroll
@@ -935,20 +964,6 @@ __global__ __launch_bounds__(512) void k_proj(const float* __restrict__ A, int64
           for (int v = 0; v < 16; ++v) {
             const int c = 32 * lt + mfma32_row(v, lane);
             if (c < r_out) o[c] = acc[lt][v];
-          }
-      } else {
-        // TR first-step remainder in the 231 layout (trdec6_singleGPU.cu:49-58, 865-877): row
-        // c = a + r0 b of column j, b < r1c, goes to out[b + r1c (j + n a)] (r0 = ldo, r1c = r_out)
-        const int r0 = (int)ldo, r1c = r_out;
-#pragma unroll
-        for (int lt = 0; lt < LT; ++lt)
-#pragma unroll
-          for (int v = 0; v < 16; ++v) {
-            const int c = 32 * lt + mfma32_row(v, lane);
-            if (c < r0 * r1c) {
-              const int a2 = c % r0, b2 = c / r0;
-              out[b2 + (int64_t)r1c * (j + n * a2)] = acc[lt][v];
-            }
           }
       }
     }
