@@ -379,11 +379,14 @@ void launch_apply_ns_out(const double* Win, int64_t rows, int k, int kp, const d
 int64_t panel_blocks(int64_t rows) { return (rows + RB64 - 1) / RB64; }
 
 // ------------------------------------------------------------------------------------------------
-// cyclic Jacobi eigensolver (round-robin ordering), FP64, k <= 64, one workgroup of 1024 threads
-// Each round rotates k2/2 disjoint pairs: the first np threads compute the angles (FP32 angle,
-// (c, s) normalised in FP64 so every applied rotation is orthogonal to FP64 accuracy), then
-// A <- J^T A J is applied on 2 x 2 blocks, one block per thread.  V <- V J of a round runs while the
-// next round's angles are computed (the pair tables are double-buffered), off the critical path.
+// cyclic Jacobi eigensolver (round-robin ordering), FP64, k <= 64, one workgroup of np^2 threads
+// (np = k2 / 2 pairs, k2 = k rounded up to even), one 2 x 2 block of A and of V per thread.
+// A round rotates the np disjoint pairs of the circle-method schedule (pair u of round t joins
+// (t + u) mod (k2 - 1) and (t - u) mod (k2 - 1), pair 0 joins t and k2 - 1), so every thread derives
+// its pairs' indices itself.  Per round: the first np threads compute the angles (FP32 angle, (c, s)
+// normalised in FP64 so every applied rotation is orthogonal to FP64 accuracy) while every thread
+// loads its A and V blocks; one barrier; each thread applies A <- J^T A J and V <- V J to its blocks;
+// a second barrier.
 // The sweep loop ends when the off-diagonal mass after a sweep is at most 0.5e-24 ||A||_F^2 (the
 // last round's blocks tile the matrix, so it comes from registers).  status[1] = sweeps, status[2] = 1
 // when the sweep cap (max_sweeps) was reached before that test passed, else 0.
@@ -393,31 +396,47 @@ __device__ __forceinline__ void jacobi_angle(double app, double aqq, double apq,
   c = 1.0;
   s = 0.0;
   if (apq == 0.0) return;
-  // the angle only needs FP32 accuracy (every applied rotation is re-normalised in FP64)
-  const float tau = (float)((aqq - app) / (2.0 * apq));
+  // tau = (aqq - app) / (2 apq) to FP32 accuracy (the rotation is re-normalised in FP64 below);
+  // v_rcp_f64 keeps the quotient independent of the matrix's scale
+  const float tau = (float)((aqq - app) * __builtin_amdgcn_rcp(2.0 * apq));
   float t;
   if (!(fabsf(tau) <= 1e18f)) t = 0.0f;
   else t = (tau >= 0.0f ? 1.0f : -1.0f) * __builtin_amdgcn_rcpf(fabsf(tau) + __builtin_amdgcn_sqrtf(1.0f + tau * tau));
   const float c0 = __builtin_amdgcn_rsqf(1.0f + t * t);
   const double cd = (double)c0, sd = (double)(t * c0);
+  // c^2 + s^2 = 1 + d with |d| < 2^-21: two Newton steps for 1/sqrt from 1 leave an error below 1e-26
   const double nrm = cd * cd + sd * sd;
-  double rs = (double)__builtin_amdgcn_rsqf((float)nrm);
-  rs = rs * (1.5 - 0.5 * nrm * rs * rs);
-  rs = rs * (1.5 - 0.5 * nrm * rs * rs);
+  const double r1 = 1.5 - 0.5 * nrm;
+  const double rs = r1 * (1.5 - 0.5 * nrm * r1 * r1);
   c = cd * rs;
   s = sd * rs;
+}
+
+// pair u of round t in the circle-method schedule over k2 indices (M = k2 - 1)
+__device__ __forceinline__ void jacobi_pair(int t, int u, int M, int& p, int& q) {
+  int x, y;
+  if (u == 0) {
+    x = t;
+    y = M;
+  } else {
+    x = t + u;
+    if (x >= M) x -= M;
+    y = t - u;
+    if (y < 0) y += M;
+  }
+  p = x < y ? x : y;
+  q = x < y ? y : x;
 }
 
 __global__ __launch_bounds__(1024) void k_syevj(const double* __restrict__ G, int k, double* __restrict__ lam,
                                                double* __restrict__ Vout, int* __restrict__ status,
                                                unsigned long long* __restrict__ keys, int max_sweeps) {
-  constexpr int NT = 1024, NW = NT / 64;
   __shared__ double a[KMAX][KMAX + 1];
   __shared__ double v[KMAX][KMAX + 1];
-  __shared__ double cs[2][KMAX / 2], sn[2][KMAX / 2];
-  __shared__ int pp[2][KMAX / 2], qq[2][KMAX / 2];
-  __shared__ double red[NW];
+  __shared__ double cs[KMAX / 2], sn[KMAX / 2];
+  __shared__ double red[16];
   __shared__ int done;
+  const int NT = blockDim.x, NW = NT >> 6;
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   if (keys && tid < 64) keys[tid] = 0ull;
   const int k2 = k + (k & 1);   // an odd size gets one zero row / column (its rotations are identities)
@@ -437,56 +456,36 @@ __global__ __launch_bounds__(1024) void k_syevj(const double* __restrict__ G, in
   if (lane == 0) red[wv] = tot;
   __syncthreads();
   tot = 0.0;
-#pragma unroll
   for (int w = 0; w < NW; ++w) tot += red[w];
-  // fixed work: the 2 x 2 block (bu, bw) and the V row pairs (vi[t], vw[t]); np^2 <= NT, k2 np <= 2 NT
   const bool hasblk = tid < np * np;
   const int bu = hasblk ? tid / np : 0, bw = hasblk ? tid % np : 0;
-  int vi[2], vw[2];
-  bool hasv[2];
-#pragma unroll
-  for (int t = 0; t < 2; ++t) {
-    const int e = tid + t * NT;
-    hasv[t] = e < k2 * np;
-    vi[t] = hasv[t] ? e / np : 0;
-    vw[t] = hasv[t] ? e % np : 0;
-  }
-  auto v_apply = [&](int b) {
-#pragma unroll
-    for (int t = 0; t < 2; ++t)
-      if (hasv[t]) {
-        const int p2 = pp[b][vw[t]], q2 = qq[b][vw[t]];
-        const double c2 = cs[b][vw[t]], s2 = sn[b][vw[t]];
-        const double vp = v[vi[t]][p2], vq = v[vi[t]][q2];
-        v[vi[t]][p2] = c2 * vp - s2 * vq;
-        v[vi[t]][q2] = s2 * vp + c2 * vq;
-      }
-  };
-  int sweeps = 0, gr = 0;   // gr: rounds so far (table parity)
+  int sweeps = 0;
   bool converged = !(tot > 0.0);   // a zero matrix is diagonal
   while (sweeps < max_sweeps && tot > 0.0) {
     ++sweeps;
     double off = 0.0;
-    for (int rnd = 0; rnd < M; ++rnd, ++gr) {
-      const int b = gr & 1;
+    for (int rnd = 0; rnd < M; ++rnd) {
+      int p1 = 0, q1 = 0, p2 = 0, q2 = 0;
+      double app = 0.0, apq = 0.0, aqp = 0.0, aqq = 0.0, vpp = 0.0, vpq = 0.0, vqp = 0.0, vqq = 0.0;
+      if (hasblk) {
+        // this round's blocks were written before the previous barrier: load them while the angles
+        // are computed
+        jacobi_pair(rnd, bu, M, p1, q1);
+        jacobi_pair(rnd, bw, M, p2, q2);
+        app = a[p1][p2]; apq = a[p1][q2]; aqp = a[q1][p2]; aqq = a[q1][q2];
+        vpp = v[p1][p2]; vpq = v[p1][q2]; vqp = v[q1][p2]; vqq = v[q1][q2];
+      }
       if (tid < np) {
-        int x, y;
-        if (tid == 0) { x = rnd; y = M; }
-        else {
-          x = rnd + tid; if (x >= M) x -= M;
-          y = rnd - tid; if (y < 0) y += M;
-        }
-        const int p = x < y ? x : y, q = x < y ? y : x;
+        int p, q;
+        jacobi_pair(rnd, tid, M, p, q);
         double c, s;
         jacobi_angle(a[p][p], a[q][q], a[p][q], c, s);
-        pp[b][tid] = p; qq[b][tid] = q; cs[b][tid] = c; sn[b][tid] = s;
+        cs[tid] = c;
+        sn[tid] = s;
       }
-      if (gr > 0) v_apply(b ^ 1);   // the previous round's V update
       __syncthreads();
       if (hasblk) {
-        const int p1 = pp[b][bu], q1 = qq[b][bu], p2 = pp[b][bw], q2 = qq[b][bw];
-        const double c1 = cs[b][bu], s1 = sn[b][bu], c2 = cs[b][bw], s2 = sn[b][bw];
-        const double app = a[p1][p2], apq = a[p1][q2], aqp = a[q1][p2], aqq = a[q1][q2];
+        const double c1 = cs[bu], s1 = sn[bu], c2 = cs[bw], s2 = sn[bw];
         const double rpp = c1 * app - s1 * aqp, rpq = c1 * apq - s1 * aqq;
         const double rqp = s1 * app + c1 * aqp, rqq = s1 * apq + c1 * aqq;
         const double npp = c2 * rpp - s2 * rpq, npq = s2 * rpp + c2 * rpq;
@@ -495,6 +494,10 @@ __global__ __launch_bounds__(1024) void k_syevj(const double* __restrict__ G, in
         a[p1][q2] = npq;
         a[q1][p2] = nqp;
         a[q1][q2] = nqq;
+        v[p1][p2] = c2 * vpp - s2 * vpq;
+        v[p1][q2] = s2 * vpp + c2 * vpq;
+        v[q1][p2] = c2 * vqp - s2 * vqq;
+        v[q1][q2] = s2 * vqp + c2 * vqq;
         if (rnd == M - 1) {
           off = npq * npq + nqp * nqp;
           if (bu != bw) off += npp * npp + nqq * nqq;
@@ -508,16 +511,11 @@ __global__ __launch_bounds__(1024) void k_syevj(const double* __restrict__ G, in
     __syncthreads();
     if (tid == 0) {
       double sum = 0.0;
-#pragma unroll
       for (int w = 0; w < NW; ++w) sum += red[w];
       done = sum <= 0.5e-24 * tot;
     }
     __syncthreads();
     if (done) { converged = true; break; }
-  }
-  if (gr > 0) {
-    v_apply((gr - 1) & 1);
-    __syncthreads();
   }
   // sort descending: rank of each eigenvalue (ties broken by index)
   for (int i = tid; i < k; i += NT) {
@@ -536,9 +534,16 @@ __global__ __launch_bounds__(1024) void k_syevj(const double* __restrict__ G, in
   }
 }
 
+// threads for a k x k problem: one per 2 x 2 block, at least one wavefront (the key clearing)
+int syevj_threads(int k) {
+  const int np = (k + (k & 1)) / 2;
+  const int t = ((np * np + 63) / 64) * 64;
+  return t < 64 ? 64 : t;
+}
+
 void launch_syevj(const double* G, int k, double* lam, double* V, int* status, unsigned long long* keys,
                   hipStream_t st, int max_sweeps) {
-  hipLaunchKernelGGL(k_syevj, dim3(1), dim3(1024), 0, st, G, k, lam, V, status, keys, max_sweeps);
+  hipLaunchKernelGGL(k_syevj, dim3(1), dim3(syevj_threads(k)), 0, st, G, k, lam, V, status, keys, max_sweeps);
 }
 
 // ------------------------------------------------------------------------------------------------
