@@ -736,6 +736,7 @@ void launch_next_from_z(const float* Z, int64_t ldz, int64_t n, int k, const flo
 // square core (ms x ms, column-major).  Row (b, i) of A is sum_a G[a][b] row (a, i) of S.
 //   mode 0: Y_A = P Y_S, Y row-major m x kp           (sketch results)
 //   mode 1: Q_S = P^T Q_A, Q column-major kp x m       (projection operands)
+//   mode 2: Q_A = P Q_S, Q column-major kp x m         (a basis taken on S, before the core assembly)
 // ------------------------------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_blockmix(const float* __restrict__ G, int ms, int m, int kp, int mode,
                                                   const float* __restrict__ X, float* __restrict__ Y) {
@@ -749,11 +750,17 @@ __global__ __launch_bounds__(256) void k_blockmix(const float* __restrict__ G, i
       float acc = 0.0f;
       for (int a2 = 0; a2 < ms; ++a2) acc += gs[a2 + ms * b] * X[(int64_t)(i0 + a2) * kp + c];
       Y[e] = acc;
-    } else {
+    } else if (mode == 1) {
       const int c = (int)(e / m), row = (int)(e % m);
       const int a2 = row % ms, i0 = row - a2;
       float acc = 0.0f;
       for (int b = 0; b < ms; ++b) acc += gs[a2 + ms * b] * X[(int64_t)c * m + i0 + b];
+      Y[e] = acc;
+    } else {
+      const int c = (int)(e / m), row = (int)(e % m);
+      const int b = row % ms, i0 = row - b;
+      float acc = 0.0f;
+      for (int a2 = 0; a2 < ms; ++a2) acc += gs[a2 + ms * b] * X[(int64_t)c * m + i0 + a2];
       Y[e] = acc;
     }
   }

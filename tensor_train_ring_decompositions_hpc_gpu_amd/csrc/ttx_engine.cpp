@@ -9,6 +9,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <map>
 #include <string>
 #include <vector>
@@ -42,6 +43,13 @@ struct Handle {
   // host-streamed first step: two column-slice buffers, per-slice Gram sums, the remainder
   DevBuf hs[2], hGs, hB, hB2, hA;
   hipStream_t cstream = nullptr;  // host-to-device copies of the slices (default priority)
+  // the first step's eigensolve and core beside the second step (a TT sweep whose first pass took
+  // both): a stream of their own (default priority), fork / eigenvalues / core events, and buffers
+  // the second step does not touch
+  hipStream_t estream = nullptr;
+  hipEvent_t eev[3] = {nullptr, nullptr, nullptr};
+  DevBuf G0, lam0, V0, keys0;
+  double* h_lam0 = nullptr;      // pinned, TTX_MAX_ELL
   hipEvent_t hev[4] = {nullptr, nullptr, nullptr, nullptr};   // copied[2], free[2]
   bool rprobe_valid = false;
   const float* rprobe_y = nullptr;
@@ -282,7 +290,19 @@ struct StepArgs {
   // rows the path decision is made on (0: m): the compacted second TR step decides as its zero-padded
   // unfolding would
   int64_t m_path = 0;
+  // fused full-rank step whose pending square core is applied late: the step runs on S itself.  With
+  // P = pend_core orthogonal (pend_ms x pend_ms), A = (I_N (x) P^T) S and S have the same sketch Gram,
+  // range residual, Z = A^T Q_A = S^T Q_S, eigenvectors and remainder, Q_A = (I_N (x) P^T) Q_S; only
+  // the core G = Q_A U_B needs P, so join() runs before the rank decision and the assembly: it waits
+  // for P, returns this step's norm2 / delta2 (known once the previous step's eigenvalues are), and
+  // returns TTX_RETRY when the previous step kept fewer than pend_ms singular values after all
+  const float* pend_core = nullptr;
+  std::function<ttx_status(double*, double*)> join;
 };
+// internal only, never returned to a caller: the step ran on a speculated left factor that did not
+// come true and is to be run again (7: above every public code, inside the enumeration's range)
+static constexpr ttx_status TTX_RETRY = static_cast<ttx_status>(7);
+static_assert(TTX_ERR_UNSUPPORTED < 7, "TTX_RETRY collides with a public status");
 struct StepOut {
   int r = 0, ell = 0, path = 0, power_iters = 0, zero_pivots = 0, sweeps = 0, kdim = 0;
   int nonconv = 0;         // the eigensolver reached its sweep cap before converging
@@ -381,21 +401,30 @@ static ttx_status step_tail(Handle* h, const StepArgs& a, const Arith3& ar, int 
                  P<unsigned long long>(h->keys), st, h->jacobi_max_sweeps);
   }
   TTX_TRY(check_launch(h, "step tail"));
+  StepArgs aj = a;
+  if (a.pend_core && a.join) TTX_TRY(a.join(&aj.norm2, &aj.delta2));
   // rank decision on the host when needed
   int r = a.r;
   if (a.rank_mode != TTX_RANK_FIXED) {
     TTX_HIP(h, hipMemcpyAsync(h->h_lam, h->lam.p, k * sizeof(double), hipMemcpyDeviceToHost, st));
     TTX_HIP(h, hipStreamSynchronize(st));
     double kept;
-    r = choose_rank(a, k, h->h_lam, a.norm2, rel2, &kept);
+    r = choose_rank(aj, k, h->h_lam, aj.norm2, rel2, &kept);
     o.kept2 = kept;
   }
   if (r > k) return fail(h, TTX_ERR_BAD_ARG, "rank exceeds the sketch width");
   TTX_TRY(ensure(h, h->Ur, (size_t)((k + 3) & ~3) * next_from_z_ld(r) * sizeof(float)));
-  // G = Q U_B[:, :r] with the sign gauge; sigma = sqrt(eig(B B^T)) (basicRandSVD_LU :717-763)
+  // G = Q U_B[:, :r] with the sign gauge; sigma = sqrt(eig(B B^T)) (basicRandSVD_LU :717-763); a late
+  // left factor turns Q_S into Q_A first (the gauge is taken on the core itself)
+  const float* Qasm = P<float>(h->Qc);
+  if (a.pend_core) {
+    KScope ks(h, TTX_K_SMALL, 0);
+    launch_blockmix(a.pend_core, a.pend_ms, m, kp, 2, P<float>(h->Qc), P<float>(h->Qp), st);
+    Qasm = P<float>(h->Qp);
+  }
   {
     KScope ks(h, TTX_K_SMALL, 0);
-    launch_assemble(P<float>(h->Qc), m, m, k, P<double>(h->lam), P<double>(h->V), r, a.core, P<float>(h->Ur),
+    launch_assemble(Qasm, m, m, k, P<double>(h->lam), P<double>(h->V), r, a.core, P<float>(h->Ur),
                     nullptr, 0, a.sig_dev, P<unsigned long long>(h->keys), st);
   }
   // remainder S V^T = U_r^T B = U_r^T Z^T (:734-739)
@@ -706,8 +735,9 @@ static ttx_status run_step(Handle* h, const StepArgs& a_in, StepOut& o) {
   const int64_t n = a.n;
   if (a.m <= 0 || a.n <= 0 || a.r <= 0) return fail(h, TTX_ERR_BAD_ARG, "step: empty unfolding or rank");
   if (a.m > (int64_t)1 << 30) return fail(h, TTX_ERR_UNSUPPORTED, "step: more than 2^30 rows");
-  if (a.pend && (a.pend_ms <= 0 || a.m % a.pend_ms != 0 || a.pend_ms > TTX_MAX_ELL))
+  if ((a.pend || a.pend_core) && (a.pend_ms <= 0 || a.m % a.pend_ms != 0 || a.pend_ms > TTX_MAX_ELL))
     return fail(h, TTX_ERR_BAD_ARG, "step: bad fused left factor");
+  if (a.pend_core) a.pend = nullptr;   // the step runs on S; the factor reaches the core only
   int ell = a.r + a.s;
   const int64_t mpath = a.m_path > 0 ? a.m_path : a.m;
   const int64_t mn = std::min<int64_t>(mpath, a.n);
@@ -729,6 +759,13 @@ static ttx_status run_step(Handle* h, const StepArgs& a_in, StepOut& o) {
   int k = 0;
   if (exact && mpath <= TTX_MAX_ELL) {
     // exact Gram path on the short side (ttdec6_singleGPU.cu:510-536): C = A A^T, eig, top r, G^T A
+    if (a.pend_core) {
+      // this path mixes the Gram itself: the left factor is needed up front
+      if (a.join) TTX_TRY(a.join(&a.norm2, &a.delta2));
+      a.pend = a.pend_core;
+      a.pend_core = nullptr;
+      a.join = nullptr;
+    }
     o.path = TTX_PATH_EXACT_M;
     o.ell = 0;
     k = m;
@@ -955,6 +992,42 @@ static ttx_status input_norm2(Handle* h, const float* A, int64_t count, double* 
   return TTX_OK;
 }
 
+// the exact 32-row first step of a sweep whose first pass took its Gram, started beside the second
+// step: its Gram reduced on the sweep's stream, then the eigensolve (and, for host ranks, the copy of
+// its eigenvalues) on estream.  The core is assembled when the second step joins.
+static ttx_status side_step0(Handle* h, const StepArgs& a, StepOut& o, bool host_ranks) {
+  const int m = (int)a.m;
+  hipStream_t st = h->stream;
+  if (!h->estream) TTX_HIP(h, hipStreamCreateWithFlags(&h->estream, hipStreamNonBlocking));
+  for (hipEvent_t& e : h->eev)
+    if (!e) TTX_HIP(h, hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  if (!h->h_lam0) TTX_HIP(h, hipHostMalloc((void**)&h->h_lam0, TTX_MAX_ELL * sizeof(double), 0));
+  TTX_TRY(ensure(h, h->G0, (size_t)m * m * sizeof(double)));
+  TTX_TRY(ensure(h, h->lam0, TTX_MAX_ELL * sizeof(double)));
+  TTX_TRY(ensure(h, h->V0, (size_t)m * m * sizeof(double)));
+  TTX_TRY(ensure(h, h->keys0, 64 * sizeof(unsigned long long)));
+  launch_reduce_f64(P<double>(h->gpart), a.gram_pre, (int64_t)m * m, P<double>(h->G0), st);
+  TTX_TRY(check_launch(h, "first step gram"));
+  TTX_HIP(h, hipEventRecord(h->eev[0], st));
+  TTX_HIP(h, hipStreamWaitEvent(h->estream, h->eev[0], 0));
+  launch_syevj(P<double>(h->G0), m, P<double>(h->lam0), P<double>(h->V0), a.stat_dev,
+               P<unsigned long long>(h->keys0), h->estream, h->jacobi_max_sweeps);
+  TTX_TRY(check_launch(h, "first step eigensolve"));
+  if (host_ranks) {
+    TTX_HIP(h, hipMemcpyAsync(h->h_lam0, h->lam0.p, m * sizeof(double), hipMemcpyDeviceToHost, h->estream));
+    TTX_HIP(h, hipEventRecord(h->eev[1], h->estream));
+  }
+  memset(o.sigma, 0, sizeof(o.sigma));
+  o.path = TTX_PATH_EXACT_M;
+  o.ell = 0;
+  o.kdim = m;
+  o.bytes += a.m * a.n * 4;
+  o.passes += 1;
+  o.r = m;            // speculated: a full rank ceiling keeps all m singular values unless the ranks
+  o.deferred = true;  // are chosen from the eigenvalues (checked when the second step joins)
+  return TTX_OK;
+}
+
 static ttx_status sweep(Handle* h, const float* d_A, const ttx_params* p, float* const* d_cores, float* h_sigma,
                         ttx_report* rep, bool tr) {
   TTX_TRY(validate(h, p, tr));
@@ -1073,6 +1146,54 @@ static ttx_status sweep(Handle* h, const float* d_A, const ttx_params* p, float*
       }
     }
   }
+  // the first step's eigensolve beside the second step: when the fused first pass ran and nothing times,
+  // captures or runs from the allocation-free arena
+  const bool overlap0 = gram_pre > 0 && !timing && !h->capture && !h->arena;
+  bool side0 = false, joined0 = false;
+  StepArgs a0;
+  struct SideGuard {   // no side work outlives the sweep, whatever path leaves it
+    Handle* h;
+    ~SideGuard() {
+      if (h->estream) (void)hipStreamSynchronize(h->estream);
+    }
+  } side_guard{h};
+  // the second step's join: the first step's rank (host ranks: from its eigenvalues), its core on
+  // estream, the sweep stream waiting for it; TTX_RETRY when fewer than m singular values are kept
+  auto join0 = [&](double* nrm, double* dlt) -> ttx_status {
+    joined0 = true;
+    StepOut& o0 = outs[0];
+    const int m0 = (int)a0.m;
+    int r0 = a0.r;
+    if (host_ranks) {
+      TTX_HIP(h, hipEventSynchronize(h->eev[1]));
+      double tot = 0.0;
+      for (int i = 0; i < m0; ++i) tot += std::max(h->h_lam0[i], 0.0);
+      StepArgs aa = a0;
+      if (aa.norm2 < 0) aa.norm2 = tot;
+      if (aa.delta2 < 0) aa.delta2 = a0.delta_rel * tot;
+      double kept = 0.0;
+      r0 = choose_rank(aa, m0, h->h_lam0, tot, 0.0, &kept);
+      o0.total2 = tot;
+      o0.kept2 = kept;
+      // the sweep's bookkeeping of a finished first step
+      if (delta2 < 0.0) {
+        norm2 = tot;
+        delta2 = delta_rel * norm2;
+      }
+      R.norm2_in[0] = tot;
+      norm2 = kept;
+    }
+    o0.r = r0;
+    if (r0 != m0) return TTX_RETRY;
+    launch_assemble(nullptr, 0, m0, m0, P<double>(h->lam0), P<double>(h->V0), r0, d_cores[0], nullptr, nullptr, 0,
+                    a0.sig_dev, P<unsigned long long>(h->keys0), h->estream);
+    TTX_TRY(check_launch(h, "first step core"));
+    TTX_HIP(h, hipEventRecord(h->eev[2], h->estream));
+    TTX_HIP(h, hipStreamWaitEvent(st, h->eev[2], 0));
+    if (nrm) *nrm = norm2;
+    if (dlt) *dlt = delta2;
+    return TTX_OK;
+  };
   for (int k = 0; k < d - 1; ++k) {
     StepArgs a;
     if (k == 0) a.gram_pre = gram_pre;
@@ -1100,8 +1221,12 @@ static ttx_status sweep(Handle* h, const float* d_A, const ttx_params* p, float*
     a.delta_rel = delta_rel;
     a.norm2 = norm2;
     a.core = d_cores[k];
-    a.pend = pend;
+    // a deferred full-rank previous step: this step runs on cur itself and its factor reaches only
+    // the core (the step beside which the first step's eigensolve runs joins it first)
+    a.pend = nullptr;
+    a.pend_core = pend;
     a.pend_ms = pend_ms;
+    if (side0 && k == 1) a.join = join0;
     a.sig_dev = sig_all + (size_t)k * TTX_MAX_ELL;
     a.sig_prev = (k > 0) ? sig_all + (size_t)(k - 1) * TTX_MAX_ELL : nullptr;
     a.r_prev = prev_kept;
@@ -1111,11 +1236,14 @@ static ttx_status sweep(Handle* h, const float* d_A, const ttx_params* p, float*
     a.sync = false;
     // range-check tolerance: a quarter of the step's share eps^2 / (d-1) of the squared error budget
     // (Oseledets 2011: the squared step errors add up), relative to ||A_k||_F^2
-    if (p->rank_mode == TTX_RANK_FIXED || norm2 <= 0.0)
+    // after a deferred full-rank first step ||A_1||^2 = ||A||^2, so delta^2 / ||A_1||^2 is eps^2 / (d-1)
+    // itself (taken as such, so a first step still running beside this one gives the same values)
+    const bool first_full = (k == 1 && pend);
+    if (p->rank_mode == TTX_RANK_FIXED || norm2 <= 0.0 || first_full)
       a.range_tol2 = 0.25 * p->eps * p->eps / (double)(d - 1);
     else
       a.range_tol2 = 0.25 * delta2 / norm2;
-    a.rel_budget = (norm2 > 0.0) ? delta2 / norm2 : p->eps * p->eps / (double)(d - 1);
+    a.rel_budget = (norm2 > 0.0 && !first_full) ? delta2 / norm2 : p->eps * p->eps / (double)(d - 1);
     const bool last = (k == d - 2);
     // A full-rank exact step (r = m, square orthogonal core) that is not the last one is not
     // projected: the next step reads this step's input with the core folded into its small factors.
@@ -1162,7 +1290,49 @@ static ttx_status sweep(Handle* h, const float* d_A, const ttx_params* p, float*
       a.core = core_compact;
     }
     StepOut& o = outs[k];
-    TTX_TRY(run_step(h, a, o));
+    if (k == 0 && overlap0 && (a.defer || a.allow_defer)) {
+      TTX_TRY(side_step0(h, a, o, host_ranks));
+      side0 = true;
+      a0 = a;
+    } else {
+      const ttx_status s1 = run_step(h, a, o);
+      if (s1 == TTX_RETRY && side0 && k == 1) {
+        // the first step keeps r0 < m singular values: project it now and run the second step on
+        // its remainder (the sweep stream's speculative work is simply superseded)
+        side0 = false;
+        TTX_HIP(h, hipStreamSynchronize(h->estream));
+        StepOut& o0 = outs[0];
+        const int m0 = (int)a0.m, r0 = o0.r;
+        const int64_t n0 = a0.n;
+        const int kpn = kp_for(r0);
+        TTX_TRY(ensure(h, h->Qn, (size_t)kpn * m0 * sizeof(float)));
+        TTX_TRY(ensure(h, h->next[0], (size_t)r0 * n0 * sizeof(float)));
+        const Arith3 ar0 = arith_for(p->precision, p->flags);
+        launch_assemble(nullptr, 0, m0, m0, P<double>(h->lam0), P<double>(h->V0), r0, d_cores[0], nullptr,
+                        P<float>(h->Qn), kpn, a0.sig_dev, P<unsigned long long>(h->keys0), st);
+        launch_proj(ar0.accurate, kpn, d_A, m0, m0, n0, P<float>(h->Qn), m0, 1, P<float>(h->next[0]), 0, r0, st);
+        TTX_TRY(check_launch(h, "first step projection"));
+        o0.deferred = false;
+        o0.bytes += (int64_t)m0 * n0 * 4 + (int64_t)r0 * n0 * 4;
+        o0.passes += 1;
+        R.bytes_alg += (int64_t)m0 * n0 * 4 + (int64_t)r0 * n0 * 4;
+        R.passes[0] = o0.passes;
+        cur = P<float>(h->next[0]);
+        cur_buf = 0;
+        r_prev = r0;
+        prev_kept = std::min(r0, TTX_MAX_ELL);
+        R.ranks[1] = r0;
+        pend = nullptr;
+        pend_ms = 0;
+        h->pre_valid = false;
+        n *= p->dims[k];   // the step's iteration divides it again
+        outs[1] = StepOut();
+        --k;
+        continue;
+      }
+      TTX_TRY(s1);
+      if (side0 && k == 1 && !joined0) return fail(h, TTX_ERR_HIP, "internal: the first step was never joined");
+    }
     if (core_compact) {
       {
         KScope ks(h, TTX_K_SMALL, 0);
@@ -1848,7 +2018,8 @@ static std::vector<ttx::DevBuf*> handle_bufs(Handle* h) {
           &h->V, &h->Ur, &h->gpart, &h->status, &h->next[0], &h->next[1], &h->sig, &h->keys,
           &h->scratch[0], &h->scratch[1], &h->scratch[2], &h->scratch[3], &h->Yt, &h->tpart,
           &h->tout, &h->mP, &h->mM1, &h->mM2, &h->mG, &h->mAct, &h->mPart, &h->mIO,
-          &h->f16s, &h->Ypre, &h->omg, &h->Rprobe, &h->hs[0], &h->hs[1], &h->hGs, &h->hB, &h->hB2, &h->hA};
+          &h->f16s, &h->Ypre, &h->omg, &h->Rprobe, &h->hs[0], &h->hs[1], &h->hGs, &h->hB, &h->hB2, &h->hA,
+          &h->G0, &h->lam0, &h->V0, &h->keys0};
 }
 
 static void free_handle(Handle* h, bool own_stream) {
@@ -1868,6 +2039,12 @@ static void free_handle(Handle* h, bool own_stream) {
   }
   for (hipEvent_t e : h->hev)
     if (e) (void)hipEventDestroy(e);
+  if (h->estream) {
+    (void)hipStreamSynchronize(h->estream);
+    (void)hipStreamDestroy(h->estream);
+  }
+  for (hipEvent_t e : h->eev)
+    if (e) (void)hipEventDestroy(e);
   for (auto* b : handle_bufs(h))
     if (b->p) (void)hipFree(b->p);
   for (float* c : h->last_cores) (void)hipFree(c);
@@ -1877,6 +2054,7 @@ static void free_handle(Handle* h, bool own_stream) {
   if (h->h_lam) (void)hipHostFree(h->h_lam);
   if (h->h_aux) (void)hipHostFree(h->h_aux);
   if (h->h_sweep) (void)hipHostFree(h->h_sweep);
+  if (h->h_lam0) (void)hipHostFree(h->h_lam0);
   if (own_stream && h->stream) (void)hipStreamDestroy(h->stream);
   delete h;
 }

@@ -107,7 +107,8 @@ int next_from_z_ld(int r);
 void launch_next_from_z(const float* Z, int64_t ldz, int64_t n, int k, const float* Ur, int r, float* out,
                         hipStream_t st);
 // pending full-rank left factor P = I_N (x) G^T of a fused step (G: ms x ms column-major core):
-// mode 0: Y = P X (X, Y row-major m x kp); mode 1: Y = P^T X (X, Y column-major kp x m)
+// mode 0: Y = P X (X, Y row-major m x kp); mode 1: Y = P^T X (X, Y column-major kp x m); mode 2: Y = P X
+// (X, Y column-major kp x m)
 void launch_blockmix(const float* G, int ms, int m, int kp, int mode, const float* X, float* Y, hipStream_t st);
 // Ca = P Cs P^T (m x m, FP64)
 void launch_gram_mix(const float* G, int ms, int m, const double* Cs, double* Ca, hipStream_t st);

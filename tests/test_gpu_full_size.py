@@ -3,8 +3,8 @@ the bench's arithmetic (BF16, adaptive range check, three-product projection, le
 ceiling 32 and eps = 1e-4), checked through properties that do not need the oracle to run at this
 size:
 
-* the first step's singular values against an FP64 Gram of the 32 x 32^5 unfolding (torch, on the
-  device, in column blocks), and their squares summing to ||A||_F^2;
+* the first step's squared singular values against the eigenvalues of an FP64 Gram of the 32 x 32^5
+  unfolding (torch, on the device, in column blocks), and their sum against ||A||_F^2;
 * steps 1..4's singular values against the FP64 singular values (torch on the device: the Gram of the
   smaller side, or for 32^3 x 32^3 a randomized SVD with 16 extra columns and two power iterations) of
   the tensor's own unfolding 32^(k+1) x 32^(5-k): the tensor has TT ranks 32
@@ -78,11 +78,17 @@ def test_headline_config_full_size(engine, tensor):
     G = _gram_fp64(A, 32)
     lam = np.clip(np.linalg.eigvalsh(G.cpu().numpy())[::-1], 0.0, None)
     s_ref = np.sqrt(lam)
-    dev_0 = float(np.max(np.abs(sig[0][:32].astype(np.float64) - s_ref[:32])) / s_ref[0])
-    print("step 0: max |sigma - sigma_fp64| / sigma_1 = %.2e" % dev_0, flush=True)
-    assert dev_0 < 2e-6, dev_0
+    # step 0 takes sigma = sqrt(eigenvalue) of the fused first pass's Gram, whose three bf16 products
+    # drop mid * mid: on the diagonal a sum of squares, so the eigenvalues are low by about 1e-6 of
+    # ||A||^2; sigma_i then carries that error divided by 2 sigma_i, so the bound is on sigma^2
+    s0 = sig[0][:32].astype(np.float64)
+    dev_0 = float(np.max(np.abs(s0 ** 2 - s_ref[:32] ** 2)) / s_ref[0] ** 2)
+    print("step 0: max |sigma^2 - sigma_fp64^2| / sigma_1^2 = %.2e" % dev_0, flush=True)
+    assert dev_0 < 1e-5, dev_0
     norm2 = float(torch.trace(G))     # ||A||_F^2
-    assert abs(float(np.sum(sig[0][:32].astype(np.float64) ** 2)) - norm2) <= 1e-4 * norm2
+    bias = (float(np.sum(s0 ** 2)) - norm2) / norm2
+    print("step 0: (sum sigma^2 - ||A||^2) / ||A||^2 = %.2e" % bias, flush=True)
+    assert abs(bias) <= 1e-5, bias
     del G
     # steps 1..4 against an FP64 randomized SVD of the unfoldings
     X64 = A.double()
