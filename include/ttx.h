@@ -170,7 +170,11 @@ ttx_status ttx_tt_svd_host(ttx_handle h, const float* h_A, const ttx_params* p, 
 ttx_status ttx_tr_svd_host(ttx_handle h, const float* h_A, const ttx_params* p, int64_t slice_cols,
                            float* const* d_cores, float* h_sigma, ttx_report* rep);
 /* Replaces ttdec_half_lu_device_6 of trdecomposition/high_order_GPUs/trdec6_singleGPU.cu:827-1004:
- * first step of rank r_0 r_1, the 231 relayout, then the TT sweep; TR core layouts above. */
+ * first step of rank r_0 r_1, the 231 relayout, then the TT sweep; TR core layouts above.  DELTA /
+ * LEARNED ranks (the commented rule of trdecomposition/third_order_GPU/unopt_GPU/
+ * trdec_gpu_float_unopt2_rank.cu:92,222-243, delta^2 = eps^2 / (d-1) ||A||_F^2 per step): ranks[1..d-1]
+ * are ceilings; the ring rank r_0 = r_d stays as given and the first step keeps r_0 ceil(R / r_0)
+ * columns for its delta rank R, so r_1 = ceil(R / r_0); the report holds the ranks used. */
 ttx_status ttx_tr_svd(ttx_handle h, const float* d_A, const ttx_params* p, float* const* d_cores, float* h_sigma,
                       ttx_report* rep);
 /* Replaces basicRandSVD_LU (ttdecomposition/third_order_GPUs/opt/ttdec_gpu_float_rsvd_lu_half.cu:557-775):

@@ -325,31 +325,65 @@ def permute_312(T, r1, n, r0):
         T.reshape((r1, n, r0), order="F").transpose(2, 0, 1).reshape(-1, order="F"))
 
 
-def tr_svd(A, dims, ranks, s=10, q=2, seed=1234, method=RSVD):
+def tr_svd(A, dims, ranks, s=10, q=2, seed=1234, method=RSVD, rank_mode="fixed", eps=1e-4):
     """Order-d TR-SVD sweep, ttdec_half_lu_device_6 (TR), trdec6_singleGPU.cu:827-1004.
 
     ranks has d+1 entries; ranks[0] = ranks[d] = r0 is the ring rank and the first step keeps
     r0*r1 columns (:865).  G1 is returned as N1 x (r0*r1) with column a + r0*b (not permuted), the
     first remainder is permuted (r0, r1, N') -> (r1, N', r0) (:873), and the last core is
-    r_{d-1} x (N_d * r0)."""
+    r_{d-1} x (N_d * r0).
+
+    rank_mode="delta": ranks[1..d-1] are ceilings and every step truncates by the delta rule with
+    delta^2 = eps^2 / (d-1) ||A||_F^2 (the reference's commented TR rule,
+    trdecomposition/third_order_GPU/unopt_GPU/trdec_gpu_float_unopt2_rank.cu:92,222-243, ep =
+    eps / sqrt(d-1)).  The ring rank r0 stays as given: the first step keeps r0 * ceil(R / r0)
+    columns for its delta rank R, so r1 = ceil(R / r0) <= ranks[1] and its error stays within delta.
+    Returns the ranks used."""
     d = len(dims)
-    r0, r1 = ranks[0], ranks[1]
-    A1 = np.asarray(A, dtype=np.float64).reshape(dims[0], -1, order="F")
+    r0 = ranks[0]
+    A = np.asarray(A, dtype=np.float64)
+    A1 = A.reshape(dims[0], -1, order="F")
     nrest = A1.shape[1]
-    G1, sig1, A2t, path = step(A1, r0 * r1, s, q, seed, 0, method)
+    norm2 = float(np.sum(A * A))
+    delta2 = eps * eps / (d - 1) * norm2
+    delta = rank_mode == "delta"
+    # as tt_svd: delta ranks run the range check, a quarter of the step's share of the squared error
+    # budget relative to ||A_k||^2, and judge the tail with its residual
+    ad = dict(tol2=0.25 * delta2 / norm2, rank_mode=rank_mode) if delta else None
+    G1, sig1, A2t, path = step(A1, r0 * ranks[1], s, q, seed, 0, method, ad)
+    r1 = ranks[1]
+    if delta:
+        rel2 = 0.0 if path == EXACT else (ad["out"]["rel2"] if "out" in ad else None)
+        R = delta_rank(sig1 ** 2, norm2, delta2, r0 * ranks[1], rel2)
+        r1 = min(ranks[1], -(-R // r0))
+        G1, A2t = G1[:, :r0 * r1], A2t[:r0 * r1]
+    total2 = float(np.sum(sig1[:r0 * r1] ** 2))
     cur = permute_231(A2t.reshape(-1, order="F"), r0, r1, nrest)
     cores, sigmas, paths = [G1], [sig1], [path]
+    used = [r0, r1]
     ncols = nrest * r0
     for k in range(1, d - 1):
-        m = ranks[k] * dims[k]
+        m = used[k] * dims[k]
         ncols //= dims[k]
         cur = np.asarray(cur).reshape(m, ncols, order="F")
-        G, sig, cur, path = step(cur, ranks[k + 1], s, q, seed, k, method)
+        if delta:
+            rk = min(ranks[k + 1], m, ncols)
+            ad = dict(tol2=0.25 * delta2 / total2, rank_mode=rank_mode)
+            G, sig, nxt, path = step(cur, rk, s, q, seed, k, method, ad)
+            rel2 = 0.0 if path == EXACT else (ad["out"]["rel2"] if "out" in ad else None)
+            rr = delta_rank(sig ** 2, total2, delta2, G.shape[1], rel2)
+            G, cur = G[:, :rr], nxt[:rr]
+        else:
+            G, sig, cur, path = step(cur, ranks[k + 1], s, q, seed, k, method)
+            rr = ranks[k + 1]
+        total2 = float(np.sum(sig[:rr] ** 2))
         cores.append(G)
         sigmas.append(sig)
         paths.append(path)
-    cores.append(np.asarray(cur).reshape(ranks[d - 1], dims[d - 1] * r0, order="F"))
-    return cores, sigmas, paths, list(ranks)
+        used.append(rr)
+    used.append(r0)
+    cores.append(np.asarray(cur).reshape(used[d - 1], dims[d - 1] * r0, order="F"))
+    return cores, sigmas, paths, used
 
 
 # ----------------------------------------------------------------------------------------------

@@ -298,6 +298,9 @@ struct StepArgs {
   // returns TTX_RETRY when the previous step kept fewer than pend_ms singular values after all
   const float* pend_core = nullptr;
   std::function<ttx_status(double*, double*)> join;
+  // DELTA / LEARNED ranks: the chosen rank is rounded up to a multiple of this (the TR first step keeps
+  // r0 * r1 columns: r0 * ceil(R / r0) for a delta rank R)
+  int rank_multiple = 1;
 };
 // internal only, never returned to a caller: the step ran on a speculated left factor that did not
 // come true and is to be run again (7: above every public code, inside the enumeration's range)
@@ -312,6 +315,7 @@ struct StepOut {
   double resid = -1.0;     // relative range residual estimate of the basis used (-1: not checked)
   int checks = 0, passes = 0, fallback = 0, pred_rank = 0;
   bool deferred = false;   // the step left A_next implicit (full-rank exact step)
+  int tr_r1c = 0;          // TR first step: rows b of the 231 layout its projection wrote
   float sigma[TTX_MAX_ELL];
 };
 
@@ -358,6 +362,7 @@ static int choose_rank(const StepArgs& a, int k, const double* lam, double total
     }
     r = std::min(r, a.r);
     r = std::max(r, 1);
+    if (a.rank_multiple > 1) r = std::min(a.r, (r + a.rank_multiple - 1) / a.rank_multiple * a.rank_multiple);
   }
   double s = 0.0;
   for (int i = 0; i < std::min(r, k); ++i) s += std::max(lam[i], 0.0);
@@ -817,10 +822,13 @@ static ttx_status run_step(Handle* h, const StepArgs& a_in, StepOut& o) {
     o.deferred = a.defer || (a.allow_defer && r == m);
     if (!o.deferred) {
       const float* Qop = proj_operand(h, a, P<float>(h->Qn), kpn);
-      // rows written: r, or (TR first step, relaid out on the way) r0 times the kept b range
-      const int64_t rw = a.tr_r1c > 0 ? (int64_t)a.tr_r0 * a.tr_r1c : r;
+      // rows written: r, or (TR first step, relaid out on the way) r0 times the kept b range (the
+      // rows b < r / r0 that the rank keeps, and of those the ones that can be nonzero)
+      const int r1c = a.tr_r1c > 0 ? std::min(a.tr_r1c, r / a.tr_r0) : 0;
+      o.tr_r1c = r1c;
+      const int64_t rw = r1c > 0 ? (int64_t)a.tr_r0 * r1c : r;
       KScope ks(h, TTX_K_PROJ, a.m * n * 4 + rw * n * 4);
-      if (a.tr_r1c > 0) launch_proj(ar.accurate, kpn, a.A, a.m, m, n, Qop, m, 2, a.next, a.tr_r0, a.tr_r1c, st);
+      if (r1c > 0) launch_proj(ar.accurate, kpn, a.A, a.m, m, n, Qop, m, 2, a.next, a.tr_r0, r1c, st);
       else launch_proj(ar.accurate, kpn, a.A, a.m, m, n, Qop, m, 1, a.next, 0, r, st);
       o.bytes += a.m * n * 4 + rw * n * 4;
       o.passes += 1;
@@ -1032,7 +1040,6 @@ static ttx_status sweep(Handle* h, const float* d_A, const ttx_params* p, float*
                         ttx_report* rep, bool tr) {
   TTX_TRY(validate(h, p, tr));
   if (!d_A || !d_cores) return fail(h, TTX_ERR_BAD_ARG, "null tensor or core pointers");
-  if (tr && p->rank_mode != TTX_RANK_FIXED) return fail(h, TTX_ERR_UNSUPPORTED, "TR-SVD supports fixed ranks only");
   const int d = p->order;
   hipStream_t st = h->stream;
   ttx_report R;
@@ -1081,6 +1088,7 @@ static ttx_status sweep(Handle* h, const float* d_A, const ttx_params* p, float*
   const float* pend = nullptr;   // square core of a deferred full-rank step
   int prev_kept = 0;             // singular values kept by the previous step
   bool tr_compact = false;       // the second TR step runs on the rows of its unfolding that can be nonzero
+  int tr_r1 = tr ? p->ranks[1] : 0;   // TR: the first step's r1 (DELTA / LEARNED: ceil(R / r0) <= ranks[1])
   int pend_ms = 0;
   R.ranks[0] = p->ranks[0];
   R.ranks[d] = p->ranks[d];
@@ -1203,8 +1211,9 @@ static ttx_status sweep(Handle* h, const float* d_A, const ttx_params* p, float*
     a.m = m;
     a.n = n;
     a.r = (tr && k == 0) ? p->ranks[0] * p->ranks[1] : p->ranks[k + 1];
+    if (tr && k == 0) a.rank_multiple = p->ranks[0];
     // the second TR step on the compacted unfolding decides as on its zero-padded rows
-    const int64_t m_pad = (tr_compact && k == 1) ? (int64_t)p->ranks[1] * p->dims[1] : m;
+    const int64_t m_pad = (tr_compact && k == 1) ? (int64_t)tr_r1 * p->dims[1] : m;
     if (tr_compact && k == 1) a.m_path = m_pad;
     // a rank above min(m, n) cannot be reached: the step keeps min(m, n) (the TR first step keeps its
     // r0 r1 columns, zero-padded, because the relayout needs them)
@@ -1336,7 +1345,7 @@ static ttx_status sweep(Handle* h, const float* d_A, const ttx_params* p, float*
     if (core_compact) {
       {
         KScope ks(h, TTX_K_SMALL, 0);
-        launch_expand_rows(core_compact, r_prev, p->ranks[1], p->dims[1], o.r, d_cores[1], st);
+        launch_expand_rows(core_compact, r_prev, tr_r1, p->dims[1], o.r, d_cores[1], st);
       }
       TTX_TRY(check_launch(h, "TR core rows"));
     }
@@ -1368,20 +1377,21 @@ static ttx_status sweep(Handle* h, const float* d_A, const ttx_params* p, float*
     }
     pend = nullptr;
     pend_ms = 0;
+    if (tr && k == 0 && (o.r % p->ranks[0] != 0 || o.r > p->ranks[0] * p->ranks[1]))
+      return fail(h, TTX_ERR_BAD_ARG, "TR first step rank mismatch");
+    if (tr && k == 0) tr_r1 = o.r / p->ranks[0];
     if (tr && k == 0 && a.tr_r1c > 0) {
       // the projection wrote (r0, r1c, N') as (r1c, N', r0) already; the ring rank joins the columns
       if (o.path != TTX_PATH_EXACT_M) return fail(h, TTX_ERR_HIP, "TR first step: relayout expected on the exact path");
-      if (o.r != p->ranks[0] * p->ranks[1]) return fail(h, TTX_ERR_BAD_ARG, "TR first step rank mismatch");
       cur = next;
       cur_buf = wbuf;
-      r_prev = a.tr_r1c;
-      tr_compact = a.tr_r1c < p->ranks[1];
+      r_prev = o.tr_r1c;
+      tr_compact = o.tr_r1c < tr_r1;
       n = n * p->ranks[0];
-      R.ranks[1] = p->ranks[1];
+      R.ranks[1] = tr_r1;
     } else if (tr && k == 0) {
       // (r0, r1, N') -> (r1, N', r0) (trdec6_singleGPU.cu:865-877); the ring rank joins the columns
-      const int64_t r0 = p->ranks[0], r1 = p->ranks[1];
-      if (o.r != r0 * r1) return fail(h, TTX_ERR_BAD_ARG, "TR first step rank mismatch");
+      const int64_t r0 = p->ranks[0], r1 = tr_r1;
       const int pbuf = (wbuf == 0) ? 1 : 0;
       TTX_TRY(ensure(h, h->next[pbuf], (size_t)o.r * n * sizeof(float)));
       {

@@ -250,6 +250,28 @@ def test_tr_svd_order6_exact_at_r0_squared_ranks():
     assert ttsvd.rel_error(T, ttsvd.tr_full(cores, dims, used)) > 1e-3
 
 
+def test_tr_svd_delta_ranks():
+    """delta-ranked TR-SVD: the first step keeps r0 * ceil(R / r0) columns for its delta rank R (a
+    first unfolding of known rank R = r0_true * r1_true), every later step the delta rank; the error
+    stays within eps; on exact steps (no range finder, whose checked widths differ from the fixed ones)
+    a tiny eps with the fixed ranks as ceilings reproduces the fixed sweep."""
+    dims, true = [8, 9, 10, 11], [2, 3, 4, 3, 2]
+    T = ttsvd.tr_full(ttsvd.make_tr_cores(dims, true, seed=5), dims, true)
+    T = ttsvd.add_noise(T, 1e-6, seed=6)
+    # the first unfolding has rank 6 (up to the noise): r0 = 2 -> r1 = 3, r0 = 4 -> r1 = 2 (8 columns)
+    for r0, r1 in ((2, 3), (4, 2)):
+        cores, _, _, used = ttsvd.tr_svd(T, dims, [r0, 8, 48, 48, r0], s=6, q=1, rank_mode="delta", eps=1e-3)
+        assert used[0] == used[4] == r0 and used[1] == r1, used
+        assert cores[0].shape == (8, r0 * r1)
+        assert ttsvd.rel_error(T, ttsvd.tr_full(cores, dims, used)) <= 1e-3
+    fixed = [2, 3, 5, 4, 2]
+    c_f, s_f, _, u_f = ttsvd.tr_svd(T, dims, fixed, method="exact")
+    c_d, s_d, _, u_d = ttsvd.tr_svd(T, dims, fixed, method="exact", rank_mode="delta", eps=1e-12)
+    assert u_d == u_f == fixed
+    for a, b in zip(c_f, c_d):
+        assert np.array_equal(a, b)
+
+
 def test_noise_and_generator_definitions():
     dims, ranks = [5, 6, 7], [1, 2, 3, 1]
     cores = ttsvd.make_tt_cores(dims, ranks, std=0.5)
