@@ -44,7 +44,8 @@ def parse():
                     help="headline leg with a fixed q instead of the a-posteriori range check")
     ap.add_argument("--rank-mode", default="learned", choices=["learned", "fixed"],
                     help="headline leg: learned ranks (eps 1e-4, ranks 32 as ceilings) or the fixed ranks 32")
-    ap.add_argument("--only", default="", help="bf16 | fp32: run one leg (profiling)")
+    ap.add_argument("--only", default="", choices=["", "bf16", "fp32", "tr", "batch", "learned", "cfg1"],
+                    help="run one leg (profiling); tr / batch / learned / cfg1 print that leg's JSON only")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-rank workload")
     ap.add_argument("--no-tr", action="store_true", help="skip the TR-SVD workload")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched workload")
@@ -382,6 +383,13 @@ def main():
 
     torch.cuda.set_device(0)
     eng = Engine()
+    side = {"tr": lambda: tr_leg(eng, torch, L), "batch": lambda: batch_leg(eng, torch),
+            "learned": lambda: learned_leg(eng, torch, L), "cfg1": lambda: cfg1_leg(eng, torch, L)}
+    if args.only in side:
+        # one secondary workload alone (profiling runs): its own JSON object, not the bench line
+        print(json.dumps({args.only: side[args.only]()}))
+        eng.close()
+        return
     A = eng.generate(DIMS, RANKS, std=1.0 / np.sqrt(32.0), noise_rel=1e-6, seed=DATA_SEED)
     torch.cuda.synchronize()
     in_bytes = A.numel() * 4
