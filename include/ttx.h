@@ -86,6 +86,9 @@ typedef struct {
  * term carries about 2^-16 relative error instead of 2^-17; a quarter fewer MFMAs in the projection,
  * a tenth fewer in the first pass.  Other exact steps and the remainders keep four products. */
 #define TTX_FLAG_TAIL_X3 16
+/* decompose this one tensor exactly as ttx_batch_tt_svd decomposes each of a batch (batch grids, the
+ * register-buffered sketch, no deferred full-rank core): its cores are then bit for bit the batch's */
+#define TTX_FLAG_BATCH_PLAN 32
 
 typedef struct {
   int32_t ranks[TTX_MAX_ORDER + 1];   /* ranks used */
@@ -225,24 +228,35 @@ ttx_status ttx_tr_full(ttx_handle h, int32_t order, const int64_t* dims, const i
 
 /* batched TT-SVD -----------------------------------------------------------------------------------
  * Many tensors of uneven shapes and ranks (the reference decomposes one tensor per call,
- * ttdec6_singleGPU.cu:799).  The tensors are spread round-robin over `streams` worker streams, each
- * with its own workspace; with TTX_BATCH_GRAPH the whole batch is captured once into a hipGraph
- * (fork / join over the worker streams) and every call with the same tensors, shapes and parameters
- * relaunches that graph as one launch.  TTX_BATCH_AUTOTUNE times candidate sketch grids per shape
- * class (ceil log2 m, ceil log2 n, sketch width, arithmetic) on the first unfolding of that class and
- * keeps the fastest for the handle's lifetime.  Fixed ranks, widths and power iterations only (no host
- * round trips inside): DELTA / LEARNED rank modes, TTX_FLAG_ADAPTIVE and TTX_FLAG_TIMING are refused
- * with TTX_ERR_UNSUPPORTED.  A captured graph is relaunched only while no worker buffer has been
- * reallocated since its capture (a larger batch on the same handle triggers a new capture).
+ * ttdec6_singleGPU.cu:799).  Fixed ranks, widths and power iterations only (no host round trips
+ * inside): DELTA / LEARNED rank modes, TTX_FLAG_ADAPTIVE and TTX_FLAG_TIMING are refused with
+ * TTX_ERR_UNSUPPORTED.
+ * Grouped engine (the default; FP32 and BF16): every tensor goes through the same sequence of step
+ * phases at once, and each phase is one launch per kernel variant whose blocks are the blocks of all
+ * the tensors' own launches (a block map and one job descriptor per tensor, built and uploaded once
+ * per batch signature), so a batch is about a hundred launches whatever its size, and each tensor's
+ * cores equal those of its own sweep in batch mode bit for bit.  With TTX_BATCH_GRAPH those launches
+ * are captured once into a hipGraph and every call with the same tensors, shapes and parameters
+ * relaunches it as one launch.  `streams` is not used.  TTX_BATCH_AUTOTUNE (any engine, not FP16):
+ * the sketch's column chunks per workgroup (8, 16, 32 or 64) are chosen per batch signature by timing
+ * one warm eager grouped run of the batch with each, and kept for the handle's lifetime (ttx_tt_svd
+ * with TTX_FLAG_BATCH_PLAN on the same handle then uses the last batch's choice).
+ * Worker-stream engine (TTX_BATCH_STREAMS, and every FP16 batch): the tensors' sweeps spread
+ * round-robin over `streams` worker streams, each with its own workspace; TTX_BATCH_GRAPH captures
+ * them (fork / join) for batches of at most TTX_BATCH_GRAPH_MAX tensors, and a graph is relaunched
+ * only while no worker buffer has been reallocated since its capture.
+ * Both engines run each tensor's sweep in batch mode: the streaming passes take batch grids (a few
+ * large workgroups per tensor), and a full-rank exact step is projected like any other.
  * dims: count x order, ranks: count x (order + 1) (r_0 = r_d = 1), d_cores: count x order device
  * pointers (ttx_core_size of each tensor's parameters). */
 #define TTX_BATCH_GRAPH 1
-/* largest batch that TTX_BATCH_GRAPH captures (the size its tests cover; the launch of one
- * 512-tensor graph faulted the GPU and no cause has been found, so larger batches are refused with
- * TTX_ERR_UNSUPPORTED: split them, or run them without TTX_BATCH_GRAPH) */
+/* largest batch that TTX_BATCH_GRAPH captures on worker streams (the size its tests cover; the launch
+ * of one 512-tensor worker-stream graph faulted the GPU, so larger ones are refused with
+ * TTX_ERR_UNSUPPORTED; the grouped engine's graph has no such limit) */
 #define TTX_BATCH_GRAPH_MAX 16
 #define TTX_BATCH_PER_TENSOR 2   /* also time each tensor alone: p50 / p99 */
 #define TTX_BATCH_AUTOTUNE 4
+#define TTX_BATCH_STREAMS 8      /* the worker-stream engine instead of the grouped one */
 typedef struct {
   double total_ms;      /* device time of the batch */
   double tensors_per_s;
@@ -250,7 +264,10 @@ typedef struct {
   int32_t streams;
   int32_t graph;        /* 1: the batch ran as one graph launch */
   int32_t graph_reused; /* 1: an identical earlier batch's graph was relaunched */
-  int32_t tuned_shapes; /* shape classes in the autotuner's table */
+  int32_t tuned_shapes; /* batches in the autotuner's table */
+  int32_t engine;       /* 1: grouped launches, 0: worker streams */
+  int32_t launches;     /* grouped engine: kernel launches per batch */
+  int32_t sketch_chunks; /* column chunks per sketch workgroup (autotuned with TTX_BATCH_AUTOTUNE) */
 } ttx_batch_report;
 ttx_status ttx_batch_tt_svd(ttx_handle h, int32_t count, int32_t order, const int64_t* dims, const int32_t* ranks,
                             const float* const* d_tensors, float* const* d_cores, const ttx_params* common,

@@ -18,7 +18,7 @@ PREC_FP32, PREC_BF16, PREC_FP16 = 0, 1, 2
 RANK_FIXED, RANK_DELTA, RANK_LEARNED = 0, 1, 2
 METHOD_AUTO, METHOD_EXACT, METHOD_RSVD = 0, 1, 2
 PATH_EXACT_M, PATH_EXACT_N, PATH_RSVD = 0, 1, 2
-FLAG_FAST_RANGE, FLAG_TIMING, FLAG_ADAPTIVE, FLAG_STREAM_TWICE, FLAG_TAIL_X3 = 1, 2, 4, 8, 16
+FLAG_FAST_RANGE, FLAG_TIMING, FLAG_ADAPTIVE, FLAG_STREAM_TWICE, FLAG_TAIL_X3, FLAG_BATCH_PLAN = 1, 2, 4, 8, 16, 32
 NUM_KCLASS = 8
 KCLASS = ["sketch", "proj", "gram", "permute", "tall", "small", "unused6", "unused7"]
 
@@ -78,10 +78,13 @@ class BatchReport(ctypes.Structure):
         ("graph", ctypes.c_int32),
         ("graph_reused", ctypes.c_int32),
         ("tuned_shapes", ctypes.c_int32),
+        ("engine", ctypes.c_int32),
+        ("launches", ctypes.c_int32),
+        ("sketch_chunks", ctypes.c_int32),
     ]
 
 
-BATCH_GRAPH, BATCH_PER_TENSOR, BATCH_AUTOTUNE = 1, 2, 4
+BATCH_GRAPH, BATCH_PER_TENSOR, BATCH_AUTOTUNE, BATCH_STREAMS = 1, 2, 4, 8
 
 _P = ctypes.c_void_p
 _I32 = ctypes.c_int32

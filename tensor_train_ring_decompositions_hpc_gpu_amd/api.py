@@ -286,10 +286,14 @@ class Engine:
         return self._sweep(self.lib.ttx_tr_svd, A, dims, ranks, True, **kw)
 
     def batch_tt_svd(self, tensors, dims_list, ranks_list, s=10, q=2, seed=1234, precision="bf16", flags=0,
-                     streams=4, graph=True, autotune=True, per_tensor=False, cores=None):
+                     streams=4, graph=True, autotune=True, per_tensor=False, cores=None, engine="grouped"):
         """TT-SVD of many tensors with fixed ranks (ttx_batch_tt_svd).  tensors: list of contiguous float32
-        CUDA tensors; all of one order.  Returns (cores per tensor [(core, shape)], report dict).  Pass the
+        CUDA tensors; all of one order.  engine: "grouped" (one launch per kernel variant and step phase
+        for the whole batch; FP16 batches run on worker streams) or "streams" (each tensor's sweep on one
+        of ``streams`` worker streams).  Returns (cores per tensor [(core, shape)], report dict).  Pass the
         returned core buffers back as ``cores`` to relaunch the same captured graph."""
+        if engine not in ("grouped", "streams"):
+            raise ValueError("ttx: engine is 'grouped' or 'streams'")
         torch = _torch()
         count = len(tensors)
         d = len(dims_list[0])
@@ -307,7 +311,7 @@ class Engine:
         cptr = L.ptr_array([c.data_ptr() for cs in cores for c in cs])
         common = self.params(dims_list[0], ranks_list[0], s=s, q=q, seed=seed, precision=precision, flags=flags)
         bflags = (L.BATCH_GRAPH if graph else 0) | (L.BATCH_AUTOTUNE if autotune else 0) | \
-                 (L.BATCH_PER_TENSOR if per_tensor else 0)
+                 (L.BATCH_PER_TENSOR if per_tensor else 0) | (L.BATCH_STREAMS if engine == "streams" else 0)
         rep = L.BatchReport()
         st = self.lib.ttx_batch_tt_svd(self.h, count, d, dims_flat, ranks_flat, tptr, cptr, ctypes.byref(common),
                                        streams, bflags, ctypes.byref(rep))
@@ -318,7 +322,8 @@ class Engine:
             out.append([(c[: a * b], (a, b)) for c, (a, b) in zip(cs, shapes)])
         info = {"total_ms": rep.total_ms, "tensors_per_s": rep.tensors_per_s, "p50_ms": rep.p50_ms,
                 "p99_ms": rep.p99_ms, "streams": rep.streams, "graph": rep.graph, "graph_reused": rep.graph_reused,
-                "tuned_shapes": rep.tuned_shapes, "core_buffers": cores}
+                "tuned_shapes": rep.tuned_shapes, "engine": "grouped" if rep.engine == 1 else "streams",
+                "launches": rep.launches, "sketch_chunks": rep.sketch_chunks, "core_buffers": cores}
         return out, info
 
     def rsvd(self, A, m, n, k, s=10, q=2, seed=1234, precision="bf16"):

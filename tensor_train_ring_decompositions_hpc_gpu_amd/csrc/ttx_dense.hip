@@ -56,13 +56,13 @@ __device__ __forceinline__ void block_gram(const double (*xs)[KMAX + 1], int nro
 }
 
 template <typename T>
-__global__ __launch_bounds__(256) void k_gram_rows(const T* __restrict__ X, int64_t ldx, int64_t rows, int k,
-                                                   int64_t rows_per_wg, double* __restrict__ part) {
+__device__ __forceinline__ void gram_rows_body(const T* __restrict__ X, int64_t ldx, int64_t rows, int k,
+                                               int64_t rows_per_wg, double* __restrict__ part, const Blk B) {
   __shared__ double xs[RB64][KMAX + 1];
-  const int64_t r0 = (int64_t)blockIdx.x * rows_per_wg;
+  const int64_t r0 = (int64_t)B.x * rows_per_wg;
   const int64_t r1 = (r0 + rows_per_wg < rows) ? r0 + rows_per_wg : rows;
   const int kb4 = ((k + 3) / 4) * 4;
-  double* out = part + (int64_t)blockIdx.x * k * k;
+  double* out = part + (int64_t)B.x * k * k;
   // accumulate chunk Grams through a small LDS buffer
   __shared__ double gacc[KMAX * KMAX];
   for (int e = threadIdx.x; e < k * k; e += 256) gacc[e] = 0.0;
@@ -81,6 +81,17 @@ __global__ __launch_bounds__(256) void k_gram_rows(const T* __restrict__ X, int6
   }
   __syncthreads();
   for (int e = threadIdx.x; e < k * k; e += 256) out[e] = gacc[e];
+}
+template <typename T>
+__global__ __launch_bounds__(256) void k_gram_rows(const T* __restrict__ X, int64_t ldx, int64_t rows, int k,
+                                                   int64_t rows_per_wg, double* __restrict__ part) {
+  gram_rows_body<T>(X, ldx, rows, k, rows_per_wg, part, hw_blk());
+}
+template <typename T>
+__global__ __launch_bounds__(256) void k_gram_rows_grp(const JobGramRows* __restrict__ jobs, const int2* __restrict__ map) {
+  Blk B;
+  const JobGramRows& j = grp_job(jobs, map, B);
+  gram_rows_body<T>(static_cast<const T*>(j.X), j.ldx, j.rows, j.k, j.rpw, j.part, B);
 }
 
 int64_t gram_tall_grid(int64_t rows) {
@@ -177,12 +188,13 @@ __device__ __forceinline__ double lane_value(double v, int lane) {
 // KP: k rounded up to a multiple of 8; the matrix is padded with the identity to KP, which leaves
 // the leading k x k factor unchanged and makes every loop a compile-time one.  INV = false (Rinv
 // null) factors only, for callers that want the pivots.
+// a: KMAX x (KMAX + 1) LDS doubles (summed Gram, then B, row c, column x, for the output); Rs: at least
+// KP x (KMAX + 2) LDS doubles (rows of R as they are produced)
 template <int KP, bool INV>
-__global__ __launch_bounds__(256) void k_chol_rinv(const double* __restrict__ part, int64_t S, int k, int64_t m,
-                                                   int shift_mode, double* __restrict__ Rinv,
-                                                   int* __restrict__ status, double* __restrict__ pivots, int kinv) {
-  __shared__ double a[KMAX][KMAX + 1];     // summed Gram, then B (row c, column x) for the output
-  __shared__ double Rs[KP][KMAX + 2];      // rows of R as they are produced
+__device__ __forceinline__ void chol_body(const double* __restrict__ part, int64_t S, int k, int64_t m,
+                                          int shift_mode, double* __restrict__ Rinv, int* __restrict__ status,
+                                          double* __restrict__ pivots, int kinv, double (*a)[KMAX + 1],
+                                          double (*Rs)[KMAX + 2]) {
   const int tid = threadIdx.x;
   sum_slabs(part, S, k, a);
   __syncthreads();
@@ -259,6 +271,27 @@ __global__ __launch_bounds__(256) void k_chol_rinv(const double* __restrict__ pa
     Rinv[e] = (c >= i) ? a[c][i] : 0.0;
   }
 }
+template <int KP, bool INV>
+__global__ __launch_bounds__(256) void k_chol_rinv(const double* __restrict__ part, int64_t S, int k, int64_t m,
+                                                   int shift_mode, double* __restrict__ Rinv,
+                                                   int* __restrict__ status, double* __restrict__ pivots, int kinv) {
+  __shared__ double a[KMAX][KMAX + 1];
+  __shared__ double Rs[KP][KMAX + 2];
+  chol_body<KP, INV>(part, S, k, m, shift_mode, Rinv, status, pivots, kinv, a, Rs);
+}
+// every job's factor size in one launch: the body for KP = k rounded up to 8 (the LDS sized for 64)
+__global__ __launch_bounds__(256) void k_chol_grp(const JobChol* __restrict__ jobs, const int2* __restrict__ map) {
+  __shared__ double a[KMAX][KMAX + 1];
+  __shared__ double Rs[KMAX][KMAX + 2];
+  Blk B;
+  const JobChol& j = grp_job(jobs, map, B);
+  switch ((j.k + 7) / 8) {
+#define TTX_CHOL(q) case q: chol_body<8 * q, true>(j.part, j.S, j.k, j.m, j.shift, j.Rinv, j.status, nullptr, j.k, a, Rs); break;
+    TTX_CHOL(1) TTX_CHOL(2) TTX_CHOL(3) TTX_CHOL(4) TTX_CHOL(5) TTX_CHOL(6) TTX_CHOL(7) TTX_CHOL(8)
+#undef TTX_CHOL
+    default: break;
+  }
+}
 
 void launch_chol_inv(const double* part, int64_t S, int k, int64_t m, int shift_mode, double* Rinv, int* status,
                      hipStream_t st, double* pivots, int kinv) {
@@ -281,11 +314,10 @@ void launch_chol_inv(const double* part, int64_t S, int k, int64_t m, int shift_
 // OUT 0: double row-major W_out (stride k); OUT 1: float column-major Qc[c * ldq + i], c < kp
 // ------------------------------------------------------------------------------------------------
 template <typename TI, int OUT>
-__global__ __launch_bounds__(256) void k_apply_gram(const TI* __restrict__ Win, int64_t ldi, int64_t rows, int k,
-                                                    int kp, const double* __restrict__ Rinv,
-                                                    double* __restrict__ Wd, float* __restrict__ Qc, int64_t ldq,
-                                                    double* __restrict__ part, const double* __restrict__ gpart,
-                                                    int64_t S, int ldr) {
+__device__ __forceinline__ void apply_body(const TI* __restrict__ Win, int64_t ldi, int64_t rows, int k, int kp,
+                                           const double* __restrict__ Rinv, double* __restrict__ Wd,
+                                           float* __restrict__ Qc, int64_t ldq, double* __restrict__ part,
+                                           const double* __restrict__ gpart, int64_t S, int ldr, const Blk B) {
   __shared__ double ri[KMAX][KMAX + 1];
   __shared__ double ws[RB64][KMAX + 1];
   __shared__ double os[RB64][KMAX + 1];
@@ -303,7 +335,7 @@ __global__ __launch_bounds__(256) void k_apply_gram(const TI* __restrict__ Win, 
     // is the inverse of its leading block
     for (int e = tid; e < k * k; e += 256) ri[e / k][e - (e / k) * k] = Rinv[(int64_t)(e / k) * ldr + (e - (e / k) * k)];
   }
-  const int64_t i0 = (int64_t)blockIdx.x * RB64;
+  const int64_t i0 = (int64_t)B.x * RB64;
   const int nr = (int)((rows - i0) < RB64 ? (rows - i0) : RB64);
   for (int e = tid; e < RB64 * k; e += 256) {
     const int rr = e / k, c = e - (e / k) * k;
@@ -349,7 +381,22 @@ __global__ __launch_bounds__(256) void k_apply_gram(const TI* __restrict__ Win, 
       if (rr < nr) Qc[(int64_t)c * ldq + i0 + rr] = (c < k) ? (float)os[rr][c] : 0.0f;
     }
   }
-  if (part) block_gram<double>(os, nr, k, part + (int64_t)blockIdx.x * k * k);
+  if (part) block_gram<double>(os, nr, k, part + (int64_t)B.x * k * k);
+}
+template <typename TI, int OUT>
+__global__ __launch_bounds__(256) void k_apply_gram(const TI* __restrict__ Win, int64_t ldi, int64_t rows, int k,
+                                                    int kp, const double* __restrict__ Rinv,
+                                                    double* __restrict__ Wd, float* __restrict__ Qc, int64_t ldq,
+                                                    double* __restrict__ part, const double* __restrict__ gpart,
+                                                    int64_t S, int ldr) {
+  apply_body<TI, OUT>(Win, ldi, rows, k, kp, Rinv, Wd, Qc, ldq, part, gpart, S, ldr, hw_blk());
+}
+template <typename TI, int OUT>
+__global__ __launch_bounds__(256) void k_apply_grp(const JobApply* __restrict__ jobs, const int2* __restrict__ map) {
+  Blk B;
+  const JobApply& j = grp_job(jobs, map, B);
+  apply_body<TI, OUT>(static_cast<const TI*>(j.Win), j.ldi, j.rows, j.k, j.kp, j.Rinv, j.Wd, j.Qc, j.ldq, j.part,
+                      j.gpart, j.S, j.ldr, B);
 }
 
 void launch_apply_gram_f32(const float* Win, int64_t ldi, int64_t rows, int k, const double* Rinv, double* Wout,
@@ -428,21 +475,23 @@ __device__ __forceinline__ void jacobi_pair(int t, int u, int M, int& p, int& q)
   q = x < y ? y : x;
 }
 
-__global__ __launch_bounds__(1024) void k_syevj(const double* __restrict__ G, int k, double* __restrict__ lam,
-                                               double* __restrict__ Vout, int* __restrict__ status,
-                                               unsigned long long* __restrict__ keys, int max_sweeps) {
+// NT: the threads that work (syevj_threads(k), a multiple of 64, at most the block's size); the
+// block's other waves only keep the barriers
+__device__ __forceinline__ void syevj_body(const double* __restrict__ G, int k, double* __restrict__ lam,
+                                           double* __restrict__ Vout, int* __restrict__ status,
+                                           unsigned long long* __restrict__ keys, int max_sweeps, const int NT) {
   __shared__ double a[KMAX][KMAX + 1];
   __shared__ double v[KMAX][KMAX + 1];
   __shared__ double cs[KMAX / 2], sn[KMAX / 2];
   __shared__ double red[16];
   __shared__ int done;
-  const int NT = blockDim.x, NW = NT >> 6;
+  const int NW = NT >> 6;
   const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
   if (keys && tid < 64) keys[tid] = 0ull;
   const int k2 = k + (k & 1);   // an odd size gets one zero row / column (its rotations are identities)
   const int np = k2 / 2, M = k2 - 1;
   double tot = 0.0;
-  for (int e = tid; e < k2 * k2; e += NT) {
+  for (int e = tid < NT ? tid : k2 * k2; e < k2 * k2; e += NT) {
     const int i = e / k2, j = e % k2;
     // the Gram passes accumulate (i, j) and (j, i) in different orders: use the symmetric part
     const double x = (i < k && j < k) ? 0.5 * (G[i * k + j] + G[j * k + i]) : 0.0;
@@ -518,7 +567,7 @@ __global__ __launch_bounds__(1024) void k_syevj(const double* __restrict__ G, in
     if (done) { converged = true; break; }
   }
   // sort descending: rank of each eigenvalue (ties broken by index)
-  for (int i = tid; i < k; i += NT) {
+  for (int i = tid < NT ? tid : k; i < k; i += NT) {
     const double li = a[i][i];
     int rank = 0;
     for (int j = 0; j < k; ++j) {
@@ -532,6 +581,18 @@ __global__ __launch_bounds__(1024) void k_syevj(const double* __restrict__ G, in
     status[1] = sweeps;
     status[2] = converged ? 0 : 1;
   }
+}
+__global__ __launch_bounds__(1024) void k_syevj(const double* __restrict__ G, int k, double* __restrict__ lam,
+                                               double* __restrict__ Vout, int* __restrict__ status,
+                                               unsigned long long* __restrict__ keys, int max_sweeps) {
+  syevj_body(G, k, lam, Vout, status, keys, max_sweeps, (int)blockDim.x);
+}
+// one 1024-thread block per job, each job working with its own syevj_threads(k)
+__global__ __launch_bounds__(1024) void k_syevj_grp(const JobSyevj* __restrict__ jobs, const int2* __restrict__ map,
+                                                   int max_sweeps) {
+  Blk B;
+  const JobSyevj& j = grp_job(jobs, map, B);
+  syevj_body(j.G, j.k, j.lam, j.V, j.status, j.keys, max_sweeps, j.nt);
 }
 
 // threads for a k x k problem: one per 2 x 2 block, at least one wavefront (the key clearing)
@@ -553,9 +614,9 @@ void launch_syevj(const double* G, int k, double* lam, double* V, int* status, u
 //   (|v| bits << 32) | ((0x7FFFFFFF - i) << 1) | sign(v)
 // pass 2 (64-row blocks): flip columns by that entry's sign; write Qn (zero-padded copy), Ur, sigma
 // ------------------------------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_assemble_g(const float* __restrict__ Qc, int64_t ldq, int m, int k,
-                                                    const double* __restrict__ V, int r, float* __restrict__ Gc,
-                                                    unsigned long long* __restrict__ keys) {
+__device__ __forceinline__ void assemble_g_body(const float* __restrict__ Qc, int64_t ldq, int m, int k,
+                                                const double* __restrict__ V, int r, float* __restrict__ Gc,
+                                                unsigned long long* __restrict__ keys, const Blk B) {
   __shared__ double vs[KMAX][KMAX + 1];
   __shared__ unsigned long long kmax[KMAX];
   const int tid = threadIdx.x;
@@ -563,7 +624,7 @@ __global__ __launch_bounds__(256) void k_assemble_g(const float* __restrict__ Qc
   for (int e = tid; e < k * k; e += 256) vs[e / k][e % k] = V[e];
   if (tid < KMAX) kmax[tid] = 0ull;
   __syncthreads();
-  const int64_t i0 = (int64_t)blockIdx.x * RB64;
+  const int64_t i0 = (int64_t)B.x * RB64;
   for (int e = tid; e < RB64 * r; e += 256) {
     const int rr = e % RB64, c = e / RB64;
     const int64_t i = i0 + rr;
@@ -587,21 +648,27 @@ __global__ __launch_bounds__(256) void k_assemble_g(const float* __restrict__ Qc
   __syncthreads();
   if (tid < rk) atomicMax(&keys[tid], kmax[tid]);
 }
+__global__ __launch_bounds__(256) void k_assemble_g(const float* __restrict__ Qc, int64_t ldq, int m, int k,
+                                                    const double* __restrict__ V, int r, float* __restrict__ Gc,
+                                                    unsigned long long* __restrict__ keys) {
+  assemble_g_body(Qc, ldq, m, k, V, r, Gc, keys, hw_blk());
+}
 
 // row stride of Ur: r rounded up to 8, 16, 32 or 64 (the remainder kernel's register width)
 __host__ __device__ constexpr int ur_ld(int r) { return r <= 8 ? 8 : r <= 16 ? 16 : r <= 32 ? 32 : 64; }
 int next_from_z_ld(int r) { return ur_ld(r); }
 
-__global__ __launch_bounds__(256) void k_assemble_fix(const unsigned long long* __restrict__ keys, int m, int k,
-                                                      const double* __restrict__ V, const double* __restrict__ lam,
-                                                      int r, float* __restrict__ Gc, float* __restrict__ Ur,
-                                                      float* __restrict__ Qn, int kpn, float* __restrict__ sigma) {
+__device__ __forceinline__ void assemble_fix_body(const unsigned long long* __restrict__ keys, int m, int k,
+                                                  const double* __restrict__ V, const double* __restrict__ lam,
+                                                  int r, float* __restrict__ Gc, float* __restrict__ Ur,
+                                                  float* __restrict__ Qn, int kpn, float* __restrict__ sigma,
+                                                  const Blk B) {
   __shared__ float sg[KMAX];
   const int tid = threadIdx.x;
   const int rk = r < k ? r : k;
   if (tid < KMAX) sg[tid] = (tid < rk && (keys[tid] & 1ull)) ? -1.0f : 1.0f;
   __syncthreads();
-  const int64_t i0 = (int64_t)blockIdx.x * RB64;
+  const int64_t i0 = (int64_t)B.x * RB64;
   for (int e = tid; e < RB64 * r; e += 256) {
     const int rr = e % RB64, c = e / RB64;
     const int64_t i = i0 + rr;
@@ -616,7 +683,7 @@ __global__ __launch_bounds__(256) void k_assemble_fix(const unsigned long long* 
       const int64_t i = i0 + rr;
       if (i < m) Qn[(int64_t)c * m + i] = 0.0f;
     }
-  if (blockIdx.x == 0) {
+  if (B.x == 0) {
     if (Ur) {
       // round4(k) x ur_ld(r), zero-padded, as k_next_from_z reads it
       const int ld = ur_ld(r), k4 = (k + 3) & ~3;
@@ -628,6 +695,23 @@ __global__ __launch_bounds__(256) void k_assemble_fix(const unsigned long long* 
     if (sigma)
       for (int i = tid; i < k; i += 256) sigma[i] = (float)sqrt(fmax(lam[i], 0.0));
   }
+}
+__global__ __launch_bounds__(256) void k_assemble_fix(const unsigned long long* __restrict__ keys, int m, int k,
+                                                      const double* __restrict__ V, const double* __restrict__ lam,
+                                                      int r, float* __restrict__ Gc, float* __restrict__ Ur,
+                                                      float* __restrict__ Qn, int kpn, float* __restrict__ sigma) {
+  assemble_fix_body(keys, m, k, V, lam, r, Gc, Ur, Qn, kpn, sigma, hw_blk());
+}
+__global__ __launch_bounds__(256) void k_assemble_g_grp(const JobAssemble* __restrict__ jobs, const int2* __restrict__ map) {
+  Blk B;
+  const JobAssemble& j = grp_job(jobs, map, B);
+  assemble_g_body(j.Qc, j.ldq, j.m, j.k, j.V, j.r, j.Gc, j.keys, B);
+}
+__global__ __launch_bounds__(256) void k_assemble_fix_grp(const JobAssemble* __restrict__ jobs,
+                                                          const int2* __restrict__ map) {
+  Blk B;
+  const JobAssemble& j = grp_job(jobs, map, B);
+  assemble_fix_body(j.keys, j.m, j.k, j.V, j.lam, j.r, j.Gc, j.Ur, j.Qn, j.kpn, j.sigma, B);
 }
 
 void launch_assemble(const float* Qc, int64_t ldq, int m, int k, const double* lam, const double* V, int r, float* Gc,
@@ -642,15 +726,16 @@ void launch_assemble(const float* Qc, int64_t ldq, int m, int k, const double* l
 // A_next = Ur^T Z^T  (r x n column-major)
 // ------------------------------------------------------------------------------------------------
 template <int RM>
-__global__ __launch_bounds__(256) void k_next_from_z(const float* __restrict__ Z, int64_t ldz, int64_t n, int k,
-                                                     const float* __restrict__ Ur, int r, float* __restrict__ out) {
+__device__ __forceinline__ void next_rows_body(const float* __restrict__ Z, int64_t ldz, int64_t n, int k,
+                                               const float* __restrict__ Ur, int r, float* __restrict__ out,
+                                               const Blk B) {
   // one output row per lane: the lane reads its own Z row with 16-byte loads (a wave reads 64
   // consecutive rows, every byte of them) and keeps RM sums in registers; Ur (round4(k) x RM,
   // zero-padded) is read at wave-uniform addresses, so its values come through scalar loads as
   // SGPR operands; the lane writes its r consecutive floats (a wave writes 64 r contiguous floats)
   const bool vec = (r % 4) == 0 && (reinterpret_cast<uintptr_t>(out) & 15) == 0;
   const int kq = (k + 3) / 4;
-  for (int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x; j < n; j += (int64_t)gridDim.x * 256) {
+  for (int64_t j = (int64_t)B.x * 256 + threadIdx.x; j < n; j += (int64_t)B.gx * 256) {
     // the whole Z row goes out first (up to 16 loads in flight per lane)
     const f32x4* z = reinterpret_cast<const f32x4*>(Z + j * ldz);
     f32x4 zr[KMAX / 4];
@@ -687,16 +772,22 @@ __global__ __launch_bounds__(256) void k_next_from_z(const float* __restrict__ Z
 // small n: RM lanes per output row (lane c computes column c), so short sweeps still fill the
 // machine; the RM lanes of one row read the same Z row with 16-byte loads
 template <int RM>
-__global__ __launch_bounds__(256) void k_next_from_z_cols(const float* __restrict__ Z, int64_t ldz, int64_t n,
-                                                          int k, const float* __restrict__ Ur, int r,
-                                                          float* __restrict__ out) {
-  __shared__ float us[(KMAX + 4) * RM];
+__global__ __launch_bounds__(256) void k_next_from_z(const float* __restrict__ Z, int64_t ldz, int64_t n, int k,
+                                                     const float* __restrict__ Ur, int r, float* __restrict__ out) {
+  next_rows_body<RM>(Z, ldz, n, k, Ur, r, out, hw_blk());
+}
+
+// us: (KMAX + 4) RM LDS floats
+template <int RM>
+__device__ __forceinline__ void next_cols_body(const float* __restrict__ Z, int64_t ldz, int64_t n, int k,
+                                               const float* __restrict__ Ur, int r, float* __restrict__ out,
+                                               float* us, const Blk B) {
   const int k4 = (k + 3) & ~3;
   for (int e = threadIdx.x; e < k4 * RM; e += 256) us[e] = Ur[e];   // Ur is k4 x RM, zero-padded
   __syncthreads();
   constexpr int ROWS = 256 / RM;
   const int jj = threadIdx.x / RM, c = threadIdx.x % RM;
-  for (int64_t j = (int64_t)blockIdx.x * ROWS + jj; j < n; j += (int64_t)gridDim.x * ROWS) {
+  for (int64_t j = (int64_t)B.x * ROWS + jj; j < n; j += (int64_t)B.gx * ROWS) {
     const f32x4* z = reinterpret_cast<const f32x4*>(Z + j * ldz);
     float a0 = 0.0f, a1 = 0.0f;
     for (int q = 0; q < k4 / 4; ++q) {
@@ -709,23 +800,57 @@ __global__ __launch_bounds__(256) void k_next_from_z_cols(const float* __restric
     if (c < r) out[j * r + c] = a0 + a1;
   }
 }
+template <int RM>
+__global__ __launch_bounds__(256) void k_next_from_z_cols(const float* __restrict__ Z, int64_t ldz, int64_t n,
+                                                          int k, const float* __restrict__ Ur, int r,
+                                                          float* __restrict__ out) {
+  __shared__ float us[(KMAX + 4) * RM];
+  next_cols_body<RM>(Z, ldz, n, k, Ur, r, out, us, hw_blk());
+}
+// both remainder layouts and every register width in one launch (jobs pick theirs)
+__global__ __launch_bounds__(256) void k_next_grp(const JobNext* __restrict__ jobs, const int2* __restrict__ map) {
+  __shared__ float us[(KMAX + 4) * 64];
+  Blk B;
+  const JobNext& j = grp_job(jobs, map, B);
+  if (j.cols) {
+    switch (j.rm) {
+      case 8: next_cols_body<8>(j.Z, j.ldz, j.n, j.k, j.Ur, j.r, j.out, us, B); break;
+      case 16: next_cols_body<16>(j.Z, j.ldz, j.n, j.k, j.Ur, j.r, j.out, us, B); break;
+      case 32: next_cols_body<32>(j.Z, j.ldz, j.n, j.k, j.Ur, j.r, j.out, us, B); break;
+      default: next_cols_body<64>(j.Z, j.ldz, j.n, j.k, j.Ur, j.r, j.out, us, B); break;
+    }
+  } else {
+    switch (j.rm) {
+      case 8: next_rows_body<8>(j.Z, j.ldz, j.n, j.k, j.Ur, j.r, j.out, B); break;
+      case 16: next_rows_body<16>(j.Z, j.ldz, j.n, j.k, j.Ur, j.r, j.out, B); break;
+      case 32: next_rows_body<32>(j.Z, j.ldz, j.n, j.k, j.Ur, j.r, j.out, B); break;
+      default: next_rows_body<64>(j.Z, j.ldz, j.n, j.k, j.Ur, j.r, j.out, B); break;
+    }
+  }
+}
+
+int64_t next_from_z_grid(int64_t n, int r, bool* cols) {
+  const int RM = ur_ld(r);
+  const int64_t cap = (int64_t)num_cus() * 16;
+  // short sweeps: a row per lane would leave most of the machine idle, so RM lanes per row
+  *cols = n < ((int64_t)1 << 17);
+  int64_t grid = *cols ? (n * RM + 255) / 256 : (n + 255) / 256;
+  if (grid > cap) grid = cap;
+  if (grid < 1) grid = 1;
+  return grid;
+}
 
 void launch_next_from_z(const float* Z, int64_t ldz, int64_t n, int k, const float* Ur, int r, float* out,
                         hipStream_t st) {
   const int RM = ur_ld(r);
-  const int64_t cap = (int64_t)num_cus() * 16;
-  if (n < (int64_t)1 << 17) {
-    // a row per lane would leave most of the machine idle: RM lanes per row
-    int64_t grid = (n * RM + 255) / 256;
-    if (grid > cap) grid = cap;
-    if (grid < 1) grid = 1;
+  bool cols = false;
+  const int64_t grid = next_from_z_grid(n, r, &cols);
+  if (cols) {
 #define TTX_NFZ(R) hipLaunchKernelGGL((k_next_from_z_cols<R>), dim3((unsigned)grid), dim3(256), 0, st, Z, ldz, n, k, Ur, r, out)
     if (RM == 8) TTX_NFZ(8); else if (RM == 16) TTX_NFZ(16); else if (RM == 32) TTX_NFZ(32); else TTX_NFZ(64);
 #undef TTX_NFZ
     return;
   }
-  int64_t grid = (n + 255) / 256;
-  if (grid > cap) grid = cap;
 #define TTX_NFZ(R) hipLaunchKernelGGL((k_next_from_z<R>), dim3((unsigned)grid), dim3(256), 0, st, Z, ldz, n, k, Ur, r, out)
   if (RM == 8) TTX_NFZ(8); else if (RM == 16) TTX_NFZ(16); else if (RM == 32) TTX_NFZ(32); else TTX_NFZ(64);
 #undef TTX_NFZ
@@ -921,19 +1046,66 @@ void launch_expand_rows(const float* compact, int r1c, int r1, int64_t N, int r,
   hipLaunchKernelGGL(k_expand_rows, dim3((unsigned)grid), dim3(256), 0, st, compact, r1c, r1, N, r, full);
 }
 
-__global__ void k_copy_cols(const float* __restrict__ A, int64_t lda, int m, int n, float* __restrict__ Y, int kp) {
-  for (int64_t e = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; e < (int64_t)m * kp;
-       e += (int64_t)gridDim.x * blockDim.x) {
+__device__ __forceinline__ void copy_cols_body(const float* __restrict__ A, int64_t lda, int m, int n,
+                                               float* __restrict__ Y, int kp, const Blk B) {
+  for (int64_t e = B.x * (int64_t)blockDim.x + threadIdx.x; e < (int64_t)m * kp; e += (int64_t)B.gx * blockDim.x) {
     const int i = (int)(e / kp), c = (int)(e % kp);
     Y[e] = (c < n) ? A[i + (int64_t)c * lda] : 0.0f;
   }
 }
+__global__ void k_copy_cols(const float* __restrict__ A, int64_t lda, int m, int n, float* __restrict__ Y, int kp) {
+  copy_cols_body(A, lda, m, n, Y, kp, hw_blk());
+}
+__global__ __launch_bounds__(256) void k_copy_cols_grp(const JobCopyCols* __restrict__ jobs, const int2* __restrict__ map) {
+  Blk B;
+  const JobCopyCols& j = grp_job(jobs, map, B);
+  copy_cols_body(j.A, j.lda, j.m, j.n, j.Y, j.kp, B);
+}
 
-void launch_copy_cols(const float* A, int64_t lda, int m, int n, float* Y, int kp, hipStream_t st) {
+int64_t copy_cols_grid(int m, int kp) {
   int64_t grid = ((int64_t)m * kp + 255) / 256;
   if (grid > 4096) grid = 4096;
   if (grid < 1) grid = 1;
+  return grid;
+}
+
+void launch_copy_cols(const float* A, int64_t lda, int m, int n, float* Y, int kp, hipStream_t st) {
+  const int64_t grid = copy_cols_grid(m, kp);
   hipLaunchKernelGGL(k_copy_cols, dim3((unsigned)grid), dim3(256), 0, st, A, lda, m, n, Y, kp);
 }
+
+// ------------------------------------------------------------------------------------------------
+// grouped launchers
+// ------------------------------------------------------------------------------------------------
+#define TTX_GRP_ARGS dim3((unsigned)g.blocks), dim3(g.threads), 0, st
+void grp_gram_rows(bool f64, const GroupLaunch& g, hipStream_t st) {
+  const JobGramRows* J = static_cast<const JobGramRows*>(g.jobs);
+  if (f64) hipLaunchKernelGGL(k_gram_rows_grp<double>, TTX_GRP_ARGS, J, g.map);
+  else hipLaunchKernelGGL(k_gram_rows_grp<float>, TTX_GRP_ARGS, J, g.map);
+}
+void grp_chol(const GroupLaunch& g, hipStream_t st) {
+  hipLaunchKernelGGL(k_chol_grp, TTX_GRP_ARGS, static_cast<const JobChol*>(g.jobs), g.map);
+}
+void grp_apply(int kind, const GroupLaunch& g, hipStream_t st) {
+  const JobApply* J = static_cast<const JobApply*>(g.jobs);
+  if (kind == 0) hipLaunchKernelGGL((k_apply_grp<float, 0>), TTX_GRP_ARGS, J, g.map);
+  else if (kind == 1) hipLaunchKernelGGL((k_apply_grp<double, 0>), TTX_GRP_ARGS, J, g.map);
+  else hipLaunchKernelGGL((k_apply_grp<double, 1>), TTX_GRP_ARGS, J, g.map);
+}
+void grp_syevj(const GroupLaunch& g, hipStream_t st, int max_sweeps) {
+  hipLaunchKernelGGL(k_syevj_grp, TTX_GRP_ARGS, static_cast<const JobSyevj*>(g.jobs), g.map, max_sweeps);
+}
+void grp_assemble(const GroupLaunch& g, hipStream_t st) {
+  const JobAssemble* J = static_cast<const JobAssemble*>(g.jobs);
+  hipLaunchKernelGGL(k_assemble_g_grp, TTX_GRP_ARGS, J, g.map);
+  hipLaunchKernelGGL(k_assemble_fix_grp, TTX_GRP_ARGS, J, g.map);
+}
+void grp_next(const GroupLaunch& g, hipStream_t st) {
+  hipLaunchKernelGGL(k_next_grp, TTX_GRP_ARGS, static_cast<const JobNext*>(g.jobs), g.map);
+}
+void grp_copy_cols(const GroupLaunch& g, hipStream_t st) {
+  hipLaunchKernelGGL(k_copy_cols_grp, TTX_GRP_ARGS, static_cast<const JobCopyCols*>(g.jobs), g.map);
+}
+#undef TTX_GRP_ARGS
 
 }  // namespace ttx

@@ -208,4 +208,24 @@ __device__ __forceinline__ float omega_round(float v) {
   return (fabsf(r) < 7.62939453125e-06f) ? 0.0f : r;
 }
 
+// ------------------------------------------------------------------------------------------
+// The block a kernel body works on.  A plain launch takes blockIdx / gridDim; a grouped launch
+// (one launch over the same kernel's work for every tensor of a batch) gives each hardware block
+// one block of one job: map[blockIdx.x] = {job, block index within that job's grid}, and the job
+// descriptor (every descriptor starts with its grid gx x gy) supplies the arguments.  A body run
+// on a job therefore does exactly what the plain launch of that job would do.
+// ------------------------------------------------------------------------------------------
+struct Blk {
+  int x, y, gx, gy;
+};
+__device__ __forceinline__ Blk hw_blk() { return Blk{(int)blockIdx.x, (int)blockIdx.y, (int)gridDim.x, (int)gridDim.y}; }
+
+template <class J>
+__device__ __forceinline__ const J& grp_job(const J* __restrict__ jobs, const int2* __restrict__ map, Blk& B) {
+  const int2 e = map[blockIdx.x];
+  const J& j = jobs[e.x];
+  B = Blk{e.y % j.gx, e.y / j.gx, j.gx, j.gy};
+  return j;
+}
+
 }  // namespace ttx

@@ -5,12 +5,14 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <functional>
 #include <map>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -23,6 +25,32 @@ namespace ttx {
 struct DevBuf {
   void* p = nullptr;
   size_t bytes = 0;
+};
+
+// one grouped launch of the batch engine: kernel kind, template choices, block size and dynamic LDS,
+// the job descriptors (bytes) and the block map; joff / moff locate them in the uploaded blob
+struct GRec {
+  int kind = 0, ar = 0, v0 = 0, v1 = 0, v2 = 0, threads = 256, lds = 0;
+  size_t jsize = 0;
+  int njobs = 0;
+  std::vector<unsigned char> jobs;
+  std::vector<int2> map;
+  size_t joff = 0, moff = 0;
+};
+// a batch's grouped plan: the launches in order, the per-tensor sigma / status words, and the graph
+// of the launches when the batch runs captured
+struct GPlan {
+  uint64_t sig = 0, ws = 0;
+  std::vector<GRec> recs;
+  float* sigstat = nullptr;     // [count][d - 1][TTX_MAX_ELL floats + 4 ints]
+  size_t sigstat_bytes = 0;
+  int steps = 0, count = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  ~GPlan() {
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    if (graph) (void)hipGraphDestroy(graph);
+  }
 };
 
 struct Handle {
@@ -85,9 +113,16 @@ struct Handle {
   // capture (no allocation either); sub-handles run the tensors on their own streams
   bool capture = false;
   bool strict = false;
-  bool autotune = false;
-  std::map<uint64_t, int> tune_own;          // autotuned sketch WG counts per shape class
-  std::map<uint64_t, int>* tune = nullptr;   // the root handle's table (sub-handles share it)
+  // batch mode (the workers of a batch, and the plans of the grouped batch engine): the streaming
+  // passes take the batch grids (a few large workgroups per tensor, since the whole batch runs at
+  // once), sketches take the register-buffered kernel with batch_cpw chunks per workgroup, and a
+  // full-rank exact step is projected like any other (no deferred core)
+  bool batch_mode = false;
+  int64_t batch_cpw = 32;
+  std::map<uint64_t, int64_t> cpw_tuned;   // autotuned batch_cpw per batch signature
+  // the grouped batch engine: its plan, the tensors' workspaces and the uploaded job blob
+  std::unique_ptr<GPlan> gplan;
+  DevBuf barena, bblob;
   std::vector<Handle*> subs;
   hipStream_t bstream = nullptr;             // capture origin of the batch graph (never the null stream)
   hipGraph_t graph = nullptr;
@@ -192,6 +227,21 @@ static ttx_status check_launch(Handle* h, const char* where) {
   } while (0)
 
 static int kp_for(int k) { return k <= 32 ? 32 : 64; }
+
+// Grids of a batch's streaming passes.  The whole batch runs at once, so each tensor takes a few large
+// workgroups instead of filling the machine by itself: the exact Gram 8192 columns per workgroup, the
+// projection about 2 MB of A per workgroup (2048 / m column blocks of 256).
+static int64_t gram_small_grid_batch(int64_t n) {
+  const int64_t g = (n + 8191) / 8192;
+  return g < 1 ? 1 : g;
+}
+static int64_t proj_grid_batch(int64_t m, int64_t n) {
+  const int64_t bpw = m >= 2048 ? 1 : 2048 / m;
+  const int64_t g = (proj_blocks(n) + bpw - 1) / bpw;
+  return g < 1 ? 1 : g;
+}
+static int64_t gram_grid_for(const Handle* h, int64_t n) { return h->batch_mode ? gram_small_grid_batch(n) : gram_small_grid(n); }
+static int64_t proj_grid_for(const Handle* h, int64_t m, int64_t n) { return h->batch_mode ? proj_grid_batch(m, n) : proj_grid(n); }
 
 // ------------------------------------------------------------------------------------------------
 // arithmetic choices
@@ -390,12 +440,12 @@ static ttx_status step_tail(Handle* h, const StepArgs& a, const Arith3& ar, int 
   float* Z = P<float>(h->Z);
   const int kz = (k + 3) & ~3;   // Z keeps only the basis columns (rounded to 16 bytes)
   // Z = A^T Q (FP32-accurate) and its Gram B B^T = Z^T Z in the same pass
-  const int64_t gg = proj_grid(n);
+  const int64_t gg = proj_grid_for(h, m, n);
   TTX_TRY(ensure(h, h->gpart, (size_t)gg * k * k * sizeof(double)));
   const float* Qop = proj_operand(h, a, P<float>(h->Qc), kp);
   {
     KScope ks(h, TTX_K_PROJ, a.m * n * 4 + n * kz * 4);
-    launch_proj(ar.tail, kp, a.A, a.m, m, n, Qop, m, 0, Z, kz, kz, st, P<double>(h->gpart), k);
+    launch_proj_grid(ar.tail, kp, a.A, a.m, m, n, Qop, m, 0, Z, kz, kz, gg, st, P<double>(h->gpart), k);
   }
   o.bytes += (int64_t)(a.m * n * 4);
   o.passes += 1;
@@ -452,68 +502,11 @@ static float* left_mix(Handle* h, const StepArgs& a, float* Y, int kp) {
   return P<float>(h->Y2);
 }
 
-// ------------------------------------------------------------------------------------------------
-// autotuner for the sketch's grid: per shape class (ceil log2 m, ceil log2 n, kp, arithmetic) the
-// number of WGs along the columns, chosen by timing candidate grids on the first unfolding of that
-// class (outputs discarded: the real launch follows on the same buffers)
-// ------------------------------------------------------------------------------------------------
-static int ceil_log2(int64_t v) {
-  int e = 0;
-  while (((int64_t)1 << e) < v) ++e;
-  return e;
-}
-static uint64_t tune_key(int m, int64_t n, int kp, int ar) {
-  return ((uint64_t)ceil_log2(m) << 32) | ((uint64_t)ceil_log2(n) << 16) | ((uint64_t)kp << 4) | (uint64_t)ar;
-}
-static std::map<uint64_t, int>& tune_table(Handle* h) { return h->tune ? *h->tune : h->tune_own; }
-
-static ttx_status autotune_sketch(Handle* h, int ar, const float* A, int m, int64_t n, int kp, int* best) {
-  const int cus = num_cus();
-  const int cand[] = {cus / 4, cus / 2, cus, 2 * cus, 4 * cus};
-  hipEvent_t e0, e1;
-  TTX_HIP(h, hipEventCreate(&e0));
-  TTX_HIP(h, hipEventCreate(&e1));
-  float best_ms = 1e30f;
-  *best = 0;
-  int last_gx = -1;
-  for (int c : cand) {
-    if (c < 1) continue;
-    const SketchPlan p = plan_sketch(m, n, kp, c);
-    if (p.grid_x == last_gx) continue;   // capped by the number of chunks
-    last_gx = p.grid_x;
-    TTX_TRY(ensure(h, h->part, (size_t)p.slabs * m * kp * sizeof(float)));
-    float total = 0.0f;
-    for (int rep = 0; rep < 3; ++rep) {
-      TTX_HIP(h, hipEventRecord(e0, h->stream));
-      launch_sketch(ar, p, A, m, m, n, nullptr, 0, 1, 0x7E57u, kp, P<float>(h->part), h->stream);
-      TTX_HIP(h, hipEventRecord(e1, h->stream));
-      TTX_HIP(h, hipEventSynchronize(e1));
-      float ms = 0.0f;
-      TTX_HIP(h, hipEventElapsedTime(&ms, e0, e1));
-      if (rep > 0) total += ms;    // the first launch warms the caches
-    }
-    if (total < best_ms) {
-      best_ms = total;
-      *best = c;
-    }
-  }
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
-  return check_launch(h, "autotune");
-}
-
+// the sketch's tiling: a batch's plan in batch mode, else the whole-machine plan
 static ttx_status plan_for(Handle* h, int ar, const float* A, int m, int64_t n, int kp, SketchPlan* out) {
-  auto& T = tune_table(h);
-  const uint64_t key = tune_key(m, n, kp, ar);
-  auto it = T.find(key);
-  int hint = 0;
-  if (it != T.end()) {
-    hint = it->second;
-  } else if (h->autotune && !h->strict) {
-    TTX_TRY(autotune_sketch(h, ar, A, m, n, kp, &hint));
-    T[key] = hint;
-  }
-  *out = plan_sketch(m, n, kp, hint);
+  (void)ar;
+  (void)A;
+  *out = h->batch_mode ? plan_sketch_batch(m, n, kp, h->batch_cpw) : plan_sketch(m, n, kp, 0);
   return TTX_OK;
 }
 
@@ -578,7 +571,8 @@ static ttx_status power_iter(Handle* h, const StepArgs& a, const Arith3& ar, int
   float* sc = fp16_scales(h, a, ar, false);
   {
     KScope ks(h, TTX_K_PROJ, a.m * n * 4 + n * kp * 4);
-    launch_proj(ar.range, kp, a.A, a.m, m, n, Qop, m, 0, P<float>(h->Z), kp, 0, st, nullptr, 0, sc);
+    launch_proj_grid(ar.range, kp, a.A, a.m, m, n, Qop, m, 0, P<float>(h->Z), kp, 0, proj_grid_for(h, m, n), st,
+                     nullptr, 0, sc);
   }
   if (sc) {
     KScope ks(h, TTX_K_SMALL, 0);
@@ -774,7 +768,7 @@ static ttx_status run_step(Handle* h, const StepArgs& a_in, StepOut& o) {
     o.path = TTX_PATH_EXACT_M;
     o.ell = 0;
     k = m;
-    int64_t grid = gram_small_grid(n);
+    int64_t grid = gram_grid_for(h, n);
     if (a.gram_pre > 0) {
       grid = a.gram_pre;   // the fused first pass left them in gpart
     } else {
@@ -829,7 +823,8 @@ static ttx_status run_step(Handle* h, const StepArgs& a_in, StepOut& o) {
       const int64_t rw = r1c > 0 ? (int64_t)a.tr_r0 * r1c : r;
       KScope ks(h, TTX_K_PROJ, a.m * n * 4 + rw * n * 4);
       if (r1c > 0) launch_proj(ar.accurate, kpn, a.A, a.m, m, n, Qop, m, 2, a.next, a.tr_r0, r1c, st);
-      else launch_proj(ar.accurate, kpn, a.A, a.m, m, n, Qop, m, 1, a.next, 0, r, st);
+      else launch_proj_grid(ar.accurate, kpn, a.A, a.m, m, n, Qop, m, 1, a.next, 0, r, proj_grid_for(h, m, n), st,
+                            nullptr, 0);
       o.bytes += a.m * n * 4 + rw * n * 4;
       o.passes += 1;
     }
@@ -1042,6 +1037,13 @@ static ttx_status sweep(Handle* h, const float* d_A, const ttx_params* p, float*
   if (!d_A || !d_cores) return fail(h, TTX_ERR_BAD_ARG, "null tensor or core pointers");
   const int d = p->order;
   hipStream_t st = h->stream;
+  // TTX_FLAG_BATCH_PLAN: this tensor as a batch decomposes it
+  struct ModeGuard {
+    Handle* h;
+    bool old;
+    ~ModeGuard() { h->batch_mode = old; }
+  } mode_guard{h, h->batch_mode};
+  if (p->flags & TTX_FLAG_BATCH_PLAN) h->batch_mode = true;
   ttx_report R;
   memset(&R, 0, sizeof(R));
   for (int k = 0; k < TTX_MAX_ORDER; ++k) R.norm2_in[k] = -1.0;
@@ -1103,7 +1105,7 @@ static ttx_status sweep(Handle* h, const float* d_A, const ttx_params* p, float*
     ~PreGuard() { h->pre_valid = false; }
   } pre_guard{h};
   int64_t gram_pre = 0;
-  if (!tr && !h->capture && d >= 3 && p->precision == TTX_PREC_BF16 && !(p->flags & TTX_FLAG_FAST_RANGE) &&
+  if (!tr && !h->capture && !h->batch_mode && d >= 3 && p->precision == TTX_PREC_BF16 && !(p->flags & TTX_FLAG_FAST_RANGE) &&
       p->ranks[0] == 1 && p->dims[0] == 32) {
     const int64_t m0 = 32, n0 = total / 32;
     const int64_t r0 = std::min<int64_t>(p->ranks[1], std::min(m0, n0));
@@ -1260,7 +1262,7 @@ static ttx_status sweep(Handle* h, const float* d_A, const ttx_params* p, float*
     const bool exact_m = (a.method != TTX_METHOD_RSVD) && ((int64_t)a.r + a.s >= mn || a.method == TTX_METHOD_EXACT) &&
                          m_pad <= TTX_MAX_ELL;
     const bool can_defer = !last && !(tr && k == 0) && !pend && exact_m && !(tr_compact && k == 1);
-    a.defer = can_defer && !host_ranks && a.r == m;
+    a.defer = can_defer && !host_ranks && a.r == m && !h->batch_mode;
     a.allow_defer = can_defer && host_ranks;
     // TR first step on the exact path (order >= 3): the projection writes the 231 layout itself (no
     // permute pass), and when r0 r1 exceeds the step's rank bound min(r0 r1, N_1) the rows b >= r1c =
@@ -2029,7 +2031,7 @@ static std::vector<ttx::DevBuf*> handle_bufs(Handle* h) {
           &h->scratch[0], &h->scratch[1], &h->scratch[2], &h->scratch[3], &h->Yt, &h->tpart,
           &h->tout, &h->mP, &h->mM1, &h->mM2, &h->mG, &h->mAct, &h->mPart, &h->mIO,
           &h->f16s, &h->Ypre, &h->omg, &h->Rprobe, &h->hs[0], &h->hs[1], &h->hGs, &h->hB, &h->hB2, &h->hA,
-          &h->G0, &h->lam0, &h->V0, &h->keys0};
+          &h->G0, &h->lam0, &h->V0, &h->keys0, &h->barena, &h->bblob};
 }
 
 static void free_handle(Handle* h, bool own_stream) {
@@ -2040,6 +2042,7 @@ static void free_handle(Handle* h, bool own_stream) {
   h->noalloc = nullptr;
   for (Handle* sub : h->subs) free_handle(sub, true);
   h->subs.clear();
+  h->gplan.reset();
   if (h->gexec) (void)hipGraphExecDestroy(h->gexec);
   if (h->graph) (void)hipGraphDestroy(h->graph);
   if (h->bstream) (void)hipStreamDestroy(h->bstream);
@@ -2478,8 +2481,8 @@ static ttx_status make_subs(Handle* h, int S) {
       free_handle(s, s->stream != nullptr);
       return fail(h, TTX_ERR_HIP, "cannot create a batch worker stream");
     }
-    s->tune = &tune_table(h);
     s->jacobi_max_sweeps = h->jacobi_max_sweeps;
+    s->batch_mode = true;
     h->subs.push_back(s);
   }
   return TTX_OK;
@@ -2515,6 +2518,517 @@ static ttx_status batch_join(Handle* h, hipStream_t origin, int S, std::vector<h
   return TTX_OK;
 }
 
+// ------------------------------------------------------------------------------------------------
+// grouped batch engine: every tensor of the batch goes through the same sequence of step phases in
+// lockstep, and each phase is one launch per kernel variant covering every tensor that has work in
+// it (the blocks of all the tensors' plain launches, concatenated).  The tensors' plans are the
+// worker sweep's in batch mode (same paths, widths, grids and arguments), so each tensor's cores are
+// those of its own sweep, bit for bit.
+// ------------------------------------------------------------------------------------------------
+enum GKind { G_GRAM_SMALL, G_REDUCE_F32, G_REDUCE_F64, G_SKETCH, G_PROJ, G_GRAM_ROWS, G_CHOL, G_APPLY, G_SYEVJ,
+             G_ASSEMBLE, G_NEXT, G_COPY_COLS };
+
+// the launches of one phase, one record per (kernel, template choices, block size), in first-use order
+struct GPhase {
+  std::vector<GRec> recs;
+  template <class J>
+  void add(int kind, int ar, int v0, int v1, int v2, int threads, int lds, const J& j) {
+    GRec* r = nullptr;
+    for (GRec& x : recs)
+      if (x.kind == kind && x.ar == ar && x.v0 == v0 && x.v1 == v1 && x.v2 == v2 && x.threads == threads) r = &x;
+    if (!r) {
+      recs.emplace_back();
+      r = &recs.back();
+      r->kind = kind; r->ar = ar; r->v0 = v0; r->v1 = v1; r->v2 = v2; r->threads = threads;
+      r->jsize = sizeof(J);
+    }
+    r->lds = std::max(r->lds, lds);
+    const unsigned char* b = reinterpret_cast<const unsigned char*>(&j);
+    r->jobs.insert(r->jobs.end(), b, b + sizeof(J));
+    const int nb = j.gx * j.gy;
+    for (int q = 0; q < nb; ++q) r->map.push_back(int2{r->njobs, q});
+    r->njobs += 1;
+  }
+  void flush(std::vector<GRec>& out) {
+    for (GRec& r : recs) out.push_back(std::move(r));
+    recs.clear();
+  }
+};
+
+// one step of one tensor in the grouped plan (the worker sweep's choices in batch mode)
+struct BStep {
+  int path = 0, m = 0, r = 0, k = 0, kp = 32, kpn = 32, kz = 4;
+  int64_t n = 0, ggram = 0, gproj = 0, nb = 0;
+  SketchPlan sp;
+  const float* A = nullptr;
+  float* next = nullptr;
+  float* core = nullptr;
+  uint32_t stream = 0;
+};
+struct BWork {   // one tensor's workspace in the batch arena
+  float *part = nullptr, *Y = nullptr, *Qc = nullptr, *Qn = nullptr, *Z = nullptr, *Ur = nullptr;
+  float* nxt[2] = {nullptr, nullptr};
+  double *gpart = nullptr, *W1 = nullptr, *W2 = nullptr, *Rinv = nullptr, *G = nullptr, *V = nullptr, *lam = nullptr;
+  unsigned long long* keys = nullptr;
+  float* sig = nullptr;   // [d - 1][TTX_MAX_ELL]
+  int* stat = nullptr;    // [d - 1][4]
+};
+
+// the schedule of one tensor (sizes only; pointers are set once the arena exists)
+static ttx_status bplan_tensor(Handle* h, const ttx_params* p, std::vector<BStep>& steps, size_t sz[16]) {
+  const int d = p->order;
+  int64_t total = 1;
+  for (int k = 0; k < d; ++k) total *= p->dims[k];
+  int64_t n = total;
+  int r_prev = 1;
+  for (int q = 0; q < 16; ++q) sz[q] = 0;
+  auto mx = [&](int q, size_t v) { sz[q] = std::max(sz[q], v); };
+  steps.assign(d - 1, BStep());
+  for (int k = 0; k < d - 1; ++k) {
+    BStep& b = steps[k];
+    const int64_t m = (int64_t)r_prev * p->dims[k];
+    n /= p->dims[k];
+    if (m > (1 << 20)) return fail(h, TTX_ERR_UNSUPPORTED, "batch: unfolding with more than 2^20 rows");
+    b.m = (int)m;
+    b.n = n;
+    const int64_t mn = std::min<int64_t>(m, n);
+    b.r = (int)std::min<int64_t>(p->ranks[k + 1], mn);
+    int64_t ell = (int64_t)b.r + p->oversampling;
+    const bool exact = p->method == TTX_METHOD_EXACT || (p->method == TTX_METHOD_AUTO && ell >= mn);
+    if (b.r > TTX_MAX_ELL) return fail(h, TTX_ERR_UNSUPPORTED, "batch: rank above TTX_MAX_ELL");
+    b.gproj = proj_grid_batch(m, n);
+    if (exact && m <= TTX_MAX_ELL) {
+      b.path = TTX_PATH_EXACT_M;
+      b.k = b.m;
+      b.kpn = kp_for(b.r);
+      b.ggram = gram_small_grid_batch(n);
+      mx(0, (size_t)b.ggram * m * m * sizeof(double));          // gpart
+      mx(1, (size_t)b.kpn * m * sizeof(float));                 // Qn
+    } else {
+      if (exact && n > TTX_MAX_ELL) return fail(h, TTX_ERR_UNSUPPORTED, "exact path needs min(m, n) <= TTX_MAX_ELL");
+      if (exact) {
+        b.path = TTX_PATH_EXACT_N;
+        b.k = (int)n;
+      } else {
+        b.path = TTX_PATH_RSVD;
+        ell = std::min<int64_t>(std::min<int64_t>(ell, TTX_MAX_ELL), mn);
+        b.k = (int)ell;
+      }
+      b.kp = kp_for(b.k);
+      b.kz = (b.k + 3) & ~3;
+      b.nb = panel_blocks(m);
+      if (b.path == TTX_PATH_RSVD) {
+        b.sp = plan_sketch_batch(b.m, n, b.kp, h->batch_cpw);
+        mx(2, (size_t)b.sp.slabs * m * b.kp * sizeof(float));    // part
+      }
+      mx(0, (size_t)b.nb * b.k * b.k * sizeof(double));
+      mx(0, (size_t)b.gproj * b.k * b.k * sizeof(double));
+      mx(3, (size_t)m * b.kp * sizeof(float));                  // Y
+      mx(4, (size_t)b.kp * m * sizeof(float));                  // Qc
+      mx(5, (size_t)n * b.kp * sizeof(float));                  // Z
+      mx(6, (size_t)m * b.k * sizeof(double));                  // W1, W2
+      mx(7, (size_t)b.k * b.k * sizeof(double));                // Rinv
+      mx(8, (size_t)b.kz * next_from_z_ld(b.r) * sizeof(float)); // Ur
+    }
+    mx(9, (size_t)b.k * b.k * sizeof(double));                  // G, V
+    if (k < d - 2) mx(10 + (k & 1), (size_t)b.r * n * sizeof(float));   // next[0..1]
+    b.stream = h->stream_base + (uint32_t)k;
+    r_prev = b.r;
+  }
+  return TTX_OK;
+}
+
+static size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
+
+// the grouped plan of a batch: schedules, workspace arena, phases, upload
+static ttx_status gplan_build(Handle* h, const std::vector<ttx_params>& P, const float* const* tensors,
+                              float* const* cores, int order, GPlan& G) {
+  const int count = (int)P.size();
+  const int S = order - 1;
+  std::vector<std::vector<BStep>> sch(count);
+  std::vector<std::array<size_t, 16>> sz(count);
+  size_t total = 0;
+  const size_t sigstat = (size_t)count * S * (TTX_MAX_ELL * sizeof(float) + 4 * sizeof(int));
+  total += align256(sigstat);
+  for (int i = 0; i < count; ++i) {
+    size_t z[16];
+    TTX_TRY(bplan_tensor(h, &P[i], sch[i], z));
+    for (int q = 0; q < 16; ++q) sz[i][q] = z[q];
+    // gpart part Y Qc Qn Z W1 W2 Rinv G V lam keys Ur next0 next1
+    const size_t fixed = align256(TTX_MAX_ELL * sizeof(double)) + align256(64 * sizeof(unsigned long long));
+    total += align256(z[0]) + align256(z[2]) + align256(z[3]) + align256(z[4]) + align256(z[1]) + align256(z[5]) +
+             2 * align256(z[6]) + align256(z[7]) + 2 * align256(z[9]) + fixed + align256(z[8]) + align256(z[10]) +
+             align256(z[11]);
+  }
+  TTX_TRY(ensure(h, h->barena, total));
+  unsigned char* base = static_cast<unsigned char*>(h->barena.p);
+  size_t off = 0;
+  auto take = [&](size_t bytes) -> void* {
+    void* p = bytes ? base + off : nullptr;
+    off += align256(bytes);
+    return p;
+  };
+  G.sigstat = static_cast<float*>(take(sigstat));
+  G.sigstat_bytes = sigstat;
+  G.steps = S;
+  G.count = count;
+  std::vector<BWork> W(count);
+  for (int i = 0; i < count; ++i) {
+    const auto& z = sz[i];
+    BWork& w = W[i];
+    w.gpart = static_cast<double*>(take(z[0]));
+    w.part = static_cast<float*>(take(z[2]));
+    w.Y = static_cast<float*>(take(z[3]));
+    w.Qc = static_cast<float*>(take(z[4]));
+    w.Qn = static_cast<float*>(take(z[1]));
+    w.Z = static_cast<float*>(take(z[5]));
+    w.W1 = static_cast<double*>(take(z[6]));
+    w.W2 = static_cast<double*>(take(z[6]));
+    w.Rinv = static_cast<double*>(take(z[7]));
+    w.G = static_cast<double*>(take(z[9]));
+    w.V = static_cast<double*>(take(z[9]));
+    w.lam = static_cast<double*>(take(TTX_MAX_ELL * sizeof(double)));
+    w.keys = static_cast<unsigned long long*>(take(64 * sizeof(unsigned long long)));
+    w.Ur = static_cast<float*>(take(z[8]));
+    w.nxt[0] = static_cast<float*>(take(z[10]));
+    w.nxt[1] = static_cast<float*>(take(z[11]));
+    float* ss = G.sigstat + (size_t)i * S * (TTX_MAX_ELL + 4);
+    w.sig = ss;
+    w.stat = reinterpret_cast<int*>(ss + (size_t)S * TTX_MAX_ELL);
+    // operands of each step: the input, then the ping-pong buffers; the last step writes the last core
+    const float* cur = tensors[i];
+    for (int k = 0; k < S; ++k) {
+      BStep& b = sch[i][k];
+      b.A = cur;
+      b.core = cores[(size_t)i * order + k];
+      b.next = (k == S - 1) ? cores[(size_t)i * order + order - 1] : w.nxt[k & 1];
+      cur = b.next;
+    }
+  }
+  if (off > total) return fail(h, TTX_ERR_HIP, "internal: batch arena overrun");
+
+  const Arith3 ar = arith_for(P[0].precision, P[0].flags);
+  G.recs.clear();
+  GPhase ph;
+  auto orth_phases = [&](int k, auto sel) {
+    // shifted CholeskyQR, CholeskyQR, Newton-Schulz (orth(), one phase per launch of it)
+    for (int stage = 0; stage < 6; ++stage) {
+      for (int i = 0; i < count; ++i) {
+        const BStep& b = sch[i][k];
+        if (!sel(i)) continue;
+        const BWork& w = W[i];
+        int* stat = w.stat + 4 * k;
+        const int64_t rpw = ((((int64_t)b.m + b.nb - 1) / b.nb + 63) / 64) * 64;
+        const int pb = (int)panel_blocks(b.m);
+        switch (stage) {
+          case 0: ph.add(G_GRAM_ROWS, 0, 0, 0, 0, 256, 0, JobGramRows{(int)b.nb, 1, w.Y, b.kp, b.m, b.k, rpw, w.gpart}); break;
+          case 1: ph.add(G_CHOL, 0, 0, 0, 0, 256, 0, JobChol{1, 1, w.gpart, b.nb, b.k, b.m, 1, w.Rinv, stat}); break;
+          case 2: ph.add(G_APPLY, 0, 0, 0, 0, 256, 0, JobApply{pb, 1, w.Y, b.kp, b.m, b.k, b.k, w.Rinv, w.W1, nullptr, 0,
+                                                              w.gpart, nullptr, 0, b.k}); break;
+          case 3: ph.add(G_CHOL, 0, 0, 0, 0, 256, 0, JobChol{1, 1, w.gpart, b.nb, b.k, b.m, 0, w.Rinv, stat}); break;
+          case 4: ph.add(G_APPLY, 0, 1, 0, 0, 256, 0, JobApply{pb, 1, w.W1, b.k, b.m, b.k, b.k, w.Rinv, w.W2, nullptr, 0,
+                                                              w.gpart, nullptr, 0, b.k}); break;
+          default: ph.add(G_APPLY, 0, 2, 0, 0, 256, 0, JobApply{pb, 1, w.W2, b.k, b.m, b.k, b.kp, nullptr, nullptr, w.Qc,
+                                                               b.m, nullptr, w.gpart, b.nb, b.k}); break;
+        }
+      }
+      ph.flush(G.recs);
+    }
+  };
+  auto is_em = [](const BStep& b) { return b.path == TTX_PATH_EXACT_M; };
+  auto is_rs = [](const BStep& b) { return b.path == TTX_PATH_RSVD; };
+  auto not_em = [](const BStep& b) { return b.path != TTX_PATH_EXACT_M; };
+  for (int k = 0; k < S; ++k) {
+    // A: exact Gram / copy of the short side / Omega sketch
+    for (int i = 0; i < count; ++i) {
+      const BStep& b = sch[i][k];
+      const BWork& w = W[i];
+      if (b.path == TTX_PATH_EXACT_M) {
+        int64_t cpw = (b.n + b.ggram - 1) / b.ggram;
+        cpw = ((cpw + 63) / 64) * 64;
+        ph.add(G_GRAM_SMALL, ar.accurate, b.m <= 32 ? 1 : 2, 0, 0, 256, 0,
+               JobGramSmall{(int)b.ggram, 1, b.A, b.m, b.m, b.n, cpw, w.gpart});
+      } else if (b.path == TTX_PATH_EXACT_N) {
+        ph.add(G_COPY_COLS, 0, 0, 0, 0, 256, 0, JobCopyCols{(int)copy_cols_grid(b.m, b.kp), 1, b.A, b.m, b.m, b.k, w.Y, b.kp});
+      } else {
+        const SketchPlan& sp = b.sp;
+        ph.add(G_SKETCH, ar.range, b.kp / 32, 1, 1, 64 * sp.RB * sp.CG, sp.lds_bytes,
+               JobSketch{sp.grid_x, sp.ysplit, b.A, b.m, b.m, b.n, nullptr, 0, P[i].omega_seed, b.stream, b.k, sp.RB,
+                         sp.CG, sp.chunks_per_wg, w.part});
+      }
+    }
+    ph.flush(G.recs);
+    // B: reductions
+    for (int i = 0; i < count; ++i) {
+      const BStep& b = sch[i][k];
+      const BWork& w = W[i];
+      if (b.path == TTX_PATH_EXACT_M) {
+        const int64_t cnt = (int64_t)b.m * b.m;
+        ph.add(G_REDUCE_F64, 0, 0, 0, 0, 256, 0, JobReduce{(int)((cnt + 31) / 32), 1, w.gpart, b.ggram, cnt, w.G});
+      } else if (b.path == TTX_PATH_RSVD) {
+        const int64_t cnt = (int64_t)b.m * b.kp;
+        ph.add(G_REDUCE_F32, 0, 0, 0, 0, 256, 0, JobReduce{(int)((cnt + 31) / 32), 1, w.part, b.sp.slabs, cnt, w.Y});
+      }
+    }
+    ph.flush(G.recs);
+    orth_phases(k, [&](int i) { return not_em(sch[i][k]); });
+    // power iterations
+    int qmax = 0;
+    for (int i = 0; i < count; ++i)
+      if (is_rs(sch[i][k])) qmax = std::max(qmax, P[i].power_iters);
+    for (int it = 0; it < qmax; ++it) {
+      auto sel = [&](int i) { return is_rs(sch[i][k]) && it < P[i].power_iters; };
+      for (int i = 0; i < count; ++i) {
+        if (!sel(i)) continue;
+        const BStep& b = sch[i][k];
+        const BWork& w = W[i];
+        ph.add(G_PROJ, ar.range, b.kp / 32, 0, 0, 512, 0,
+               JobProj{(int)b.gproj, 1, b.A, b.m, b.m, b.n, w.Qc, b.m, 0, w.Z, b.kp, 0, proj_blocks(b.n), nullptr, 0});
+      }
+      ph.flush(G.recs);
+      for (int i = 0; i < count; ++i) {
+        if (!sel(i)) continue;
+        const BStep& b = sch[i][k];
+        const BWork& w = W[i];
+        const SketchPlan& sp = b.sp;
+        const int fq = b.kp / (16 * sp.RB);
+        ph.add(G_SKETCH, ar.range, b.kp / 32, 0, fq <= 1 ? 1 : (fq == 2 ? 2 : 4), 64 * sp.RB * sp.CG, sp.lds_bytes,
+               JobSketch{sp.grid_x, sp.ysplit, b.A, b.m, b.m, b.n, w.Z, b.kp, 0, 0, b.k, sp.RB, sp.CG,
+                         sp.chunks_per_wg, w.part});
+      }
+      ph.flush(G.recs);
+      for (int i = 0; i < count; ++i) {
+        if (!sel(i)) continue;
+        const BStep& b = sch[i][k];
+        const BWork& w = W[i];
+        const int64_t cnt = (int64_t)b.m * b.kp;
+        ph.add(G_REDUCE_F32, 0, 0, 0, 0, 256, 0, JobReduce{(int)((cnt + 31) / 32), 1, w.part, b.sp.slabs, cnt, w.Y});
+      }
+      ph.flush(G.recs);
+      orth_phases(k, sel);
+    }
+    // E: the tail's projection with its Gram, and its reduction
+    for (int i = 0; i < count; ++i) {
+      const BStep& b = sch[i][k];
+      if (!not_em(b)) continue;
+      const BWork& w = W[i];
+      ph.add(G_PROJ, ar.tail, b.kp / 32, 1, 0, 512, 0,
+             JobProj{(int)b.gproj, 1, b.A, b.m, b.m, b.n, w.Qc, b.m, 0, w.Z, b.kz, b.kz, proj_blocks(b.n), w.gpart, b.k});
+    }
+    ph.flush(G.recs);
+    for (int i = 0; i < count; ++i) {
+      const BStep& b = sch[i][k];
+      if (!not_em(b)) continue;
+      const BWork& w = W[i];
+      const int64_t cnt = (int64_t)b.k * b.k;
+      ph.add(G_REDUCE_F64, 0, 0, 0, 0, 256, 0, JobReduce{(int)((cnt + 31) / 32), 1, w.gpart, b.gproj, cnt, w.G});
+    }
+    ph.flush(G.recs);
+    // F: eigensolver (every tensor)
+    for (int i = 0; i < count; ++i) {
+      const BStep& b = sch[i][k];
+      const BWork& w = W[i];
+      ph.add(G_SYEVJ, 0, 0, 0, 0, 1024, 0,
+             JobSyevj{1, 1, w.G, b.k, w.lam, w.V, w.stat + 4 * k, w.keys, syevj_threads(b.k)});
+    }
+    ph.flush(G.recs);
+    // G: core assembly
+    for (int i = 0; i < count; ++i) {
+      const BStep& b = sch[i][k];
+      const BWork& w = W[i];
+      const int gb = (int)((b.m + 63) / 64);
+      float* sig = w.sig + (size_t)k * TTX_MAX_ELL;
+      if (is_em(b))
+        ph.add(G_ASSEMBLE, 0, 0, 0, 0, 256, 0,
+               JobAssemble{gb, 1, nullptr, 0, b.m, b.m, w.V, w.lam, b.r, b.core, nullptr, w.Qn, b.kpn, sig, w.keys});
+      else
+        ph.add(G_ASSEMBLE, 0, 0, 0, 0, 256, 0,
+               JobAssemble{gb, 1, w.Qc, b.m, b.m, b.k, w.V, w.lam, b.r, b.core, w.Ur, nullptr, 0, sig, w.keys});
+    }
+    ph.flush(G.recs);
+    // H: the next unfolding (exact projection, or the remainder from Z)
+    for (int i = 0; i < count; ++i) {
+      const BStep& b = sch[i][k];
+      const BWork& w = W[i];
+      if (is_em(b)) {
+        ph.add(G_PROJ, ar.accurate, b.kpn / 32, 0, 0, 512, 0,
+               JobProj{(int)b.gproj, 1, b.A, b.m, b.m, b.n, w.Qn, b.m, 1, b.next, 0, b.r, proj_blocks(b.n), nullptr, 0});
+      } else {
+        bool cols = false;
+        const int64_t g = next_from_z_grid(b.n, b.r, &cols);
+        ph.add(G_NEXT, 0, 0, 0, 0, 256, 0,
+               JobNext{(int)g, 1, w.Z, b.kz, b.n, b.k, w.Ur, b.r, b.next, next_from_z_ld(b.r), cols ? 1 : 0});
+      }
+    }
+    ph.flush(G.recs);
+  }
+  // upload every descriptor and block map in one copy
+  size_t blob = 0;
+  for (GRec& r : G.recs) {
+    r.joff = blob;
+    blob = align256(blob + r.jobs.size());
+    r.moff = blob;
+    blob = align256(blob + r.map.size() * sizeof(int2));
+  }
+  TTX_TRY(ensure(h, h->bblob, blob));
+  TTX_HIP(h, hipStreamSynchronize(h->stream));   // nothing queued earlier still reads the blob
+  std::vector<unsigned char> host(blob, 0);
+  for (const GRec& r : G.recs) {
+    memcpy(host.data() + r.joff, r.jobs.data(), r.jobs.size());
+    memcpy(host.data() + r.moff, r.map.data(), r.map.size() * sizeof(int2));
+  }
+  TTX_HIP(h, hipMemcpy(h->bblob.p, host.data(), blob, hipMemcpyHostToDevice));
+  return TTX_OK;
+}
+
+// queue the plan's launches on st
+static ttx_status gplan_queue(Handle* h, const GPlan& G, hipStream_t st) {
+  launch_zero(G.sigstat, (int64_t)G.sigstat_bytes, st);
+  const unsigned char* blob = static_cast<const unsigned char*>(h->bblob.p);
+  for (const GRec& r : G.recs) {
+    if (r.map.empty()) continue;
+    GroupLaunch g;
+    g.jobs = blob + r.joff;
+    g.map = reinterpret_cast<const int2*>(blob + r.moff);
+    g.blocks = (int)r.map.size();
+    g.threads = r.threads;
+    g.lds = r.lds;
+    switch (r.kind) {
+      case G_GRAM_SMALL: grp_gram_small(r.ar, r.v0, g, st); break;
+      case G_REDUCE_F32: grp_reduce(false, g, st); break;
+      case G_REDUCE_F64: grp_reduce(true, g, st); break;
+      case G_SKETCH: grp_sketch(r.ar, r.v0, r.v1 != 0, r.v2, g, st); break;
+      case G_PROJ: grp_proj(r.ar, r.v0, r.v1 != 0, g, st); break;
+      case G_GRAM_ROWS: grp_gram_rows(false, g, st); break;
+      case G_CHOL: grp_chol(g, st); break;
+      case G_APPLY: grp_apply(r.v0, g, st); break;
+      case G_SYEVJ: grp_syevj(g, st, h->jacobi_max_sweeps); break;
+      case G_ASSEMBLE: grp_assemble(g, st); break;
+      case G_NEXT: grp_next(g, st); break;
+      case G_COPY_COLS: grp_copy_cols(g, st); break;
+      default: return fail(h, TTX_ERR_HIP, "internal: unknown grouped kernel");
+    }
+  }
+  return check_launch(h, "grouped batch");
+}
+
+// run a batch on the grouped engine: build (or reuse) its plan, then queue it eagerly on the handle's
+// stream or, with TTX_BATCH_GRAPH, as one launch of the plan captured into a graph
+static ttx_status grouped_run(Handle* h, const std::vector<ttx_params>& P, const float* const* tensors,
+                              float* const* cores, int order, int bflags, uint64_t sig, ttx_batch_report* rep) {
+  const int count = (int)P.size();
+  const bool graph = (bflags & TTX_BATCH_GRAPH) != 0;
+  if (!h->gplan || h->gplan->sig != sig) {
+    h->gplan.reset();
+    std::unique_ptr<GPlan> G(new GPlan());
+    TTX_TRY(gplan_build(h, P, tensors, cores, order, *G));
+    G->sig = sig;
+    h->gplan = std::move(G);
+  }
+  GPlan& G = *h->gplan;
+  const bool reused = graph && G.gexec != nullptr;
+  if (!h->bstream && hipStreamCreateWithFlags(&h->bstream, hipStreamNonBlocking) != hipSuccess) {
+    h->bstream = nullptr;
+    return fail(h, TTX_ERR_HIP, "cannot create the batch stream");
+  }
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};   // e0, e1, fork, join
+  struct EvGuard {
+    hipEvent_t* e;
+    ~EvGuard() {
+      for (int i = 0; i < 4; ++i)
+        if (e[i]) (void)hipEventDestroy(e[i]);
+    }
+  } evg{ev};
+  for (int i = 0; i < 4; ++i)
+    TTX_HIP(h, i < 2 ? hipEventCreate(&ev[i]) : hipEventCreateWithFlags(&ev[i], hipEventDisableTiming));
+  if (graph && !G.gexec) {
+    if (hipStreamBeginCapture(h->bstream, hipStreamCaptureModeRelaxed) != hipSuccess) {
+      (void)hipGetLastError();
+      return fail(h, TTX_ERR_HIP, "batch: begin capture");
+    }
+    const ttx_status qs = gplan_queue(h, G, h->bstream);
+    hipGraph_t g = nullptr;
+    const hipError_t ce = hipStreamEndCapture(h->bstream, &g);
+    if (qs != TTX_OK || ce != hipSuccess || !g) {
+      if (g) (void)hipGraphDestroy(g);
+      (void)hipGetLastError();
+      return qs != TTX_OK ? qs : fail(h, TTX_ERR_HIP, "batch: end capture");
+    }
+    G.graph = g;
+    if (hipGraphInstantiate(&G.gexec, g, nullptr, nullptr, 0) != hipSuccess) {
+      G.gexec = nullptr;
+      (void)hipGetLastError();
+      return fail(h, TTX_ERR_HIP, "batch: graph instantiation");
+    }
+  }
+  TTX_HIP(h, hipEventRecord(ev[0], h->stream));
+  if (graph) {
+    // the caller's stream -> the graph on the batch stream -> the caller's stream
+    TTX_HIP(h, hipEventRecord(ev[2], h->stream));
+    TTX_HIP(h, hipStreamWaitEvent(h->bstream, ev[2], 0));
+    TTX_HIP(h, hipGraphLaunch(G.gexec, h->bstream));
+    TTX_HIP(h, hipEventRecord(ev[3], h->bstream));
+    TTX_HIP(h, hipStreamWaitEvent(h->stream, ev[3], 0));
+  } else {
+    TTX_TRY(gplan_queue(h, G, h->stream));
+  }
+  TTX_HIP(h, hipEventRecord(ev[1], h->stream));
+  TTX_HIP(h, hipEventSynchronize(ev[1]));
+  float ms = 0.0f;
+  (void)hipEventElapsedTime(&ms, ev[0], ev[1]);
+  // the eigensolvers' convergence flags (status word 2 of every step of every tensor)
+  std::vector<float> ss(G.sigstat_bytes / sizeof(float));
+  TTX_HIP(h, hipMemcpy(ss.data(), G.sigstat, G.sigstat_bytes, hipMemcpyDeviceToHost));
+  ttx_batch_report R;
+  memset(&R, 0, sizeof(R));
+  R.total_ms = ms;
+  R.tensors_per_s = ms > 0 ? count / (ms * 1e-3) : 0.0;
+  R.p50_ms = R.p99_ms = -1.0;
+  R.streams = 1;
+  R.graph = graph ? 1 : 0;
+  R.graph_reused = reused ? 1 : 0;
+  R.tuned_shapes = (int)h->cpw_tuned.size();
+  R.sketch_chunks = (int)h->batch_cpw;
+  R.engine = 1;
+  R.launches = 0;
+  for (const GRec& r : G.recs) R.launches += r.map.empty() ? 0 : (r.kind == G_ASSEMBLE ? 2 : 1);
+  if (bflags & TTX_BATCH_PER_TENSOR) {
+    // each tensor alone through its worker sweep (the same plan), queued work only
+    Handle* sub = h->subs[0];
+    sub->capture = true;
+    std::vector<double> t(count);
+    ttx_status st = TTX_OK;
+    for (int i = 0; i < count && st == TTX_OK; ++i) {
+      if (hipEventRecord(ev[0], sub->stream) != hipSuccess) { st = fail(h, TTX_ERR_HIP, "batch: event"); break; }
+      const ttx_status s2 = sweep(sub, tensors[i], &P[i], cores + (size_t)i * order, nullptr, nullptr, false);
+      if (s2 != TTX_OK) { st = fail(h, s2, sub->err); break; }
+      if (hipEventRecord(ev[1], sub->stream) != hipSuccess || hipEventSynchronize(ev[1]) != hipSuccess) {
+        st = fail(h, TTX_ERR_HIP, "batch: per-tensor run");
+        break;
+      }
+      float ti = 0.0f;
+      (void)hipEventElapsedTime(&ti, ev[0], ev[1]);
+      t[i] = ti;
+    }
+    sub->capture = false;
+    TTX_TRY(st);
+    std::sort(t.begin(), t.end());
+    R.p50_ms = t[(size_t)((count - 1) * 0.50)];
+    R.p99_ms = t[(size_t)((count - 1) * 0.99)];
+  }
+  if (rep) *rep = R;
+  const int S = G.steps;
+  for (int i = 0; i < count; ++i) {
+    const int* stat = reinterpret_cast<const int*>(ss.data() + ((size_t)i * S * (TTX_MAX_ELL + 4) + (size_t)S * TTX_MAX_ELL));
+    for (int k = 0; k < S; ++k)
+      if (stat[4 * k + 2])
+        return fail(h, TTX_ERR_NO_CONVERGE, "batch: tensor " + std::to_string(i) + " step " + std::to_string(k) +
+                                                ": the Jacobi eigensolver reached its cap of " +
+                                                std::to_string(h->jacobi_max_sweeps) + " sweeps before converging");
+  }
+  return TTX_OK;
+}
+
 static ttx_status batch_run(Handle* h, int count, int order, const int64_t* dims, const int32_t* ranks,
                             const float* const* tensors, float* const* cores, const ttx_params* common, int S,
                             int bflags, ttx_batch_report* rep) {
@@ -2527,9 +3041,13 @@ static ttx_status batch_run(Handle* h, int count, int order, const int64_t* dims
   // it: a host round trip per check, which a captured graph could neither contain nor replay
   if (common->flags & TTX_FLAG_ADAPTIVE)
     return fail(h, TTX_ERR_UNSUPPORTED, "batch: no adaptive range finder (fixed widths and power iterations only)");
-  if ((bflags & TTX_BATCH_GRAPH) && count > TTX_BATCH_GRAPH_MAX)
-    return fail(h, TTX_ERR_UNSUPPORTED, "batch: graph capture takes at most " + std::to_string(TTX_BATCH_GRAPH_MAX) +
-                                            " tensors; split the batch or run it without TTX_BATCH_GRAPH");
+  // the grouped engine takes every arithmetic but FP16 (whose operand scales are per-step values the
+  // grouped jobs do not carry); FP16 and TTX_BATCH_STREAMS run the worker streams
+  const bool grouped = !(bflags & TTX_BATCH_STREAMS) && common->precision != TTX_PREC_FP16;
+  if (!grouped && (bflags & TTX_BATCH_GRAPH) && count > TTX_BATCH_GRAPH_MAX)
+    return fail(h, TTX_ERR_UNSUPPORTED, "batch: graph capture over worker streams takes at most " +
+                                            std::to_string(TTX_BATCH_GRAPH_MAX) +
+                                            " tensors; split the batch or use the grouped engine");
   std::vector<ttx_params> P(count);
   for (int i = 0; i < count; ++i) {
     ttx_params_init(&P[i], order, dims + (size_t)i * order, ranks + (size_t)i * (order + 1));
@@ -2546,7 +3064,7 @@ static ttx_status batch_run(Handle* h, int count, int order, const int64_t* dims
     for (int k = 0; k < order; ++k)
       if (!cores[(size_t)i * order + k]) return fail(h, TTX_ERR_BAD_ARG, "batch: null core");
   }
-  TTX_TRY(make_subs(h, S));
+  TTX_TRY(make_subs(h, grouped ? 1 : S));
   const bool graph = (bflags & TTX_BATCH_GRAPH) != 0;
   uint64_t sig = 0xCBF29CE484222325ull;
   sig = fnv(sig, &count, sizeof(count));
@@ -2560,6 +3078,33 @@ static ttx_status batch_run(Handle* h, int count, int order, const int64_t* dims
   sig = fnv(sig, &P[0].omega_seed, sizeof(uint64_t));
   sig = fnv(sig, &P[0].precision, sizeof(int32_t) * 2);
   sig = fnv(sig, &P[0].flags, sizeof(int32_t));
+  sig = fnv(sig, &h->jacobi_max_sweeps, sizeof(int));
+  // autotuner: the sketch's chunks per workgroup (its tile along the columns, which also sets the
+  // partial-sum traffic), chosen for this batch by timing one warm eager grouped run per candidate
+  if ((bflags & TTX_BATCH_AUTOTUNE) && common->precision != TTX_PREC_FP16 && !h->cpw_tuned.count(sig)) {
+    const int64_t cand[] = {8, 16, 32, 64};
+    double best_ms = 1e300;
+    int64_t best = h->batch_cpw;
+    for (int64_t c : cand) {
+      h->batch_cpw = c;
+      const uint64_t sc = fnv(sig, &c, sizeof(c));
+      ttx_batch_report r;
+      TTX_TRY(grouped_run(h, P, tensors, cores, order, 0, sc, &r));   // builds the plan, warms
+      TTX_TRY(grouped_run(h, P, tensors, cores, order, 0, sc, &r));
+      if (r.total_ms < best_ms) {
+        best_ms = r.total_ms;
+        best = c;
+      }
+    }
+    h->cpw_tuned[sig] = best;
+  }
+  {
+    auto it = h->cpw_tuned.find(sig);
+    if (it != h->cpw_tuned.end()) h->batch_cpw = it->second;
+  }
+  for (Handle* sub : h->subs) sub->batch_cpw = h->batch_cpw;
+  sig = fnv(sig, &h->batch_cpw, sizeof(int64_t));
+  if (grouped) return grouped_run(h, P, tensors, cores, order, bflags, sig, rep);
   // the workers' buffers the graph's kernels were captured with: any reallocation since (a larger
   // batch, the per-tensor timing loop) invalidates it
   auto workspace_gen = [&]() {
@@ -2572,7 +3117,7 @@ static ttx_status batch_run(Handle* h, int count, int order, const int64_t* dims
   hipEvent_t fork = nullptr, e0 = nullptr, e1 = nullptr;
   std::vector<hipEvent_t> joins(S, nullptr);
   auto cleanup = [&]() {
-    for (Handle* sub : h->subs) { sub->capture = false; sub->strict = false; sub->autotune = false; }
+    for (Handle* sub : h->subs) { sub->capture = false; sub->strict = false; }
     if (fork) (void)hipEventDestroy(fork);
     for (hipEvent_t e : joins) if (e) (void)hipEventDestroy(e);
     if (e0) (void)hipEventDestroy(e0);
@@ -2588,9 +3133,9 @@ static ttx_status batch_run(Handle* h, int count, int order, const int64_t* dims
         break;
       }
     if (st != TTX_OK) break;
-    for (Handle* sub : h->subs) { sub->capture = true; sub->autotune = (bflags & TTX_BATCH_AUTOTUNE) != 0; }
+    for (Handle* sub : h->subs) sub->capture = true;
     if (!reuse) {
-      // eager pass: sizes every worker's workspace and runs the autotuner
+      // eager pass: sizes every worker's workspace
       if ((st = batch_fork(h, h->stream, S, fork)) != TTX_OK) break;
       if ((st = batch_queue(h, S, P, tensors, cores, order)) != TTX_OK) break;
       if ((st = batch_join(h, h->stream, S, joins)) != TTX_OK) break;
@@ -2655,7 +3200,10 @@ static ttx_status batch_run(Handle* h, int count, int order, const int64_t* dims
     R.streams = S;
     R.graph = graph ? 1 : 0;
     R.graph_reused = reuse ? 1 : 0;
-    R.tuned_shapes = (int)tune_table(h).size();
+    R.tuned_shapes = (int)h->cpw_tuned.size();
+    R.sketch_chunks = (int)h->batch_cpw;
+    R.engine = 0;
+    R.launches = 0;
     if (bflags & TTX_BATCH_PER_TENSOR) {
       // each tensor alone on one worker stream (queued work only, as in the batch)
       Handle* sub = h->subs[0];

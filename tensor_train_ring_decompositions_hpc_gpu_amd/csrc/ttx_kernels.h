@@ -11,8 +11,12 @@ int num_cus();
 struct SketchPlan {
   int RB = 1, CG = 1, ysplit = 1, grid_x = 1, kp = 32, lds_bytes = 0;
   int64_t chunks_per_wg = 1, slabs = 1;
+  bool plain = false;   // the register-buffered kernel only (the one batches group)
 };
 SketchPlan plan_sketch(int m, int64_t n, int kp, int wgs_hint);
+// a batch's plan: the same tiling, cpw chunks per WG (partial slabs of about kp / (16 cpw) of A's size),
+// the register-buffered kernel
+SketchPlan plan_sketch_batch(int m, int64_t n, int kp, int64_t cpw);
 // Y partials: part[slab][m][kp].  X == nullptr selects the in-kernel Philox sketch (seed, stream, ell).
 // FP16 arithmetic: scales (optional) holds power-of-two operand scales, [0] for A and [1 + c] for
 // column c of X, undone on the partial sums.
@@ -159,5 +163,60 @@ void mlp_backward(const float* P, int in, int hid, int out, const float* X, cons
                   float* dZ, float* grad, double* part, hipStream_t st);
 int mlp_loss_blocks(int64_t count);
 void mlp_adam(float* P, const float* g, float* m1, float* m2, int count, float lr, int t, hipStream_t st);
+
+// ---- grouped launches (batches) ---------------------------------------------------------------
+// One launch runs one kernel over many jobs: hardware block b runs block map[b].y of job map[b].x.
+// Each job descriptor starts with its grid (gx x gy, the grid its plain launch would have) and
+// carries that launch's arguments, so a job's results are those of its plain launch, bit for bit.
+struct GroupLaunch {
+  const void* jobs = nullptr;   // device array of descriptors
+  const int2* map = nullptr;    // device array, one entry per block
+  int blocks = 0, threads = 0, lds = 0;
+};
+struct JobGramSmall { int gx, gy; const float* A; int64_t lda; int m; int64_t n; int64_t cpw; double* part; };
+struct JobReduce { int gx, gy; const void* part; int64_t S; int64_t count; void* out; };
+struct JobSketch {
+  int gx, gy; const float* A; int64_t lda; int m; int64_t n; const float* X; int64_t ldx; uint64_t seed;
+  uint32_t stream; int ell; int RB, CG; int64_t cpw; float* part;
+};
+struct JobProj {
+  int gx, gy; const float* A; int64_t lda; int m; int64_t n; const float* Qc; int64_t ldq; int mode; float* out;
+  int64_t ldo; int r_out; int64_t blocks; double* gpart; int kg;
+};
+struct JobGramRows { int gx, gy; const void* X; int64_t ldx; int64_t rows; int k; int64_t rpw; double* part; };
+struct JobChol { int gx, gy; const double* part; int64_t S; int k; int64_t m; int shift; double* Rinv; int* status; };
+struct JobApply {
+  int gx, gy; const void* Win; int64_t ldi; int64_t rows; int k; int kp; const double* Rinv; double* Wd; float* Qc;
+  int64_t ldq; double* part; const double* gpart; int64_t S; int ldr;
+};
+struct JobSyevj { int gx, gy; const double* G; int k; double* lam; double* V; int* status; unsigned long long* keys; int nt; };
+struct JobAssemble {
+  int gx, gy; const float* Qc; int64_t ldq; int m; int k; const double* V; const double* lam; int r; float* Gc;
+  float* Ur; float* Qn; int kpn; float* sigma; unsigned long long* keys;
+};
+struct JobNext { int gx, gy; const float* Z; int64_t ldz; int64_t n; int k; const float* Ur; int r; float* out; int rm; int cols; };
+struct JobCopyCols { int gx, gy; const float* A; int64_t lda; int m; int n; float* Y; int kp; };
+
+// grids the plain launches use, so a batch plan can build the same jobs
+int64_t proj_blocks(int64_t n);
+int syevj_threads(int k);
+int64_t next_from_z_grid(int64_t n, int r, bool* cols);
+int64_t copy_cols_grid(int m, int kp);
+// plain launches with an explicit grid (the batch plans' grids; grid 0: the default)
+void launch_proj_grid(int ar, int kp, const float* A, int64_t lda, int m, int64_t n, const float* Qc, int64_t ldq,
+                      int mode, float* out, int64_t ldo, int r_out, int64_t grid, hipStream_t st, double* gpart, int kg,
+                      const float* scales = nullptr);
+// grouped launchers (ar: the Arith of every job; variant: the template choice the jobs share)
+void grp_gram_small(int ar, int mt, const GroupLaunch& g, hipStream_t st);
+void grp_reduce(bool f64, const GroupLaunch& g, hipStream_t st);
+void grp_sketch(int ar, int lt, bool omega, int fq, const GroupLaunch& g, hipStream_t st);
+void grp_proj(int ar, int lt, bool gram, const GroupLaunch& g, hipStream_t st);
+void grp_gram_rows(bool f64, const GroupLaunch& g, hipStream_t st);
+void grp_chol(const GroupLaunch& g, hipStream_t st);
+void grp_apply(int kind, const GroupLaunch& g, hipStream_t st);   // 0: float in, 1: double in, 2: Newton-Schulz out
+void grp_syevj(const GroupLaunch& g, hipStream_t st, int max_sweeps);
+void grp_assemble(const GroupLaunch& g, hipStream_t st);   // both passes (two launches)
+void grp_next(const GroupLaunch& g, hipStream_t st);
+void grp_copy_cols(const GroupLaunch& g, hipStream_t st);
 
 }  // namespace ttx

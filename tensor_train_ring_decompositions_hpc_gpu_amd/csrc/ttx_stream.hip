@@ -221,10 +221,10 @@ __device__ __forceinline__ void sketch_fill(float* __restrict__ xs, int pitch, i
 }
 
 template <int LT, int AR, bool OMEGA, int FQ>
-__global__ __launch_bounds__(512) void k_sketch(const float* __restrict__ A, int64_t lda, int m, int64_t n,
-                                                const float* __restrict__ X, int64_t ldx, uint64_t seed,
-                                                uint32_t stream, int ell, int RB, int CG, int64_t chunks_per_wg,
-                                                float* __restrict__ part, const float* __restrict__ scales) {
+__device__ __forceinline__ void sketch_body(const float* __restrict__ A, int64_t lda, int m, int64_t n,
+                                            const float* __restrict__ X, int64_t ldx, uint64_t seed,
+                                            uint32_t stream, int ell, int RB, int CG, int64_t chunks_per_wg,
+                                            float* __restrict__ part, const float* __restrict__ scales, const Blk B) {
   constexpr int KP = 32 * LT;
   const int KC = 16 * CG;
   const int pitch = KC + 4;
@@ -240,11 +240,11 @@ __global__ __launch_bounds__(512) void k_sketch(const float* __restrict__ A, int
   const int cg = w / RB;
   const int r = lane & 31;
   const int b = lane >> 5;
-  const int row = blockIdx.y * RW * RB + RW * rb + NS * r;
+  const int row = B.y * RW * RB + RW * rb + NS * r;
   const bool vec = ((lda & (NS - 1)) == 0) && (row + NS - 1 < m);
 
   const int64_t nchunks = (n + KC - 1) / KC;
-  const int64_t c0 = (int64_t)blockIdx.x * chunks_per_wg;
+  const int64_t c0 = (int64_t)B.x * chunks_per_wg;
   const int64_t c1 = (c0 + chunks_per_wg < nchunks) ? c0 + chunks_per_wg : nchunks;
 
   SketchAcc<LT> S;
@@ -327,7 +327,7 @@ __global__ __launch_bounds__(512) void k_sketch(const float* __restrict__ A, int
   }
 
   // partial output (row-major, kp columns), the FP16 operand scales undone (exact powers of two)
-  const int64_t slab = (int64_t)blockIdx.x * CG + cg;
+  const int64_t slab = (int64_t)B.x * CG + cg;
   float* out = part + slab * (int64_t)m * KP;
   float unscale[LT];
 #pragma unroll
@@ -338,9 +338,24 @@ __global__ __launch_bounds__(512) void k_sketch(const float* __restrict__ A, int
     for (int lt = 0; lt < LT; ++lt)
 #pragma unroll
       for (int v = 0; v < 16; ++v) {
-        const int prow = blockIdx.y * RW * RB + RW * rb + NS * mfma32_row(v, lane) + s;
+        const int prow = B.y * RW * RB + RW * rb + NS * mfma32_row(v, lane) + s;
         if (prow < m) out[(int64_t)prow * KP + 32 * lt + r] = S.acc[s][lt][v] * unscale[lt];
       }
+}
+
+template <int LT, int AR, bool OMEGA, int FQ>
+__global__ __launch_bounds__(512) void k_sketch(const float* __restrict__ A, int64_t lda, int m, int64_t n,
+                                                const float* __restrict__ X, int64_t ldx, uint64_t seed,
+                                                uint32_t stream, int ell, int RB, int CG, int64_t chunks_per_wg,
+                                                float* __restrict__ part, const float* __restrict__ scales) {
+  sketch_body<LT, AR, OMEGA, FQ>(A, lda, m, n, X, ldx, seed, stream, ell, RB, CG, chunks_per_wg, part, scales, hw_blk());
+}
+template <int LT, int AR, bool OMEGA, int FQ>
+__global__ __launch_bounds__(512) void k_sketch_grp(const JobSketch* __restrict__ jobs, const int2* __restrict__ map) {
+  Blk B;
+  const JobSketch& j = grp_job(jobs, map, B);
+  sketch_body<LT, AR, OMEGA, FQ>(j.A, j.lda, j.m, j.n, j.X, j.ldx, j.seed, j.stream, j.ell, j.RB, j.CG, j.cpw, j.part,
+                                 nullptr, B);
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -674,11 +689,11 @@ __global__ __launch_bounds__(512) void k_sketch_dl(const float* __restrict__ A, 
 // pass): each wave transposes its tile through LDS and runs FP32 MFMAs over its columns; at the end
 // the 8 wave sums are added in wave order in FP64 and gpart[blockIdx.x][kg][kg] is written.
 template <int LT, int AR, bool GRAM>
-__global__ __launch_bounds__(512) void k_proj(const float* __restrict__ A, int64_t lda, int m, int64_t n,
-                                              const float* __restrict__ Qc, int64_t ldq,
-                                              int out_mode, float* __restrict__ out, int64_t ldo, int r_out,
-                                              int64_t blocks_total, double* __restrict__ gpart, int kg,
-                                              const float* __restrict__ scales) {
+__device__ __forceinline__ void proj_body(const float* __restrict__ A, int64_t lda, int m, int64_t n,
+                                          const float* __restrict__ Qc, int64_t ldq,
+                                          int out_mode, float* __restrict__ out, int64_t ldo, int r_out,
+                                          int64_t blocks_total, double* __restrict__ gpart, int kg,
+                                          const float* __restrict__ scales, const Blk B) {
   constexpr int KP = 32 * LT;
   constexpr int QP = 64 + 8;   // LDS pitch of a Q column chunk (floats; bf16 halves use the same pitch)
   // bf16 arithmetic keeps the Q chunk in LDS already split: hi in the first half of a buffer, lo in
@@ -702,16 +717,16 @@ __global__ __launch_bounds__(512) void k_proj(const float* __restrict__ A, int64
   const int jl = lane & 31;
   const int b = lane >> 5;
   const int nk = (m + 63) / 64;
-  // each WG runs its K loop from chunk blockIdx.x mod nk (wrapping): at a power-of-two column stride
+  // each WG runs its K loop from chunk (its block index) mod nk (wrapping): at a power-of-two column stride
   // every WG reading the same chunk index at once puts all requests on the same address bits below
   // the stride, i.e. on a few DRAM channels; staggered starts spread them (measured 1.13 -> 0.93 ms
   // for the 1024 x 1M step)
-  const int kofs = (int)(blockIdx.x % nk);
+  const int kofs = (int)(B.x % nk);
   const bool vec = ((lda & 3) == 0) && ((m & 3) == 0);
   // whole 64-row chunks and 16-byte rows of A and Q: no per-load row guards
   const bool rows_whole = vec && ((m & 63) == 0) && ((ldq & 3) == 0);
 
-  for (int64_t blk = blockIdx.x; blk < blocks_total; blk += gridDim.x) {
+  for (int64_t blk = B.x; blk < blocks_total; blk += B.gx) {
     const int64_t j = blk * 256 + 32 * w + jl;
     // an interior block (all 256 columns below n) loads without guards (a wave-uniform branch)
     const bool fast = rows_whole && blk * 256 + 256 <= n;
@@ -1014,24 +1029,40 @@ __global__ __launch_bounds__(512) void k_proj(const float* __restrict__ A, int64
       }
       __syncthreads();
     }
-    double* o = gpart + (int64_t)blockIdx.x * kg * kg;
+    double* o = gpart + (int64_t)B.x * kg * kg;
     for (int e = threadIdx.x; e < kg * kg; e += 512) o[e] = gs[(e / kg) * KP + (e % kg)];
   }
+}
+
+template <int LT, int AR, bool GRAM>
+__global__ __launch_bounds__(512) void k_proj(const float* __restrict__ A, int64_t lda, int m, int64_t n,
+                                              const float* __restrict__ Qc, int64_t ldq,
+                                              int out_mode, float* __restrict__ out, int64_t ldo, int r_out,
+                                              int64_t blocks_total, double* __restrict__ gpart, int kg,
+                                              const float* __restrict__ scales) {
+  proj_body<LT, AR, GRAM>(A, lda, m, n, Qc, ldq, out_mode, out, ldo, r_out, blocks_total, gpart, kg, scales, hw_blk());
+}
+template <int LT, int AR, bool GRAM>
+__global__ __launch_bounds__(512) void k_proj_grp(const JobProj* __restrict__ jobs, const int2* __restrict__ map) {
+  Blk B;
+  const JobProj& j = grp_job(jobs, map, B);
+  proj_body<LT, AR, GRAM>(j.A, j.lda, j.m, j.n, j.Qc, j.ldq, j.mode, j.out, j.ldo, j.r_out, j.blocks, j.gpart, j.kg,
+                          nullptr, B);
 }
 
 // ------------------------------------------------------------------------------------------------
 // exact-path Gram: C = A A^T for m <= 64 (MT = number of 32-row tiles), FP64 partial per WG
 // ------------------------------------------------------------------------------------------------
 template <int MT, int AR>
-__global__ __launch_bounds__(256) void k_gram_small(const float* __restrict__ A, int64_t lda, int m, int64_t n,
-                                                    int64_t cols_per_wg, double* __restrict__ part,
-                                                    float* __restrict__ absmax) {
+__device__ __forceinline__ void gram_small_body(const float* __restrict__ A, int64_t lda, int m, int64_t n,
+                                                int64_t cols_per_wg, double* __restrict__ part,
+                                                float* __restrict__ absmax, const Blk B) {
   constexpr int NP = MT * (MT + 1) / 2;
   const int lane = threadIdx.x & 63;
   const int w = threadIdx.x >> 6;
   const int r = lane & 31;
   const int b = lane >> 5;
-  const int64_t j0 = (int64_t)blockIdx.x * cols_per_wg;
+  const int64_t j0 = (int64_t)B.x * cols_per_wg;
   const int64_t j1 = (j0 + cols_per_wg < n) ? j0 + cols_per_wg : n;
 
   double dacc[NP][16];
@@ -1115,14 +1146,14 @@ __global__ __launch_bounds__(256) void k_gram_small(const float* __restrict__ A,
 #pragma unroll
     for (int q = 0; q < 16; ++q) dacc[p][q] += (double)acc[p][q];
 
-  // combine the 4 waves through LDS; one slab per WG: part[blockIdx.x][m][m]
+  // combine the 4 waves through LDS; one slab per WG: part[block][m][m]
   __shared__ double red[4][NP][1024];
 #pragma unroll
   for (int p = 0; p < NP; ++p)
 #pragma unroll
     for (int v = 0; v < 16; ++v) red[w][p][mfma32_row(v, lane) * 32 + r] = dacc[p][v];
   __syncthreads();
-  double* o = part + (int64_t)blockIdx.x * m * m;
+  double* o = part + (int64_t)B.x * m * m;
   for (int e = threadIdx.x; e < NP * 1024; e += 256) {
     const int p = e >> 10, rc = e & 1023;
     const double sum = red[0][p][rc] + red[1][p][rc] + red[2][p][rc] + red[3][p][rc];
@@ -1141,17 +1172,31 @@ __global__ __launch_bounds__(256) void k_gram_small(const float* __restrict__ A,
   }
 }
 
+template <int MT, int AR>
+__global__ __launch_bounds__(256) void k_gram_small(const float* __restrict__ A, int64_t lda, int m, int64_t n,
+                                                    int64_t cols_per_wg, double* __restrict__ part,
+                                                    float* __restrict__ absmax) {
+  gram_small_body<MT, AR>(A, lda, m, n, cols_per_wg, part, absmax, hw_blk());
+}
+template <int MT, int AR>
+__global__ __launch_bounds__(256) void k_gram_small_grp(const JobGramSmall* __restrict__ jobs,
+                                                        const int2* __restrict__ map) {
+  Blk B;
+  const JobGramSmall& j = grp_job(jobs, map, B);
+  gram_small_body<MT, AR>(j.A, j.lda, j.m, j.n, j.cpw, j.part, nullptr, B);
+}
+
 // ------------------------------------------------------------------------------------------------
 // deterministic slab reductions
 // ------------------------------------------------------------------------------------------------
 // out[e] = sum_s part[s * count + e] in a fixed order: WG handles 32 elements; its 8 thread groups
 // take slabs g, g + 8, ...; the 8 group sums are added in group order.
 template <typename TI, typename TO>
-__global__ __launch_bounds__(256) void k_reduce(const TI* __restrict__ part, int64_t S, int64_t count,
-                                                TO* __restrict__ out) {
+__device__ __forceinline__ void reduce_body(const TI* __restrict__ part, int64_t S, int64_t count,
+                                            TO* __restrict__ out, const Blk B) {
   __shared__ double red[8][33];
   const int el = threadIdx.x & 31, g = threadIdx.x >> 5;
-  const int64_t e = (int64_t)blockIdx.x * 32 + el;
+  const int64_t e = (int64_t)B.x * 32 + el;
   double acc = 0.0;
   // unrolled: sixteen independent loads in flight per thread, added in slab order
   if (e < count) {
@@ -1166,6 +1211,18 @@ __global__ __launch_bounds__(256) void k_reduce(const TI* __restrict__ part, int
     for (int i = 0; i < 8; ++i) t += red[i][el];
     out[e] = (TO)t;
   }
+}
+
+template <typename TI, typename TO>
+__global__ __launch_bounds__(256) void k_reduce(const TI* __restrict__ part, int64_t S, int64_t count,
+                                                TO* __restrict__ out) {
+  reduce_body<TI, TO>(part, S, count, out, hw_blk());
+}
+template <typename T>
+__global__ __launch_bounds__(256) void k_reduce_grp(const JobReduce* __restrict__ jobs, const int2* __restrict__ map) {
+  Blk B;
+  const JobReduce& j = grp_job(jobs, map, B);
+  reduce_body<T, T>(static_cast<const T*>(j.part), j.S, j.count, static_cast<T*>(j.out), B);
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -1219,6 +1276,17 @@ SketchPlan plan_sketch(int m, int64_t n, int kp, int wgs_hint) {
   return p;
 }
 
+SketchPlan plan_sketch_batch(int m, int64_t n, int kp, int64_t cpw) {
+  SketchPlan p = plan_sketch(m, n, kp, 1);
+  const int64_t nchunks = (n + 16 * p.CG - 1) / (16 * p.CG);
+  p.chunks_per_wg = cpw < 1 ? 1 : cpw;
+  p.grid_x = (int)((nchunks + p.chunks_per_wg - 1) / p.chunks_per_wg);
+  if (p.grid_x < 1) p.grid_x = 1;
+  p.slabs = (int64_t)p.grid_x * p.CG;
+  p.plain = true;
+  return p;
+}
+
 template <int LT, int AR, bool OM>
 static void launch_sketch_t(const SketchPlan& p, const float* A, int64_t lda, int m, int64_t n, const float* X,
                             int64_t ldx, uint64_t seed, uint32_t stream, int ell, float* part, hipStream_t st,
@@ -1231,7 +1299,7 @@ static void launch_sketch_t(const SketchPlan& p, const float* A, int64_t lda, in
     const int nw = p.RB * p.CG;
     const int tile_bytes = 16 * RW * (int)sizeof(float) * nw;
     constexpr int LDS_MAX = 160 * 1024;
-    if ((lda & 3) == 0 && (m & 3) == 0 && g_sketch_dl) {
+    if ((lda & 3) == 0 && (m & 3) == 0 && g_sketch_dl && !p.plain) {
       if (p.lds_bytes + 4 * tile_bytes <= LDS_MAX) {
         hipLaunchKernelGGL((k_sketch_dl<LT, AR, 4, false>), grid, dim3(64 * nw), p.lds_bytes + 4 * tile_bytes, st, A, lda, m,
                            n, seed, stream, ell, p.RB, p.CG, p.chunks_per_wg, part, scales, nullptr);
@@ -1403,12 +1471,14 @@ int64_t proj_grid(int64_t n) {
   return grid;
 }
 
+int64_t proj_blocks(int64_t n) { return (n + 255) / 256; }
+
 template <int LT, int AR>
 static void launch_proj_t(const float* A, int64_t lda, int m, int64_t n, const float* Qc, int64_t ldq, int mode,
                           float* out, int64_t ldo, int r_out, double* gpart, int kg, hipStream_t st,
-                          const float* sc) {
-  const int64_t blocks = (n + 255) / 256;
-  const int64_t grid = proj_grid(n);
+                          const float* sc, int64_t grid_in = 0) {
+  const int64_t blocks = proj_blocks(n);
+  const int64_t grid = grid_in > 0 ? grid_in : proj_grid(n);
   if (gpart)
     hipLaunchKernelGGL((k_proj<LT, AR, true>), dim3((unsigned)grid), dim3(512), 0, st, A, lda, m, n, Qc, ldq, mode,
                        out, ldo, r_out, blocks, gpart, kg, sc);
@@ -1420,21 +1490,28 @@ static void launch_proj_t(const float* A, int64_t lda, int m, int64_t n, const f
 template <int LT>
 static void launch_proj_ar(int ar, const float* A, int64_t lda, int m, int64_t n, const float* Qc, int64_t ldq,
                            int mode, float* out, int64_t ldo, int r_out, double* gpart, int kg, hipStream_t st,
-                           const float* sc) {
+                           const float* sc, int64_t grid) {
   switch (ar) {
-    case AR_FP32: launch_proj_t<LT, AR_FP32>(A, lda, m, n, Qc, ldq, mode, out, ldo, r_out, gpart, kg, st, sc); break;
-    case AR_BF16: launch_proj_t<LT, AR_BF16>(A, lda, m, n, Qc, ldq, mode, out, ldo, r_out, gpart, kg, st, sc); break;
-    case AR_FP16: launch_proj_t<LT, AR_FP16>(A, lda, m, n, Qc, ldq, mode, out, ldo, r_out, gpart, kg, st, sc); break;
-    case AR_BF16X4: launch_proj_t<LT, AR_BF16X4>(A, lda, m, n, Qc, ldq, mode, out, ldo, r_out, gpart, kg, st, sc); break;
-    default: launch_proj_t<LT, AR_BF16X3>(A, lda, m, n, Qc, ldq, mode, out, ldo, r_out, gpart, kg, st, sc); break;
+    case AR_FP32: launch_proj_t<LT, AR_FP32>(A, lda, m, n, Qc, ldq, mode, out, ldo, r_out, gpart, kg, st, sc, grid); break;
+    case AR_BF16: launch_proj_t<LT, AR_BF16>(A, lda, m, n, Qc, ldq, mode, out, ldo, r_out, gpart, kg, st, sc, grid); break;
+    case AR_FP16: launch_proj_t<LT, AR_FP16>(A, lda, m, n, Qc, ldq, mode, out, ldo, r_out, gpart, kg, st, sc, grid); break;
+    case AR_BF16X4: launch_proj_t<LT, AR_BF16X4>(A, lda, m, n, Qc, ldq, mode, out, ldo, r_out, gpart, kg, st, sc, grid); break;
+    default: launch_proj_t<LT, AR_BF16X3>(A, lda, m, n, Qc, ldq, mode, out, ldo, r_out, gpart, kg, st, sc, grid); break;
   }
 }
 
 void launch_proj(int ar, int kp, const float* A, int64_t lda, int m, int64_t n, const float* Qc, int64_t ldq,
                  int mode, float* out, int64_t ldo, int r_out, hipStream_t st, double* gpart, int kg,
                  const float* scales) {
-  if (kp == 32) launch_proj_ar<1>(ar, A, lda, m, n, Qc, ldq, mode, out, ldo, r_out, gpart, kg, st, scales);
-  else launch_proj_ar<2>(ar, A, lda, m, n, Qc, ldq, mode, out, ldo, r_out, gpart, kg, st, scales);
+  if (kp == 32) launch_proj_ar<1>(ar, A, lda, m, n, Qc, ldq, mode, out, ldo, r_out, gpart, kg, st, scales, 0);
+  else launch_proj_ar<2>(ar, A, lda, m, n, Qc, ldq, mode, out, ldo, r_out, gpart, kg, st, scales, 0);
+}
+
+void launch_proj_grid(int ar, int kp, const float* A, int64_t lda, int m, int64_t n, const float* Qc, int64_t ldq,
+                      int mode, float* out, int64_t ldo, int r_out, int64_t grid, hipStream_t st, double* gpart, int kg,
+                      const float* scales) {
+  if (kp == 32) launch_proj_ar<1>(ar, A, lda, m, n, Qc, ldq, mode, out, ldo, r_out, gpart, kg, st, scales, grid);
+  else launch_proj_ar<2>(ar, A, lda, m, n, Qc, ldq, mode, out, ldo, r_out, gpart, kg, st, scales, grid);
 }
 
 int64_t gram_small_grid(int64_t n) {
@@ -1475,5 +1552,73 @@ void launch_reduce_f64(const double* part, int64_t S, int64_t count, double* out
   const int64_t grid = (count + 31) / 32;
   hipLaunchKernelGGL((k_reduce<double, double>), dim3((unsigned)grid), dim3(256), 0, st, part, S, count, out);
 }
+
+// ------------------------------------------------------------------------------------------------
+// grouped launchers
+// ------------------------------------------------------------------------------------------------
+#define TTX_GRP_ARGS dim3((unsigned)g.blocks), dim3(g.threads), g.lds, st
+void grp_gram_small(int ar, int mt, const GroupLaunch& g, hipStream_t st) {
+  const JobGramSmall* J = static_cast<const JobGramSmall*>(g.jobs);
+#define TTX_G(MT, AR) hipLaunchKernelGGL((k_gram_small_grp<MT, AR>), TTX_GRP_ARGS, J, g.map)
+  if (mt == 1) {
+    switch (ar) {
+      case AR_FP32: TTX_G(1, AR_FP32); break;
+      case AR_BF16: TTX_G(1, AR_BF16); break;
+      case AR_BF16X4: TTX_G(1, AR_BF16X4); break;
+      default: TTX_G(1, AR_BF16X3); break;
+    }
+  } else {
+    switch (ar) {
+      case AR_FP32: TTX_G(2, AR_FP32); break;
+      case AR_BF16: TTX_G(2, AR_BF16); break;
+      case AR_BF16X4: TTX_G(2, AR_BF16X4); break;
+      default: TTX_G(2, AR_BF16X3); break;
+    }
+  }
+#undef TTX_G
+}
+
+void grp_reduce(bool f64, const GroupLaunch& g, hipStream_t st) {
+  const JobReduce* J = static_cast<const JobReduce*>(g.jobs);
+  if (f64) hipLaunchKernelGGL(k_reduce_grp<double>, TTX_GRP_ARGS, J, g.map);
+  else hipLaunchKernelGGL(k_reduce_grp<float>, TTX_GRP_ARGS, J, g.map);
+}
+
+template <int LT, int AR>
+static void grp_sketch_t(bool omega, int fq, const GroupLaunch& g, hipStream_t st) {
+  const JobSketch* J = static_cast<const JobSketch*>(g.jobs);
+  if (omega) hipLaunchKernelGGL((k_sketch_grp<LT, AR, true, 1>), TTX_GRP_ARGS, J, g.map);
+  else if (fq <= 1) hipLaunchKernelGGL((k_sketch_grp<LT, AR, false, 1>), TTX_GRP_ARGS, J, g.map);
+  else if (fq == 2) hipLaunchKernelGGL((k_sketch_grp<LT, AR, false, 2>), TTX_GRP_ARGS, J, g.map);
+  else hipLaunchKernelGGL((k_sketch_grp<LT, AR, false, 4>), TTX_GRP_ARGS, J, g.map);
+}
+template <int LT>
+static void grp_sketch_ar(int ar, bool omega, int fq, const GroupLaunch& g, hipStream_t st) {
+  switch (ar) {
+    case AR_FP32: grp_sketch_t<LT, AR_FP32>(omega, fq, g, st); break;
+    case AR_BF16: grp_sketch_t<LT, AR_BF16>(omega, fq, g, st); break;
+    default: grp_sketch_t<LT, AR_BF16X3>(omega, fq, g, st); break;
+  }
+}
+void grp_sketch(int ar, int lt, bool omega, int fq, const GroupLaunch& g, hipStream_t st) {
+  if (lt == 1) grp_sketch_ar<1>(ar, omega, fq, g, st);
+  else grp_sketch_ar<2>(ar, omega, fq, g, st);
+}
+
+template <int LT, bool GRAM>
+static void grp_proj_t(int ar, const GroupLaunch& g, hipStream_t st) {
+  const JobProj* J = static_cast<const JobProj*>(g.jobs);
+  switch (ar) {
+    case AR_FP32: hipLaunchKernelGGL((k_proj_grp<LT, AR_FP32, GRAM>), TTX_GRP_ARGS, J, g.map); break;
+    case AR_BF16: hipLaunchKernelGGL((k_proj_grp<LT, AR_BF16, GRAM>), TTX_GRP_ARGS, J, g.map); break;
+    case AR_BF16X4: hipLaunchKernelGGL((k_proj_grp<LT, AR_BF16X4, GRAM>), TTX_GRP_ARGS, J, g.map); break;
+    default: hipLaunchKernelGGL((k_proj_grp<LT, AR_BF16X3, GRAM>), TTX_GRP_ARGS, J, g.map); break;
+  }
+}
+void grp_proj(int ar, int lt, bool gram, const GroupLaunch& g, hipStream_t st) {
+  if (lt == 1) { if (gram) grp_proj_t<1, true>(ar, g, st); else grp_proj_t<1, false>(ar, g, st); }
+  else { if (gram) grp_proj_t<2, true>(ar, g, st); else grp_proj_t<2, false>(ar, g, st); }
+}
+#undef TTX_GRP_ARGS
 
 }  // namespace ttx
