@@ -245,8 +245,8 @@ def host_leg(eng, A, torch, L, steps=2):
 
 def batch_bytes_alg(dims_l, ranks_l, s=10, q=2):
     """Algorithmic HBM bytes of a batch: per tensor and step, P passes over the unfolding A_k plus the
-    write of A_{k+1}, with P = 2 on an exact step (ell = r + s >= min(m, n)) and 2 q + 2 on a randomized
-    one (the batch's fixed schedule)."""
+    write of A_{k+1}, with P = 2 on an exact step and 2 q + 2 on a randomized one.  A batch step is exact
+    when ell = r + s >= min(m, n) or min(m, n) <= 64 (the batch plan's rule)."""
     total = 0
     for dims, ranks in zip(dims_l, ranks_l):
         n = int(np.prod(dims))
@@ -255,7 +255,7 @@ def batch_bytes_alg(dims_l, ranks_l, s=10, q=2):
             m = r_prev * dims[k]
             n //= dims[k]
             r = min(ranks[k + 1], m, n)
-            passes = 2 if r + s >= min(m, n) else 2 * q + 2
+            passes = 2 if (r + s >= min(m, n) or min(m, n) <= 64) else 2 * q + 2
             total += passes * m * n * 4 + r * n * 4
             r_prev = r
     return total

@@ -829,11 +829,12 @@ __global__ __launch_bounds__(256) void k_next_grp(const JobNext* __restrict__ jo
   }
 }
 
-int64_t next_from_z_grid(int64_t n, int r, bool* cols) {
+int64_t next_from_z_grid(int64_t n, int r, bool* cols, bool batch) {
   const int RM = ur_ld(r);
   const int64_t cap = (int64_t)num_cus() * 16;
-  // short sweeps: a row per lane would leave most of the machine idle, so RM lanes per row
-  *cols = n < ((int64_t)1 << 17);
+  // short sweeps: a row per lane would leave most of the machine idle, so RM lanes per row (not in a
+  // batch, whose tensors together fill it)
+  *cols = !batch && n < ((int64_t)1 << 17);
   int64_t grid = *cols ? (n * RM + 255) / 256 : (n + 255) / 256;
   if (grid > cap) grid = cap;
   if (grid < 1) grid = 1;
@@ -841,10 +842,10 @@ int64_t next_from_z_grid(int64_t n, int r, bool* cols) {
 }
 
 void launch_next_from_z(const float* Z, int64_t ldz, int64_t n, int k, const float* Ur, int r, float* out,
-                        hipStream_t st) {
+                        hipStream_t st, bool batch) {
   const int RM = ur_ld(r);
   bool cols = false;
-  const int64_t grid = next_from_z_grid(n, r, &cols);
+  const int64_t grid = next_from_z_grid(n, r, &cols, batch);
   if (cols) {
 #define TTX_NFZ(R) hipLaunchKernelGGL((k_next_from_z_cols<R>), dim3((unsigned)grid), dim3(256), 0, st, Z, ldz, n, k, Ur, r, out)
     if (RM == 8) TTX_NFZ(8); else if (RM == 16) TTX_NFZ(16); else if (RM == 32) TTX_NFZ(32); else TTX_NFZ(64);

@@ -485,7 +485,7 @@ static ttx_status step_tail(Handle* h, const StepArgs& a, const Arith3& ar, int 
   // remainder S V^T = U_r^T B = U_r^T Z^T (:734-739)
   {
     KScope ks(h, TTX_K_TALL, n * kz * 4 + (int64_t)r * n * 4);
-    launch_next_from_z(Z, kz, n, k, P<float>(h->Ur), r, a.next, st);
+    launch_next_from_z(Z, kz, n, k, P<float>(h->Ur), r, a.next, st, h->batch_mode);
   }
   o.bytes += (int64_t)(n * kz * 4 + (int64_t)r * n * 4);
   o.r = r;
@@ -740,7 +740,11 @@ static ttx_status run_step(Handle* h, const StepArgs& a_in, StepOut& o) {
   int ell = a.r + a.s;
   const int64_t mpath = a.m_path > 0 ? a.m_path : a.m;
   const int64_t mn = std::min<int64_t>(mpath, a.n);
-  const bool exact = (a.method == TTX_METHOD_EXACT) || (a.method == TTX_METHOD_AUTO && ell >= mn);
+  // AUTO: exact when the sketch would be as wide as the short side; in batch mode also whenever the
+  // short side is at most TTX_MAX_ELL (one streaming pass for its Gram, or none for its basis, against
+  // 2 q + 2 for the randomized range finder)
+  const bool exact = (a.method == TTX_METHOD_EXACT) ||
+                     (a.method == TTX_METHOD_AUTO && (ell >= mn || (h->batch_mode && mn <= TTX_MAX_ELL)));
   memset(o.sigma, 0, sizeof(o.sigma));
   o.bytes = 0;
   TTX_TRY(ensure(h, h->status, 4 * sizeof(int)));
@@ -2594,7 +2598,8 @@ static ttx_status bplan_tensor(Handle* h, const ttx_params* p, std::vector<BStep
     const int64_t mn = std::min<int64_t>(m, n);
     b.r = (int)std::min<int64_t>(p->ranks[k + 1], mn);
     int64_t ell = (int64_t)b.r + p->oversampling;
-    const bool exact = p->method == TTX_METHOD_EXACT || (p->method == TTX_METHOD_AUTO && ell >= mn);
+    const bool exact = p->method == TTX_METHOD_EXACT ||
+                       (p->method == TTX_METHOD_AUTO && (ell >= mn || mn <= TTX_MAX_ELL));
     if (b.r > TTX_MAX_ELL) return fail(h, TTX_ERR_UNSUPPORTED, "batch: rank above TTX_MAX_ELL");
     b.gproj = proj_grid_batch(m, n);
     if (exact && m <= TTX_MAX_ELL) {
@@ -2855,7 +2860,7 @@ static ttx_status gplan_build(Handle* h, const std::vector<ttx_params>& P, const
                JobProj{(int)b.gproj, 1, b.A, b.m, b.m, b.n, w.Qn, b.m, 1, b.next, 0, b.r, proj_blocks(b.n), nullptr, 0});
       } else {
         bool cols = false;
-        const int64_t g = next_from_z_grid(b.n, b.r, &cols);
+        const int64_t g = next_from_z_grid(b.n, b.r, &cols, true);
         ph.add(G_NEXT, 0, 0, 0, 0, 256, 0,
                JobNext{(int)g, 1, w.Z, b.kz, b.n, b.k, w.Ur, b.r, b.next, next_from_z_ld(b.r), cols ? 1 : 0});
       }

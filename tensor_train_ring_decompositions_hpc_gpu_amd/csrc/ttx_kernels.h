@@ -109,7 +109,7 @@ void launch_assemble(const float* Qc, int64_t ldq, int m, int k, const double* l
 // 16-byte aligned rows, columns k .. round_up(k, 4) - 1 readable); Ur as launch_assemble writes it
 int next_from_z_ld(int r);
 void launch_next_from_z(const float* Z, int64_t ldz, int64_t n, int k, const float* Ur, int r, float* out,
-                        hipStream_t st);
+                        hipStream_t st, bool batch = false);
 // pending full-rank left factor P = I_N (x) G^T of a fused step (G: ms x ms column-major core):
 // mode 0: Y = P X (X, Y row-major m x kp); mode 1: Y = P^T X (X, Y column-major kp x m); mode 2: Y = P X
 // (X, Y column-major kp x m)
@@ -200,7 +200,8 @@ struct JobCopyCols { int gx, gy; const float* A; int64_t lda; int m; int n; floa
 // grids the plain launches use, so a batch plan can build the same jobs
 int64_t proj_blocks(int64_t n);
 int syevj_threads(int k);
-int64_t next_from_z_grid(int64_t n, int r, bool* cols);
+// batch: one row per lane whatever n (the whole batch fills the machine), at most 4096 workgroups
+int64_t next_from_z_grid(int64_t n, int r, bool* cols, bool batch = false);
 int64_t copy_cols_grid(int m, int kp);
 // plain launches with an explicit grid (the batch plans' grids; grid 0: the default)
 void launch_proj_grid(int ar, int kp, const float* A, int64_t lda, int m, int64_t n, const float* Qc, int64_t ldq,
