@@ -20,8 +20,8 @@ constexpr int RB64 = 64;   // rows per WG of the panel kernels
 // Gram of a panel block: part[blockIdx.x][a][b] = sum over rows i of the block of X[i][a] X[i][b]
 // 4 x 4 register blocks over the upper triangle; rows staged through LDS in chunks of 32.
 // ------------------------------------------------------------------------------------------------
-template <typename T>
-__device__ __forceinline__ void block_gram(const double (*xs)[KMAX + 1], int nrows, int k, double* __restrict__ out) {
+template <typename T, int KC = KMAX>
+__device__ __forceinline__ void block_gram(const double (*xs)[KC + 1], int nrows, int k, double* __restrict__ out) {
   const int kb = (k + 3) / 4;
   const int nb = kb * (kb + 1) / 2;
   for (int bi = threadIdx.x; bi < nb; bi += blockDim.x) {
@@ -55,18 +55,20 @@ __device__ __forceinline__ void block_gram(const double (*xs)[KMAX + 1], int nro
   }
 }
 
-template <typename T>
+// KC: the widest k the LDS is sized for (the plain launch takes KMAX; a grouped launch of narrow jobs
+// a smaller KC, so more of its workgroups share a CU)
+template <typename T, int KC = KMAX>
 __device__ __forceinline__ void gram_rows_body(const T* __restrict__ X, int64_t ldx, int64_t rows, int k,
                                                int64_t rows_per_wg, double* __restrict__ part, const Blk B) {
-  __shared__ double xs[RB64][KMAX + 1];
+  __shared__ double xs[RB64][KC + 1];
   const int64_t r0 = (int64_t)B.x * rows_per_wg;
   const int64_t r1 = (r0 + rows_per_wg < rows) ? r0 + rows_per_wg : rows;
   const int kb4 = ((k + 3) / 4) * 4;
   double* out = part + (int64_t)B.x * k * k;
   // accumulate chunk Grams through a small LDS buffer
-  __shared__ double gacc[KMAX * KMAX];
+  __shared__ double gacc[KC * KC];
   for (int e = threadIdx.x; e < k * k; e += 256) gacc[e] = 0.0;
-  __shared__ double gchunk[KMAX * KMAX];
+  __shared__ double gchunk[KC * KC];
   for (int64_t rc = r0; rc < r1; rc += RB64) {
     const int nr = (int)((r1 - rc) < RB64 ? (r1 - rc) : RB64);
     __syncthreads();
@@ -75,7 +77,7 @@ __device__ __forceinline__ void gram_rows_body(const T* __restrict__ X, int64_t 
       xs[rr][c] = (rr < nr && c < k) ? (double)X[(rc + rr) * ldx + c] : 0.0;
     }
     __syncthreads();
-    block_gram<T>(xs, nr, k, gchunk);
+    block_gram<T, KC>(xs, nr, k, gchunk);
     __syncthreads();
     for (int e = threadIdx.x; e < k * k; e += 256) gacc[e] += gchunk[e];
   }
@@ -87,11 +89,11 @@ __global__ __launch_bounds__(256) void k_gram_rows(const T* __restrict__ X, int6
                                                    int64_t rows_per_wg, double* __restrict__ part) {
   gram_rows_body<T>(X, ldx, rows, k, rows_per_wg, part, hw_blk());
 }
-template <typename T>
+template <typename T, int KC>
 __global__ __launch_bounds__(256) void k_gram_rows_grp(const JobGramRows* __restrict__ jobs, const int2* __restrict__ map) {
   Blk B;
   const JobGramRows& j = grp_job(jobs, map, B);
-  gram_rows_body<T>(static_cast<const T*>(j.X), j.ldx, j.rows, j.k, j.rpw, j.part, B);
+  gram_rows_body<T, KC>(static_cast<const T*>(j.X), j.ldx, j.rows, j.k, j.rpw, j.part, B);
 }
 
 int64_t gram_tall_grid(int64_t rows) {
@@ -118,8 +120,9 @@ void launch_gram_tall_f64(const double* X, int64_t ldx, int64_t rows, int k, dou
 
 // dst[i][j] = sum_q part[q][i * k + j]: slabs are read whole (coalesced), two at a time, so each
 // thread keeps several independent loads in flight
+template <int KC = KMAX>
 __device__ __forceinline__ void sum_slabs(const double* __restrict__ part, int64_t S, int k,
-                                          double (*dst)[KMAX + 1]) {
+                                          double (*dst)[KC + 1]) {
   const int kk = k * k;
   double acc[16];
 #pragma unroll
@@ -313,18 +316,19 @@ void launch_chol_inv(const double* part, int64_t S, int k, int64_t m, int shift_
 // W_out = W_in Rinv for a 64-row block, plus the Gram partial of W_out (next CholeskyQR round)
 // OUT 0: double row-major W_out (stride k); OUT 1: float column-major Qc[c * ldq + i], c < kp
 // ------------------------------------------------------------------------------------------------
-template <typename TI, int OUT>
+// KC: the widest k the LDS is sized for (as gram_rows_body)
+template <typename TI, int OUT, int KC = KMAX>
 __device__ __forceinline__ void apply_body(const TI* __restrict__ Win, int64_t ldi, int64_t rows, int k, int kp,
                                            const double* __restrict__ Rinv, double* __restrict__ Wd,
                                            float* __restrict__ Qc, int64_t ldq, double* __restrict__ part,
                                            const double* __restrict__ gpart, int64_t S, int ldr, const Blk B) {
-  __shared__ double ri[KMAX][KMAX + 1];
-  __shared__ double ws[RB64][KMAX + 1];
-  __shared__ double os[RB64][KMAX + 1];
+  __shared__ double ri[KC][KC + 1];
+  __shared__ double ws[RB64][KC + 1];
+  __shared__ double os[RB64][KC + 1];
   const int tid = threadIdx.x;
   if (gpart) {
     // Newton-Schulz step towards the orthogonal polar factor: M = (3 I - W^T W) / 2 (full matrix)
-    sum_slabs(gpart, S, k, ri);
+    sum_slabs<KC>(gpart, S, k, ri);
     __syncthreads();
     for (int e = tid; e < k * k; e += 256) {
       const int i = e / k, j = e - (e / k) * k;
@@ -381,7 +385,7 @@ __device__ __forceinline__ void apply_body(const TI* __restrict__ Win, int64_t l
       if (rr < nr) Qc[(int64_t)c * ldq + i0 + rr] = (c < k) ? (float)os[rr][c] : 0.0f;
     }
   }
-  if (part) block_gram<double>(os, nr, k, part + (int64_t)B.x * k * k);
+  if (part) block_gram<double, KC>(os, nr, k, part + (int64_t)B.x * k * k);
 }
 template <typename TI, int OUT>
 __global__ __launch_bounds__(256) void k_apply_gram(const TI* __restrict__ Win, int64_t ldi, int64_t rows, int k,
@@ -391,12 +395,12 @@ __global__ __launch_bounds__(256) void k_apply_gram(const TI* __restrict__ Win, 
                                                     int64_t S, int ldr) {
   apply_body<TI, OUT>(Win, ldi, rows, k, kp, Rinv, Wd, Qc, ldq, part, gpart, S, ldr, hw_blk());
 }
-template <typename TI, int OUT>
+template <typename TI, int OUT, int KC>
 __global__ __launch_bounds__(256) void k_apply_grp(const JobApply* __restrict__ jobs, const int2* __restrict__ map) {
   Blk B;
   const JobApply& j = grp_job(jobs, map, B);
-  apply_body<TI, OUT>(static_cast<const TI*>(j.Win), j.ldi, j.rows, j.k, j.kp, j.Rinv, j.Wd, j.Qc, j.ldq, j.part,
-                      j.gpart, j.S, j.ldr, B);
+  apply_body<TI, OUT, KC>(static_cast<const TI*>(j.Win), j.ldi, j.rows, j.k, j.kp, j.Rinv, j.Wd, j.Qc, j.ldq, j.part,
+                          j.gpart, j.S, j.ldr, B);
 }
 
 void launch_apply_gram_f32(const float* Win, int64_t ldi, int64_t rows, int k, const double* Rinv, double* Wout,
@@ -1079,19 +1083,24 @@ void launch_copy_cols(const float* A, int64_t lda, int m, int n, float* Y, int k
 // grouped launchers
 // ------------------------------------------------------------------------------------------------
 #define TTX_GRP_ARGS dim3((unsigned)g.blocks), dim3(g.threads), 0, st
-void grp_gram_rows(bool f64, const GroupLaunch& g, hipStream_t st) {
+int grp_kcap(int k) { return k <= 32 ? 32 : (k <= 48 ? 48 : 64); }
+void grp_gram_rows(bool f64, int kc, const GroupLaunch& g, hipStream_t st) {
   const JobGramRows* J = static_cast<const JobGramRows*>(g.jobs);
-  if (f64) hipLaunchKernelGGL(k_gram_rows_grp<double>, TTX_GRP_ARGS, J, g.map);
-  else hipLaunchKernelGGL(k_gram_rows_grp<float>, TTX_GRP_ARGS, J, g.map);
+#define TTX_G(KC) { if (f64) hipLaunchKernelGGL((k_gram_rows_grp<double, KC>), TTX_GRP_ARGS, J, g.map); \
+                    else hipLaunchKernelGGL((k_gram_rows_grp<float, KC>), TTX_GRP_ARGS, J, g.map); }
+  if (kc <= 32) TTX_G(32) else if (kc <= 48) TTX_G(48) else TTX_G(64)
+#undef TTX_G
 }
 void grp_chol(const GroupLaunch& g, hipStream_t st) {
   hipLaunchKernelGGL(k_chol_grp, TTX_GRP_ARGS, static_cast<const JobChol*>(g.jobs), g.map);
 }
-void grp_apply(int kind, const GroupLaunch& g, hipStream_t st) {
+void grp_apply(int kind, int kc, const GroupLaunch& g, hipStream_t st) {
   const JobApply* J = static_cast<const JobApply*>(g.jobs);
-  if (kind == 0) hipLaunchKernelGGL((k_apply_grp<float, 0>), TTX_GRP_ARGS, J, g.map);
-  else if (kind == 1) hipLaunchKernelGGL((k_apply_grp<double, 0>), TTX_GRP_ARGS, J, g.map);
-  else hipLaunchKernelGGL((k_apply_grp<double, 1>), TTX_GRP_ARGS, J, g.map);
+#define TTX_G(KC) { if (kind == 0) hipLaunchKernelGGL((k_apply_grp<float, 0, KC>), TTX_GRP_ARGS, J, g.map); \
+                    else if (kind == 1) hipLaunchKernelGGL((k_apply_grp<double, 0, KC>), TTX_GRP_ARGS, J, g.map); \
+                    else hipLaunchKernelGGL((k_apply_grp<double, 1, KC>), TTX_GRP_ARGS, J, g.map); }
+  if (kc <= 32) TTX_G(32) else if (kc <= 48) TTX_G(48) else TTX_G(64)
+#undef TTX_G
 }
 void grp_syevj(const GroupLaunch& g, hipStream_t st, int max_sweeps) {
   hipLaunchKernelGGL(k_syevj_grp, TTX_GRP_ARGS, static_cast<const JobSyevj*>(g.jobs), g.map, max_sweeps);

@@ -2725,15 +2725,16 @@ static ttx_status gplan_build(Handle* h, const std::vector<ttx_params>& P, const
         int* stat = w.stat + 4 * k;
         const int64_t rpw = ((((int64_t)b.m + b.nb - 1) / b.nb + 63) / 64) * 64;
         const int pb = (int)panel_blocks(b.m);
+        const int kc = grp_kcap(b.k);   // LDS capacity class of the panel kernels
         switch (stage) {
-          case 0: ph.add(G_GRAM_ROWS, 0, 0, 0, 0, 256, 0, JobGramRows{(int)b.nb, 1, w.Y, b.kp, b.m, b.k, rpw, w.gpart}); break;
+          case 0: ph.add(G_GRAM_ROWS, 0, 0, kc, 0, 256, 0, JobGramRows{(int)b.nb, 1, w.Y, b.kp, b.m, b.k, rpw, w.gpart}); break;
           case 1: ph.add(G_CHOL, 0, 0, 0, 0, 256, 0, JobChol{1, 1, w.gpart, b.nb, b.k, b.m, 1, w.Rinv, stat}); break;
-          case 2: ph.add(G_APPLY, 0, 0, 0, 0, 256, 0, JobApply{pb, 1, w.Y, b.kp, b.m, b.k, b.k, w.Rinv, w.W1, nullptr, 0,
+          case 2: ph.add(G_APPLY, 0, 0, kc, 0, 256, 0, JobApply{pb, 1, w.Y, b.kp, b.m, b.k, b.k, w.Rinv, w.W1, nullptr, 0,
                                                               w.gpart, nullptr, 0, b.k}); break;
           case 3: ph.add(G_CHOL, 0, 0, 0, 0, 256, 0, JobChol{1, 1, w.gpart, b.nb, b.k, b.m, 0, w.Rinv, stat}); break;
-          case 4: ph.add(G_APPLY, 0, 1, 0, 0, 256, 0, JobApply{pb, 1, w.W1, b.k, b.m, b.k, b.k, w.Rinv, w.W2, nullptr, 0,
+          case 4: ph.add(G_APPLY, 0, 1, kc, 0, 256, 0, JobApply{pb, 1, w.W1, b.k, b.m, b.k, b.k, w.Rinv, w.W2, nullptr, 0,
                                                               w.gpart, nullptr, 0, b.k}); break;
-          default: ph.add(G_APPLY, 0, 2, 0, 0, 256, 0, JobApply{pb, 1, w.W2, b.k, b.m, b.k, b.kp, nullptr, nullptr, w.Qc,
+          default: ph.add(G_APPLY, 0, 2, kc, 0, 256, 0, JobApply{pb, 1, w.W2, b.k, b.m, b.k, b.kp, nullptr, nullptr, w.Qc,
                                                                b.m, nullptr, w.gpart, b.nb, b.k}); break;
         }
       }
@@ -2904,9 +2905,9 @@ static ttx_status gplan_queue(Handle* h, const GPlan& G, hipStream_t st) {
       case G_REDUCE_F64: grp_reduce(true, g, st); break;
       case G_SKETCH: grp_sketch(r.ar, r.v0, r.v1 != 0, r.v2, g, st); break;
       case G_PROJ: grp_proj(r.ar, r.v0, r.v1 != 0, g, st); break;
-      case G_GRAM_ROWS: grp_gram_rows(false, g, st); break;
+      case G_GRAM_ROWS: grp_gram_rows(false, r.v1, g, st); break;
       case G_CHOL: grp_chol(g, st); break;
-      case G_APPLY: grp_apply(r.v0, g, st); break;
+      case G_APPLY: grp_apply(r.v0, r.v1, g, st); break;
       case G_SYEVJ: grp_syevj(g, st, h->jacobi_max_sweeps); break;
       case G_ASSEMBLE: grp_assemble(g, st); break;
       case G_NEXT: grp_next(g, st); break;

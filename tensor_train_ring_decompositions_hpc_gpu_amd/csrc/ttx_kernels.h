@@ -106,7 +106,8 @@ void launch_assemble(const float* Qc, int64_t ldq, int m, int k, const double* l
                      float* Gc, float* Ur, float* Qn, int kpn, float* sigma, unsigned long long* keys,
                      hipStream_t st);
 // A_next[c + r j] = sum_a Ur[a][c] Z[j][a]  (c < r, j < n); Z row-major with stride ldz (a multiple of 4,
-// 16-byte aligned rows, columns k .. round_up(k, 4) - 1 readable); Ur as launch_assemble writes it
+// 16-byte aligned rows, columns k .. round_up(k, 4) - 1 readable); Ur as launch_assemble writes it.
+// batch: one output row per lane for every n
 int next_from_z_ld(int r);
 void launch_next_from_z(const float* Z, int64_t ldz, int64_t n, int k, const float* Ur, int r, float* out,
                         hipStream_t st, bool batch = false);
@@ -212,9 +213,11 @@ void grp_gram_small(int ar, int mt, const GroupLaunch& g, hipStream_t st);
 void grp_reduce(bool f64, const GroupLaunch& g, hipStream_t st);
 void grp_sketch(int ar, int lt, bool omega, int fq, const GroupLaunch& g, hipStream_t st);
 void grp_proj(int ar, int lt, bool gram, const GroupLaunch& g, hipStream_t st);
-void grp_gram_rows(bool f64, const GroupLaunch& g, hipStream_t st);
+// kc: the widest job's k (the LDS is sized for 32, 48 or 64 by grp_kcap)
+int grp_kcap(int k);
+void grp_gram_rows(bool f64, int kc, const GroupLaunch& g, hipStream_t st);
 void grp_chol(const GroupLaunch& g, hipStream_t st);
-void grp_apply(int kind, const GroupLaunch& g, hipStream_t st);   // 0: float in, 1: double in, 2: Newton-Schulz out
+void grp_apply(int kind, int kc, const GroupLaunch& g, hipStream_t st);   // kind 0: float in, 1: double in, 2: Newton-Schulz out
 void grp_syevj(const GroupLaunch& g, hipStream_t st, int max_sweeps);
 void grp_assemble(const GroupLaunch& g, hipStream_t st);   // both passes (two launches)
 void grp_next(const GroupLaunch& g, hipStream_t st);
