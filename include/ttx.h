@@ -220,6 +220,17 @@ ttx_status ttx_tt_rel_error(ttx_handle h, int32_t order, const int64_t* dims, co
                             const float* const* d_cores, const float* d_A, double* err);
 ttx_status ttx_tr_rel_error(ttx_handle h, int32_t order, const int64_t* dims, const int32_t* ranks,
                             const float* const* d_cores, const float* d_A, double* err);
+/* TT rounding (no counterpart in the reference, whose cores are only freed, ttdec6_singleGPU.cu:947-952;
+ * the consumer that recompresses them, e.g. after adding two TTs): Oseledets 2011, TT-rounding
+ * (Algorithm 2), right-to-left orthogonalization (shifted CholeskyQR2 of each core's transpose) then
+ * left-to-right exact truncation steps with the delta rule, delta^2 = eps^2 ||A||_F^2 / (d - 1), so
+ * ||A - round(A)||_F <= eps ||A||_F.  max_ranks (optional, d + 1 entries) caps the ranks.  The input
+ * cores are not modified; d_cores_out[k] must hold r_{k-1} N_k r_k floats at the input ranks; ranks_out
+ * (d + 1) receives the new ranks, h_sigma (optional, (d - 1) x TTX_MAX_ELL) the singular values of each
+ * bond.  TT ranks at most TTX_MAX_ELL, r_{k-1} <= N_k r_k. */
+ttx_status ttx_tt_round(ttx_handle h, int32_t order, const int64_t* dims, const int32_t* ranks_in,
+                        const float* const* d_cores_in, double eps, const int32_t* max_ranks, float* const* d_cores_out,
+                        int32_t* ranks_out, float* h_sigma);
 /* contract cores into a full tensor (d_out: prod N floats) */
 ttx_status ttx_tt_full(ttx_handle h, int32_t order, const int64_t* dims, const int32_t* ranks,
                        const float* const* d_cores, float* d_out);

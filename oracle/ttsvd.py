@@ -28,6 +28,9 @@ Functions and the reference code each one restates:
                             test columns (Halko, Martinsson, Tropp 2011, sec. 4.3-4.4); the
                             reference has none (its q and ell are fixed, rsvd_lu_half.cu:557-683), so
                             this is the engine's own rule, pinned by closed forms in the tests
+* ``tt_round``/``tt_add``-- TT rounding (Oseledets 2011, Algorithm 2: QR right to left, truncated SVD left
+                            to right) and the TT sum whose ranks it recompresses; the reference has no
+                            consumer of its cores (they are freed, ttdec6_singleGPU.cu:947-952)
 * ``delta_rank``         -- the commented truncation rule,
                             trdecomposition/third_order_GPU/unopt_GPU/trdec_gpu_float_unopt2_rank.cu:92,222-243
 """
@@ -413,6 +416,61 @@ def tr_full(cores, dims, ranks):
     mid = permute_312(R.reshape(-1, order="F"), r1, nrest, r0).reshape(r0 * r1, nrest, order="F")
     G1 = np.asarray(cores[0], dtype=np.float64).reshape(dims[0], r0 * r1, order="F")
     return (G1 @ mid).reshape(-1, order="F")
+
+
+def tt_add(ca, cb, dims, ra, rb):
+    """Cores of the sum of two TTs (Oseledets 2011, sec. 4.1): block-diagonal cores of ranks ra + rb, the
+    first core [A1 B1], the last [Ad; Bd]."""
+    d = len(dims)
+    rs = [1] + [ra[k] + rb[k] for k in range(1, d)] + [1]
+    out = []
+    for k in range(d):
+        A = np.asarray(ca[k], dtype=np.float64).reshape(ra[k], dims[k], ra[k + 1], order="F")
+        B = np.asarray(cb[k], dtype=np.float64).reshape(rb[k], dims[k], rb[k + 1], order="F")
+        C = np.zeros((rs[k], dims[k], rs[k + 1]))
+        if k == 0:
+            C[:, :, :ra[1]] = A
+            C[:, :, ra[1]:] = B
+        elif k == d - 1:
+            C[:ra[k], :, :] = A
+            C[ra[k]:, :, :] = B
+        else:
+            C[:ra[k], :, :ra[k + 1]] = A
+            C[ra[k]:, :, ra[k + 1]:] = B
+        out.append(C.reshape(rs[k] * dims[k], rs[k + 1], order="F"))
+    return out, rs
+
+
+def tt_round(cores, dims, ranks, eps, max_ranks=None):
+    """TT rounding in FP64 (Oseledets 2011, Algorithm 2): right to left, core k (r_{k-1} x N_k r_k) = R^T Q^T
+    from the QR of its transpose, core k <- Q^T and R^T moves left; then left to right, the SVD of core k's
+    (r_{k-1} N_k) x r_k unfolding truncated by the delta rule, delta^2 = eps^2 ||A||^2 / (d - 1) (the
+    tail sum of squared singular values at most delta^2), core k <- U_r and S V^T moves right.
+    Returns (cores, ranks, sigma per bond)."""
+    d = len(dims)
+    r = list(ranks)
+    G = [np.asarray(c, dtype=np.float64).reshape(r[k] * dims[k], r[k + 1], order="F").copy()
+         for k, c in enumerate(cores)]
+    for k in range(d - 1, 0, -1):
+        M = G[k].reshape(r[k], dims[k] * r[k + 1], order="F")
+        Q, R = np.linalg.qr(M.T)
+        rk = Q.shape[1]
+        G[k] = Q.T.reshape(rk * dims[k], r[k + 1], order="F")
+        G[k - 1] = G[k - 1] @ R.T
+        r[k] = rk
+    norm2 = float(np.sum(G[0] ** 2))
+    delta2 = eps * eps * norm2 / (d - 1)
+    sig = []
+    for k in range(d - 1):
+        U, S, Vt = np.linalg.svd(G[k], full_matrices=False)
+        cap = r[k + 1] if max_ranks is None else max(1, min(max_ranks[k + 1], r[k + 1]))
+        rn = delta_rank(S ** 2, float(np.sum(S ** 2)), delta2, min(cap, len(S)), rel2=0.0)
+        sig.append(S)
+        G[k] = U[:, :rn]
+        nxt = (S[:rn, None] * Vt[:rn]) @ G[k + 1].reshape(r[k + 1], dims[k + 1] * r[k + 2], order="F")
+        G[k + 1] = nxt.reshape(rn * dims[k + 1], r[k + 2], order="F")
+        r[k + 1] = rn
+    return G, r, sig
 
 
 def rel_error(A, Ahat):

@@ -125,6 +125,8 @@ SIGNATURES = {
     "ttx_tr_rel_error": (_I32, [_P, _I32, ctypes.POINTER(_I64), ctypes.POINTER(_I32), _FPP, _P,
                                 ctypes.POINTER(_D)]),
     "ttx_tt_full": (_I32, [_P, _I32, ctypes.POINTER(_I64), ctypes.POINTER(_I32), _FPP, _P]),
+    "ttx_tt_round": (_I32, [_P, _I32, ctypes.POINTER(_I64), ctypes.POINTER(_I32), _FPP, ctypes.c_double,
+                            ctypes.POINTER(_I32), _FPP, ctypes.POINTER(_I32), _P]),
     "ttx_tr_full": (_I32, [_P, _I32, ctypes.POINTER(_I64), ctypes.POINTER(_I32), _FPP, _P]),
     "ttx_batch_tt_svd": (_I32, [_P, _I32, _I32, ctypes.POINTER(_I64), ctypes.POINTER(_I32), _FPP, _FPP,
                                 ctypes.POINTER(Params), _I32, _I32, ctypes.POINTER(BatchReport)]),

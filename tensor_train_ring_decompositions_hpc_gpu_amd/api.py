@@ -420,6 +420,32 @@ class Engine:
         _check(self.lib, self.h, st)
         return float(err.value)
 
+    def tt_round(self, cores, dims, ranks, eps, max_ranks=None):
+        """TT rounding of cores (ttx_tt_round): recompress to the smallest ranks with relative error at most
+        eps (capped by max_ranks).  cores: device tensors in the TT layout (or (tensor, shape) pairs).
+        Returns ([(core, (rows, cols))], new ranks, sigma (d-1) x TTX_MAX_ELL per bond)."""
+        torch = _torch()
+        d = len(dims)
+        if len(cores) != d or len(ranks) != d + 1:
+            raise ValueError("ttx: need d cores and d + 1 ranks")
+        ins = [(c[0] if isinstance(c, tuple) else c) for c in cores]
+        for k, c in enumerate(ins):
+            _dev_ptr(c)
+            if c.numel() < ranks[k] * dims[k] * ranks[k + 1]:
+                raise ValueError("ttx: core %d holds fewer values than its ranks need" % k)
+        outs = [torch.empty(ranks[k] * dims[k] * ranks[k + 1], device="cuda", dtype=torch.float32) for k in range(d)]
+        rout = (ctypes.c_int32 * (d + 1))()
+        sigma = np.zeros((d - 1) * L.MAX_ELL, dtype=np.float32)
+        rmax = L.i32_array(max_ranks) if max_ranks is not None else None
+        st = self.lib.ttx_tt_round(self.h, d, L.i64_array(dims), L.i32_array(ranks),
+                                   L.ptr_array([c.data_ptr() for c in ins]), float(eps), rmax,
+                                   L.ptr_array([c.data_ptr() for c in outs]), rout,
+                                   sigma.ctypes.data_as(ctypes.c_void_p))
+        _check(self.lib, self.h, st)
+        rn = [int(x) for x in rout]
+        shapes = [(rn[k] * dims[k], rn[k + 1]) for k in range(d)]
+        return [(c[: a * b], (a, b)) for c, (a, b) in zip(outs, shapes)], rn, sigma.reshape(d - 1, L.MAX_ELL)
+
     def full(self, cores, dims, ranks, tr=False):
         torch = _torch()
         out = torch.empty(int(np.prod(dims)), device="cuda", dtype=torch.float32)

@@ -123,6 +123,7 @@ struct Handle {
   // the grouped batch engine: its plan, the tensors' workspaces and the uploaded job blob
   std::unique_ptr<GPlan> gplan;
   DevBuf barena, bblob;
+  DevBuf rwork;   // TT rounding: working copies of the cores and a contraction buffer
   std::vector<Handle*> subs;
   hipStream_t bstream = nullptr;             // capture origin of the batch graph (never the null stream)
   hipGraph_t graph = nullptr;
@@ -351,6 +352,10 @@ struct StepArgs {
   // DELTA / LEARNED ranks: the chosen rank is rounded up to a multiple of this (the TR first step keeps
   // r0 * r1 columns: r0 * ceil(R / r0) for a delta rank R)
   int rank_multiple = 1;
+  // exact paths only: the eigensolver's Gram (A A^T of the exact-m path, Z^T Z of the tail) summed in
+  // FP64 from the FP32 operands instead of FP32 MFMA products, so singular values far below the largest
+  // keep their accuracy (sqrt of the Gram's relative error: about 1e-8 instead of 3e-4 of sigma_1)
+  bool precise = false;
 };
 // internal only, never returned to a caller: the step ran on a speculated left factor that did not
 // come true and is to be run again (7: above every public code, inside the enumeration's range)
@@ -440,12 +445,20 @@ static ttx_status step_tail(Handle* h, const StepArgs& a, const Arith3& ar, int 
   float* Z = P<float>(h->Z);
   const int kz = (k + 3) & ~3;   // Z keeps only the basis columns (rounded to 16 bytes)
   // Z = A^T Q (FP32-accurate) and its Gram B B^T = Z^T Z in the same pass
-  const int64_t gg = proj_grid_for(h, m, n);
+  int64_t gg = proj_grid_for(h, m, n);
   TTX_TRY(ensure(h, h->gpart, (size_t)gg * k * k * sizeof(double)));
   const float* Qop = proj_operand(h, a, P<float>(h->Qc), kp);
   {
     KScope ks(h, TTX_K_PROJ, a.m * n * 4 + n * kz * 4);
-    launch_proj_grid(ar.tail, kp, a.A, a.m, m, n, Qop, m, 0, Z, kz, kz, gg, st, P<double>(h->gpart), k);
+    launch_proj_grid(ar.tail, kp, a.A, a.m, m, n, Qop, m, 0, Z, kz, kz, gg, st, a.precise ? nullptr : P<double>(h->gpart),
+                     a.precise ? 0 : k);
+  }
+  if (a.precise) {
+    // Z^T Z in FP64 from the FP32 rows of Z
+    gg = gram_tall_grid(n);
+    TTX_TRY(ensure(h, h->gpart, (size_t)gg * k * k * sizeof(double)));
+    KScope ks(h, TTX_K_SMALL, 0);
+    launch_gram_tall_f32(Z, kz, n, k, P<double>(h->gpart), gg, st);
   }
   o.bytes += (int64_t)(a.m * n * 4);
   o.passes += 1;
@@ -775,6 +788,12 @@ static ttx_status run_step(Handle* h, const StepArgs& a_in, StepOut& o) {
     int64_t grid = gram_grid_for(h, n);
     if (a.gram_pre > 0) {
       grid = a.gram_pre;   // the fused first pass left them in gpart
+    } else if (a.precise) {
+      // A A^T as the FP64 Gram of the rows of A^T (A column-major m x n is A^T row-major, stride m)
+      grid = gram_tall_grid(n);
+      TTX_TRY(ensure(h, h->gpart, (size_t)grid * m * m * sizeof(double)));
+      KScope ks(h, TTX_K_GRAM, a.m * n * 4);
+      launch_gram_tall_f32(a.A, m, n, m, P<double>(h->gpart), grid, st);
     } else {
       TTX_TRY(ensure(h, h->gpart, (size_t)grid * m * m * sizeof(double)));
       KScope ks(h, TTX_K_GRAM, a.m * n * 4);
@@ -1805,6 +1824,120 @@ static ttx_status contract_halves(Handle* h, int d, const int64_t* dims, const i
   return TTX_OK;
 }
 
+// ------------------------------------------------------------------------------------------------
+// TT rounding (Oseledets 2011, TT-rounding, Algorithm 2): a consumer of the sweep's cores that
+// recompresses a TT (for instance the sum of two TTs, whose ranks add up) to the smallest ranks whose
+// relative error is within eps.  Right to left, core k (r_{k-1} x N_k r_k) becomes Q^T with Q the
+// orthonormal basis of its transpose (shifted CholeskyQR2, which stays orthonormal on a rank-deficient
+// core), and R^T = M Q moves into core k - 1.  Left to right, each core's unfolding then takes one
+// exact step of the sweep with the delta rule (delta^2 = eps^2 ||A||^2 / (d - 1), ||A|| being the
+// first core's norm once the others are right-orthonormal): its core is U_r and S V^T moves into
+// core k + 1.  Ranks at most TTX_MAX_ELL; FP32 data, FP64 solves.
+// ------------------------------------------------------------------------------------------------
+static ttx_status tt_round(Handle* h, int d, const int64_t* dims, const int32_t* rin, const float* const* cin,
+                           double eps, const int32_t* rmax, float* const* cout, int32_t* rout, float* h_sigma) {
+  hipStream_t st = h->stream;
+  if (d < 2 || d > TTX_MAX_ORDER || !(eps >= 0.0)) return fail(h, TTX_ERR_BAD_ARG, "round: bad order or eps");
+  if (rin[0] != 1 || rin[d] != 1) return fail(h, TTX_ERR_BAD_ARG, "round: TT ranks need r_0 = r_d = 1");
+  for (int k = 0; k < d; ++k) {
+    if (dims[k] <= 0 || rin[k + 1] <= 0 || !cin[k] || !cout[k]) return fail(h, TTX_ERR_BAD_ARG, "round: bad core");
+    if (rin[k + 1] > TTX_MAX_ELL) return fail(h, TTX_ERR_UNSUPPORTED, "round: rank above TTX_MAX_ELL");
+    if (k > 0 && rin[k] > dims[k] * rin[k + 1])
+      return fail(h, TTX_ERR_UNSUPPORTED, "round: r_{k-1} above N_k r_k (reduce the core first)");
+  }
+  // working copies W[k] (each sized for its input ranks; ranks only shrink), then a contraction buffer
+  std::vector<size_t> off(d + 1, 0);
+  size_t tmp_floats = 0;
+  for (int k = 0; k < d; ++k) {
+    const size_t e = (size_t)rin[k] * dims[k] * rin[k + 1];
+    off[k + 1] = off[k] + ((e + 63) & ~(size_t)63);
+    tmp_floats = std::max(tmp_floats, e);
+  }
+  TTX_TRY(ensure(h, h->rwork, (off[d] + tmp_floats + 2 * 64 * 64) * sizeof(float)));
+  float* base = P<float>(h->rwork);
+  std::vector<float*> W(d);
+  for (int k = 0; k < d; ++k) {
+    W[k] = base + off[k];
+    TTX_HIP(h, hipMemcpyAsync(W[k], cin[k], (size_t)rin[k] * dims[k] * rin[k + 1] * sizeof(float),
+                              hipMemcpyDeviceToDevice, st));
+  }
+  float* T = base + off[d];               // contraction output
+  float* RT = T + tmp_floats;             // R^T (r x r) of the orthogonalization
+  std::vector<int> r(rin, rin + d + 1);
+  TTX_TRY(ensure(h, h->status, 4 * sizeof(int)));
+  TTX_HIP(h, hipMemsetAsync(h->status.p, 0, 4 * sizeof(int), st));
+  // right to left: core k = Q^T, core k - 1 times R^T
+  for (int k = d - 1; k >= 1; --k) {
+    const int rk = r[k];
+    const int64_t L = dims[k] * (int64_t)r[k + 1];
+    const int kp = kp_for(rk);
+    TTX_TRY(ensure(h, h->Qc, (size_t)kp * L * sizeof(float)));
+    // core k, r_{k-1} x L column-major, is M^T as an L x r_{k-1} row-major matrix
+    TTX_TRY(orth(h, W[k], rk, (int)L, rk, kp, P<float>(h->Qc), P<int>(h->status)));
+    launch_gemm(rk, rk, L, 1.0f, W[k], rk, P<float>(h->Qc), L, 0.0f, RT, rk, st);   // R^T = M Q
+    const int64_t tdims[2] = {L, rk};
+    const int tperm[2] = {1, 0};
+    launch_permute(P<float>(h->Qc), W[k], 2, tdims, tperm, st);                    // core k = Q^T
+    const int64_t mk = (int64_t)r[k - 1] * dims[k - 1];
+    launch_gemm(mk, rk, rk, 1.0f, W[k - 1], mk, RT, rk, 0.0f, T, mk, st);
+    TTX_HIP(h, hipMemcpyAsync(W[k - 1], T, (size_t)mk * rk * sizeof(float), hipMemcpyDeviceToDevice, st));
+    TTX_TRY(check_launch(h, "round: orthogonalization"));
+  }
+  double norm2 = 0.0;
+  TTX_TRY(input_norm2(h, W[0], (int64_t)dims[0] * r[1], &norm2));
+  const double delta2 = eps * eps * norm2 / (double)(d - 1);
+  // left to right: one exact step with the delta rule per core
+  TTX_TRY(ensure(h, h->scratch[3], (size_t)TTX_MAX_ELL * (TTX_MAX_ELL + 4) * sizeof(float)));
+  for (int k = 0; k < d - 1; ++k) {
+    StepArgs a;
+    a.A = W[k];
+    a.m = (int64_t)r[k] * dims[k];
+    a.n = r[k + 1];
+    const int cap = rmax ? std::max(1, rmax[k + 1]) : r[k + 1];
+    a.r = (int)std::min<int64_t>(cap, std::min<int64_t>(a.m, a.n));
+    a.s = 0;
+    a.q = 0;
+    a.precision = TTX_PREC_FP32;
+    a.method = TTX_METHOD_EXACT;
+    a.rank_mode = TTX_RANK_DELTA;
+    a.delta2 = delta2;
+    a.precise = true;
+    a.core = cout[k];
+    a.next = T;
+    a.sig_dev = P<float>(h->scratch[3]);
+    a.stat_dev = reinterpret_cast<int*>(P<float>(h->scratch[3]) + TTX_MAX_ELL);
+    TTX_HIP(h, hipMemsetAsync(a.stat_dev, 0, 4 * sizeof(int), st));
+    a.sync = true;
+    StepOut o;
+    TTX_TRY(run_step(h, a, o));
+    if (o.nonconv)
+      return fail(h, TTX_ERR_NO_CONVERGE, "round: the Jacobi eigensolver reached its sweep cap at bond " +
+                                              std::to_string(k + 1));
+    if (h_sigma)
+      for (int i = 0; i < TTX_MAX_ELL; ++i) h_sigma[(size_t)k * TTX_MAX_ELL + i] = i < o.kdim ? o.sigma[i] : 0.0f;
+    const int rn = o.r;
+    // core k + 1 = (S V^T) times core k + 1 (r[k+1] x N_{k+1} r_{k+2})
+    const int64_t nk = dims[k + 1] * (int64_t)r[k + 2];
+    float* dst = (k + 1 == d - 1) ? cout[d - 1] : W[k + 1];
+    if (k + 1 == d - 1) {
+      launch_gemm(rn, nk, r[k + 1], 1.0f, T, rn, W[k + 1], r[k + 1], 0.0f, dst, rn, st);
+    } else {
+      // the product may not overwrite its operand: behind S V^T in the contraction buffer (which runs on
+      // into the R^T space, free in this sweep)
+      float* prod = T + (size_t)rn * r[k + 1];
+      if ((size_t)rn * r[k + 1] + (size_t)rn * nk > tmp_floats + 2 * 64 * 64)
+        return fail(h, TTX_ERR_HIP, "internal: round contraction buffer");
+      launch_gemm(rn, nk, r[k + 1], 1.0f, T, rn, W[k + 1], r[k + 1], 0.0f, prod, rn, st);
+      TTX_HIP(h, hipMemcpyAsync(W[k + 1], prod, (size_t)rn * nk * sizeof(float), hipMemcpyDeviceToDevice, st));
+    }
+    TTX_TRY(check_launch(h, "round: truncation"));
+    r[k + 1] = rn;
+  }
+  TTX_HIP(h, hipStreamSynchronize(st));
+  for (int k = 0; k <= d; ++k) rout[k] = r[k];
+  return TTX_OK;
+}
+
 static ttx_status rel_error(Handle* h, int d, const int64_t* dims, const int32_t* ranks, const float* const* cores,
                             const float* A, bool tr, double* err) {
   Halves hv;
@@ -2035,7 +2168,7 @@ static std::vector<ttx::DevBuf*> handle_bufs(Handle* h) {
           &h->scratch[0], &h->scratch[1], &h->scratch[2], &h->scratch[3], &h->Yt, &h->tpart,
           &h->tout, &h->mP, &h->mM1, &h->mM2, &h->mG, &h->mAct, &h->mPart, &h->mIO,
           &h->f16s, &h->Ypre, &h->omg, &h->Rprobe, &h->hs[0], &h->hs[1], &h->hGs, &h->hB, &h->hB2, &h->hA,
-          &h->G0, &h->lam0, &h->V0, &h->keys0, &h->barena, &h->bblob};
+          &h->G0, &h->lam0, &h->V0, &h->keys0, &h->barena, &h->bblob, &h->rwork};
 }
 
 static void free_handle(Handle* h, bool own_stream) {
@@ -2290,6 +2423,13 @@ ttx_status ttx_tr_rel_error(ttx_handle h, int32_t order, const int64_t* dims, co
                             const float* const* d_cores, const float* d_A, double* err) {
   if (!h || !dims || !ranks || !d_cores || !d_A || !err || order < 2 || order > TTX_MAX_ORDER) return TTX_ERR_BAD_ARG;
   return ttx::rel_error(H(h), order, dims, ranks, d_cores, d_A, true, err);
+}
+
+ttx_status ttx_tt_round(ttx_handle h, int32_t order, const int64_t* dims, const int32_t* ranks_in,
+                        const float* const* d_cores_in, double eps, const int32_t* max_ranks, float* const* d_cores_out,
+                        int32_t* ranks_out, float* h_sigma) {
+  if (!h || !dims || !ranks_in || !d_cores_in || !d_cores_out || !ranks_out) return TTX_ERR_BAD_ARG;
+  return ttx::tt_round(H(h), order, dims, ranks_in, d_cores_in, eps, max_ranks, d_cores_out, ranks_out, h_sigma);
 }
 
 ttx_status ttx_tt_full(ttx_handle h, int32_t order, const int64_t* dims, const int32_t* ranks,
