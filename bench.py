@@ -50,6 +50,8 @@ def parse():
     ap.add_argument("--no-tr", action="store_true", help="skip the TR-SVD workload")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched workload")
     ap.add_argument("--no-host", action="store_true", help="skip the host-streamed workload")
+    ap.add_argument("--batch-grouped-only", action="store_true",
+                    help="batched workload: the grouped engine's runs only (no per-tensor or worker-stream timing)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run one 1 GiB device-to-device copy (scale reference for rocprof HBM counters)")
     return ap.parse_args()
@@ -261,7 +263,7 @@ def batch_bytes_alg(dims_l, ranks_l, s=10, q=2):
     return total
 
 
-def batch_leg(eng, torch, count=512, streams=8, reps=5):
+def batch_leg(eng, torch, count=512, streams=8, reps=5, grouped_only=False):
     """Batched workload: 512 order-4 tensors, each mode size uniform in {16..64} and each TT rank uniform
     in {4..32} per tensor (seeded), N(0,1) cores, 1e-5 relative noise; BF16 range finder, s = 10, q = 2.
     Grouped engine: every step phase of the whole batch is one launch per kernel variant, the launches
@@ -279,7 +281,7 @@ def batch_leg(eng, torch, count=512, streams=8, reps=5):
         ranks_l.append(reachable_ranks(dims, ranks))   # a drawn rank above min(m, n) is unreachable
     elems = sum(int(np.prod(d)) for d in dims_l)
     kw = dict(s=10, q=2, precision="bf16", graph=True, autotune=True, streams=streams)
-    out, info = eng.batch_tt_svd(tensors, dims_l, ranks_l, per_tensor=True, **kw)   # tune, plan, capture, alone times
+    out, info = eng.batch_tt_svd(tensors, dims_l, ranks_l, per_tensor=not grouped_only, **kw)   # tune, plan, capture
     bufs = info["core_buffers"]
     ts = []
     for _ in range(reps):
@@ -289,9 +291,11 @@ def batch_leg(eng, torch, count=512, streams=8, reps=5):
     errs = [eng.rel_error(c, d, r, A) for c, d, r, A in zip(out[:32], dims_l[:32], ranks_l[:32], tensors[:32])]
     balg = batch_bytes_alg(dims_l, ranks_l)
     # the worker-stream engine on the same batch (the same tuned plan; the second call is timed)
-    kws = dict(s=10, q=2, precision="bf16", streams=streams, graph=False, autotune=True, engine="streams")
-    eng.batch_tt_svd(tensors, dims_l, ranks_l, **kws)
-    _, info_s = eng.batch_tt_svd(tensors, dims_l, ranks_l, cores=bufs, **kws)
+    info_s = {"total_ms": float("nan"), "tensors_per_s": float("nan")}
+    if not grouped_only:
+        kws = dict(s=10, q=2, precision="bf16", streams=streams, graph=False, autotune=True, engine="streams")
+        eng.batch_tt_svd(tensors, dims_l, ranks_l, **kws)
+        _, info_s = eng.batch_tt_svd(tensors, dims_l, ranks_l, cores=bufs, **kws)
     res = {"workload": "512 order-4 TT tensors, N_i~U{16..64}, r_i~U{4..32}, N(0,1) cores + 1e-5 noise, BF16, s=10, q=2",
            "engine": info2["engine"], "launches_per_batch": info2["launches"],
            "tensors_per_s": round(count / (t * 1e-3), 1), "batch_ms": round(t, 3),
@@ -403,6 +407,15 @@ def cfg1_leg(eng, torch, L):
     return out
 
 
+def copy_probe(torch):
+    """one 1 GiB device-to-device copy: the scale reference of rocprof's FETCH_SIZE / WRITE_SIZE"""
+    a = torch.ones(1 << 28, device="cuda", dtype=torch.float32)
+    b = torch.empty_like(a)
+    b.copy_(a)
+    torch.cuda.synchronize()
+    del a, b
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -415,11 +428,14 @@ def main():
 
     torch.cuda.set_device(0)
     eng = Engine()
-    side = {"tr": lambda: tr_leg(eng, torch, L), "batch": lambda: batch_leg(eng, torch),
+    side = {"tr": lambda: tr_leg(eng, torch, L), "batch": lambda: batch_leg(eng, torch, grouped_only=args.batch_grouped_only),
             "learned": lambda: learned_leg(eng, torch, L), "cfg1": lambda: cfg1_leg(eng, torch, L)}
     if args.only in side:
         # one secondary workload alone (profiling runs): its own JSON object, not the bench line
-        print(json.dumps({args.only: side[args.only]()}))
+        res = side[args.only]()
+        if args.copy_probe:
+            copy_probe(torch)
+        print(json.dumps({args.only: res}))
         eng.close()
         return
     A = eng.generate(DIMS, RANKS, std=1.0 / np.sqrt(32.0), noise_rel=1e-6, seed=DATA_SEED)
@@ -535,11 +551,7 @@ def main():
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(A, torch)
     if args.copy_probe:
-        a = torch.ones(1 << 28, device="cuda", dtype=torch.float32)
-        b = torch.empty_like(a)
-        b.copy_(a)
-        torch.cuda.synchronize()
-        del a, b
+        copy_probe(torch)
     print(json.dumps(out))
     eng.close()
 

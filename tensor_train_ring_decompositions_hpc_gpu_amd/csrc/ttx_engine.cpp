@@ -845,9 +845,10 @@ static ttx_status run_step(Handle* h, const StepArgs& a_in, StepOut& o) {
       o.tr_r1c = r1c;
       const int64_t rw = r1c > 0 ? (int64_t)a.tr_r0 * r1c : r;
       KScope ks(h, TTX_K_PROJ, a.m * n * 4 + rw * n * 4);
-      if (r1c > 0) launch_proj(ar.accurate, kpn, a.A, a.m, m, n, Qop, m, 2, a.next, a.tr_r0, r1c, st);
-      else launch_proj_grid(ar.accurate, kpn, a.A, a.m, m, n, Qop, m, 1, a.next, 0, r, proj_grid_for(h, m, n), st,
-                            nullptr, 0);
+      // the exact path's m <= 64 rows: the short-unfolding projection
+      if (r1c > 0) launch_proj_short(ar.accurate, kpn, a.A, a.m, m, n, Qop, m, 2, a.next, a.tr_r0, r1c, 0, st);
+      else launch_proj_short(ar.accurate, kpn, a.A, a.m, m, n, Qop, m, 1, a.next, 0, r,
+                             h->batch_mode ? proj_short_grid(n, true) : 0, st);
       o.bytes += a.m * n * 4 + rw * n * 4;
       o.passes += 1;
     }
@@ -1344,7 +1345,8 @@ static ttx_status sweep(Handle* h, const float* d_A, const ttx_params* p, float*
         const Arith3 ar0 = arith_for(p->precision, p->flags);
         launch_assemble(nullptr, 0, m0, m0, P<double>(h->lam0), P<double>(h->V0), r0, d_cores[0], nullptr,
                         P<float>(h->Qn), kpn, a0.sig_dev, P<unsigned long long>(h->keys0), st);
-        launch_proj(ar0.accurate, kpn, d_A, m0, m0, n0, P<float>(h->Qn), m0, 1, P<float>(h->next[0]), 0, r0, st);
+        launch_proj_short(ar0.accurate, kpn, d_A, m0, m0, n0, P<float>(h->Qn), m0, 1, P<float>(h->next[0]), 0, r0, 0,
+                          st);
         TTX_TRY(check_launch(h, "first step projection"));
         o0.deferred = false;
         o0.bytes += (int64_t)m0 * n0 * 4 + (int64_t)r0 * n0 * 4;
@@ -1674,11 +1676,11 @@ static ttx_status sweep_host(Handle* h, const float* h_A, const ttx_params* p, i
   TTX_TRY(ensure(h, h->hB, (size_t)r * n * sizeof(float)));
   float* B = P<float>(h->hB);
   if (resident) {
-    launch_proj(ar.accurate, kpn, Ad, m, m, n, P<float>(h->Qn), m, 1, B, 0, r, st);
+    launch_proj_short(ar.accurate, kpn, Ad, m, m, n, P<float>(h->Qn), m, 1, B, 0, r, 0, st);
     TTX_TRY(check_launch(h, "host-streamed projection"));
   } else {
     TTX_TRY(stream_pass([&](int64_t, const float* X, int64_t c0, int64_t ws) {
-      launch_proj(ar.accurate, kpn, X, m, m, ws, P<float>(h->Qn), m, 1, B + c0 * r, 0, r, st);
+      launch_proj_short(ar.accurate, kpn, X, m, m, ws, P<float>(h->Qn), m, 1, B + c0 * r, 0, r, 0, st);
     }));
   }
   float sig0[TTX_MAX_ELL];
@@ -2670,7 +2672,7 @@ static ttx_status batch_join(Handle* h, hipStream_t origin, int S, std::vector<h
 // those of its own sweep, bit for bit.
 // ------------------------------------------------------------------------------------------------
 enum GKind { G_GRAM_SMALL, G_REDUCE_F32, G_REDUCE_F64, G_SKETCH, G_PROJ, G_GRAM_ROWS, G_CHOL, G_APPLY, G_SYEVJ,
-             G_ASSEMBLE, G_NEXT, G_COPY_COLS };
+             G_ASSEMBLE, G_NEXT, G_COPY_COLS, G_PROJ_SHORT };
 
 // the launches of one phase, one record per (kernel, template choices, block size), in first-use order
 struct GPhase {
@@ -2997,8 +2999,9 @@ static ttx_status gplan_build(Handle* h, const std::vector<ttx_params>& P, const
       const BStep& b = sch[i][k];
       const BWork& w = W[i];
       if (is_em(b)) {
-        ph.add(G_PROJ, ar.accurate, b.kpn / 32, 0, 0, 512, 0,
-               JobProj{(int)b.gproj, 1, b.A, b.m, b.m, b.n, w.Qn, b.m, 1, b.next, 0, b.r, proj_blocks(b.n), nullptr, 0});
+        ph.add(G_PROJ_SHORT, ar.accurate, b.kpn / 32, (b.m + 15) / 16, 0, 256, 0,
+               JobProj{(int)proj_short_grid(b.n, true), 1, b.A, b.m, b.m, b.n, w.Qn, b.m, 1, b.next, 0, b.r, 0, nullptr,
+                       0});
       } else {
         bool cols = false;
         const int64_t g = next_from_z_grid(b.n, b.r, &cols, true);
@@ -3045,6 +3048,7 @@ static ttx_status gplan_queue(Handle* h, const GPlan& G, hipStream_t st) {
       case G_REDUCE_F64: grp_reduce(true, g, st); break;
       case G_SKETCH: grp_sketch(r.ar, r.v0, r.v1 != 0, r.v2, g, st); break;
       case G_PROJ: grp_proj(r.ar, r.v0, r.v1 != 0, g, st); break;
+      case G_PROJ_SHORT: grp_proj_short(r.ar, r.v0, r.v1, g, st); break;
       case G_GRAM_ROWS: grp_gram_rows(false, r.v1, g, st); break;
       case G_CHOL: grp_chol(g, st); break;
       case G_APPLY: grp_apply(r.v0, r.v1, g, st); break;

@@ -53,6 +53,12 @@ int64_t proj_grid(int64_t n);
 void launch_proj(int ar, int kp, const float* A, int64_t lda, int m, int64_t n, const float* Qc, int64_t ldq,
                  int mode, float* out, int64_t ldo, int r_out, hipStream_t st, double* gpart = nullptr, int kg = 0,
                  const float* scales = nullptr);
+// D = Q^T A for a short unfolding (m <= 64 rows; the exact step's G^T A): mode 1 (r_out x n column-major)
+// or mode 2 (the TR first step's 231 layout, as launch_proj); arithmetic AR_FP32 or AR_BF16X4 (any other
+// takes AR_BF16X4); kp = 32 or 64 columns of Qc (ldq); grid 0: proj_short_grid(n, false)
+int64_t proj_short_grid(int64_t n, bool batch);
+void launch_proj_short(int ar, int kp, const float* A, int64_t lda, int m, int64_t n, const float* Qc, int64_t ldq,
+                       int mode, float* out, int64_t ldo, int r_out, int64_t grid, hipStream_t st);
 int64_t gram_small_grid(int64_t n);
 // partial A A^T: part[grid][m][m] (FP64); absmax (optional) receives max |A| as float bits
 void launch_gram_small(int ar, const float* A, int64_t lda, int m, int64_t n, double* part, int64_t grid,
@@ -213,6 +219,7 @@ void grp_gram_small(int ar, int mt, const GroupLaunch& g, hipStream_t st);
 void grp_reduce(bool f64, const GroupLaunch& g, hipStream_t st);
 void grp_sketch(int ar, int lt, bool omega, int fq, const GroupLaunch& g, hipStream_t st);
 void grp_proj(int ar, int lt, bool gram, const GroupLaunch& g, hipStream_t st);
+void grp_proj_short(int ar, int lt, int nk, const GroupLaunch& g, hipStream_t st);
 // kc: the widest job's k (the LDS is sized for 32, 48 or 64 by grp_kcap)
 int grp_kcap(int k);
 void grp_gram_rows(bool f64, int kc, const GroupLaunch& g, hipStream_t st);

@@ -1051,6 +1051,174 @@ __global__ __launch_bounds__(512) void k_proj_grp(const JobProj* __restrict__ jo
 }
 
 // ------------------------------------------------------------------------------------------------
+// projection of a short unfolding: D = Q^T A for m <= 64 rows (the exact step's G^T A)
+// ------------------------------------------------------------------------------------------------
+// k_proj stages 64-row chunks of 256 columns; at m <= 32 half of each chunk is padding and a block's
+// loads are a few hundred bytes per column.  Here a wave computes 32 columns at a time with the rows as
+// the MFMA's K dimension: lane (n = lane % 32, hb = lane / 32) holds rows 16 kk + 8 hb + [0, 8) of
+// column j0 + n (32 contiguous bytes; the tile's 32 columns are one contiguous block of 32 m floats, so
+// its loads use every byte of the lines they touch) as the B operand, and the matching rows of Q's
+// columns c = 32 lt + n, loaded and split once per wave, as the A operand.  The next tile's loads are
+// issued before this tile's MFMAs.  NK = ceil(m / 16) K-steps; rows i >= m are zero.  Output as
+// k_proj's mode 1 (r_out x n column-major) or mode 2 (the TR first step's 231 layout).
+// Arithmetic: AR_FP32 (v_mfma_f32_32x32x2_f32, eight per K-step) or AR_BF16X4 (both operands split in
+// two bf16 terms, four products: hi hi, lo hi, hi lo, lo lo).
+template <int LT, int NK, int AR, int MODE>
+__device__ __forceinline__ void proj_short_body(const float* __restrict__ A, int64_t lda, int m, int64_t n,
+                                                const float* __restrict__ Qc, int64_t ldq,
+                                                float* __restrict__ out, int64_t ldo, int r_out, const Blk B) {
+  static_assert(AR == AR_FP32 || AR == AR_BF16X4, "short projection: FP32 or BF16X4");
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  const int cn = lane & 31, hb = lane >> 5;
+  // Q operand: Q[i][c] for i = 16 kk + 8 hb + t, c = 32 lt + cn (Qc column-major, ldq; kp = 32 LT columns)
+  float qv[LT][NK][8];
+#pragma unroll
+  for (int lt = 0; lt < LT; ++lt)
+#pragma unroll
+    for (int kk = 0; kk < NK; ++kk)
+#pragma unroll
+      for (int t = 0; t < 8; ++t) {
+        const int i = 16 * kk + 8 * hb + t;
+        qv[lt][kk][t] = (i < m) ? Qc[(int64_t)(32 * lt + cn) * ldq + i] : 0.0f;
+      }
+  [[maybe_unused]] bf16x8 qh[LT][NK], ql[LT][NK];
+  if constexpr (AR == AR_BF16X4) {
+#pragma unroll
+    for (int lt = 0; lt < LT; ++lt)
+#pragma unroll
+      for (int kk = 0; kk < NK; ++kk) split8(qv[lt][kk], qh[lt][kk], ql[lt][kk]);
+  }
+  const int64_t tiles = (n + 31) / 32;
+  // whole 8-row groups as two 16-byte loads: m a multiple of 8 and 16-byte aligned columns
+  const bool vec = ((m & 7) == 0) && ((lda & 3) == 0) && ((reinterpret_cast<uintptr_t>(A) & 15) == 0);
+  auto load = [&](float (&x)[NK][8], int64_t tile) {
+    const int64_t j = tile * 32 + cn;
+    const float* col = A + j * lda;
+#pragma unroll
+    for (int kk = 0; kk < NK; ++kk) {
+      const int i0 = 16 * kk + 8 * hb;
+      if (vec) {
+        if (j < n && i0 < m) {
+          const f32x4 u0 = *reinterpret_cast<const f32x4*>(col + i0);
+          const f32x4 u1 = *reinterpret_cast<const f32x4*>(col + i0 + 4);
+          x[kk][0] = u0[0]; x[kk][1] = u0[1]; x[kk][2] = u0[2]; x[kk][3] = u0[3];
+          x[kk][4] = u1[0]; x[kk][5] = u1[1]; x[kk][6] = u1[2]; x[kk][7] = u1[3];
+        } else {
+#pragma unroll
+          for (int t = 0; t < 8; ++t) x[kk][t] = 0.0f;
+        }
+      } else {
+#pragma unroll
+        for (int t = 0; t < 8; ++t) x[kk][t] = (j < n && i0 + t < m) ? col[i0 + t] : 0.0f;
+      }
+    }
+  };
+  auto compute = [&](const float (&x)[NK][8], f32x16 (&acc)[LT]) {
+#pragma unroll
+    for (int kk = 0; kk < NK; ++kk) {
+      if constexpr (AR == AR_FP32) {
+#pragma unroll
+        for (int lt = 0; lt < LT; ++lt)
+#pragma unroll
+          for (int t = 0; t < 8; ++t) acc[lt] = mfma_f32(qv[lt][kk][t], x[kk][t], acc[lt]);
+      } else {
+        bf16x8 xh, xl;
+        split8(x[kk], xh, xl);
+#pragma unroll
+        for (int lt = 0; lt < LT; ++lt) {
+          acc[lt] = mfma_bf16(qh[lt][kk], xh, acc[lt]);
+          acc[lt] = mfma_bf16(ql[lt][kk], xh, acc[lt]);
+          acc[lt] = mfma_bf16(qh[lt][kk], xl, acc[lt]);
+          acc[lt] = mfma_bf16(ql[lt][kk], xl, acc[lt]);
+        }
+      }
+    }
+  };
+  const bool vout = (MODE == 1) && ((r_out & 3) == 0) && ((reinterpret_cast<uintptr_t>(out) & 15) == 0);
+  // mode 2: the tile goes through the wave's LDS area (32 LT rows x 32 columns, pitch 33) so that every
+  // a writes its r1c x 32 floats as one contiguous run
+  constexpr int DP = 33;
+  __shared__ float dstage[MODE == 2 ? 4 * 32 * LT * DP : 1];
+  auto store = [&](const f32x16 (&acc)[LT], int64_t tile) {
+    if constexpr (MODE == 1) {
+      const int64_t j = tile * 32 + cn;
+      if (j >= n) return;
+      float* o = out + j * (int64_t)r_out;
+#pragma unroll
+      for (int lt = 0; lt < LT; ++lt)
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
+          const int c0 = 32 * lt + 8 * g + 4 * hb;   // rows c0 .. c0 + 3 of D in acc[lt][4 g .. 4 g + 3]
+          if (vout && c0 + 3 < r_out) {
+            *reinterpret_cast<f32x4*>(o + c0) =
+                f32x4{acc[lt][4 * g], acc[lt][4 * g + 1], acc[lt][4 * g + 2], acc[lt][4 * g + 3]};
+          } else {
+#pragma unroll
+            for (int e = 0; e < 4; ++e)
+              if (c0 + e < r_out) o[c0 + e] = acc[lt][4 * g + e];
+          }
+        }
+    } else {
+      // row c = a + r0 b (b < r1c) of column j to out[b + r1c (j + n a)] (trdec6_singleGPU.cu:49-58)
+      float* d = dstage + w * (32 * LT * DP);
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+#pragma unroll
+      for (int lt = 0; lt < LT; ++lt)
+#pragma unroll
+        for (int v = 0; v < 16; ++v) d[(32 * lt + mfma32_row(v, lane)) * DP + cn] = acc[lt][v];
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      const int r0 = (int)ldo, r1c = r_out;
+      const int64_t j0 = tile * 32;
+      const int ncols = (n - j0 >= 32) ? 32 : (int)(n - j0);
+      for (int a2 = 0; a2 < r0; ++a2) {
+        float* o = out + (int64_t)r1c * (j0 + n * a2);
+        for (int e = lane; e < r1c * ncols; e += 64) {
+          const int b2 = e % r1c, jj = e / r1c;
+          o[e] = d[(a2 + r0 * b2) * DP + jj];
+        }
+      }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+    }
+  };
+  int64_t tile = (int64_t)B.x * nw + w;
+  const int64_t stride = (int64_t)B.gx * nw;
+  float x0[NK][8], x1[NK][8];
+  if (tile < tiles) load(x0, tile);
+  for (; tile < tiles; tile += 2 * stride) {
+    // even tile from x0, the next one's loads in flight; then the odd tile from x1
+    if (tile + stride < tiles) load(x1, tile + stride);
+    f32x16 acc[LT];
+#pragma unroll
+    for (int lt = 0; lt < LT; ++lt) acc[lt] = zero16();
+    compute(x0, acc);
+    store(acc, tile);
+    if (tile + stride >= tiles) break;
+    if (tile + 2 * stride < tiles) load(x0, tile + 2 * stride);
+#pragma unroll
+    for (int lt = 0; lt < LT; ++lt) acc[lt] = zero16();
+    compute(x1, acc);
+    store(acc, tile + stride);
+  }
+}
+template <int LT, int NK, int AR, int MODE>
+__global__ __launch_bounds__(256) void k_proj_short(const float* __restrict__ A, int64_t lda, int m, int64_t n,
+                                                    const float* __restrict__ Qc, int64_t ldq,
+                                                    float* __restrict__ out, int64_t ldo, int r_out) {
+  proj_short_body<LT, NK, AR, MODE>(A, lda, m, n, Qc, ldq, out, ldo, r_out, hw_blk());
+}
+// grouped: mode 1 jobs (a batch's exact steps)
+template <int LT, int NK, int AR>
+__global__ __launch_bounds__(256) void k_proj_short_grp(const JobProj* __restrict__ jobs, const int2* __restrict__ map) {
+  Blk B;
+  const JobProj& j = grp_job(jobs, map, B);
+  proj_short_body<LT, NK, AR, 1>(j.A, j.lda, j.m, j.n, j.Qc, j.ldq, j.out, j.ldo, j.r_out, B);
+}
+
+// ------------------------------------------------------------------------------------------------
 // exact-path Gram: C = A A^T for m <= 64 (MT = number of 32-row tiles), FP64 partial per WG
 // ------------------------------------------------------------------------------------------------
 template <int MT, int AR>
@@ -1514,6 +1682,39 @@ void launch_proj_grid(int ar, int kp, const float* A, int64_t lda, int m, int64_
   else launch_proj_ar<2>(ar, A, lda, m, n, Qc, ldq, mode, out, ldo, r_out, gpart, kg, st, scales, grid);
 }
 
+// one 256-thread block per 4 x 32 columns, at most 16 per CU (plain), or 16 tiles per wave (batch)
+int64_t proj_short_grid(int64_t n, bool batch) {
+  const int64_t tiles = (n + 31) / 32;
+  int64_t g = batch ? (tiles + 63) / 64 : (tiles + 3) / 4;
+  if (!batch && g > (int64_t)num_cus() * 16) g = (int64_t)num_cus() * 16;
+  return g < 1 ? 1 : g;
+}
+
+template <int LT, int AR>
+static void launch_proj_short_nk(int nk, dim3 g, hipStream_t st, const float* A, int64_t lda, int m, int64_t n,
+                                 const float* Qc, int64_t ldq, int mode, float* out, int64_t ldo, int r_out) {
+#define TTX_PS(NK, MODE) hipLaunchKernelGGL((k_proj_short<LT, NK, AR, MODE>), g, dim3(256), 0, st, A, lda, m, n, Qc, ldq, \
+                                            out, ldo, r_out)
+#define TTX_PSM(MODE) { if (nk <= 1) TTX_PS(1, MODE); else if (nk == 2) TTX_PS(2, MODE); \
+                        else if (nk == 3) TTX_PS(3, MODE); else TTX_PS(4, MODE); }
+  if (mode == 2) TTX_PSM(2) else TTX_PSM(1)
+#undef TTX_PSM
+#undef TTX_PS
+}
+
+void launch_proj_short(int ar, int kp, const float* A, int64_t lda, int m, int64_t n, const float* Qc, int64_t ldq,
+                       int mode, float* out, int64_t ldo, int r_out, int64_t grid, hipStream_t st) {
+  const int nk = (m + 15) / 16;
+  const dim3 g((unsigned)(grid > 0 ? grid : proj_short_grid(n, false)));
+  if (ar == AR_FP32) {
+    if (kp == 32) launch_proj_short_nk<1, AR_FP32>(nk, g, st, A, lda, m, n, Qc, ldq, mode, out, ldo, r_out);
+    else launch_proj_short_nk<2, AR_FP32>(nk, g, st, A, lda, m, n, Qc, ldq, mode, out, ldo, r_out);
+  } else {
+    if (kp == 32) launch_proj_short_nk<1, AR_BF16X4>(nk, g, st, A, lda, m, n, Qc, ldq, mode, out, ldo, r_out);
+    else launch_proj_short_nk<2, AR_BF16X4>(nk, g, st, A, lda, m, n, Qc, ldq, mode, out, ldo, r_out);
+  }
+}
+
 int64_t gram_small_grid(int64_t n) {
   // four 4-wave WGs per CU (the kernel's registers allow five)
   int64_t g = (int64_t)num_cus() * 4;
@@ -1615,6 +1816,17 @@ static void grp_proj_t(int ar, const GroupLaunch& g, hipStream_t st) {
     default: hipLaunchKernelGGL((k_proj_grp<LT, AR_BF16X3, GRAM>), TTX_GRP_ARGS, J, g.map); break;
   }
 }
+void grp_proj_short(int ar, int lt, int nk, const GroupLaunch& g, hipStream_t st) {
+  const JobProj* J = static_cast<const JobProj*>(g.jobs);
+#define TTX_PS(LT, NK, AR) hipLaunchKernelGGL((k_proj_short_grp<LT, NK, AR>), TTX_GRP_ARGS, J, g.map)
+#define TTX_PSN(LT, AR) { if (nk <= 1) TTX_PS(LT, 1, AR); else if (nk == 2) TTX_PS(LT, 2, AR); \
+                          else if (nk == 3) TTX_PS(LT, 3, AR); else TTX_PS(LT, 4, AR); }
+  if (ar == AR_FP32) { if (lt == 1) TTX_PSN(1, AR_FP32) else TTX_PSN(2, AR_FP32) }
+  else { if (lt == 1) TTX_PSN(1, AR_BF16X4) else TTX_PSN(2, AR_BF16X4) }
+#undef TTX_PSN
+#undef TTX_PS
+}
+
 void grp_proj(int ar, int lt, bool gram, const GroupLaunch& g, hipStream_t st) {
   if (lt == 1) { if (gram) grp_proj_t<1, true>(ar, g, st); else grp_proj_t<1, false>(ar, g, st); }
   else { if (gram) grp_proj_t<2, true>(ar, g, st); else grp_proj_t<2, false>(ar, g, st); }
