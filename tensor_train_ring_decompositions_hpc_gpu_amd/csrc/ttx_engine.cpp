@@ -584,12 +584,13 @@ static ttx_status power_iter(Handle* h, const StepArgs& a, const Arith3& ar, int
   float* sc = fp16_scales(h, a, ar, false);
   {
     KScope ks(h, TTX_K_PROJ, a.m * n * 4 + n * kp * 4);
+    // FP16: the projection also takes Z's column maxima (the power pass's operand scales)
     launch_proj_grid(ar.range, kp, a.A, a.m, m, n, Qop, m, 0, P<float>(h->Z), kp, 0, proj_grid_for(h, m, n), st,
-                     nullptr, 0, sc);
+                     nullptr, 0, sc, sc ? reinterpret_cast<unsigned*>(sc + 65) : nullptr);
   }
   if (sc) {
     KScope ks(h, TTX_K_SMALL, 0);
-    launch_fp16_xscale(P<float>(h->Z), n, kp, kp, reinterpret_cast<unsigned*>(sc + 65), sc, st);
+    launch_fp16_colscale(reinterpret_cast<unsigned*>(sc + 65), kp, sc, st);
   }
   {
     KScope ks(h, TTX_K_SKETCH, a.m * n * 4 + n * kp * 4);

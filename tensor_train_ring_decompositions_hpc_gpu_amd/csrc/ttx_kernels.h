@@ -42,6 +42,8 @@ void launch_omega_chunks(uint64_t seed, uint32_t stream, int64_t n, int ell, voi
 void launch_fp16_ascale(const float* sig_prev, int r_prev, double numel, float* scales, hipStream_t st);
 // scales[1 + c] from max_j |X[j][c]| (to [128, 256)); mx: kp zeroed words, zeroed again afterwards
 void launch_fp16_xscale(const float* X, int64_t n, int64_t ldx, int kp, unsigned* mx, float* scales, hipStream_t st);
+// scales[1 + c] from the column maxima mx[c] (float bits) alone; mx zeroed again afterwards
+void launch_fp16_colscale(unsigned* mx, int kp, float* scales, hipStream_t st);
 // D = Q^T A; mode 0 writes Z (n x ldo row-major: kp columns, or the first r_out when r_out > 0, a multiple
 // of 4), mode 1 writes r_out x n column-major, mode 2 writes the TR first-step remainder straight into the
 // 231 layout: row c = a + r0 b (b < r1c) of column j to out[b + r1c (j + n a)], with r0 = ldo and
@@ -211,9 +213,11 @@ int syevj_threads(int k);
 int64_t next_from_z_grid(int64_t n, int r, bool* cols, bool batch = false);
 int64_t copy_cols_grid(int m, int kp);
 // plain launches with an explicit grid (the batch plans' grids; grid 0: the default)
+// colmax (FP16 arithmetic, mode 0 only): also atomicMax of max_j |Z[j][c]| (float bits) into colmax[c],
+// the input of launch_fp16_colscale (the column scales of Z without launch_fp16_xscale's pass over it)
 void launch_proj_grid(int ar, int kp, const float* A, int64_t lda, int m, int64_t n, const float* Qc, int64_t ldq,
                       int mode, float* out, int64_t ldo, int r_out, int64_t grid, hipStream_t st, double* gpart, int kg,
-                      const float* scales = nullptr);
+                      const float* scales = nullptr, unsigned* colmax = nullptr);
 // grouped launchers (ar: the Arith of every job; variant: the template choice the jobs share)
 void grp_gram_small(int ar, int mt, const GroupLaunch& g, hipStream_t st);
 void grp_reduce(bool f64, const GroupLaunch& g, hipStream_t st);

@@ -688,12 +688,15 @@ __global__ __launch_bounds__(512) void k_sketch_dl(const float* __restrict__ A, 
 // GRAM: also accumulate the Gram D D^T of the written tile (the projected Gram B B^T of the final
 // pass): each wave transposes its tile through LDS and runs FP32 MFMAs over its columns; at the end
 // the 8 wave sums are added in wave order in FP64 and gpart[blockIdx.x][kg][kg] is written.
-template <int LT, int AR, bool GRAM>
+// COLMAX (mode 0): also max_j |Z[j][c]| per column c as float bits, atomicMax into colmax[c] (the FP16
+// range finder's column scales of Z, without a pass of their own)
+template <int LT, int AR, bool GRAM, bool COLMAX = false>
 __device__ __forceinline__ void proj_body(const float* __restrict__ A, int64_t lda, int m, int64_t n,
                                           const float* __restrict__ Qc, int64_t ldq,
                                           int out_mode, float* __restrict__ out, int64_t ldo, int r_out,
                                           int64_t blocks_total, double* __restrict__ gpart, int kg,
-                                          const float* __restrict__ scales, const Blk B) {
+                                          const float* __restrict__ scales, const Blk B,
+                                          unsigned* __restrict__ colmax = nullptr) {
   constexpr int KP = 32 * LT;
   constexpr int QP = 64 + 8;   // LDS pitch of a Q column chunk (floats; bf16 halves use the same pitch)
   // bf16 arithmetic keeps the Q chunk in LDS already split: hi in the first half of a buffer, lo in
@@ -707,6 +710,16 @@ __device__ __forceinline__ void proj_body(const float* __restrict__ A, int64_t l
   // per-wave staging of its A tile (32 columns x 64 rows); the Gram tile reuses it after the K loop
   __shared__ float stg[8][32 * SP];
   __shared__ double gs[GRAM ? KP * KP : 1];
+  __shared__ unsigned cmx[COLMAX ? KP : 1];
+  // COLMAX: running max |D[c][j]| of the lane's rows c over its columns, reduced across lanes at the end
+  [[maybe_unused]] float amax[COLMAX ? LT : 1][16];
+  if constexpr (COLMAX) {
+    if (threadIdx.x < KP) cmx[threadIdx.x] = 0u;
+#pragma unroll
+    for (int lt = 0; lt < LT; ++lt)
+#pragma unroll
+      for (int v = 0; v < 16; ++v) amax[lt][v] = 0.0f;
+  }
   f32x16 gacc[GRAM ? NPG : 1];
   if constexpr (GRAM) {
 #pragma unroll
@@ -956,7 +969,16 @@ __device__ __forceinline__ void proj_body(const float* __restrict__ A, int64_t l
       }
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
       __builtin_amdgcn_wave_barrier();
-    } else if (j < n) {
+    } else {
+      if constexpr (COLMAX) {
+        // columns past n hold zeros
+#pragma unroll
+        for (int lt = 0; lt < LT; ++lt)
+#pragma unroll
+          for (int v = 0; v < 16; ++v) amax[lt][v] = fmaxf(amax[lt][v], fabsf(acc[lt][v]));
+      }
+    }
+    if (out_mode != 2 && j < n) {
       if (out_mode == 0) {
         // Z = D^T, row-major n x ldo: kp columns, or the first r_out (a multiple of 4)
         float* o = out + j * ldo;
@@ -1008,6 +1030,22 @@ __device__ __forceinline__ void proj_body(const float* __restrict__ A, int64_t l
       }
     }
   }
+  if constexpr (COLMAX) {
+    // the lanes of a half share their rows c: reduce over the 32 columns, then over the waves in LDS
+    __syncthreads();
+    const int jl0 = threadIdx.x & 31;
+#pragma unroll
+    for (int lt = 0; lt < LT; ++lt)
+#pragma unroll
+      for (int v = 0; v < 16; ++v) {
+        float x = amax[lt][v];
+#pragma unroll
+        for (int off = 1; off < 32; off <<= 1) x = fmaxf(x, __shfl_xor(x, off));
+        if (jl0 == 0) atomicMax(&cmx[32 * lt + mfma32_row(v, threadIdx.x & 63)], __float_as_uint(x));
+      }
+    __syncthreads();
+    if (threadIdx.x < KP) atomicMax(&colmax[threadIdx.x], cmx[threadIdx.x]);
+  }
   if constexpr (GRAM) {
     for (int e = threadIdx.x; e < KP * KP; e += 512) gs[e] = 0.0;
     __syncthreads();
@@ -1034,20 +1072,21 @@ __device__ __forceinline__ void proj_body(const float* __restrict__ A, int64_t l
   }
 }
 
-template <int LT, int AR, bool GRAM>
+template <int LT, int AR, bool GRAM, bool COLMAX = false>
 __global__ __launch_bounds__(512) void k_proj(const float* __restrict__ A, int64_t lda, int m, int64_t n,
                                               const float* __restrict__ Qc, int64_t ldq,
                                               int out_mode, float* __restrict__ out, int64_t ldo, int r_out,
                                               int64_t blocks_total, double* __restrict__ gpart, int kg,
-                                              const float* __restrict__ scales) {
-  proj_body<LT, AR, GRAM>(A, lda, m, n, Qc, ldq, out_mode, out, ldo, r_out, blocks_total, gpart, kg, scales, hw_blk());
+                                              const float* __restrict__ scales, unsigned* __restrict__ colmax) {
+  proj_body<LT, AR, GRAM, COLMAX>(A, lda, m, n, Qc, ldq, out_mode, out, ldo, r_out, blocks_total, gpart, kg, scales,
+                                  hw_blk(), colmax);
 }
 template <int LT, int AR, bool GRAM>
 __global__ __launch_bounds__(512) void k_proj_grp(const JobProj* __restrict__ jobs, const int2* __restrict__ map) {
   Blk B;
   const JobProj& j = grp_job(jobs, map, B);
   proj_body<LT, AR, GRAM>(j.A, j.lda, j.m, j.n, j.Qc, j.ldq, j.mode, j.out, j.ldo, j.r_out, j.blocks, j.gpart, j.kg,
-                          nullptr, B);
+                          nullptr, B, nullptr);
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -1623,6 +1662,10 @@ void launch_fp16_ascale(const float* sig_prev, int r_prev, double numel, float* 
 }
 
 // mx: kp zeroed words (left zeroed again afterwards)
+void launch_fp16_colscale(unsigned* mx, int kp, float* scales, hipStream_t st) {
+  hipLaunchKernelGGL(k_colscale, dim3(1), dim3(64), 0, st, mx, kp, scales);
+}
+
 void launch_fp16_xscale(const float* X, int64_t n, int64_t ldx, int kp, unsigned* mx, float* scales, hipStream_t st) {
   int64_t grid = (n * kp + 255) / 256;
   if (grid > 1024) grid = 1024;
@@ -1644,25 +1687,29 @@ int64_t proj_blocks(int64_t n) { return (n + 255) / 256; }
 template <int LT, int AR>
 static void launch_proj_t(const float* A, int64_t lda, int m, int64_t n, const float* Qc, int64_t ldq, int mode,
                           float* out, int64_t ldo, int r_out, double* gpart, int kg, hipStream_t st,
-                          const float* sc, int64_t grid_in = 0) {
+                          const float* sc, int64_t grid_in = 0, unsigned* colmax = nullptr) {
   const int64_t blocks = proj_blocks(n);
   const int64_t grid = grid_in > 0 ? grid_in : proj_grid(n);
   if (gpart)
     hipLaunchKernelGGL((k_proj<LT, AR, true>), dim3((unsigned)grid), dim3(512), 0, st, A, lda, m, n, Qc, ldq, mode,
-                       out, ldo, r_out, blocks, gpart, kg, sc);
+                       out, ldo, r_out, blocks, gpart, kg, sc, (unsigned*)nullptr);
+  else if (AR == AR_FP16 && colmax && mode == 0)
+    hipLaunchKernelGGL((k_proj<LT, AR, false, AR == AR_FP16>), dim3((unsigned)grid), dim3(512), 0, st, A, lda, m, n, Qc,
+                       ldq, mode, out, ldo, r_out, blocks, gpart, kg, sc, colmax);
   else
     hipLaunchKernelGGL((k_proj<LT, AR, false>), dim3((unsigned)grid), dim3(512), 0, st, A, lda, m, n, Qc, ldq, mode,
-                       out, ldo, r_out, blocks, gpart, kg, sc);
+                       out, ldo, r_out, blocks, gpart, kg, sc, (unsigned*)nullptr);
 }
 
 template <int LT>
 static void launch_proj_ar(int ar, const float* A, int64_t lda, int m, int64_t n, const float* Qc, int64_t ldq,
                            int mode, float* out, int64_t ldo, int r_out, double* gpart, int kg, hipStream_t st,
-                           const float* sc, int64_t grid) {
+                           const float* sc, int64_t grid, unsigned* colmax = nullptr) {
   switch (ar) {
     case AR_FP32: launch_proj_t<LT, AR_FP32>(A, lda, m, n, Qc, ldq, mode, out, ldo, r_out, gpart, kg, st, sc, grid); break;
     case AR_BF16: launch_proj_t<LT, AR_BF16>(A, lda, m, n, Qc, ldq, mode, out, ldo, r_out, gpart, kg, st, sc, grid); break;
-    case AR_FP16: launch_proj_t<LT, AR_FP16>(A, lda, m, n, Qc, ldq, mode, out, ldo, r_out, gpart, kg, st, sc, grid); break;
+    case AR_FP16: launch_proj_t<LT, AR_FP16>(A, lda, m, n, Qc, ldq, mode, out, ldo, r_out, gpart, kg, st, sc, grid,
+                                             colmax); break;
     case AR_BF16X4: launch_proj_t<LT, AR_BF16X4>(A, lda, m, n, Qc, ldq, mode, out, ldo, r_out, gpart, kg, st, sc, grid); break;
     default: launch_proj_t<LT, AR_BF16X3>(A, lda, m, n, Qc, ldq, mode, out, ldo, r_out, gpart, kg, st, sc, grid); break;
   }
@@ -1677,9 +1724,9 @@ void launch_proj(int ar, int kp, const float* A, int64_t lda, int m, int64_t n, 
 
 void launch_proj_grid(int ar, int kp, const float* A, int64_t lda, int m, int64_t n, const float* Qc, int64_t ldq,
                       int mode, float* out, int64_t ldo, int r_out, int64_t grid, hipStream_t st, double* gpart, int kg,
-                      const float* scales) {
-  if (kp == 32) launch_proj_ar<1>(ar, A, lda, m, n, Qc, ldq, mode, out, ldo, r_out, gpart, kg, st, scales, grid);
-  else launch_proj_ar<2>(ar, A, lda, m, n, Qc, ldq, mode, out, ldo, r_out, gpart, kg, st, scales, grid);
+                      const float* scales, unsigned* colmax) {
+  if (kp == 32) launch_proj_ar<1>(ar, A, lda, m, n, Qc, ldq, mode, out, ldo, r_out, gpart, kg, st, scales, grid, colmax);
+  else launch_proj_ar<2>(ar, A, lda, m, n, Qc, ldq, mode, out, ldo, r_out, gpart, kg, st, scales, grid, colmax);
 }
 
 // one 256-thread block per 4 x 32 columns, at most 16 per CU (plain), or 16 tiles per wave (batch)
