@@ -430,14 +430,15 @@ void launch_apply_ns_out(const double* Win, int64_t rows, int k, int kp, const d
 int64_t panel_blocks(int64_t rows) { return (rows + RB64 - 1) / RB64; }
 
 // ------------------------------------------------------------------------------------------------
-// cyclic Jacobi eigensolver (round-robin ordering), FP64, k <= 64, one workgroup of np^2 threads
-// (np = k2 / 2 pairs, k2 = k rounded up to even), one 2 x 2 block of A and of V per thread.
+// cyclic Jacobi eigensolver (round-robin ordering), FP64, k <= 64, one workgroup: one thread per
+// 2 x 2 block of A and one per block of V (np = k2 / 2 pairs, k2 = k rounded up to even; 2 np^2
+// threads up to k = 44, else np^2 threads that rotate both).
 // A round rotates the np disjoint pairs of the circle-method schedule (pair u of round t joins
-// (t + u) mod (k2 - 1) and (t - u) mod (k2 - 1), pair 0 joins t and k2 - 1), so every thread derives
-// its pairs' indices itself.  Per round: the first np threads compute the angles (FP32 angle, (c, s)
-// normalised in FP64 so every applied rotation is orthogonal to FP64 accuracy) while every thread
-// loads its A and V blocks; one barrier; each thread applies A <- J^T A J and V <- V J to its blocks;
-// a second barrier.
+// (t + u) mod (k2 - 1) and (t - u) mod (k2 - 1), pair 0 joins t and k2 - 1), so every thread tracks
+// its pairs' indices itself.  Per round: every thread loads its block, the threads of the diagonal
+// blocks compute the angles (FP32 angle, (c, s) normalised in FP64 so every applied rotation is
+// orthogonal to FP64 accuracy); one barrier; A <- J^T A J and V <- V J on the blocks; a second
+// barrier.
 // The sweep loop ends when the off-diagonal mass after a sweep is at most 0.5e-24 ||A||_F^2 (the
 // last round's blocks tile the matrix, so it comes from registers).  status[1] = sweeps, status[2] = 1
 // when the sweep cap (max_sweeps) was reached before that test passed, else 0.
@@ -463,24 +464,12 @@ __device__ __forceinline__ void jacobi_angle(double app, double aqq, double apq,
   s = sd * rs;
 }
 
-// pair u of round t in the circle-method schedule over k2 indices (M = k2 - 1)
-__device__ __forceinline__ void jacobi_pair(int t, int u, int M, int& p, int& q) {
-  int x, y;
-  if (u == 0) {
-    x = t;
-    y = M;
-  } else {
-    x = t + u;
-    if (x >= M) x -= M;
-    y = t - u;
-    if (y < 0) y += M;
-  }
-  p = x < y ? x : y;
-  q = x < y ? y : x;
-}
-
 // NT: the threads that work (syevj_threads(k), a multiple of 64, at most the block's size); the
-// block's other waves only keep the barriers
+// block's other waves only keep the barriers.  When 2 np^2 <= NT, threads [0, np^2) rotate the A
+// blocks and threads [np^2, 2 np^2) the V blocks (the V update needs only the column angles, so it
+// never delays the next round's angles); otherwise each thread rotates its A and its V block.
+// The angle of pair u is computed by the thread of the diagonal block (u, u), from the entries it
+// has just loaded: no separate angle threads (which all sat in wave 0, the round's critical path).
 __device__ __forceinline__ void syevj_body(const double* __restrict__ G, int k, double* __restrict__ lam,
                                            double* __restrict__ Vout, int* __restrict__ status,
                                            unsigned long long* __restrict__ keys, int max_sweeps, const int NT) {
@@ -510,34 +499,38 @@ __device__ __forceinline__ void syevj_body(const double* __restrict__ G, int k, 
   __syncthreads();
   tot = 0.0;
   for (int w = 0; w < NW; ++w) tot += red[w];
-  const bool hasblk = tid < np * np;
-  const int bu = hasblk ? tid / np : 0, bw = hasblk ? tid % np : 0;
+  const int nb = np * np;
+  const bool split = 2 * nb <= NT;
+  const bool doA = tid < nb;
+  const bool doV = split ? (tid >= nb && tid < 2 * nb) : doA;
+  const int bid = (split && tid >= nb) ? tid - nb : tid;
+  const int bu = (doA || doV) ? bid / np : 0, bw = (doA || doV) ? bid % np : 0;
+  // circle-method pair u at round t: (t, M) for u = 0, else ((t + u) mod M, (t - u) mod M); both
+  // ends advance by one (mod M) per round, except the fixed end M of pair 0
+  auto wrap = [M](int x) { return x >= M ? x - M : x; };
   int sweeps = 0;
   bool converged = !(tot > 0.0);   // a zero matrix is diagonal
   while (sweeps < max_sweeps && tot > 0.0) {
     ++sweeps;
     double off = 0.0;
+    int x1 = bu, y1 = bu == 0 ? M : M - bu, x2 = bw, y2 = bw == 0 ? M : M - bw;
     for (int rnd = 0; rnd < M; ++rnd) {
-      int p1 = 0, q1 = 0, p2 = 0, q2 = 0;
+      const int p1 = x1 < y1 ? x1 : y1, q1 = x1 < y1 ? y1 : x1;
+      const int p2 = x2 < y2 ? x2 : y2, q2 = x2 < y2 ? y2 : x2;
       double app = 0.0, apq = 0.0, aqp = 0.0, aqq = 0.0, vpp = 0.0, vpq = 0.0, vqp = 0.0, vqq = 0.0;
-      if (hasblk) {
-        // this round's blocks were written before the previous barrier: load them while the angles
-        // are computed
-        jacobi_pair(rnd, bu, M, p1, q1);
-        jacobi_pair(rnd, bw, M, p2, q2);
+      // this round's blocks were written before the previous barrier
+      if (doA) {
         app = a[p1][p2]; apq = a[p1][q2]; aqp = a[q1][p2]; aqq = a[q1][q2];
-        vpp = v[p1][p2]; vpq = v[p1][q2]; vqp = v[q1][p2]; vqq = v[q1][q2];
+        if (bu == bw) {
+          double c, s;
+          jacobi_angle(app, aqq, apq, c, s);
+          cs[bu] = c;
+          sn[bu] = s;
+        }
       }
-      if (tid < np) {
-        int p, q;
-        jacobi_pair(rnd, tid, M, p, q);
-        double c, s;
-        jacobi_angle(a[p][p], a[q][q], a[p][q], c, s);
-        cs[tid] = c;
-        sn[tid] = s;
-      }
+      if (doV) { vpp = v[p1][p2]; vpq = v[p1][q2]; vqp = v[q1][p2]; vqq = v[q1][q2]; }
       __syncthreads();
-      if (hasblk) {
+      if (doA) {
         const double c1 = cs[bu], s1 = sn[bu], c2 = cs[bw], s2 = sn[bw];
         const double rpp = c1 * app - s1 * aqp, rpq = c1 * apq - s1 * aqq;
         const double rqp = s1 * app + c1 * aqp, rqq = s1 * apq + c1 * aqq;
@@ -547,16 +540,23 @@ __device__ __forceinline__ void syevj_body(const double* __restrict__ G, int k, 
         a[p1][q2] = npq;
         a[q1][p2] = nqp;
         a[q1][q2] = nqq;
-        v[p1][p2] = c2 * vpp - s2 * vpq;
-        v[p1][q2] = s2 * vpp + c2 * vpq;
-        v[q1][p2] = c2 * vqp - s2 * vqq;
-        v[q1][q2] = s2 * vqp + c2 * vqq;
         if (rnd == M - 1) {
           off = npq * npq + nqp * nqp;
           if (bu != bw) off += npp * npp + nqq * nqq;
         }
       }
+      if (doV) {
+        const double c2 = cs[bw], s2 = sn[bw];
+        v[p1][p2] = c2 * vpp - s2 * vpq;
+        v[p1][q2] = s2 * vpp + c2 * vpq;
+        v[q1][p2] = c2 * vqp - s2 * vqq;
+        v[q1][q2] = s2 * vqp + c2 * vqq;
+      }
       __syncthreads();
+      x1 = wrap(x1 + 1);
+      x2 = wrap(x2 + 1);
+      if (bu != 0) y1 = wrap(y1 + 1);
+      if (bw != 0) y2 = wrap(y2 + 1);
     }
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) off += __shfl_xor(off, o);
@@ -599,10 +599,12 @@ __global__ __launch_bounds__(1024) void k_syevj_grp(const JobSyevj* __restrict__
   syevj_body(j.G, j.k, j.lam, j.V, j.status, j.keys, max_sweeps, j.nt);
 }
 
-// threads for a k x k problem: one per 2 x 2 block, at least one wavefront (the key clearing)
+// threads for a k x k problem: one per 2 x 2 block of A and one per block of V when both fit in
+// a 1024-thread block, else one per block for both; at least one wavefront (the key clearing)
 int syevj_threads(int k) {
   const int np = (k + (k & 1)) / 2;
-  const int t = ((np * np + 63) / 64) * 64;
+  const int nb = np * np;
+  const int t = ((2 * nb <= 1024 ? 2 * nb : nb) + 63) / 64 * 64;
   return t < 64 ? 64 : t;
 }
 

@@ -12,6 +12,7 @@
 // (rows 4r..4r+3 of one column per lane); the MFMA row index r of tile s then maps to physical
 // row 4r+s.  The 32x32 MFMA output layout is documented in ttx_device.h.
 #include <cstdlib>
+#include <type_traits>
 
 #include "ttx_device.h"
 #include "ttx_kernels.h"
@@ -739,200 +740,171 @@ __device__ __forceinline__ void proj_body(const float* __restrict__ A, int64_t l
   // whole 64-row chunks and 16-byte rows of A and Q: no per-load row guards
   const bool rows_whole = vec && ((m & 63) == 0) && ((ldq & 3) == 0);
 
-  for (int64_t blk = B.x; blk < blocks_total; blk += B.gx) {
-    const int64_t j = blk * 256 + 32 * w + jl;
-    // an interior block (all 256 columns below n) loads without guards (a wave-uniform branch)
-    const bool fast = rows_whole && blk * 256 + 256 <= n;
-    f32x16 acc[LT];
+  // Q chunk kc (64 rows x KP columns): QF float4 pieces per thread, loaded to registers one chunk
+  // ahead (qload) and written to LDS after the MFMAs of the current chunk (qstore), so the A loads
+  // and the MFMAs never wait on these (L2) loads
+  constexpr int QF = KP * 16 / 512;
+  auto qload = [&](f32x4 (&qr)[QF], int kc, auto whole) {
 #pragma unroll
-    for (int lt = 0; lt < LT; ++lt) acc[lt] = zero16();
-
-    // Q chunk kc (64 rows x KP columns): QF float4 pieces per thread, loaded to registers one chunk
-    // ahead (qload) and written to LDS after the MFMAs of the current chunk (qstore), so the A loads
-    // and the MFMAs never wait on these (L2) loads
-    constexpr int QF = KP * 16 / 512;
-    auto qload = [&](f32x4 (&qr)[QF], int kc) {
+    for (int u = 0; u < QF; ++u) {
+      const int idx = threadIdx.x + 512 * u;
+      const int c = idx / 16;
+      const int i = ((kc + kofs) % nk) * 64 + (idx % 16) * 4;
+      f32x4 v;
+      if (decltype(whole)::value || (((ldq & 3) == 0) && i + 3 < m)) {
+        v = *reinterpret_cast<const f32x4*>(Qc + (int64_t)c * ldq + i);
+      } else {
 #pragma unroll
-      for (int u = 0; u < QF; ++u) {
-        const int idx = threadIdx.x + 512 * u;
-        const int c = idx / 16;
-        const int i = ((kc + kofs) % nk) * 64 + (idx % 16) * 4;
-        f32x4 v;
-        if (rows_whole || (((ldq & 3) == 0) && i + 3 < m)) {
-          v = *reinterpret_cast<const f32x4*>(Qc + (int64_t)c * ldq + i);
-        } else {
-#pragma unroll
-          for (int e = 0; e < 4; ++e) v[e] = (i + e < m) ? Qc[(int64_t)c * ldq + i + e] : 0.0f;
-        }
-        qr[u] = v;
+        for (int e = 0; e < 4; ++e) v[e] = (i + e < m) ? Qc[(int64_t)c * ldq + i + e] : 0.0f;
       }
-    };
-    auto qstore = [&](const f32x4 (&qr)[QF], float* q) {
-#pragma unroll
-      for (int u = 0; u < QF; ++u) {
-        const int idx = threadIdx.x + 512 * u;
-        if constexpr (QSPLIT) {
-          __bf16* qh = reinterpret_cast<__bf16*>(q);
-          __bf16* ql = qh + KP * QP;
-          typedef __attribute__((__vector_size__(4 * sizeof(__bf16)))) __bf16 bf16x4;
-          bf16x4 h, l;
-#pragma unroll
-          for (int e = 0; e < 4; ++e) {
-            const __bf16 hv = (__bf16)qr[u][e];
-            h[e] = hv;
-            l[e] = (__bf16)(qr[u][e] - (float)hv);
-          }
-          const int o = (idx / 16) * QP + (idx % 16) * 4;
-          *reinterpret_cast<bf16x4*>(qh + o) = h;
-          *reinterpret_cast<bf16x4*>(ql + o) = l;
-        } else {
-          *reinterpret_cast<f32x4*>(q + (idx / 16) * QP + (idx % 16) * 4) = qr[u];
-        }
-      }
-    };
-    // coalesced loads of the wave's tile: load u, lane l takes rows 4 (l % 16) .. +3 of tile column
-    // 4 u + l / 16 (16 lanes read one column's 256 bytes); the tile is transposed to the MFMA layout
-    // through the wave's LDS staging area
-    const int lc = lane >> 4, lr = (lane & 15) * 4;
-    auto load = [&](float (&x)[32], int kc) {
-      const int i = ((kc + kofs) % nk) * 64 + lr;
-      if (fast) {
-        const float* p0 = A + (blk * 256 + 32 * w + lc) * lda + i;
-#pragma unroll
-        for (int u = 0; u < 8; ++u) {
-          const f32x4 v = *reinterpret_cast<const f32x4*>(p0 + (int64_t)(4 * u) * lda);
-          x[4 * u] = v[0]; x[4 * u + 1] = v[1]; x[4 * u + 2] = v[2]; x[4 * u + 3] = v[3];
-        }
-        return;
-      }
-#pragma unroll
-      for (int u = 0; u < 8; ++u) {
-        const int64_t jj = blk * 256 + 32 * w + 4 * u + lc;
-        if (jj < n) {
-          const float* p = A + jj * lda + i;
-          if (vec && i + 3 < m) {
-            const f32x4 v = *reinterpret_cast<const f32x4*>(p);
-            x[4 * u] = v[0]; x[4 * u + 1] = v[1]; x[4 * u + 2] = v[2]; x[4 * u + 3] = v[3];
-          } else {
-#pragma unroll
-            for (int e = 0; e < 4; ++e) x[4 * u + e] = (i + e < m) ? p[e] : 0.0f;
-          }
-        } else {
-#pragma unroll
-          for (int e = 0; e < 4; ++e) x[4 * u + e] = 0.0f;
-        }
-      }
-    };
-    float* sw = stg[w];
-    auto stage = [&](const float (&x)[32]) {
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-      __builtin_amdgcn_wave_barrier();
-#pragma unroll
-      for (int u = 0; u < 8; ++u) {
-        f32x4 v;
-        v[0] = x[4 * u]; v[1] = x[4 * u + 1]; v[2] = x[4 * u + 2]; v[3] = x[4 * u + 3];
-        *reinterpret_cast<f32x4*>(sw + (4 * u + lc) * SP + lr) = v;
-      }
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    };
-    auto compute = [&](const float (&xin)[32], const float* q) {
-      stage(xin);
-#pragma unroll
-      for (int t = 0; t < 4; ++t) {
-        // B operand: rows 32 b + 8 t + [0,8) of column j, from the staged tile
-        float xb[8];
-        {
-          const float* sp = sw + jl * SP + 32 * b + 8 * t;
-          const f32x4 u0 = *reinterpret_cast<const f32x4*>(sp);
-          const f32x4 u1 = *reinterpret_cast<const f32x4*>(sp + 4);
-          xb[0] = u0[0]; xb[1] = u0[1]; xb[2] = u0[2]; xb[3] = u0[3];
-          xb[4] = u1[0]; xb[5] = u1[1]; xb[6] = u1[2]; xb[7] = u1[3];
-        }
-        if constexpr (AR == AR_FP32) {
-#pragma unroll
-          for (int lt = 0; lt < LT; ++lt) {
-            const float* qp = q + (32 * lt + jl) * QP + 32 * b + 8 * t;
-#pragma unroll
-            for (int u = 0; u < 8; ++u) {
-              // MFMA k index = lane half b; K-step (t, u) pairs row 32 b + 8 t + u of both operands
-              acc[lt] = mfma_f32(qp[u], xb[u], acc[lt]);
-            }
-          }
-        } else if constexpr (AR == AR_FP16) {
-          f16x8 bh;
-          float xsc[8];
-#pragma unroll
-          for (int u = 0; u < 8; ++u) xsc[u] = xb[u] * sa;
-          round8h(xsc, bh);
-#pragma unroll
-          for (int lt = 0; lt < LT; ++lt) {
-            const float* qp = q + (32 * lt + jl) * QP + 32 * b + 8 * t;
-            float qv[8];
-#pragma unroll
-            for (int u = 0; u < 8; ++u) qv[u] = qp[u];
-            f16x8 ah;
-            round8h(qv, ah);
-            acc[lt] = mfma_f16(ah, bh, acc[lt]);
-          }
-        } else {
-          bf16x8 bh, bl;
-          if (AR == AR_BF16X3 || AR == AR_BF16X4) split8(xb, bh, bl);
-          else round8(xb, bh);
-#pragma unroll
-          for (int lt = 0; lt < LT; ++lt) {
-            const __bf16* qh = reinterpret_cast<const __bf16*>(q);
-            const int o = (32 * lt + jl) * QP + 32 * b + 8 * t;
-            const bf16x8 ah = *reinterpret_cast<const bf16x8*>(qh + o);
-            bf16x8 al;
-            if (AR == AR_BF16X3 || AR == AR_BF16X4) al = *reinterpret_cast<const bf16x8*>(qh + KP * QP + o);
-            acc[lt] = mfma_bf16(ah, bh, acc[lt]);
-            if (AR == AR_BF16X3 || AR == AR_BF16X4) {
-              acc[lt] = mfma_bf16(al, bh, acc[lt]);
-              acc[lt] = mfma_bf16(ah, bl, acc[lt]);
-            }
-            if (AR == AR_BF16X4) acc[lt] = mfma_bf16(al, bl, acc[lt]);
-          }
-        }
-      }
-    };
-
-    // workgroup barrier for the Q chunks in LDS only: their writes complete (lgkmcnt(0)), then
-    // s_barrier.  __syncthreads would also wait for the next A chunk's loads (vmcnt(0)) and drain
-    // the wave's only chunk in flight at every Q chunk.
-    auto q_barrier = [] {
-      asm volatile("" ::: "memory");
-      __builtin_amdgcn_s_waitcnt(0xC07F);   // lgkmcnt(0); vmcnt, expcnt unchanged
-      __builtin_amdgcn_s_barrier();
-      asm volatile("" ::: "memory");
-    };
-    float x0[32], x1[32];
-    f32x4 qr[QF];
-    qload(qr, 0);
-    load(x0, 0);
-    qstore(qr, qs[0]);
-    q_barrier();
-    for (int kc = 0; kc < nk; kc += 2) {
-      if (kc + 1 < nk) {
-        qload(qr, kc + 1);
-        load(x1, kc + 1);
-      }
-      compute(x0, qs[0]);
-      __builtin_amdgcn_sched_barrier(0);
-      if (kc + 1 < nk) qstore(qr, qs[1]);
-      q_barrier();
-      if (kc + 1 >= nk) break;
-      if (kc + 2 < nk) {
-        qload(qr, kc + 2);
-        load(x0, kc + 2);
-      }
-      compute(x1, qs[1]);
-      __builtin_amdgcn_sched_barrier(0);
-      if (kc + 2 < nk) qstore(qr, qs[0]);
-      q_barrier();
+      qr[u] = v;
     }
+  };
+  auto qstore = [&](const f32x4 (&qr)[QF], float* q) {
+#pragma unroll
+    for (int u = 0; u < QF; ++u) {
+      const int idx = threadIdx.x + 512 * u;
+      if constexpr (QSPLIT) {
+        __bf16* qh = reinterpret_cast<__bf16*>(q);
+        __bf16* ql = qh + KP * QP;
+        typedef __attribute__((__vector_size__(4 * sizeof(__bf16)))) __bf16 bf16x4;
+        bf16x4 h, l;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const __bf16 hv = (__bf16)qr[u][e];
+          h[e] = hv;
+          l[e] = (__bf16)(qr[u][e] - (float)hv);
+        }
+        const int o = (idx / 16) * QP + (idx % 16) * 4;
+        *reinterpret_cast<bf16x4*>(qh + o) = h;
+        *reinterpret_cast<bf16x4*>(ql + o) = l;
+      } else {
+        *reinterpret_cast<f32x4*>(q + (idx / 16) * QP + (idx % 16) * 4) = qr[u];
+      }
+    }
+  };
+  // coalesced loads of the wave's tile: load u, lane l takes rows 4 (l % 16) .. +3 of tile column
+  // 4 u + l / 16 (16 lanes read one column's 256 bytes); the tile is transposed to the MFMA layout
+  // through the wave's LDS staging area
+  const int lc = lane >> 4, lr = (lane & 15) * 4;
+  // an interior block (all 256 columns below n, whole 64-row chunks): no guards
+  auto load_fast = [&](float (&x)[32], int64_t blk, int kc) {
+    const int i = ((kc + kofs) % nk) * 64 + lr;
+    const float* p0 = A + (blk * 256 + 32 * w + lc) * lda + i;
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const f32x4 v = *reinterpret_cast<const f32x4*>(p0 + (int64_t)(4 * u) * lda);
+      x[4 * u] = v[0]; x[4 * u + 1] = v[1]; x[4 * u + 2] = v[2]; x[4 * u + 3] = v[3];
+    }
+  };
+  auto load_guarded = [&](float (&x)[32], int64_t blk, int kc) {
+    const int i = ((kc + kofs) % nk) * 64 + lr;
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int64_t jj = blk * 256 + 32 * w + 4 * u + lc;
+      if (jj < n) {
+        const float* p = A + jj * lda + i;
+        if (vec && i + 3 < m) {
+          const f32x4 v = *reinterpret_cast<const f32x4*>(p);
+          x[4 * u] = v[0]; x[4 * u + 1] = v[1]; x[4 * u + 2] = v[2]; x[4 * u + 3] = v[3];
+        } else {
+#pragma unroll
+          for (int e = 0; e < 4; ++e) x[4 * u + e] = (i + e < m) ? p[e] : 0.0f;
+        }
+      } else {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) x[4 * u + e] = 0.0f;
+      }
+    }
+  };
+  float* sw = stg[w];
+  auto stage = [&](const float (&x)[32]) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      f32x4 v;
+      v[0] = x[4 * u]; v[1] = x[4 * u + 1]; v[2] = x[4 * u + 2]; v[3] = x[4 * u + 3];
+      *reinterpret_cast<f32x4*>(sw + (4 * u + lc) * SP + lr) = v;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  };
+  auto compute = [&](f32x16 (&acc)[LT], const float (&xin)[32], const float* q) {
+    stage(xin);
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      // B operand: rows 32 b + 8 t + [0,8) of column j, from the staged tile
+      float xb[8];
+      {
+        const float* sp = sw + jl * SP + 32 * b + 8 * t;
+        const f32x4 u0 = *reinterpret_cast<const f32x4*>(sp);
+        const f32x4 u1 = *reinterpret_cast<const f32x4*>(sp + 4);
+        xb[0] = u0[0]; xb[1] = u0[1]; xb[2] = u0[2]; xb[3] = u0[3];
+        xb[4] = u1[0]; xb[5] = u1[1]; xb[6] = u1[2]; xb[7] = u1[3];
+      }
+      if constexpr (AR == AR_FP32) {
+#pragma unroll
+        for (int lt = 0; lt < LT; ++lt) {
+          const float* qp = q + (32 * lt + jl) * QP + 32 * b + 8 * t;
+#pragma unroll
+          for (int u = 0; u < 8; ++u) {
+            // MFMA k index = lane half b; K-step (t, u) pairs row 32 b + 8 t + u of both operands
+            acc[lt] = mfma_f32(qp[u], xb[u], acc[lt]);
+          }
+        }
+      } else if constexpr (AR == AR_FP16) {
+        f16x8 bh;
+        float xsc[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) xsc[u] = xb[u] * sa;
+        round8h(xsc, bh);
+#pragma unroll
+        for (int lt = 0; lt < LT; ++lt) {
+          const float* qp = q + (32 * lt + jl) * QP + 32 * b + 8 * t;
+          float qv[8];
+#pragma unroll
+          for (int u = 0; u < 8; ++u) qv[u] = qp[u];
+          f16x8 ah;
+          round8h(qv, ah);
+          acc[lt] = mfma_f16(ah, bh, acc[lt]);
+        }
+      } else {
+        bf16x8 bh, bl;
+        if (AR == AR_BF16X3 || AR == AR_BF16X4) split8(xb, bh, bl);
+        else round8(xb, bh);
+#pragma unroll
+        for (int lt = 0; lt < LT; ++lt) {
+          const __bf16* qh = reinterpret_cast<const __bf16*>(q);
+          const int o = (32 * lt + jl) * QP + 32 * b + 8 * t;
+          const bf16x8 ah = *reinterpret_cast<const bf16x8*>(qh + o);
+          bf16x8 al;
+          if (AR == AR_BF16X3 || AR == AR_BF16X4) al = *reinterpret_cast<const bf16x8*>(qh + KP * QP + o);
+          acc[lt] = mfma_bf16(ah, bh, acc[lt]);
+          if (AR == AR_BF16X3 || AR == AR_BF16X4) {
+            acc[lt] = mfma_bf16(al, bh, acc[lt]);
+            acc[lt] = mfma_bf16(ah, bl, acc[lt]);
+          }
+          if (AR == AR_BF16X4) acc[lt] = mfma_bf16(al, bl, acc[lt]);
+        }
+      }
+    }
+  };
 
+  // workgroup barrier for the Q chunks in LDS only: their writes complete (lgkmcnt(0)), then
+  // s_barrier.  __syncthreads would also wait for the next A chunk's loads (vmcnt(0)) and drain
+  // the wave's only chunk in flight at every Q chunk.
+  auto q_barrier = [] {
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_s_waitcnt(0xC07F);   // lgkmcnt(0); vmcnt, expcnt unchanged
+    __builtin_amdgcn_s_barrier();
+    asm volatile("" ::: "memory");
+  };
+
+  // a finished block: FP16 unscale, the outputs, the column maxima and the Gram of its D tile
+  auto epilogue = [&](int64_t blk, f32x16 (&acc)[LT]) {
+    const int64_t j = blk * 256 + 32 * w + jl;
     if (AR == AR_FP16 && scales) {
       const float inv = 1.0f / sa;
 #pragma unroll
@@ -1005,17 +977,22 @@ __device__ __forceinline__ void proj_body(const float* __restrict__ A, int64_t l
       }
     }
     if constexpr (GRAM) {
-      // tile D (kp x 32 columns, zero for columns j >= n) -> LDS dt[j][c], then
-      // G += D D^T with MFMA 32x32x2: A = D[c][j] (lane: c = l%32, j = 2 kk + l/32), B = the same
-      // row pitch KP + 1: the 32 lanes of a column write to distinct banks
+      // tile D (kp x 32 columns, zero for columns j >= n) -> the wave's own LDS dt[j][c] (a wave-level
+      // fence suffices: no other wave reads it), then G += D D^T with MFMA 32x32x2: A = D[c][j] (lane:
+      // c = l%32, j = 2 kk + l/32), B = the same.  Row pitch KP + 1: the 32 lanes of a column write to
+      // distinct banks
       constexpr int DP = KP + 1;
       static_assert(32 * DP <= 32 * SP, "D tile fits the staging area");
       float* d = stg[w];
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
 #pragma unroll
       for (int lt = 0; lt < LT; ++lt)
 #pragma unroll
         for (int v = 0; v < 16; ++v) d[jl * DP + 32 * lt + mfma32_row(v, lane)] = acc[lt][v];
-      __syncthreads();
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 #pragma unroll
       for (int kk = 0; kk < 16; ++kk) {
         const int jr = 2 * kk + b;
@@ -1029,6 +1006,97 @@ __device__ __forceinline__ void proj_body(const float* __restrict__ A, int64_t l
           for (int tb = ta; tb < LT; ++tb) { gacc[pidx] = mfma_f32(x[ta], x[tb], gacc[pidx]); ++pidx; }
       }
     }
+#pragma unroll
+    for (int lt = 0; lt < LT; ++lt) acc[lt] = zero16();
+  };
+
+  // Interior blocks (this workgroup's blocks B.x + t gx, t < cnt, all below nfast) run as one chunk
+  // sequence g = t nk + kc, software-pipelined across block boundaries: the next chunk's A and Q loads
+  // are issued unconditionally before this chunk's MFMAs (the index is clamped at the end of the
+  // sequence, where the redundant reload of the last chunk comes from L2).  A load issued under a
+  // branch leaves the compiler unable to count the loads in flight at the merge, and it then waits
+  // for all of them (vmcnt(0)) before the current chunk, which serialised load and compute.
+  const int64_t nfast = rows_whole ? n / 256 : 0;
+  const int64_t cnt = (B.x < nfast) ? (nfast - 1 - B.x) / B.gx + 1 : 0;
+  if (cnt > 0) {
+    const int64_t total = cnt * nk;
+    using whole_t = std::true_type;
+    f32x16 acc[LT];
+#pragma unroll
+    for (int lt = 0; lt < LT; ++lt) acc[lt] = zero16();
+    float x0[32], x1[32];
+    f32x4 qr[QF];
+    // chunk g: block B.x + (g / nk) gx, K chunk g % nk (tracked incrementally)
+    int kc0 = 0;
+    int64_t blk0 = B.x;
+    auto succ = [&](int kc, int64_t bk, int& kn, int64_t& bn) {
+      kn = (kc + 1 == nk) ? 0 : kc + 1;
+      bn = (kc + 1 == nk) ? bk + B.gx : bk;
+    };
+    qload(qr, 0, whole_t{});
+    load_fast(x0, blk0, 0);
+    qstore(qr, qs[0]);
+    q_barrier();
+    for (int64_t g = 0; g < total; g += 2) {
+      int kc1;
+      int64_t blk1;
+      succ(kc0, blk0, kc1, blk1);
+      if (g + 1 >= total) { kc1 = kc0; blk1 = blk0; }
+      qload(qr, kc1, whole_t{});
+      load_fast(x1, blk1, kc1);
+      compute(acc, x0, qs[0]);
+      __builtin_amdgcn_sched_barrier(0);
+      qstore(qr, qs[1]);
+      q_barrier();
+      if (kc0 == nk - 1) epilogue(blk0, acc);
+      if (g + 1 >= total) break;
+      int kc2;
+      int64_t blk2;
+      succ(kc1, blk1, kc2, blk2);
+      if (g + 2 >= total) { kc2 = kc1; blk2 = blk1; }
+      qload(qr, kc2, whole_t{});
+      load_fast(x0, blk2, kc2);
+      compute(acc, x1, qs[1]);
+      __builtin_amdgcn_sched_barrier(0);
+      qstore(qr, qs[0]);
+      q_barrier();
+      if (kc1 == nk - 1) epilogue(blk1, acc);
+      kc0 = kc2;
+      blk0 = blk2;
+    }
+  }
+  // the other blocks (the tail block, or every block when the rows are not whole): guarded loads
+  for (int64_t blk = B.x + cnt * B.gx; blk < blocks_total; blk += B.gx) {
+    using guarded_t = std::false_type;
+    f32x16 acc[LT];
+#pragma unroll
+    for (int lt = 0; lt < LT; ++lt) acc[lt] = zero16();
+    float x0[32], x1[32];
+    f32x4 qr[QF];
+    qload(qr, 0, guarded_t{});
+    load_guarded(x0, blk, 0);
+    qstore(qr, qs[0]);
+    q_barrier();
+    for (int kc = 0; kc < nk; kc += 2) {
+      if (kc + 1 < nk) {
+        qload(qr, kc + 1, guarded_t{});
+        load_guarded(x1, blk, kc + 1);
+      }
+      compute(acc, x0, qs[0]);
+      __builtin_amdgcn_sched_barrier(0);
+      if (kc + 1 < nk) qstore(qr, qs[1]);
+      q_barrier();
+      if (kc + 1 >= nk) break;
+      if (kc + 2 < nk) {
+        qload(qr, kc + 2, guarded_t{});
+        load_guarded(x0, blk, kc + 2);
+      }
+      compute(acc, x1, qs[1]);
+      __builtin_amdgcn_sched_barrier(0);
+      if (kc + 2 < nk) qstore(qr, qs[0]);
+      q_barrier();
+    }
+    epilogue(blk, acc);
   }
   if constexpr (COLMAX) {
     // the lanes of a half share their rows c: reduce over the 32 columns, then over the waves in LDS
