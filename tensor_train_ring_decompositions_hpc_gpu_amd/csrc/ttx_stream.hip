@@ -751,7 +751,13 @@ __device__ __forceinline__ void proj_body(const float* __restrict__ A, int64_t l
       const int c = idx / 16;
       const int i = ((kc + kofs) % nk) * 64 + (idx % 16) * 4;
       f32x4 v;
-      if (decltype(whole)::value || (((ldq & 3) == 0) && i + 3 < m)) {
+      if constexpr (decltype(whole)::value) {
+        // Q's kp columns from one descriptor: the lane's fixed offset plus the chunk's scalar offset
+        const __amdgpu_buffer_rsrc_t rs =
+            __builtin_amdgcn_make_buffer_rsrc((void*)Qc, 0, (int)(KP * ldq * 4), 0x00020000);
+        v = __builtin_bit_cast(f32x4, __builtin_amdgcn_raw_buffer_load_b128(
+                                          rs, (int)((c * ldq + (idx % 16) * 4) * 4), ((kc + kofs) % nk) * 256, 0));
+      } else if (((ldq & 3) == 0) && i + 3 < m) {
         v = *reinterpret_cast<const f32x4*>(Qc + (int64_t)c * ldq + i);
       } else {
 #pragma unroll
@@ -787,13 +793,19 @@ __device__ __forceinline__ void proj_body(const float* __restrict__ A, int64_t l
   // 4 u + l / 16 (16 lanes read one column's 256 bytes); the tile is transposed to the MFMA layout
   // through the wave's LDS staging area
   const int lc = lane >> 4, lr = (lane & 15) * 4;
-  // an interior block (all 256 columns below n, whole 64-row chunks): no guards
+  // an interior block (all 256 columns below n, whole 64-row chunks): no guards.  Buffer loads from
+  // a descriptor at the block's first column: the lane's 32-bit offset never changes and the chunk
+  // and load offsets are scalars, so the lane holds one address register instead of a 64-bit pair
+  // per load (with the pairs, register pressure put addresses in registers that loads in flight
+  // still had to write, and the next chunk's loads waited for this chunk's)
+  const int aoff = (int)(((int64_t)(32 * w + lc) * lda + lr) * 4);
   auto load_fast = [&](float (&x)[32], int64_t blk, int kc) {
-    const int i = ((kc + kofs) % nk) * 64 + lr;
-    const float* p0 = A + (blk * 256 + 32 * w + lc) * lda + i;
+    const __amdgpu_buffer_rsrc_t rs =
+        __builtin_amdgcn_make_buffer_rsrc((void*)(A + blk * 256 * lda), 0, (int)(256 * lda * 4), 0x00020000);
+    const int soff = ((kc + kofs) % nk) * 256;
 #pragma unroll
     for (int u = 0; u < 8; ++u) {
-      const f32x4 v = *reinterpret_cast<const f32x4*>(p0 + (int64_t)(4 * u) * lda);
+      const f32x4 v = __builtin_bit_cast(f32x4, __builtin_amdgcn_raw_buffer_load_b128(rs, aoff, soff + (int)(16 * u * lda), 0));
       x[4 * u] = v[0]; x[4 * u + 1] = v[1]; x[4 * u + 2] = v[2]; x[4 * u + 3] = v[3];
     }
   };
@@ -1016,7 +1028,9 @@ __device__ __forceinline__ void proj_body(const float* __restrict__ A, int64_t l
   // sequence, where the redundant reload of the last chunk comes from L2).  A load issued under a
   // branch leaves the compiler unable to count the loads in flight at the merge, and it then waits
   // for all of them (vmcnt(0)) before the current chunk, which serialised load and compute.
-  const int64_t nfast = rows_whole ? n / 256 : 0;
+  // (the buffer offsets are 32-bit: a block of A and the whole of Q stay below 2^31 bytes)
+  const bool offs32 = 256 * lda * 4 < (int64_t(1) << 31) && (int64_t)KP * ldq * 4 < (int64_t(1) << 31);
+  const int64_t nfast = (rows_whole && offs32) ? n / 256 : 0;
   const int64_t cnt = (B.x < nfast) ? (nfast - 1 - B.x) / B.gx + 1 : 0;
   if (cnt > 0) {
     const int64_t total = cnt * nk;
