@@ -17,6 +17,9 @@
 #include "ttx_device.h"
 #include "ttx_kernels.h"
 
+// cache-policy operand of the load builtins on gfx950: nt (stream past the caches)
+constexpr int CPOL_NT = 2;
+
 namespace ttx {
 
 // ------------------------------------------------------------------------------------------------
@@ -495,8 +498,10 @@ __global__ __launch_bounds__(512) void k_sketch_dl(const float* __restrict__ A, 
     for (int q = 0; q < NI; ++q) {
       int64_t j = c * KC + lcol + CPI * q;
       if (j >= n) j = n - 1;
+      // nontemporal (cache policy nt): the tensor is streamed once per pass (measured 0.89 -> 0.84 ms for
+      // the 4.3 GB first pass)
       __builtin_amdgcn_global_load_lds((const void*)(A + j * lda + lrow),
-                                       (__attribute__((address_space(3))) void*)(dst + q * (256 + PAD)), 16, 0, 0);
+                                       (__attribute__((address_space(3))) void*)(dst + q * (256 + PAD)), 16, 0, CPOL_NT);
     }
   };
   auto compute = [&](int64_t c, const float* xs, int kofs) {
