@@ -735,43 +735,58 @@ template <int RM>
 __device__ __forceinline__ void next_rows_body(const float* __restrict__ Z, int64_t ldz, int64_t n, int k,
                                                const float* __restrict__ Ur, int r, float* __restrict__ out,
                                                const Blk B) {
-  // one output row per lane: the lane reads its own Z row with 16-byte loads (a wave reads 64
-  // consecutive rows, every byte of them) and keeps RM sums in registers; Ur (round4(k) x RM,
-  // zero-padded) is read at wave-uniform addresses, so its values come through scalar loads as
-  // SGPR operands; the lane writes its r consecutive floats (a wave writes 64 r contiguous floats)
-  const bool vec = (r % 4) == 0 && (reinterpret_cast<uintptr_t>(out) & 15) == 0;
-  const int kq = (k + 3) / 4;
-  for (int64_t j = (int64_t)B.x * 256 + threadIdx.x; j < n; j += (int64_t)B.gx * 256) {
-    // the whole Z row goes out first (up to 16 loads in flight per lane)
-    const f32x4* z = reinterpret_cast<const f32x4*>(Z + j * ldz);
-    f32x4 zr[KMAX / 4];
+  // A_next^T = Z U_r in 32-row tiles on the matrix cores: a wave stages rows j0 .. j0 + 31 of Z (one
+  // contiguous run of 32 ldz floats, read with 16-byte loads by consecutive lanes) in its LDS tile,
+  // then runs k4 / 2 FP32 MFMAs (32x32x2) per 32 output columns: A = the tile (lane: row l % 32,
+  // k = 2 s + l / 32), B = U_r (round4(k) x RM, zero-padded), held in registers for the whole call.
+  // A lane writes 32 consecutive floats of one output row per accumulator register.
+  constexpr int NT = RM > 32 ? 2 : 1;   // 32-column output tiles
+  constexpr int P = KMAX + 1;           // LDS pitch of a staged Z row (odd: the column reads are conflict-free)
+  __shared__ float zs[4][32 * P];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int k4 = (k + 3) & ~3, kq = k4 / 4;
+  float ub[NT][KMAX / 2];
 #pragma unroll
-    for (int q = 0; q < KMAX / 4; ++q)
-      if (q < kq) zr[q] = z[q];
-    float acc[RM];
+  for (int nt = 0; nt < NT; ++nt)
 #pragma unroll
-    for (int c = 0; c < RM; ++c) acc[c] = 0.0f;
+    for (int st = 0; st < KMAX / 2; ++st) {
+      const int kk = 2 * st + (lane >> 5), c = 32 * nt + (lane & 31);
+      ub[nt][st] = (kk < k4 && c < RM) ? Ur[kk * RM + c] : 0.0f;
+    }
+  float* zt = zs[w];
+  const int64_t tiles = (n + 31) / 32;
+  for (int64_t t = (int64_t)B.x * 4 + w; t < tiles; t += (int64_t)B.gx * 4) {
+    const int64_t j0 = t * 32;
+    const int rows = (n - j0 >= 32) ? 32 : (int)(n - j0);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    for (int e = lane; e < 32 * kq; e += 64) {
+      const int rr = e / kq, q = e - rr * kq;
+      f32x4 v = {0.0f, 0.0f, 0.0f, 0.0f};
+      if (rr < rows) v = *reinterpret_cast<const f32x4*>(Z + (j0 + rr) * ldz + 4 * q);
+      float* d = zt + rr * P + 4 * q;
+      d[0] = v[0]; d[1] = v[1]; d[2] = v[2]; d[3] = v[3];
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    f32x16 acc[NT];
 #pragma unroll
-    for (int q = 0; q < KMAX / 4; ++q) {
-      if (q < kq) {
-        const float* u = Ur + 4 * q * RM;
+    for (int nt = 0; nt < NT; ++nt) acc[nt] = zero16();
 #pragma unroll
-        for (int t = 0; t < 4; ++t)
+    for (int st = 0; st < KMAX / 2; ++st) {
+      if (2 * st >= k4) break;
+      const float a = zt[(lane & 31) * P + 2 * st + (lane >> 5)];
 #pragma unroll
-          for (int c = 0; c < RM; ++c) acc[c] += zr[q][t] * u[t * RM + c];
+      for (int nt = 0; nt < NT; ++nt) acc[nt] = mfma_f32(a, ub[nt][st], acc[nt]);
+    }
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt)
+#pragma unroll
+      for (int v = 0; v < 16; ++v) {
+        const int rr = mfma32_row(v, lane), c = 32 * nt + (lane & 31);
+        if (rr < rows && c < r) out[(j0 + rr) * r + c] = acc[nt][v];
       }
-    }
-    float* o = out + j * r;
-    if (vec) {
-#pragma unroll
-      for (int c4 = 0; c4 < RM / 4; ++c4)
-        if (4 * c4 < r)
-          *reinterpret_cast<f32x4*>(o + 4 * c4) = f32x4{acc[4 * c4], acc[4 * c4 + 1], acc[4 * c4 + 2], acc[4 * c4 + 3]};
-    } else {
-#pragma unroll
-      for (int c = 0; c < RM; ++c)
-        if (c < r) o[c] = acc[c];
-    }
   }
 }
 
