@@ -735,8 +735,10 @@ static ttx_status predict(Handle* h, const StepArgs& a, const Arith3& ar, int el
   const double r = std::isfinite(hy[0]) ? 64.0 * hy[0] : 1.0;
   const double w = std::isfinite(hy[1]) ? 64.0 * hy[1] : (double)ell;
   *pred_r = std::max(1, std::min(a.r, (int)std::lround(r)));
-  // two columns of margin over the predicted width
-  *pred_w = std::max(1, std::min(ell, (int)std::ceil(w) + 2));
+  // four columns of margin over the predicted width: on 32 order-5 tensors (64 steps) the range check
+  // failed for 36% of the steps with two, 9% with four and 2% with six columns, and four cost no time
+  // against two on either the learned or the 32^6 workload (six: 2% slower on the learned one)
+  *pred_w = std::max(1, std::min(ell, (int)std::ceil(w) + 4));
   return TTX_OK;
 }
 
