@@ -214,13 +214,33 @@ class Engine:
         p.flags = int(flags)
         return p
 
+    def _plan(self, dims, ranks, tr, kw):
+        """Parameter block and core sizes of a call, kept per call signature: the entry points take the
+        block as const, so repeated calls with the same arguments reuse it (a loop of decompositions
+        then spends its host time in the engine, not in building ctypes structures)."""
+        try:
+            key = (tuple(dims), tuple(ranks), tr, tuple(kw.items()))
+            hash(key)
+        except TypeError:
+            key = None
+        cache = self.__dict__.setdefault("_plans", {})
+        hit = cache.get(key) if key is not None else None
+        if hit is not None:
+            return hit
+        p = self.params(dims, ranks, **kw)
+        sizes = [int(self.lib.ttx_core_size(ctypes.byref(p), int(tr), k)) for k in range(len(dims))]
+        if key is not None:
+            if len(cache) >= 64:
+                cache.clear()
+            cache[key] = (p, sizes)
+        return p, sizes
+
     def _sweep(self, fn, A, dims, ranks, tr, cores=None, host_slice=None, **kw):
         torch = _torch()
-        p = self.params(dims, ranks, **kw)
+        p, sizes = self._plan(dims, ranks, tr, kw)
         if p.rank_mode == L.RANK_MODES["learned"] and not self.model_loaded:
             self.load_rank_model()
         d = len(dims)
-        sizes = [int(self.lib.ttx_core_size(ctypes.byref(p), int(tr), k)) for k in range(d)]
         if cores is not None:
             # caller-owned output buffers (core_buffers), written in place
             if len(cores) != d or any(not (c.is_cuda and c.dtype == torch.float32 and c.is_contiguous()) or
@@ -239,7 +259,7 @@ class Engine:
             st = fn(self.h, ctypes.c_void_p(_host_ptr(A)), ctypes.byref(p), int(host_slice), core_p, sig_p,
                     ctypes.byref(rep))
         _check(self.lib, self.h, st)
-        used = [int(rep.ranks[k]) for k in range(d + 1)]
+        used = rep.ranks[:d + 1]
         out = []
         for k in range(d):
             if tr and k == 0:
@@ -468,24 +488,33 @@ def reachable_ranks(dims, ranks):
 
 
 def report_dict(rep, d):
+    # ctypes array slices convert in one call (an indexed read per element costs a Python object each)
+    n = d - 1
     return {
-        "ranks": [int(rep.ranks[k]) for k in range(d + 1)],
-        "ell": [int(rep.ell[k]) for k in range(d - 1)],
-        "path": [int(rep.path[k]) for k in range(d - 1)],
-        "power_iters": [int(rep.power_iters[k]) for k in range(d - 1)],
-        "zero_pivots": [int(rep.zero_pivots[k]) for k in range(d - 1)],
-        "jacobi_sweeps": [int(rep.jacobi_sweeps[k]) for k in range(d - 1)],
-        "norm2_in": [float(rep.norm2_in[k]) for k in range(d - 1)],
-        "step_ms": [float(rep.step_ms[k]) for k in range(d - 1)],
+        "ranks": rep.ranks[:d + 1],
+        "ell": rep.ell[:n],
+        "path": rep.path[:n],
+        "power_iters": rep.power_iters[:n],
+        "zero_pivots": rep.zero_pivots[:n],
+        "jacobi_sweeps": rep.jacobi_sweeps[:n],
+        "norm2_in": rep.norm2_in[:n],
+        "step_ms": rep.step_ms[:n],
         "total_ms": float(rep.total_ms),
         "bytes_alg": int(rep.bytes_alg),
-        "range_resid": [float(rep.range_resid[k]) for k in range(d - 1)],
-        "range_checks": [int(rep.range_checks[k]) for k in range(d - 1)],
-        "passes": [int(rep.passes[k]) for k in range(d - 1)],
-        "fallback": [int(rep.fallback[k]) for k in range(d - 1)],
-        "predicted_rank": [int(rep.predicted_rank[k]) for k in range(d - 1)],
-        "kernels": {L.KCLASS[c]: {"ms": float(rep.kernel_ms[c]), "launches": int(rep.kernel_launches[c]),
-                                  "bytes": int(rep.kernel_bytes[c]), "top_bytes": int(rep.kernel_top_bytes[c]),
-                                  "top_ms": float(rep.kernel_top_ms[c])}
-                    for c in range(L.NUM_KCLASS) if rep.kernel_launches[c] > 0},
+        "range_resid": rep.range_resid[:n],
+        "range_checks": rep.range_checks[:n],
+        "passes": rep.passes[:n],
+        "fallback": rep.fallback[:n],
+        "predicted_rank": rep.predicted_rank[:n],
+        "kernels": _kernel_classes(rep),
     }
+
+
+def _kernel_classes(rep):
+    launches = rep.kernel_launches[:]
+    if not any(launches):
+        return {}
+    return {L.KCLASS[c]: {"ms": float(rep.kernel_ms[c]), "launches": int(launches[c]),
+                          "bytes": int(rep.kernel_bytes[c]), "top_bytes": int(rep.kernel_top_bytes[c]),
+                          "top_ms": float(rep.kernel_top_ms[c])}
+            for c in range(L.NUM_KCLASS) if launches[c] > 0}
