@@ -586,10 +586,167 @@ __device__ __forceinline__ void syevj_body(const double* __restrict__ G, int k, 
     status[2] = converged ? 0 : 1;
   }
 }
+// ------------------------------------------------------------------------------------------------
+// the same eigensolver for a single problem (k_syevj), one barrier per round.  np = k2 / 2 pairs
+// (k2 = k rounded up to even).  A round rotates the np disjoint pairs of the circle-method schedule (pair u of round t joins (t + u) mod (k2 - 1) and (t - u) mod (k2 - 1),
+// pair 0 joins t and k2 - 1), so every lane tracks its pairs' indices itself.
+// Lane (bu, bw) owns the 2 x 2 block (row pair bu, column pair bw) of A and of V.  The lanes of a
+// row pair share a wavefront: GP = floor(64 / np) row pairs per wave, lane g np + bw of wave w
+// holding row pair w GP + g.  Per round every lane
+//   * loads its A block, its V block and the entries (p, p), (q, q), (p, q) of its column pair;
+//   * computes its column pair's angle (FP32 angle, (c, s) normalised in FP64 so every applied
+//     rotation is orthogonal to FP64 accuracy) and reads its row pair's angle from the lane of the
+//     diagonal block (bu, bu) in its own wave, a cross-lane read that needs no barrier;
+//   * writes its block of J^T A J into the other copy of A and its block of V J into V;
+// then one barrier.  A is double-buffered, so no write of a round can reach an entry that another
+// lane still reads in that round.  Every lane that computes the angle of pair u gets the same bits
+// (same inputs, same instructions): the rotations are those a single owner per pair would apply.
+// The sweep loop ends when the off-diagonal mass after a sweep is at most 0.5e-24 ||A||_F^2 (the
+// last round's blocks tile the matrix, so it comes from registers).  status[1] = sweeps, status[2] = 1
+// when the sweep cap (max_sweeps) was reached before that test passed, else 0.
+// keys[0..63] (optional) are zeroed for the core assembly that follows.
+// Same rotations as the two-barrier body above, so the same results to the bit.  Its double-buffered
+// A (100 KB of LDS) and 68 VGPRs allow one 1024-thread workgroup per CU, so the grouped launch, which
+// runs many problems side by side, keeps the two-barrier body (67 KB: two workgroups per CU).
+// ------------------------------------------------------------------------------------------------
+// Branch-free (apq = 0 selects the identity at the end), so the loads of the three entries are not
+// sunk below a branch and waited for one after the other.
+__device__ __forceinline__ void jacobi_angle_sel(double app, double aqq, double apq, double& c, double& s) {
+  // tau = (aqq - app) / (2 apq) to FP32 accuracy (the rotation is re-normalised in FP64 below);
+  // v_rcp_f64 keeps the quotient independent of the matrix's scale
+  const float tau = (float)((aqq - app) * __builtin_amdgcn_rcp(2.0 * apq));
+  float t;
+  if (!(fabsf(tau) <= 1e18f)) t = 0.0f;
+  else t = (tau >= 0.0f ? 1.0f : -1.0f) * __builtin_amdgcn_rcpf(fabsf(tau) + __builtin_amdgcn_sqrtf(1.0f + tau * tau));
+  const float c0 = __builtin_amdgcn_rsqf(1.0f + t * t);
+  const double cd = (double)c0, sd = (double)(t * c0);
+  // c^2 + s^2 = 1 + d with |d| < 2^-21: two Newton steps for 1/sqrt from 1 leave an error below 1e-26
+  const double nrm = cd * cd + sd * sd;
+  const double r1 = 1.5 - 0.5 * nrm;
+  const double rs = r1 * (1.5 - 0.5 * nrm * r1 * r1);
+  const bool rot = apq != 0.0;
+  c = rot ? cd * rs : 1.0;
+  s = rot ? sd * rs : 0.0;
+}
+
+// NT: the threads that work (syevj1_threads(k), a multiple of 64, at most the block's size); the
+// block's other waves only keep the barriers.
+__device__ __forceinline__ void syevj1_body(const double* __restrict__ G, int k, double* __restrict__ lam,
+                                           double* __restrict__ Vout, int* __restrict__ status,
+                                           unsigned long long* __restrict__ keys, int max_sweeps, const int NT) {
+  __shared__ double a[2][KMAX][KMAX + 1];
+  __shared__ double v[KMAX][KMAX + 1];
+  __shared__ double red[16];
+  __shared__ int done;
+  const int NW = NT >> 6;
+  const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+  if (keys && tid < 64) keys[tid] = 0ull;
+  const int k2 = k + (k & 1);   // an odd size gets one zero row / column (its rotations are identities)
+  const int np = k2 / 2, M = k2 - 1;
+  double tot = 0.0;
+  for (int e = tid < NT ? tid : k2 * k2; e < k2 * k2; e += NT) {
+    const int i = e / k2, j = e % k2;
+    // the Gram passes accumulate (i, j) and (j, i) in different orders: use the symmetric part
+    const double x = (i < k && j < k) ? 0.5 * (G[i * k + j] + G[j * k + i]) : 0.0;
+    a[0][i][j] = x;
+    v[i][j] = (i == j) ? 1.0 : 0.0;
+    tot += x * x;
+  }
+  // ||A||_F^2 is invariant under the rotations: one reduction up front
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) tot += __shfl_xor(tot, o);
+  if (lane == 0) red[wv] = tot;
+  __syncthreads();
+  tot = 0.0;
+  for (int w = 0; w < NW; ++w) tot += red[w];
+  const int GP = 64 / np;
+  const int g = lane / np, bwl = lane - g * np, bul = wv * GP + g;
+  const bool act = tid < NT && g < GP && bul < np;
+  // idle lanes follow pair 0's indices (in bounds) and write nothing
+  const int bu = act ? bul : 0, bw = act ? bwl : 0;
+  const int src = act ? g * np + bu : lane;   // the lane of the diagonal block (bu, bu)
+  // circle-method pair u at round t: (t, M) for u = 0, else ((t + u) mod M, (t - u) mod M); both
+  // ends advance by one (mod M) per round, except the fixed end M of pair 0
+  auto wrap = [M](int x) { return x >= M ? x - M : x; };
+  int sweeps = 0, cur = 0;
+  bool converged = !(tot > 0.0);   // a zero matrix is diagonal
+  while (sweeps < max_sweeps && tot > 0.0) {
+    ++sweeps;
+    double off = 0.0;
+    int x1 = bu, y1 = bu == 0 ? M : M - bu, x2 = bw, y2 = bw == 0 ? M : M - bw;
+    for (int rnd = 0; rnd < M; ++rnd) {
+      const int p1 = x1 < y1 ? x1 : y1, q1 = x1 < y1 ? y1 : x1;
+      const int p2 = x2 < y2 ? x2 : y2, q2 = x2 < y2 ? y2 : x2;
+      const double(*A0)[KMAX + 1] = a[cur];
+      double(*A1)[KMAX + 1] = a[cur ^ 1];
+      // this round's entries were written before the previous barrier
+      double dpp = A0[p2][p2], dqq = A0[q2][q2], dpq = A0[p2][q2];
+      double app = A0[p1][p2], apq = A0[p1][q2], aqp = A0[q1][p2], aqq = A0[q1][q2];
+      // all seven loads in flight together: left alone, the compiler sinks the block's four into the
+      // store branch, a second LDS round trip after the angles
+      asm volatile("" : "+v"(dpp), "+v"(dqq), "+v"(dpq), "+v"(app), "+v"(apq), "+v"(aqp), "+v"(aqq));
+      const double vpp = v[p1][p2], vpq = v[p1][q2], vqp = v[q1][p2], vqq = v[q1][q2];
+      double c2, s2;
+      jacobi_angle_sel(dpp, dqq, dpq, c2, s2);
+      const double c1 = __shfl(c2, src), s1 = __shfl(s2, src);
+      if (act) {
+        v[p1][p2] = c2 * vpp - s2 * vpq;
+        v[p1][q2] = s2 * vpp + c2 * vpq;
+        v[q1][p2] = c2 * vqp - s2 * vqq;
+        v[q1][q2] = s2 * vqp + c2 * vqq;
+        const double rpp = c1 * app - s1 * aqp, rpq = c1 * apq - s1 * aqq;
+        const double rqp = s1 * app + c1 * aqp, rqq = s1 * apq + c1 * aqq;
+        const double npp = c2 * rpp - s2 * rpq, npq = s2 * rpp + c2 * rpq;
+        const double nqp = c2 * rqp - s2 * rqq, nqq = s2 * rqp + c2 * rqq;
+        A1[p1][p2] = npp;
+        A1[p1][q2] = npq;
+        A1[q1][p2] = nqp;
+        A1[q1][q2] = nqq;
+        if (rnd == M - 1) {
+          off = npq * npq + nqp * nqp;
+          if (bu != bw) off += npp * npp + nqq * nqq;
+        }
+      }
+      __syncthreads();
+      cur ^= 1;
+      x1 = wrap(x1 + 1);
+      x2 = wrap(x2 + 1);
+      if (bu != 0) y1 = wrap(y1 + 1);
+      if (bw != 0) y2 = wrap(y2 + 1);
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) off += __shfl_xor(off, o);
+    if (lane == 0) red[wv] = off;
+    __syncthreads();
+    if (tid == 0) {
+      double sum = 0.0;
+      for (int w = 0; w < NW; ++w) sum += red[w];
+      done = sum <= 0.5e-24 * tot;
+    }
+    __syncthreads();
+    if (done) { converged = true; break; }
+  }
+  // sort descending: rank of each eigenvalue (ties broken by index)
+  const double(*AF)[KMAX + 1] = a[cur];
+  for (int i = tid < NT ? tid : k; i < k; i += NT) {
+    const double li = AF[i][i];
+    int rank = 0;
+    for (int j = 0; j < k; ++j) {
+      const double lj = AF[j][j];
+      if (lj > li || (lj == li && j < i)) ++rank;
+    }
+    lam[rank] = li;
+    for (int r = 0; r < k; ++r) Vout[r * k + rank] = v[r][i];
+  }
+  if (tid == 0 && status) {
+    status[1] = sweeps;
+    status[2] = converged ? 0 : 1;
+  }
+}
 __global__ __launch_bounds__(1024) void k_syevj(const double* __restrict__ G, int k, double* __restrict__ lam,
                                                double* __restrict__ Vout, int* __restrict__ status,
                                                unsigned long long* __restrict__ keys, int max_sweeps) {
-  syevj_body(G, k, lam, Vout, status, keys, max_sweeps, (int)blockDim.x);
+  syevj1_body(G, k, lam, Vout, status, keys, max_sweeps, (int)blockDim.x);
 }
 // one 1024-thread block per job, each job working with its own syevj_threads(k)
 __global__ __launch_bounds__(1024) void k_syevj_grp(const JobSyevj* __restrict__ jobs, const int2* __restrict__ map,
@@ -608,9 +765,17 @@ int syevj_threads(int k) {
   return t < 64 ? 64 : t;
 }
 
+// threads for a k x k problem: whole waves of floor(64 / np) row pairs of np lanes each (at most
+// 16 waves for k = 64)
+static int syevj1_threads(int k) {
+  const int np = k > 1 ? (k + (k & 1)) / 2 : 1;
+  const int gp = 64 / np;
+  return 64 * ((np + gp - 1) / gp);
+}
+
 void launch_syevj(const double* G, int k, double* lam, double* V, int* status, unsigned long long* keys,
                   hipStream_t st, int max_sweeps) {
-  hipLaunchKernelGGL(k_syevj, dim3(1), dim3(syevj_threads(k)), 0, st, G, k, lam, V, status, keys, max_sweeps);
+  hipLaunchKernelGGL(k_syevj, dim3(1), dim3(syevj1_threads(k)), 0, st, G, k, lam, V, status, keys, max_sweeps);
 }
 
 // ------------------------------------------------------------------------------------------------
