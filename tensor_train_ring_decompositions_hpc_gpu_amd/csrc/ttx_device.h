@@ -111,8 +111,11 @@ struct u32x4 { uint32_t x, y, z, w; };
 __device__ __forceinline__ u32x4 philox4x32_10(u32x4 c, uint32_t k0, uint32_t k1) {
 #pragma unroll
   for (int r = 0; r < 10; ++r) {
-    const uint32_t lo0 = 0xD2511F53u * c.x, hi0 = __umulhi(0xD2511F53u, c.x);
-    const uint32_t lo1 = 0xCD9E8D57u * c.z, hi1 = __umulhi(0xCD9E8D57u, c.z);
+    // each 32 x 32 -> 64-bit product as one multiply (v_mad_u64_u32) rather than its low and high
+    // halves apart (v_mul_lo_u32 and v_mul_hi_u32)
+    const uint64_t p0 = (uint64_t)0xD2511F53u * c.x, p1 = (uint64_t)0xCD9E8D57u * c.z;
+    const uint32_t lo0 = (uint32_t)p0, hi0 = (uint32_t)(p0 >> 32);
+    const uint32_t lo1 = (uint32_t)p1, hi1 = (uint32_t)(p1 >> 32);
     u32x4 n;
     n.x = hi1 ^ c.y ^ k0;
     n.y = lo1;
