@@ -920,17 +920,44 @@ __device__ __forceinline__ void next_rows_body(const float* __restrict__ Z, int6
     }
   float* zt = zs[w];
   const int64_t tiles = (n + 31) / 32;
-  for (int64_t t = (int64_t)B.x * 4 + w; t < tiles; t += (int64_t)B.gx * 4) {
+  // a tile's 32 kq float4 pieces, piece e = lane + 64 u at row e / kq, columns 4 (e % kq) .. + 3: all
+  // loads of a tile are issued at once, unconditionally (row and piece clamped into the matrix, the
+  // pieces outside it zeroed after the load), and the next tile's are in flight while this one is
+  // staged, multiplied and written (a load-wait-store loop per piece left one load in flight)
+  constexpr int NU = 32 * (KMAX / 4) / 64;
+  const int npc = 32 * kq;
+  auto load_tile = [&](int64_t j0, f32x4 (&v)[NU]) {
+#pragma unroll
+    for (int u = 0; u < NU; ++u) {
+      const int e = lane + 64 * u;
+      const int ec = e < npc ? e : npc - 1;
+      const int rr = ec / kq, q = ec - rr * kq;
+      const int64_t row = (j0 + rr < n) ? j0 + rr : n - 1;
+      v[u] = *reinterpret_cast<const f32x4*>(Z + row * ldz + 4 * q);
+    }
+  };
+  int64_t t = (int64_t)B.x * 4 + w;
+  f32x4 vn[NU];
+  if (t < tiles) load_tile(t * 32, vn);
+  for (; t < tiles; t += (int64_t)B.gx * 4) {
     const int64_t j0 = t * 32;
     const int rows = (n - j0 >= 32) ? 32 : (int)(n - j0);
+    f32x4 v[NU];
+#pragma unroll
+    for (int u = 0; u < NU; ++u) v[u] = vn[u];
+    const int64_t tn = t + (int64_t)B.gx * 4;
+    load_tile((tn < tiles ? tn : t) * 32, vn);
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
-    for (int e = lane; e < 32 * kq; e += 64) {
-      const int rr = e / kq, q = e - rr * kq;
-      f32x4 v = {0.0f, 0.0f, 0.0f, 0.0f};
-      if (rr < rows) v = *reinterpret_cast<const f32x4*>(Z + (j0 + rr) * ldz + 4 * q);
-      float* d = zt + rr * P + 4 * q;
-      d[0] = v[0]; d[1] = v[1]; d[2] = v[2]; d[3] = v[3];
+#pragma unroll
+    for (int u = 0; u < NU; ++u) {
+      const int e = lane + 64 * u;
+      if (e < npc) {
+        const int rr = e / kq, q = e - rr * kq;
+        const bool ok = rr < rows;
+        float* d = zt + rr * P + 4 * q;
+        d[0] = ok ? v[u][0] : 0.0f; d[1] = ok ? v[u][1] : 0.0f; d[2] = ok ? v[u][2] : 0.0f; d[3] = ok ? v[u][3] : 0.0f;
+      }
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
@@ -948,9 +975,9 @@ __device__ __forceinline__ void next_rows_body(const float* __restrict__ Z, int6
 #pragma unroll
     for (int nt = 0; nt < NT; ++nt)
 #pragma unroll
-      for (int v = 0; v < 16; ++v) {
-        const int rr = mfma32_row(v, lane), c = 32 * nt + (lane & 31);
-        if (rr < rows && c < r) out[(j0 + rr) * r + c] = acc[nt][v];
+      for (int v2 = 0; v2 < 16; ++v2) {
+        const int rr = mfma32_row(v2, lane), c = 32 * nt + (lane & 31);
+        if (rr < rows && c < r) out[(j0 + rr) * r + c] = acc[nt][v2];
       }
   }
 }
